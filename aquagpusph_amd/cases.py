@@ -1,0 +1,151 @@
+"""Particle-case generators (the role of the reference's examples/*/Create.py).
+
+Each generator returns a plain dict:
+    dims, N, r, normal, tangent, u, dudt, rho, drhodt, m, imove, iset
+    params: h, cs, courant, g, refd/visc_dyn/delta (per set), dt_min, dt_Ma,
+            p0, domain_min, domain_max, dr
+Positions are float64 lattices; consumers narrow them to float32.
+
+Geometry of the dam break follows the reference's 2D example
+(examples/2D/lobovsky_etal_2014/src/Create.py:41-140): a fluid column of
+width B and height H at the left end of a tank of length L and height D,
+cell-centred lattice of spacing dr, one layer of boundary-integral elements
+(imove = -3, area stored in m, outward normals) on every tank face.
+Physical set-up: rho0 = 998, c0 = 10 sqrt(g H), h = hfac dr, hydrostatic
+initial pressure with the density from the linear EOS, artificial viscosity
+alpha folded into visc_dyn = max(alpha/8 rho0 h c0, mu) in 2D and alpha/10 in
+3D (examples/2D/spheric_testcase5_dambreak/src/Create.py:63,
+examples/3D/spheric_testcase2_dambreak/src/Create.py:73).
+"""
+import math
+
+import numpy as np
+
+G = 9.81
+
+
+def _lattice(*axes):
+    grids = np.meshgrid(*axes, indexing="ij")
+    return np.stack([g.ravel() for g in grids], axis=1)
+
+
+def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
+              refd=998.0, alpha=0.02, mu=8.9e-4, delta=0.1, courant=0.1,
+              walls=("bottom", "top", "left", "right", "front", "back")):
+    """Dam break with one layer of BI wall elements.
+
+    2D: tank [-B, L-B] x [0, D], column [-B, 0] x [0, H] (g along -y).
+    3D: tank [-B, L-B] x [0, W] x [0, D], column [-B, 0] x [0, W] x [0, H]
+        (g along -z).
+    """
+    if L is None:
+        L = 1.61 if dims == 2 else 1.6
+    nx = int(round(B / dr))
+    nh = int(round(H / dr))
+    Nx = int(round(L / dr))
+    Nd = int(round(D / dr))
+    A = Nx * dr - B
+    Dt = Nd * dr
+    h = hfac * dr
+    cs = 10.0 * math.sqrt(G * H)
+    vs = 2 if dims == 2 else 4
+    vert = dims - 1
+    xs = -B + (np.arange(nx) + 0.5) * dr
+    hs = (np.arange(nh) + 0.5) * dr
+    if dims == 2:
+        fluid = _lattice(xs, hs)
+    else:
+        ny = int(round(W / dr))
+        Wt = ny * dr
+        ys = (np.arange(ny) + 0.5) * dr
+        fluid = _lattice(xs, ys, hs)
+    nf = fluid.shape[0]
+
+    # wall elements: (positions, normal, tangent)
+    els = []
+    xe = -B + (np.arange(Nx) + 0.5) * dr
+    de = (np.arange(Nd) + 0.5) * dr
+    if dims == 2:
+        if "bottom" in walls:
+            els.append((_lattice(xe, [0.0]), (0, -1), (0, 0)))
+        if "top" in walls:
+            els.append((_lattice(xe, [Dt]), (0, 1), (0, 0)))
+        if "left" in walls:
+            els.append((_lattice([-B], de), (-1, 0), (0, 0)))
+        if "right" in walls:
+            els.append((_lattice([A], de), (1, 0), (0, 0)))
+    else:
+        ye = (np.arange(ny) + 0.5) * dr
+        if "bottom" in walls:
+            els.append((_lattice(xe, ye, [0.0]), (0, 0, -1), (1, 0, 0)))
+        if "top" in walls:
+            els.append((_lattice(xe, ye, [Dt]), (0, 0, 1), (1, 0, 0)))
+        if "left" in walls:
+            els.append((_lattice([-B], ye, de), (-1, 0, 0), (0, 1, 0)))
+        if "right" in walls:
+            els.append((_lattice([A], ye, de), (1, 0, 0), (0, 1, 0)))
+        if "front" in walls:
+            els.append((_lattice(xe, [0.0], de), (0, -1, 0), (1, 0, 0)))
+        if "back" in walls:
+            els.append((_lattice(xe, [Wt], de), (0, 1, 0), (1, 0, 0)))
+    ne = sum(e[0].shape[0] for e in els)
+    N = nf + ne
+
+    r = np.zeros((N, vs))
+    normal = np.zeros((N, vs))
+    tangent = np.zeros((N, vs))
+    r[:nf, :dims] = fluid
+    o = nf
+    for pos, n, t in els:
+        k = pos.shape[0]
+        r[o:o + k, :dims] = pos
+        normal[o:o + k, :dims] = n[:dims]
+        tangent[o:o + k, :dims] = t[:dims]
+        o += k
+    imove = np.full(N, -3, np.int32)
+    imove[:nf] = 1
+    press = np.zeros(N)
+    press[:nf] = refd * G * (H - fluid[:, vert])
+    rho = refd + press / cs ** 2
+    rho[nf:] = refd
+    m = np.empty(N)
+    m[:nf] = refd * dr ** dims
+    m[nf:] = dr ** (dims - 1)
+
+    visc = max(alpha / (8.0 if dims == 2 else 10.0) * refd * h * cs, mu)
+    g = [0.0] * dims
+    g[vert] = -G
+    lo = [-B - 0.25 * L] + [-0.25 * D] * (dims - 1)
+    hi = [A + 0.25 * L] + [1.5 * D] * (dims - 1)
+    if dims == 3:
+        lo[1], hi[1] = -0.25 * W, 1.25 * W
+    params = dict(h=h, cs=cs, courant=courant, g=g, refd=[refd],
+                  visc_dyn=[visc], delta=[delta], p0=0.0,
+                  dt_min=0.25 * courant * h / cs, dt_Ma=0.1,
+                  domain_min=lo, domain_max=hi, dr=dr, hfac=hfac)
+    return dict(dims=dims, N=N, n_fluid=nf, r=r, normal=normal,
+                tangent=tangent, u=np.zeros((N, vs)), dudt=np.zeros((N, vs)),
+                rho=rho, drhodt=np.zeros(N), m=m, imove=imove,
+                iset=np.zeros(N, np.uint32), params=params)
+
+
+def dam_break_dr_for(n_fluid, dims=3, B=0.6, H=0.3, W=0.6):
+    """Lattice spacing that gives about n_fluid particles in the column."""
+    vol = B * H * (W if dims == 3 else 1.0)
+    return (vol / n_fluid) ** (1.0 / dims)
+
+
+def random_box(dims, n, seed=0, lo=0.0, hi=1.0, dr=None):
+    """Seeded random fluid particles in a box (interaction unit tests)."""
+    rng = np.random.default_rng(seed)
+    vs = 2 if dims == 2 else 4
+    r = np.zeros((n, vs))
+    r[:, :dims] = rng.uniform(lo, hi, (n, dims))
+    if dr is None:
+        dr = (hi - lo) / n ** (1.0 / dims)
+    u = np.zeros((n, vs))
+    u[:, :dims] = rng.normal(0.0, 0.1, (n, dims))
+    rho = 1000.0 + rng.normal(0.0, 1.0, n)
+    return dict(dims=dims, N=n, r=r, u=u, rho=rho,
+                m=np.full(n, 1000.0 * dr ** dims),
+                imove=np.ones(n, np.int32), dr=dr)

@@ -1,0 +1,194 @@
+// Device-side building blocks: vec storage, the Wendland C2 kernel and the
+// link-list neighbour walk.  All FP32 expressions keep the association of the
+// reference OpenCL sources (the library is compiled with -ffp-contract=off).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+extern "C" __device__ float __ocml_acospi_f32(float);
+
+namespace aq {
+
+// ---- vec storage: float2 in 2D, float4 in 3D (types/2D.h:23, 3D.h:23) ----
+template <int D> struct VecT;
+template <> struct VecT<2> {
+    typedef float2 T;
+};
+template <> struct VecT<3> {
+    typedef float4 T;
+};
+
+struct f3 {
+    float x, y, z;
+};
+
+template <int D> __device__ __forceinline__ f3 ldv(const float* a, uint32_t i)
+{
+    f3 v;
+    if constexpr (D == 2) {
+        const float2 t = reinterpret_cast<const float2*>(a)[i];
+        v.x = t.x;
+        v.y = t.y;
+        v.z = 0.f;
+    } else {
+        const float4 t = reinterpret_cast<const float4*>(a)[i];
+        v.x = t.x;
+        v.y = t.y;
+        v.z = t.z;
+    }
+    return v;
+}
+
+// store the XYZ part, leaving w untouched (the reference writes .XYZ)
+template <int D>
+__device__ __forceinline__ void stxyz(float* a, uint32_t i, const f3& v)
+{
+    if constexpr (D == 2) {
+        reinterpret_cast<float2*>(a)[i] = make_float2(v.x, v.y);
+    } else {
+        float* p = a + 4 * (size_t)i;
+        p[0] = v.x;
+        p[1] = v.y;
+        p[2] = v.z;
+    }
+}
+
+template <int D> __device__ __forceinline__ f3 sub(const f3& a, const f3& b)
+{
+    f3 v;
+    v.x = a.x - b.x;
+    v.y = a.y - b.y;
+    v.z = (D == 3) ? a.z - b.z : 0.f;
+    return v;
+}
+
+// dot over the first D components, left to right
+template <int D> __device__ __forceinline__ float dot(const f3& a, const f3& b)
+{
+    float s = a.x * b.x;
+    s = s + a.y * b.y;
+    if constexpr (D == 3) s = s + a.z * b.z;
+    return s;
+}
+
+template <int D>
+__device__ __forceinline__ void axpy(f3& acc, float s, const f3& v)
+{
+    acc.x += s * v.x;
+    acc.y += s * v.y;
+    if constexpr (D == 3) acc.z += s * v.z;
+}
+
+// ---- Wendland C2 (KernelFunctions/Wendland{2D,3D}.hcl) --------------------
+#define AQ_IPI 0.318309886f
+
+template <int D> __device__ __forceinline__ float kernelW(float q)
+{
+    const float wcon = (D == 2) ? 0.109375f * AQ_IPI : 0.08203125f * AQ_IPI;
+    const float a = 2.f - q;
+    return wcon * (1.f + 2.f * q) * a * a * a * a;
+}
+
+template <int D> __device__ __forceinline__ float kernelF(float q)
+{
+    const float wcon = (D == 2) ? 1.09375f * AQ_IPI : 0.8203125f * AQ_IPI;
+    const float a = 2.f - q;
+    return wcon * a * a * a;
+}
+
+template <int D> __device__ __forceinline__ float kernelS_P(float q)
+{
+    const float q2 = q * q;
+    const float q3 = q2 * q;
+    if constexpr (D == 2) {
+        const float wcon = 0.109375f * AQ_IPI;
+        return wcon * (0.285714f * q3 * q2 - 2.5f * q2 * q2 + 8.f * q3 -
+                       10.f * q2 + 8.f);
+    } else {
+        const float wcon = 0.08203125f * AQ_IPI;
+        return wcon * (0.25f * q3 * q2 - 2.142857f * q2 * q2 +
+                       6.666667f * q3 - 8.f * q2 + 5.333333f);
+    }
+}
+
+__device__ __forceinline__ float cl_sign(float x)
+{
+    if (x > 0.f) return 1.f;
+    if (x < 0.f) return -1.f;
+    if (x != x) return 0.f;
+    return x;
+}
+__device__ __forceinline__ float cl_min(float x, float y)
+{
+    return (y < x) ? y : x;
+}
+__device__ __forceinline__ float cl_max(float x, float y)
+{
+    return (x < y) ? y : x;
+}
+
+__device__ __forceinline__ float omega3(float a, float b)
+{
+    const float a2 = a * a;
+    const float b2 = b * b;
+    return __ocml_acospi_f32(cl_min(
+        sqrtf((1.f + a2 + b2) / ((1.f + a2) * (1.f + b2))), 1.f));
+}
+
+template <int D>
+__device__ __forceinline__ float kernelS_D(float d, float t, float b, float s)
+{
+    if constexpr (D == 2) {
+        const float wcon = 0.5f * AQ_IPI;
+        const float dr = 0.5f * s;
+        return -wcon * (atanf((t + dr) / d) - atanf((t - dr) / d));
+    } else {
+        const float wcon = 0.25f;
+        const float dr = 0.5f * sqrtf(s);
+        const float t1 = (t - dr) / d, t2 = (t + dr) / d;
+        const float b1 = (b - dr) / d, b2 = (b + dr) / d;
+        const float st = cl_sign(t1), sb = cl_sign(b1);
+        return -wcon * (omega3(t2, b2) - st * omega3(t1, b2) -
+                        sb * omega3(t2, b1) + st * sb * omega3(t1, b1));
+    }
+}
+
+// ---- neighbour walk (types/3D.h:197-219, types/2D.h:174-193) --------------
+// Calls body(j) for every particle j of the 3^D cells around c_i, in the
+// reference order: x offset outermost, then y, then z; in each cell the
+// sorted indices ascending while icell[j] == cell.
+template <int D, class Body>
+__device__ __forceinline__ void walk(uint32_t c_i, uint32_t N, uint32_t nx,
+                                     uint32_t ny, const uint32_t* __restrict__ icell,
+                                     const uint32_t* __restrict__ ihoc, Body body)
+{
+#pragma unroll 1
+    for (int ci = -1; ci <= 1; ci++) {
+#pragma unroll 1
+        for (int cj = -1; cj <= 1; cj++) {
+            if constexpr (D == 2) {
+                const uint32_t c_j = c_i + (uint32_t)ci + (uint32_t)cj * nx;
+                uint32_t j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    body(j);
+                    j++;
+                }
+            } else {
+#pragma unroll 1
+                for (int ck = -1; ck <= 1; ck++) {
+                    const uint32_t c_j = c_i + (uint32_t)ci +
+                                         (uint32_t)cj * nx +
+                                         (uint32_t)ck * nx * ny;
+                    uint32_t j = ihoc[c_j];
+                    while ((j < N) && (icell[j] == c_j)) {
+                        body(j);
+                        j++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace aq

@@ -1,0 +1,353 @@
+// Link-list: bounds, cell hash and head-of-chain table.
+//
+// Replaces the reference LinkList tool (aquagpusph/CalcServer/LinkList.cpp
+// :326-494, kernels LinkList.cl.in:32-113).  The reference reads r_min and
+// r_max back to the host every step to size the grid and reallocate ihoc;
+// here the grid size n_cells is computed on the device with the same float
+// arithmetic and kept in device memory, and ihoc lives in a buffer of fixed
+// capacity, so the whole link-list can be replayed from a HIP graph.  An
+// overflowing grid sets bit 0 of a sticky status word and the later kernels
+// skip their work; the caller grows the capacity and re-runs.
+#include "aq_device.hpp"
+#include "aq_internal.hpp"
+
+namespace aq {
+
+void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
+                unsigned key_bits, void* workspace, hipStream_t s);
+size_t radix_sort_workspace(uint32_t n);
+unsigned radix_key_bits(uint64_t max_val);
+
+constexpr int MM_BS = 256;
+constexpr int MM_MAXB = 1024;
+
+// float -> uint with saturation (NaN and negatives -> 0, >= 2^32 ->
+// UINT_MAX); the OpenCL cast is implementation-defined outside [0, 2^32)
+__device__ __forceinline__ uint32_t cvt_u32(float x)
+{
+    if (!(x > 0.f)) return 0u;
+    if (x >= 4294967296.f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+template <int D>
+__global__ void __launch_bounds__(MM_BS)
+    minmax_partial(const float* __restrict__ r, uint32_t N,
+                   float* __restrict__ part)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    float mn[VS], mx[VS];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        mn[k] = (k == 3) ? 0.f : INFINITY;
+        mx[k] = (k == 3) ? -0.f : -INFINITY;
+    }
+    for (uint32_t i = blockIdx.x * MM_BS + threadIdx.x; i < N;
+         i += gridDim.x * MM_BS) {
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            const float v = r[(size_t)i * VS + k];
+            mn[k] = cl_min(mn[k], v);
+            mx[k] = cl_max(mx[k], v);
+        }
+    }
+    __shared__ float smn[VS][MM_BS], smx[VS][MM_BS];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        smn[k][threadIdx.x] = mn[k];
+        smx[k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int o = MM_BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+#pragma unroll
+            for (int k = 0; k < VS; k++) {
+                smn[k][threadIdx.x] =
+                    cl_min(smn[k][threadIdx.x], smn[k][threadIdx.x + o]);
+                smx[k][threadIdx.x] =
+                    cl_max(smx[k][threadIdx.x], smx[k][threadIdx.x + o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            part[(size_t)blockIdx.x * 2 * VS + k] = smn[k][0];
+            part[(size_t)blockIdx.x * 2 * VS + VS + k] = smx[k][0];
+        }
+    }
+}
+
+// final min/max + LinkList::nCells (LinkList.cpp:185-232) + capacity check
+template <int D>
+__global__ void __launch_bounds__(MM_BS)
+    grid_final(const float* __restrict__ part, uint32_t nb, float* r_min,
+               float* r_max, const float* support, const float* h,
+               uint32_t* n_cells, uint32_t capacity, uint32_t* status)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    __shared__ float smn[VS][MM_BS], smx[VS][MM_BS];
+    float mn[VS], mx[VS];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        mn[k] = (k == 3) ? 0.f : INFINITY;
+        mx[k] = (k == 3) ? -0.f : -INFINITY;
+    }
+    for (uint32_t b = threadIdx.x; b < nb; b += MM_BS) {
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            mn[k] = cl_min(mn[k], part[(size_t)b * 2 * VS + k]);
+            mx[k] = cl_max(mx[k], part[(size_t)b * 2 * VS + VS + k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        smn[k][threadIdx.x] = mn[k];
+        smx[k][threadIdx.x] = mx[k];
+    }
+    __syncthreads();
+    for (int o = MM_BS / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+#pragma unroll
+            for (int k = 0; k < VS; k++) {
+                smn[k][threadIdx.x] =
+                    cl_min(smn[k][threadIdx.x], smn[k][threadIdx.x + o]);
+                smx[k][threadIdx.x] =
+                    cl_max(smx[k][threadIdx.x], smx[k][threadIdx.x + o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float lo[VS], hi[VS];
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            lo[k] = smn[k][0];
+            hi[k] = smx[k][0];
+            r_min[k] = lo[k];
+            r_max[k] = hi[k];
+        }
+        const float cell_length = (*support) * (*h);
+        uint64_t n[3] = { 1, 1, 1 };
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const float v = (hi[k] - lo[k]) / cell_length;
+            n[k] = (uint64_t)cvt_u32(v) + 6;
+        }
+        const uint64_t w = n[0] * n[1] * n[2];
+        if (w > (uint64_t)capacity) {
+            // keep the previous, consistent grid; flag the overflow
+            *status |= 1u;
+        } else {
+            n_cells[0] = (uint32_t)n[0];
+            n_cells[1] = (uint32_t)n[1];
+            n_cells[2] = (uint32_t)n[2];
+            n_cells[3] = (uint32_t)w;
+        }
+    }
+}
+
+// iCell, LinkList.cl.in:54-85
+template <int D>
+__global__ void icell_k(uint32_t* __restrict__ icell,
+                        const float* __restrict__ r, uint32_t N,
+                        const float* r_min, const float* support,
+                        const float* h, const uint32_t* n_cells,
+                        const uint32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (status && (*status & 1u)) return;
+    const float idist = 1.f / ((*support) * (*h));
+    const uint32_t nx = n_cells[0], ny = n_cells[1];
+    if constexpr (D == 2) {
+        const float2 p = reinterpret_cast<const float2*>(r)[i];
+        const uint32_t cx =
+            cvt_u32((p.x - r_min[0]) * idist) + 3u;
+        const uint32_t cy =
+            cvt_u32((p.y - r_min[1]) * idist) + 3u;
+        icell[i] = cx - 1u + (cy - 1u) * nx;
+    } else {
+        const float4 p = reinterpret_cast<const float4*>(r)[i];
+        const uint32_t cx =
+            cvt_u32((p.x - r_min[0]) * idist) + 3u;
+        const uint32_t cy =
+            cvt_u32((p.y - r_min[1]) * idist) + 3u;
+        const uint32_t cz =
+            cvt_u32((p.z - r_min[2]) * idist) + 3u;
+        icell[i] = cx - 1u + (cy - 1u) * nx + (cz - 1u) * nx * ny;
+    }
+}
+
+// iHoc, LinkList.cl.in:32-42
+__global__ void ihoc_fill(uint32_t* __restrict__ ihoc, uint32_t N,
+                          const uint32_t* n_cells, uint32_t capacity,
+                          const uint32_t* status)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= capacity || c >= n_cells[3]) return;
+    if (status && (*status & 1u)) return;
+    ihoc[c] = N;
+}
+
+// linkList, LinkList.cl.in:92-113
+__global__ void ihoc_heads(const uint32_t* __restrict__ icell,
+                           uint32_t* __restrict__ ihoc, uint32_t N,
+                           const uint32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N - 1) return;
+    if (status && (*status & 1u)) return;
+    const uint32_t c = icell[i];
+    const uint32_t c2 = icell[i + 1];
+    if (i == 0) ihoc[c] = 0;
+    if (c2 != c) ihoc[c2] = i + 1;
+}
+
+// ---- host drivers ---------------------------------------------------------
+size_t linklist_workspace(uint32_t N)
+{
+    const size_t mm = (size_t)MM_MAXB * 2 * 4 * sizeof(float);
+    return ((mm + 255) & ~(size_t)255) + radix_sort_workspace(N);
+}
+
+void linklist_grid(int dims, const float* r, uint32_t N, float* r_min,
+                   float* r_max, const float* support, const float* h,
+                   uint32_t* n_cells, uint32_t cap, uint32_t* status,
+                   void* ws, hipStream_t s)
+{
+    const unsigned nb =
+        std::min<unsigned>(MM_MAXB, blocks_for(N, MM_BS * 8));
+    float* part = (float*)ws;
+    if (dims == 2) {
+        hipLaunchKernelGGL(minmax_partial<2>, dim3(nb), dim3(MM_BS), 0, s, r,
+                           N, part);
+        hipLaunchKernelGGL(grid_final<2>, dim3(1), dim3(MM_BS), 0, s, part,
+                           nb, r_min, r_max, support, h, n_cells, cap,
+                           status);
+    } else {
+        hipLaunchKernelGGL(minmax_partial<3>, dim3(nb), dim3(MM_BS), 0, s, r,
+                           N, part);
+        hipLaunchKernelGGL(grid_final<3>, dim3(1), dim3(MM_BS), 0, s, part,
+                           nb, r_min, r_max, support, h, n_cells, cap,
+                           status);
+    }
+}
+
+void linklist_icell(int dims, uint32_t* icell, const float* r, uint32_t N,
+                    const float* r_min, const float* support, const float* h,
+                    const uint32_t* n_cells, const uint32_t* status,
+                    hipStream_t s)
+{
+    const dim3 g(blocks_for(N, 256)), b(256);
+    if (dims == 2)
+        hipLaunchKernelGGL(icell_k<2>, g, b, 0, s, icell, r, N, r_min,
+                           support, h, n_cells, status);
+    else
+        hipLaunchKernelGGL(icell_k<3>, g, b, 0, s, icell, r, N, r_min,
+                           support, h, n_cells, status);
+}
+
+void linklist_ihoc(const uint32_t* icell, uint32_t* ihoc, uint32_t N,
+                   const uint32_t* n_cells, uint32_t cap,
+                   const uint32_t* status, hipStream_t s)
+{
+    hipLaunchKernelGGL(ihoc_fill, dim3(blocks_for(cap, 256)), dim3(256), 0, s,
+                       ihoc, N, n_cells, cap, status);
+    if (N > 1)
+        hipLaunchKernelGGL(ihoc_heads, dim3(blocks_for(N - 1, 256)),
+                           dim3(256), 0, s, icell, ihoc, N, status);
+}
+
+void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
+                  float* r_max, const float* support, const float* h,
+                  uint32_t* n_cells, uint32_t* icell, uint32_t* id_unsorted,
+                  uint32_t* id_sorted, uint32_t* ihoc, uint32_t cap,
+                  uint32_t* status, void* ws, hipStream_t s)
+{
+    linklist_grid(dims, r, N, r_min, r_max, support, h, n_cells, cap, status,
+                  ws, s);
+    linklist_icell(dims, icell, r, N, r_min, support, h, n_cells, status, s);
+    const size_t mm = (size_t)MM_MAXB * 2 * 4 * sizeof(float);
+    void* sws = (char*)ws + ((mm + 255) & ~(size_t)255);
+    // icell < n_cells.w <= cap, so sorting on the bits of cap gives the
+    // reference result (it sorts on the bits of n_cells.w)
+    radix_sort(icell, id_unsorted, id_sorted, N, radix_key_bits(cap), sws, s);
+    linklist_ihoc(icell, ihoc, N, n_cells, cap, status, s);
+}
+
+}  // namespace aq
+
+using namespace aq;
+
+extern "C" {
+
+AQ_API size_t aq_linklist_workspace(uint32_t N) { return linklist_workspace(N); }
+
+AQ_API int aq_linklist_grid(int dims, const float* r, uint32_t N,
+                            float* r_min, float* r_max, const float* support,
+                            const float* h, uint32_t* n_cells,
+                            uint32_t ihoc_capacity, uint32_t* status,
+                            void* workspace, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        AQ_REQUIRE(N > 0, "aq_linklist_grid: N must be > 0");
+        AQ_REQUIRE(r && r_min && r_max && support && h && n_cells && status &&
+                       workspace,
+                   "aq_linklist_grid: null pointer");
+        linklist_grid(dims, r, N, r_min, r_max, support, h, n_cells,
+                      ihoc_capacity, status, workspace, S(stream));
+    });
+}
+
+AQ_API int aq_linklist_icell(int dims, uint32_t* icell, const float* r,
+                             uint32_t N, const float* r_min,
+                             const float* support, const float* h,
+                             const uint32_t* n_cells, const uint32_t* status,
+                             void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        AQ_REQUIRE(icell && r && r_min && support && h && n_cells,
+                   "aq_linklist_icell: null pointer");
+        if (N) linklist_icell(dims, icell, r, N, r_min, support, h, n_cells,
+                              status, S(stream));
+    });
+}
+
+AQ_API int aq_linklist_ihoc(const uint32_t* icell, uint32_t* ihoc,
+                            uint32_t N, const uint32_t* n_cells,
+                            uint32_t ihoc_capacity, const uint32_t* status,
+                            void* stream)
+{
+    return guard([&] {
+        AQ_REQUIRE(icell && ihoc && n_cells, "aq_linklist_ihoc: null pointer");
+        AQ_REQUIRE(N > 0, "aq_linklist_ihoc: N must be > 0");
+        linklist_ihoc(icell, ihoc, N, n_cells, ihoc_capacity, status,
+                      S(stream));
+    });
+}
+
+AQ_API int aq_linklist(int dims, const float* r, uint32_t N, float* r_min,
+                       float* r_max, const float* support, const float* h,
+                       uint32_t* n_cells, uint32_t* icell,
+                       uint32_t* id_unsorted, uint32_t* id_sorted,
+                       uint32_t* ihoc, uint32_t ihoc_capacity,
+                       uint32_t* status, void* workspace, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        AQ_REQUIRE(N > 0, "aq_linklist: N must be > 0");
+        AQ_REQUIRE(r && r_min && r_max && support && h && n_cells && icell &&
+                       id_unsorted && id_sorted && ihoc && status &&
+                       workspace,
+                   "aq_linklist: null pointer");
+        linklist_all(dims, r, N, r_min, r_max, support, h, n_cells, icell,
+                     id_unsorted, id_sorted, ihoc, ihoc_capacity, status,
+                     workspace, S(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,1251 @@
+// SPH particle kernels of the cfd preset stack on gfx950.
+//
+// Streaming kernels (one thread per particle) and neighbour-walk kernels
+// (one thread per particle walking the 3^D link-list cells in the reference
+// order).  Each kernel cites the reference script it replaces and keeps its
+// expression trees, early exits and accumulation order, so FP32 results
+// match the reference kernel operation by operation.
+//
+// Candidate rejection: the reference tests q = length(r_ij)/H >= 2 for every
+// candidate.  With correctly rounded sqrt and division q is a monotone
+// function of s = dot(r_ij, r_ij), so the same test is "s > s_max" with
+// s_max precomputed on the host (support_threshold); sqrt and division run
+// only for accepted pairs.
+#include <cfloat>
+#include <cstring>
+#include <cmath>
+#include <mutex>
+#include <unordered_map>
+
+#include "aq_device.hpp"
+#include "aq_internal.hpp"
+
+namespace aq {
+
+constexpr int BS = 256;
+
+#define AQ_STATUS_SKIP                                                         \
+    if (ll.status && (*ll.status & 1u)) return
+
+float support_threshold(float H)
+{
+    static std::mutex mtx;
+    static std::unordered_map<uint32_t, float> cache;
+    uint32_t key;
+    std::memcpy(&key, &H, 4);
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    // q(s) = fl(fl(sqrt(s)) / H) is non-decreasing in s; binary search the
+    // largest float s >= 0 with q(s) < 2 over the ordered bit patterns.
+    volatile float vH = H;
+    auto accept = [&](uint32_t bits) {
+        float s;
+        std::memcpy(&s, &bits, 4);
+        volatile float root = std::sqrt(s);
+        volatile float q = root / vH;
+        return q < 2.f;
+    };
+    uint32_t lo = 0, hi = 0x7F800000u;  // accept(lo) true, accept(hi) false
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (accept(mid))
+            lo = mid;
+        else
+            hi = mid;
+    }
+    float smax;
+    std::memcpy(&smax, &lo, 4);
+    cache[key] = smax;
+    return smax;
+}
+
+// ===========================================================================
+// Streaming kernels
+// ===========================================================================
+
+// basic/time_scheme/improved_euler.cl:75-103
+template <int D>
+__global__ void predictor_k(const int* __restrict__ imove,
+                            const float* __restrict__ r,
+                            const float* __restrict__ u,
+                            const float* __restrict__ dudt,
+                            const float* __restrict__ rho,
+                            const float* __restrict__ drhodt,
+                            float* __restrict__ r_in, float* __restrict__ u_in,
+                            float* __restrict__ dudt_in,
+                            float* __restrict__ rho_in,
+                            float* __restrict__ drhodt_in, uint32_t N,
+                            const float* dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float DT = *dtp;
+    if (imove[i] <= 0) DT = 0.f;
+    const float hdt2 = 0.5f * DT * DT;
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        const float a_ = dudt[a];
+        const float u_ = u[a];
+        dudt_in[a] = a_;
+        u_in[a] = u_ + DT * a_;
+        r_in[a] = r[a] + DT * u_ + hdt2 * a_;
+    }
+    drhodt_in[i] = drhodt[i];
+    rho_in[i] = rho[i] + DT * drhodt[i];
+}
+
+// basic/time_scheme/improved_euler.cl:125-147
+template <int D>
+__global__ void corrector_k(const int* __restrict__ imove,
+                            float* __restrict__ r, float* __restrict__ u,
+                            const float* __restrict__ dudt,
+                            float* __restrict__ rho,
+                            const float* __restrict__ drhodt,
+                            const float* __restrict__ dudt_in,
+                            const float* __restrict__ drhodt_in, uint32_t N,
+                            const float* dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] > 0) {
+        const float DT = 0.5f * (*dtp);
+        const float DT2 = DT * DT;
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            const size_t a = (size_t)i * VS + k;
+            const float da = dudt[a] - dudt_in[a];
+            u[a] += DT * da;
+            r[a] += DT2 * da;
+        }
+        rho[i] += DT * (drhodt[i] - drhodt_in[i]);
+    }
+}
+
+// basic/Domain.cl:48-90
+template <int D>
+__global__ void domain_k(int* __restrict__ imove, float* __restrict__ r_in,
+                         float* __restrict__ u_in, float* __restrict__ dudt_in,
+                         float* __restrict__ m, uint32_t N,
+                         const float* dmin, const float* dmax)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= -255) return;
+    bool out = false;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const float c = r_in[(size_t)i * VS + k];
+        if (isnan(c) || isinf(c) || c < dmin[k] || c > dmax[k]) out = true;
+    }
+    if (out) {
+        imove[i] = -256;
+        m[i] = 0.f;
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            u_in[(size_t)i * VS + k] = 0.f;
+            dudt_in[(size_t)i * VS + k] = 0.f;
+            r_in[(size_t)i * VS + k] = dmax[k];
+        }
+    }
+}
+
+// basic/EOS.cl:57-73
+__global__ void eos_k(const uint32_t* __restrict__ iset,
+                      const int* __restrict__ imove,
+                      const float* __restrict__ rho, float* __restrict__ p,
+                      const float* __restrict__ refd, uint32_t N,
+                      const float* csp, const float* p0p)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int im = imove[i];
+    if ((im <= 0) && (im != -1)) return;
+    const float cs = *csp;
+    p[i] = (*p0p) + cs * cs * (rho[i] - refd[iset[i]]);
+}
+
+// basic/Binormal.cl:37-53
+template <int D>
+__global__ void binormal_k(const float* __restrict__ normal,
+                           float* __restrict__ tangent,
+                           float* __restrict__ binormal, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if constexpr (D == 2) {
+        const float2 n = reinterpret_cast<const float2*>(normal)[i];
+        reinterpret_cast<float2*>(binormal)[i] = make_float2(0.f, 0.f);
+        reinterpret_cast<float2*>(tangent)[i] = make_float2(n.y, -n.x);
+    } else {
+        const float4 a = reinterpret_cast<const float4*>(normal)[i];
+        const float4 b = reinterpret_cast<const float4*>(tangent)[i];
+        reinterpret_cast<float4*>(binormal)[i] =
+            make_float4(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
+                        a.x * b.y - a.y * b.x, 0.f);
+    }
+}
+
+// cfd/Rates.cl:55-77
+template <int D>
+__global__ void rates_k(const uint32_t* __restrict__ iset,
+                        const int* __restrict__ imove,
+                        const float* __restrict__ grad_p,
+                        const float* __restrict__ lap_u,
+                        const float* __restrict__ div_u,
+                        float* __restrict__ dudt, float* __restrict__ drhodt,
+                        const float* __restrict__ visc_dyn, uint32_t N,
+                        const float* g)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) return;
+    const float mu = visc_dyn[iset[i]];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        dudt[a] = -grad_p[a] + mu * lap_u[a] + g[k];
+    }
+    drhodt[i] = -div_u[i];
+}
+
+// basic/deltaSPH.cl:330-350, cfd/deltaSPH.cl:29 (fluid only)
+__global__ void delta_rate_k(const uint32_t* __restrict__ iset,
+                             const int* __restrict__ imove,
+                             const float* __restrict__ rho,
+                             const float* __restrict__ lap_p,
+                             float* __restrict__ drhodt,
+                             const float* __restrict__ refd,
+                             const float* __restrict__ delta, uint32_t N,
+                             const float* dtp)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) return;
+    const uint32_t s = iset[i];
+    const float delta_f = delta[s] * (*dtp) * rho[i] / refd[s];
+    drhodt[i] += delta_f * lap_p[i];
+}
+
+// cfd/TimeStep.cl:56-77 (length of the whole vec, w included)
+template <int D>
+__global__ void timestep_k(const int* __restrict__ imove,
+                           const float* __restrict__ u,
+                           float* __restrict__ dt_var, uint32_t N,
+                           const float* dtp, const float* dt_minp,
+                           const float* courantp, const float* dt_Map,
+                           const float* hp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float dt = *dtp;
+    if (imove[i] <= 0) {
+        dt_var[i] = dt;
+        return;
+    }
+    const float dr_max = (*dt_Map) * (*hp);
+    float s = u[(size_t)i * VS] * u[(size_t)i * VS];
+#pragma unroll
+    for (int k = 1; k < VS; k++)
+        s = s + u[(size_t)i * VS + k] * u[(size_t)i * VS + k];
+    const float dt_u = (*courantp) * dr_max / sqrtf(s);
+    dt_var[i] = cl_max(cl_min(dt, dt_u), *dt_minp);
+}
+
+// cfd/SensorsRenormalization.cl:42-67
+template <int D>
+__global__ void sensors_renorm_k(const int* __restrict__ imove,
+                                 const float* __restrict__ shepard,
+                                 float* __restrict__ u, float* __restrict__ rho,
+                                 float* __restrict__ p, uint32_t N)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 0) return;
+    float s = shepard[i];
+    if (s < 1.0E-6f) s = 1.f;
+#pragma unroll
+    for (int k = 0; k < VS; k++) u[(size_t)i * VS + k] /= s;
+    rho[i] /= s;
+    p[i] /= s;
+}
+
+// cfd/Boundary/BI/Shepard.cl:173-195 (apply)
+template <int D>
+__global__ void bi_shepard_apply_k(const int* __restrict__ imove,
+                                   const float* __restrict__ shepard,
+                                   float* __restrict__ grad_p,
+                                   float* __restrict__ lap_u,
+                                   float* __restrict__ div_u, uint32_t N)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) return;
+    const float s = shepard[i];
+#pragma unroll
+    for (int k = 0; k < VS; k++) grad_p[(size_t)i * VS + k] /= s;
+#pragma unroll
+    for (int k = 0; k < VS; k++) lap_u[(size_t)i * VS + k] /= s;
+    div_u[i] /= s;
+}
+
+// cfd/Boundary/BI/GradP.cl:55-83 (freeslip)
+template <int D>
+__global__ void bi_gradp_k(const uint32_t* __restrict__ iset,
+                           const int* __restrict__ imove,
+                           const float* __restrict__ shepard,
+                           const float* __restrict__ lap_u,
+                           const float* __restrict__ dudt,
+                           float* __restrict__ grad_p,
+                           const float* __restrict__ visc_dyn,
+                           const float* __restrict__ refd, uint32_t N,
+                           const float* g)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != -3) return;
+    float s = shepard[i];
+    if (s < 1.0E-6f) s = 1.f;
+    const float f = visc_dyn[iset[i]] / refd[iset[i]];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        grad_p[a] = f * lap_u[a] / s + g[k] - dudt[a];
+    }
+}
+
+// cfd/Boundary/BI/InterpolationShepard.cl:48-76
+__global__ void bi_interp_shepard_k(const uint32_t* __restrict__ iset,
+                                    const int* __restrict__ imove,
+                                    const float* __restrict__ shepard,
+                                    float* __restrict__ rho,
+                                    float* __restrict__ p,
+                                    const float* __restrict__ refd,
+                                    uint32_t N, const float* csp,
+                                    const float* p0p)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != -3) return;
+    float s = shepard[i];
+    if (s < 1.0E-6f) s = 1.f;
+    const float pi = p[i] / s;
+    p[i] = pi;
+    const float cs = *csp;
+    rho[i] = refd[iset[i]] + (pi - (*p0p)) / (cs * cs);
+}
+
+// ===========================================================================
+// Neighbour-walk kernels
+// ===========================================================================
+struct LLArgs {
+    const uint32_t* icell;
+    const uint32_t* ihoc;
+    const uint32_t* n_cells;
+    const uint32_t* status;
+    uint32_t N;
+};
+
+// basic/Shepard.cl:76-125 (cfd/Shepard.cl when cfd != 0)
+template <int D>
+__global__ void __launch_bounds__(BS)
+    shepard_k(const int* __restrict__ imove, const float* __restrict__ r,
+              const float* __restrict__ rho, const float* __restrict__ m,
+              float* __restrict__ shepard, LLArgs ll, float H, float CONW,
+              float smax, int cfd)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    const int im = imove[i];
+    auto excl = [&](int v) { return cfd ? (v != 1) : (v >= 3); };
+    if ((im < -3) || ((im > 0) && excl(im))) return;
+    const f3 r_i = ldv<D>(r, i);
+    float acc = shepard[i];
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if (excl(imove[j])) return;
+                const float q = sqrtf(s) / H;
+                acc += kernelW<D>(q) * CONW * m[j] / rho[j];
+            });
+    shepard[i] = acc;
+}
+
+// cfd/Interactions.cl:60-145
+template <int D>
+__global__ void __launch_bounds__(BS)
+    interactions_k(const int* __restrict__ imove, const float* __restrict__ r,
+                   const float* __restrict__ u, const float* __restrict__ rho,
+                   const float* __restrict__ m, const float* __restrict__ p,
+                   float* __restrict__ grad_p, float* __restrict__ lap_u,
+                   float* __restrict__ div_u, LLArgs ll, float H, float CONF,
+                   float smax, int lap)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != 1) return;
+    constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
+    const f3 r_i = ldv<D>(r, i);
+    const f3 u_i = ldv<D>(u, i);
+    const float p_i = p[i];
+    const float rho_i = rho[i];
+    f3 gp = ldv<D>(grad_p, i), lu = ldv<D>(lap_u, i);
+    float dv = div_u[i];
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if (i == j) return;
+                if (imove[j] != 1) return;
+                const float q = sqrtf(s) / H;
+                const float rho_j = rho[j];
+                const float p_j = p[j];
+                const f3 u_ij = sub<D>(ldv<D>(u, j), u_i);
+                const float udr = dot<D>(u_ij, r_ij);
+                const float f_ij = kernelF<D>(q) * CONF * m[j];
+                axpy<D>(gp, (p_i + p_j) / (rho_i * rho_j) * f_ij, r_ij);
+                if (lap == AQ_LAP_MONAGHAN) {
+                    const float r2 = (q * q + 0.01f) * H * H;
+                    axpy<D>(lu,
+                            f_ij * CLEARY * udr / (r2 * rho_i * rho_j), r_ij);
+                } else {
+                    axpy<D>(lu, f_ij * 2.f / (rho_i * rho_j), u_ij);
+                }
+                dv += udr * f_ij * rho_i / rho_j;
+            });
+    stxyz<D>(grad_p, i, gp);
+    stxyz<D>(lap_u, i, lu);
+    div_u[i] = dv;
+}
+
+// basic/deltaSPH.cl:191-242 (lapp), fluid only
+template <int D>
+__global__ void __launch_bounds__(BS)
+    lapp_k(const int* __restrict__ imove, const float* __restrict__ r,
+           const float* __restrict__ rho, const float* __restrict__ m,
+           const float* __restrict__ p, float* __restrict__ lap_p, LLArgs ll,
+           float H, float CONF, float smax)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != 1) return;
+    const f3 r_i = ldv<D>(r, i);
+    const float p_i = p[i];
+    float acc = lap_p[i];
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if ((i == j) || (imove[j] != 1)) return;
+                const float q = sqrtf(s) / H;
+                const float f_ij = kernelF<D>(q) * CONF * m[j] / rho[j];
+                acc += (p[j] - p_i) * f_ij;
+            });
+    lap_p[i] = acc;
+}
+
+// cfd/Sensors.cl:57-130
+template <int D>
+__global__ void __launch_bounds__(BS)
+    sensors_k(const int* __restrict__ imove, const float* __restrict__ r,
+              const float* __restrict__ m, float* __restrict__ u,
+              float* __restrict__ rho, float* __restrict__ p, LLArgs ll,
+              float H, float CONW, float smax)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != 0) return;
+    const f3 r_i = ldv<D>(r, i);
+    f3 uu = { 0.f, 0.f, 0.f };
+    float rr = 0.f, pp = 0.f;
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if (i == j) return;
+                if (imove[j] != 1) return;
+                const float q = sqrtf(s) / H;
+                const float rho_j = rho[j];
+                const float w_ij = kernelW<D>(q) * CONW * m[j] / rho_j;
+                const f3 u_j = ldv<D>(u, j);
+                axpy<D>(uu, w_ij, u_j);
+                rr += rho_j * w_ij;
+                pp += p[j] * w_ij;
+            });
+    stxyz<D>(u, i, uu);
+    rho[i] = rr;
+    p[i] = pp;
+}
+
+// cfd/Boundary/BI/Shepard.cl:61-150 (compute)
+template <int D>
+__global__ void __launch_bounds__(BS)
+    bi_shepard_compute_k(const int* __restrict__ imove,
+                         const float* __restrict__ r,
+                         const float* __restrict__ normal,
+                         const float* __restrict__ tangent,
+                         const float* __restrict__ binormal,
+                         const float* __restrict__ m,
+                         float* __restrict__ shepard, LLArgs ll, float H,
+                         float CONW, float DIMS, float smax)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    const int im = imove[i];
+    if ((im < -3) || (im > 1)) return;
+    const f3 r_i = ldv<D>(r, i);
+    float acc = 1.f;
+    bool self_added = false;
+    const float eps = 1E-8f * H;
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                if (imove[j] != -3) return;
+                if (i == j) {
+                    if (!self_added) {
+                        self_added = true;
+                        acc -= 0.5f;
+                    }
+                    return;
+                }
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                const float q = sqrtf(s) / H;
+                const f3 n_j = ldv<D>(normal, j);
+                const f3 t_j = ldv<D>(tangent, j);
+                const f3 b_j = ldv<D>(binormal, j);
+                const float r_n = dot<D>(r_ij, n_j);
+                const float area_j = m[j];
+                if ((r_n > -eps) && (r_n < eps)) {
+                    if (!self_added) {
+                        const float r_t = fabsf(dot<D>(r_ij, t_j));
+                        const float r_b = fabsf(dot<D>(r_ij, b_j));
+                        const float dr =
+                            0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                        if ((r_t <= dr) && (r_b <= dr)) {
+                            self_added = true;
+                            acc -= 0.5f;
+                        }
+                    }
+                    return;
+                }
+                const float r_t = fabsf(dot<D>(r_ij, t_j));
+                const float r_b = fabsf(dot<D>(r_ij, b_j));
+                acc += r_n * CONW * kernelS_P<D>(q) * area_j +
+                       kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
+            });
+    shepard[i] = acc;
+}
+
+// cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
+template <int D>
+__global__ void __launch_bounds__(BS)
+    bi_lapu_k(const int* __restrict__ imove, const float* __restrict__ r,
+              const float* __restrict__ u, const float* __restrict__ rho,
+              const float* __restrict__ m, float* __restrict__ lap_u,
+              LLArgs ll, float H, float CONF, float smax, int lap)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != -3) return;
+    constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
+    const f3 r_i = ldv<D>(r, i);
+    const f3 u_i = ldv<D>(u, i);
+    f3 lu = ldv<D>(lap_u, i);
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if (i == j) return;
+                if (imove[j] != 1) return;
+                const float q = sqrtf(s) / H;
+                const float rho_j = rho[j];
+                const float f_ij = kernelF<D>(q) * CONF * m[j];
+                const f3 u_ij = sub<D>(ldv<D>(u, j), u_i);
+                if (lap == AQ_LAP_MONAGHAN) {
+                    const float udr = dot<D>(u_ij, r_ij);
+                    const float r2 = (q * q + 0.01f) * H * H;
+                    axpy<D>(lu, f_ij * CLEARY * udr / (r2 * rho_j), r_ij);
+                } else {
+                    axpy<D>(lu, f_ij * 2.f / rho_j, u_ij);
+                }
+            });
+    stxyz<D>(lap_u, i, lu);
+}
+
+// cfd/Boundary/BI/Interpolation.cl:54-107
+template <int D>
+__global__ void __launch_bounds__(BS)
+    bi_interp_k(const int* __restrict__ imove, const float* __restrict__ r,
+                const float* __restrict__ m, const float* __restrict__ rho,
+                const float* __restrict__ grad_p, float* __restrict__ p,
+                LLArgs ll, float H, float CONW, float smax)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != -3) return;
+    const f3 r_i = ldv<D>(r, i);
+    const float rho_i = rho[i];
+    f3 gp_i = ldv<D>(grad_p, i);
+    gp_i.x = rho_i * gp_i.x;
+    gp_i.y = rho_i * gp_i.y;
+    gp_i.z = rho_i * gp_i.z;
+    float acc = 0.f;
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                if (imove[j] != 1) return;
+                const float q = sqrtf(s) / H;
+                const float w_ij = kernelW<D>(q) * CONW * m[j] / rho[j];
+                acc += (p[j] - dot<D>(gp_i, r_ij)) * w_ij;
+            });
+    p[i] = acc;
+}
+
+// cfd/Boundary/BI/Interactions.cl:51-133
+template <int D>
+__global__ void __launch_bounds__(BS)
+    bi_interactions_k(const int* __restrict__ imove,
+                      const float* __restrict__ r,
+                      const float* __restrict__ normal,
+                      const float* __restrict__ u,
+                      const float* __restrict__ rho,
+                      const float* __restrict__ m,
+                      const float* __restrict__ p, float* __restrict__ grad_p,
+                      float* __restrict__ div_u, LLArgs ll, float H,
+                      float CONW, float smax)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != 1) return;
+    const f3 r_i = ldv<D>(r, i);
+    const f3 u_i = ldv<D>(u, i);
+    const float p_i = p[i];
+    const float rho_i = rho[i];
+    f3 gp = ldv<D>(grad_p, i);
+    float dv = div_u[i];
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                if (imove[j] != -3) return;
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                const float q = sqrtf(s) / H;
+                const f3 n_j = ldv<D>(normal, j);
+                const float area_j = m[j];
+                const float p_j = p[j];
+                const f3 du = sub<D>(ldv<D>(u, j), u_i);
+                const float w_ij = kernelW<D>(q) * CONW * area_j;
+                axpy<D>(gp, (p_i + p_j) / rho_i * w_ij, n_j);
+                dv += rho_i * dot<D>(du, n_j) * w_ij;
+            });
+    stxyz<D>(grad_p, i, gp);
+    div_u[i] = dv;
+}
+
+// basic/neighs.cl:52-91
+template <int D>
+__global__ void __launch_bounds__(BS)
+    neighs_k(const int* __restrict__ imove, uint32_t* __restrict__ n_neighs,
+             const uint32_t* limitp, LLArgs ll)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] <= -255) {
+        n_neighs[i] = 0;
+        return;
+    }
+    const uint32_t limit = *limitp;
+    uint32_t n = 0;
+    bool stop = false;
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t) {
+                if (stop) return;
+                n += 1;
+                if (n >= limit) stop = true;
+            });
+    n_neighs[i] = n;
+}
+
+// ===========================================================================
+// Launchers (also used by the runtime tools)
+// ===========================================================================
+#define AQ_GRID(N) dim3(blocks_for((N), BS)), dim3(BS), 0, s
+#define AQ_DISPATCH(dims, KER, ...)                                            \
+    do {                                                                       \
+        if ((dims) == 2)                                                       \
+            hipLaunchKernelGGL(KER<2>, __VA_ARGS__);                           \
+        else                                                                   \
+            hipLaunchKernelGGL(KER<3>, __VA_ARGS__);                           \
+    } while (0)
+
+static LLArgs mkll(const uint32_t* icell, const uint32_t* ihoc,
+                   const uint32_t* n_cells, const uint32_t* status, uint32_t N)
+{
+    LLArgs a;
+    a.icell = icell;
+    a.ihoc = ihoc;
+    a.n_cells = n_cells;
+    a.status = status;
+    a.N = N;
+    return a;
+}
+
+void launch_predictor(int dims, const int32_t* imove, const float* r,
+                      const float* u, const float* dudt, const float* rho,
+                      const float* drhodt, float* r_in, float* u_in,
+                      float* dudt_in, float* rho_in, float* drhodt_in,
+                      uint32_t N, const float* dt, hipStream_t s)
+{
+    AQ_DISPATCH(dims, predictor_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
+                r_in, u_in, dudt_in, rho_in, drhodt_in, N, dt);
+}
+void launch_corrector(int dims, const int32_t* imove, float* r, float* u,
+                      const float* dudt, float* rho, const float* drhodt,
+                      const float* dudt_in, const float* drhodt_in,
+                      uint32_t N, const float* dt, hipStream_t s)
+{
+    AQ_DISPATCH(dims, corrector_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
+                dudt_in, drhodt_in, N, dt);
+}
+void launch_domain(int dims, int32_t* imove, float* r_in, float* u_in,
+                   float* dudt_in, float* m, uint32_t N, const float* dmin,
+                   const float* dmax, hipStream_t s)
+{
+    AQ_DISPATCH(dims, domain_k, AQ_GRID(N), imove, r_in, u_in, dudt_in, m, N,
+                dmin, dmax);
+}
+void launch_eos(const uint32_t* iset, const int32_t* imove, const float* rho,
+                float* p, const float* refd, uint32_t N, const float* cs,
+                const float* p0, hipStream_t s)
+{
+    hipLaunchKernelGGL(eos_k, AQ_GRID(N), iset, imove, rho, p, refd, N, cs,
+                       p0);
+}
+void launch_binormal(int dims, const float* normal, float* tangent,
+                     float* binormal, uint32_t N, hipStream_t s)
+{
+    AQ_DISPATCH(dims, binormal_k, AQ_GRID(N), normal, tangent, binormal, N);
+}
+void launch_rates(int dims, const uint32_t* iset, const int32_t* imove,
+                  const float* grad_p, const float* lap_u, const float* div_u,
+                  float* dudt, float* drhodt, const float* visc_dyn,
+                  uint32_t N, const float* g, hipStream_t s)
+{
+    AQ_DISPATCH(dims, rates_k, AQ_GRID(N), iset, imove, grad_p, lap_u, div_u,
+                dudt, drhodt, visc_dyn, N, g);
+}
+void launch_delta_rate(const uint32_t* iset, const int32_t* imove,
+                       const float* rho, const float* lap_p, float* drhodt,
+                       const float* refd, const float* delta, uint32_t N,
+                       const float* dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(delta_rate_k, AQ_GRID(N), iset, imove, rho, lap_p,
+                       drhodt, refd, delta, N, dt);
+}
+void launch_timestep(int dims, const int32_t* imove, const float* u,
+                     float* dt_var, uint32_t N, const float* dt,
+                     const float* dt_min, const float* courant,
+                     const float* dt_Ma, const float* h, hipStream_t s)
+{
+    AQ_DISPATCH(dims, timestep_k, AQ_GRID(N), imove, u, dt_var, N, dt, dt_min,
+                courant, dt_Ma, h);
+}
+void launch_sensors_renorm(int dims, const int32_t* imove,
+                           const float* shepard, float* u, float* rho,
+                           float* p, uint32_t N, hipStream_t s)
+{
+    AQ_DISPATCH(dims, sensors_renorm_k, AQ_GRID(N), imove, shepard, u, rho, p,
+                N);
+}
+void launch_bi_shepard_apply(int dims, const int32_t* imove,
+                             const float* shepard, float* grad_p,
+                             float* lap_u, float* div_u, uint32_t N,
+                             hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_shepard_apply_k, AQ_GRID(N), imove, shepard, grad_p,
+                lap_u, div_u, N);
+}
+void launch_bi_gradp(int dims, const uint32_t* iset, const int32_t* imove,
+                     const float* shepard, const float* lap_u,
+                     const float* dudt, float* grad_p, const float* visc_dyn,
+                     const float* refd, uint32_t N, const float* g,
+                     hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_gradp_k, AQ_GRID(N), iset, imove, shepard, lap_u,
+                dudt, grad_p, visc_dyn, refd, N, g);
+}
+void launch_bi_interp_shepard(const uint32_t* iset, const int32_t* imove,
+                              const float* shepard, float* rho, float* p,
+                              const float* refd, uint32_t N, const float* cs,
+                              const float* p0, hipStream_t s)
+{
+    hipLaunchKernelGGL(bi_interp_shepard_k, AQ_GRID(N), iset, imove, shepard,
+                       rho, p, refd, N, cs, p0);
+}
+
+void launch_shepard(int dims, const int32_t* imove, const float* r,
+                    const float* rho, const float* m, float* shepard,
+                    uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                    const uint32_t* n_cells, const uint32_t* status,
+                    aq_defs d, int cfd, hipStream_t s)
+{
+    AQ_DISPATCH(dims, shepard_k, AQ_GRID(N), imove, r, rho, m, shepard,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
+                support_threshold(d.H), cfd);
+}
+void launch_interactions(int dims, const int32_t* imove, const float* r,
+                         const float* u, const float* rho, const float* m,
+                         const float* p, float* grad_p, float* lap_u,
+                         float* div_u, uint32_t N, const uint32_t* icell,
+                         const uint32_t* ihoc, const uint32_t* n_cells,
+                         const uint32_t* status, aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, interactions_k, AQ_GRID(N), imove, r, u, rho, m, p,
+                grad_p, lap_u, div_u, mkll(icell, ihoc, n_cells, status, N),
+                d.H, d.CONF, support_threshold(d.H), d.lap_formulation);
+}
+void launch_lapp(int dims, const int32_t* imove, const float* r,
+                 const float* rho, const float* m, const float* p,
+                 float* lap_p, uint32_t N, const uint32_t* icell,
+                 const uint32_t* ihoc, const uint32_t* n_cells,
+                 const uint32_t* status, aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, lapp_k, AQ_GRID(N), imove, r, rho, m, p, lap_p,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONF,
+                support_threshold(d.H));
+}
+void launch_sensors(int dims, const int32_t* imove, const float* r,
+                    const float* m, float* u, float* rho, float* p,
+                    uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                    const uint32_t* n_cells, const uint32_t* status,
+                    aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, sensors_k, AQ_GRID(N), imove, r, m, u, rho, p,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
+                support_threshold(d.H));
+}
+void launch_bi_shepard_compute(int dims, const int32_t* imove, const float* r,
+                               const float* normal, const float* tangent,
+                               const float* binormal, const float* m,
+                               float* shepard, uint32_t N,
+                               const uint32_t* icell, const uint32_t* ihoc,
+                               const uint32_t* n_cells,
+                               const uint32_t* status, aq_defs d,
+                               hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_shepard_compute_k, AQ_GRID(N), imove, r, normal,
+                tangent, binormal, m, shepard,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW, d.DIMS,
+                support_threshold(d.H));
+}
+void launch_bi_lapu(int dims, const int32_t* imove, const float* r,
+                    const float* u, const float* rho, const float* m,
+                    float* lap_u, uint32_t N, const uint32_t* icell,
+                    const uint32_t* ihoc, const uint32_t* n_cells,
+                    const uint32_t* status, aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_lapu_k, AQ_GRID(N), imove, r, u, rho, m, lap_u,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONF,
+                support_threshold(d.H), d.lap_formulation);
+}
+void launch_bi_interp(int dims, const int32_t* imove, const float* r,
+                      const float* m, const float* rho, const float* grad_p,
+                      float* p, uint32_t N, const uint32_t* icell,
+                      const uint32_t* ihoc, const uint32_t* n_cells,
+                      const uint32_t* status, aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_interp_k, AQ_GRID(N), imove, r, m, rho, grad_p, p,
+                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
+                support_threshold(d.H));
+}
+void launch_bi_interactions(int dims, const int32_t* imove, const float* r,
+                            const float* normal, const float* u,
+                            const float* rho, const float* m, const float* p,
+                            float* grad_p, float* div_u, uint32_t N,
+                            const uint32_t* icell, const uint32_t* ihoc,
+                            const uint32_t* n_cells, const uint32_t* status,
+                            aq_defs d, hipStream_t s)
+{
+    AQ_DISPATCH(dims, bi_interactions_k, AQ_GRID(N), imove, r, normal, u, rho,
+                m, p, grad_p, div_u, mkll(icell, ihoc, n_cells, status, N),
+                d.H, d.CONW, support_threshold(d.H));
+}
+void launch_neighs(int dims, const int32_t* imove, uint32_t* n_neighs,
+                   const uint32_t* limit, uint32_t N, const uint32_t* icell,
+                   const uint32_t* ihoc, const uint32_t* n_cells,
+                   const uint32_t* status, hipStream_t s)
+{
+    AQ_DISPATCH(dims, neighs_k, AQ_GRID(N), imove, n_neighs, limit,
+                mkll(icell, ihoc, n_cells, status, N));
+}
+
+}  // namespace aq
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace aq;
+
+#define AQ_PTRS(...) AQ_REQUIRE(all_set({ __VA_ARGS__ }), "null pointer argument")
+static bool all_set(std::initializer_list<const void*> l)
+{
+    for (auto p : l)
+        if (!p) return false;
+    return true;
+}
+static void check_defs(const aq_defs& d)
+{
+    AQ_REQUIRE(d.H > 0.f && std::isfinite(d.H), "defs.H must be > 0");
+    AQ_REQUIRE(d.lap_formulation == AQ_LAP_MONAGHAN ||
+                   d.lap_formulation == AQ_LAP_MORRIS,
+               "defs.lap_formulation must be AQ_LAP_MONAGHAN or AQ_LAP_MORRIS");
+}
+
+extern "C" {
+
+AQ_API int aq_predictor(int dims, const int32_t* imove, const float* r,
+                        const float* u, const float* dudt, const float* rho,
+                        const float* drhodt, float* r_in, float* u_in,
+                        float* dudt_in, float* rho_in, float* drhodt_in,
+                        uint32_t N, const float* dt, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, r, u, dudt, rho, drhodt, r_in, u_in, dudt_in, rho_in,
+                drhodt_in, dt);
+        launch_predictor(dims, imove, r, u, dudt, rho, drhodt, r_in, u_in,
+                         dudt_in, rho_in, drhodt_in, N, dt, S(stream));
+    });
+}
+
+AQ_API int aq_corrector(int dims, const int32_t* imove, float* r, float* u,
+                        const float* dudt, float* rho, const float* drhodt,
+                        const float* dudt_in, const float* drhodt_in,
+                        uint32_t N, const float* dt, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, r, u, dudt, rho, drhodt, dudt_in, drhodt_in, dt);
+        launch_corrector(dims, imove, r, u, dudt, rho, drhodt, dudt_in,
+                         drhodt_in, N, dt, S(stream));
+    });
+}
+
+AQ_API int aq_domain(int dims, int32_t* imove, float* r_in, float* u_in,
+                     float* dudt_in, float* m, uint32_t N,
+                     const float* domain_min, const float* domain_max,
+                     void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, r_in, u_in, dudt_in, m, domain_min, domain_max);
+        launch_domain(dims, imove, r_in, u_in, dudt_in, m, N, domain_min,
+                      domain_max, S(stream));
+    });
+}
+
+AQ_API int aq_eos(const uint32_t* iset, const int32_t* imove,
+                  const float* rho, float* p, const float* refd, uint32_t N,
+                  const float* cs, const float* p0, void* stream)
+{
+    return guard([&] {
+        if (!N) return;
+        AQ_PTRS(iset, imove, rho, p, refd, cs, p0);
+        launch_eos(iset, imove, rho, p, refd, N, cs, p0, S(stream));
+    });
+}
+
+AQ_API int aq_binormal(int dims, const float* normal, float* tangent,
+                       float* binormal, uint32_t N, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(normal, tangent, binormal);
+        launch_binormal(dims, normal, tangent, binormal, N, S(stream));
+    });
+}
+
+AQ_API int aq_rates(int dims, const uint32_t* iset, const int32_t* imove,
+                    const float* rho, const float* grad_p, const float* lap_u,
+                    const float* div_u, float* dudt, float* drhodt,
+                    const float* visc_dyn, uint32_t N, const float* g,
+                    void* stream)
+{
+    (void)rho;
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(iset, imove, grad_p, lap_u, div_u, dudt, drhodt, visc_dyn, g);
+        launch_rates(dims, iset, imove, grad_p, lap_u, div_u, dudt, drhodt,
+                     visc_dyn, N, g, S(stream));
+    });
+}
+
+AQ_API int aq_delta_rate(const uint32_t* iset, const int32_t* imove,
+                         const float* rho, const float* lap_p, float* drhodt,
+                         const float* refd, const float* delta, uint32_t N,
+                         const float* dt, void* stream)
+{
+    return guard([&] {
+        if (!N) return;
+        AQ_PTRS(iset, imove, rho, lap_p, drhodt, refd, delta, dt);
+        launch_delta_rate(iset, imove, rho, lap_p, drhodt, refd, delta, N, dt,
+                          S(stream));
+    });
+}
+
+AQ_API int aq_timestep(int dims, const int32_t* imove, const float* u,
+                       float* dt_var, uint32_t N, const float* dt,
+                       const float* dt_min, const float* courant,
+                       const float* dt_Ma, const float* h, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, u, dt_var, dt, dt_min, courant, dt_Ma, h);
+        launch_timestep(dims, imove, u, dt_var, N, dt, dt_min, courant, dt_Ma,
+                        h, S(stream));
+    });
+}
+
+AQ_API int aq_sensors_renorm(int dims, const int32_t* imove,
+                             const float* shepard, float* u, float* rho,
+                             float* p, uint32_t N, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, shepard, u, rho, p);
+        launch_sensors_renorm(dims, imove, shepard, u, rho, p, N, S(stream));
+    });
+}
+
+AQ_API int aq_bi_shepard_apply(int dims, const int32_t* imove,
+                               const float* shepard, float* grad_p,
+                               float* lap_u, float* div_u, uint32_t N,
+                               void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, shepard, grad_p, lap_u, div_u);
+        launch_bi_shepard_apply(dims, imove, shepard, grad_p, lap_u, div_u, N,
+                                S(stream));
+    });
+}
+
+AQ_API int aq_bi_gradp(int dims, const uint32_t* iset, const int32_t* imove,
+                       const float* shepard, const float* rho,
+                       const float* lap_u, const float* dudt, float* grad_p,
+                       const float* visc_dyn, const float* refd, uint32_t N,
+                       const float* g, void* stream)
+{
+    (void)rho;
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(iset, imove, shepard, lap_u, dudt, grad_p, visc_dyn, refd, g);
+        launch_bi_gradp(dims, iset, imove, shepard, lap_u, dudt, grad_p,
+                        visc_dyn, refd, N, g, S(stream));
+    });
+}
+
+AQ_API int aq_bi_interp_shepard(const uint32_t* iset, const int32_t* imove,
+                                const float* shepard, float* rho, float* p,
+                                const float* refd, uint32_t N,
+                                const float* cs, const float* p0,
+                                void* stream)
+{
+    return guard([&] {
+        if (!N) return;
+        AQ_PTRS(iset, imove, shepard, rho, p, refd, cs, p0);
+        launch_bi_interp_shepard(iset, imove, shepard, rho, p, refd, N, cs,
+                                 p0, S(stream));
+    });
+}
+
+AQ_API int aq_shepard(int dims, const int32_t* imove, const float* r,
+                      const float* rho, const float* m, float* shepard,
+                      uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                      const uint32_t* n_cells, aq_defs defs, int cfd_variant,
+                      void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, rho, m, shepard, icell, ihoc, n_cells);
+        launch_shepard(dims, imove, r, rho, m, shepard, N, icell, ihoc,
+                       n_cells, nullptr, defs, cfd_variant ? 1 : 0,
+                       S(stream));
+    });
+}
+
+AQ_API int aq_interactions(int dims, const int32_t* imove, const float* r,
+                           const float* u, const float* rho, const float* m,
+                           const float* p, float* grad_p, float* lap_u,
+                           float* div_u, uint32_t N, const uint32_t* icell,
+                           const uint32_t* ihoc, const uint32_t* n_cells,
+                           aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, u, rho, m, p, grad_p, lap_u, div_u, icell, ihoc,
+                n_cells);
+        launch_interactions(dims, imove, r, u, rho, m, p, grad_p, lap_u, div_u,
+                            N, icell, ihoc, n_cells, nullptr, defs,
+                            S(stream));
+    });
+}
+
+AQ_API int aq_lapp(int dims, const int32_t* imove, const float* r,
+                   const float* rho, const float* m, const float* p,
+                   float* lap_p, uint32_t N, const uint32_t* icell,
+                   const uint32_t* ihoc, const uint32_t* n_cells,
+                   aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, rho, m, p, lap_p, icell, ihoc, n_cells);
+        launch_lapp(dims, imove, r, rho, m, p, lap_p, N, icell, ihoc, n_cells,
+                    nullptr, defs, S(stream));
+    });
+}
+
+AQ_API int aq_sensors(int dims, const int32_t* imove, const float* r,
+                      const float* m, float* u, float* rho, float* p,
+                      uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                      const uint32_t* n_cells, aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, m, u, rho, p, icell, ihoc, n_cells);
+        launch_sensors(dims, imove, r, m, u, rho, p, N, icell, ihoc, n_cells,
+                       nullptr, defs, S(stream));
+    });
+}
+
+AQ_API int aq_bi_shepard_compute(int dims, const int32_t* imove,
+                                 const float* r, const float* normal,
+                                 const float* tangent, const float* binormal,
+                                 const float* m, float* shepard, uint32_t N,
+                                 const uint32_t* icell, const uint32_t* ihoc,
+                                 const uint32_t* n_cells, aq_defs defs,
+                                 void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, normal, tangent, binormal, m, shepard, icell, ihoc,
+                n_cells);
+        launch_bi_shepard_compute(dims, imove, r, normal, tangent, binormal, m,
+                                  shepard, N, icell, ihoc, n_cells, nullptr,
+                                  defs, S(stream));
+    });
+}
+
+AQ_API int aq_bi_lapu(int dims, const int32_t* imove, const float* r,
+                      const float* u, const float* rho, const float* m,
+                      float* lap_u, uint32_t N, const uint32_t* icell,
+                      const uint32_t* ihoc, const uint32_t* n_cells,
+                      aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, u, rho, m, lap_u, icell, ihoc, n_cells);
+        launch_bi_lapu(dims, imove, r, u, rho, m, lap_u, N, icell, ihoc,
+                       n_cells, nullptr, defs, S(stream));
+    });
+}
+
+AQ_API int aq_bi_interp(int dims, const int32_t* imove, const float* r,
+                        const float* m, const float* rho,
+                        const float* grad_p, float* p, uint32_t N,
+                        const uint32_t* icell, const uint32_t* ihoc,
+                        const uint32_t* n_cells, aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, m, rho, grad_p, p, icell, ihoc, n_cells);
+        launch_bi_interp(dims, imove, r, m, rho, grad_p, p, N, icell, ihoc,
+                         n_cells, nullptr, defs, S(stream));
+    });
+}
+
+AQ_API int aq_bi_interactions(int dims, const int32_t* imove, const float* r,
+                              const float* normal, const float* u,
+                              const float* rho, const float* m,
+                              const float* p, float* grad_p, float* div_u,
+                              uint32_t N, const uint32_t* icell,
+                              const uint32_t* ihoc, const uint32_t* n_cells,
+                              aq_defs defs, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        check_defs(defs);
+        if (!N) return;
+        AQ_PTRS(imove, r, normal, u, rho, m, p, grad_p, div_u, icell, ihoc,
+                n_cells);
+        launch_bi_interactions(dims, imove, r, normal, u, rho, m, p, grad_p,
+                               div_u, N, icell, ihoc, n_cells, nullptr, defs,
+                               S(stream));
+    });
+}
+
+AQ_API int aq_neighs(int dims, const int32_t* imove, uint32_t* n_neighs,
+                     const uint32_t* neighs_limit, uint32_t N,
+                     const uint32_t* icell, const uint32_t* ihoc,
+                     const uint32_t* n_cells, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(imove, n_neighs, neighs_limit, icell, ihoc, n_cells);
+        launch_neighs(dims, imove, n_neighs, neighs_limit, N, icell, ihoc,
+                      n_cells, nullptr, S(stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,296 @@
+/*
+ * aquagpusph_amd — MI355X-native (gfx950, HIP) implementation of the
+ * AQUAgpusph per-timestep SPH pipeline.
+ *
+ * Plain C ABI: device pointers, sizes and a HIP stream passed as void*.
+ * Every entry point returns 0 on success or a negative AQ_E* code; the text
+ * of the last error on the calling thread is aq_last_error().  Nothing here
+ * throws across the boundary.
+ *
+ * Conventions (mirroring the reference's variable contract,
+ * aquagpusph/Variable.cpp:1548-1575 with addr_bits = 32):
+ *   - `vec` arrays hold float2 (2D) or float4 (3D, w carried) per particle;
+ *   - indices (`usize`, `size_t*` variables) are uint32;
+ *   - `imove` is int32, `iset` is uint32;
+ *   - per-set arrays (refd, visc_dyn, delta) are float[n_sets];
+ *   - scalars that can change during a run (dt, t, n_cells, r_min, cs, g,
+ *     ...) are passed as DEVICE pointers so that a step can be replayed from
+ *     a captured HIP graph without re-binding;
+ *   - preset definitions (H, CONW, CONF, DIMS, Laplacian formulation) are
+ *     compile-time constants in the reference and are passed by value in
+ *     aq_defs.
+ * Each particle kernel launches over N particles and reproduces the
+ * reference kernel's early exits on imove.
+ */
+#ifndef AQUAGPUSPH_AMD_H
+#define AQUAGPUSPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AQ_API __attribute__((visibility("default")))
+
+/* error codes */
+#define AQ_OK 0
+#define AQ_EINVAL -1    /* invalid argument */
+#define AQ_EHIP -2      /* HIP runtime error */
+#define AQ_ENOMEM -3    /* allocation failure or capacity exceeded */
+#define AQ_EPARSE -4    /* XML / expression parse error */
+#define AQ_ENOTFOUND -5 /* unknown variable, tool or script */
+#define AQ_EASSERT -6   /* an assert tool failed on the device */
+#define AQ_EUNSUP -7    /* feature not supported by this build */
+
+/* Laplacian formulation (cfd.xml __LAP_FORMULATION__) */
+#define AQ_LAP_MONAGHAN 1
+#define AQ_LAP_MORRIS 2
+
+/* Preset definitions, as the reference passes them to its kernels with -D
+ * after "%#G" formatting (aquagpusph/CalcServer/CalcServer.cpp:241-260). */
+typedef struct aq_defs {
+    float H;    /* kernel length h */
+    float CONW; /* 1/h^dims */
+    float CONF; /* 1/h^(dims+2) */
+    float DIMS; /* dims as a float */
+    int lap_formulation; /* AQ_LAP_MONAGHAN or AQ_LAP_MORRIS */
+} aq_defs;
+
+/* ---- library ---------------------------------------------------------- */
+AQ_API const char* aq_version(void);
+AQ_API const char* aq_strerror(int code);
+AQ_API const char* aq_last_error(void);
+/* Select the HIP device used by subsequent calls on this thread. */
+AQ_API int aq_set_device(int device);
+
+/* ---- link-list: aquagpusph/CalcServer/LinkList.{cpp,cl.in} ------------ */
+
+/* r_min/r_max reductions + LinkList::nCells (LinkList.cpp:185-232,
+ * 326-391): component-wise min/max of r[0..N) into r_min/r_max (device vec
+ * scalars), then n_cells.xyz = (uint)((max-min)/cell_length) + 6 and
+ * n_cells.w = x*y*z into the device uint4 n_cells.  If n_cells.w exceeds
+ * ihoc_capacity, *status (device uint) gets bit 0 set; the following
+ * link-list kernels then do nothing.  workspace: aq_linklist_workspace(). */
+AQ_API int aq_linklist_grid(int dims, const float* r, uint32_t N,
+                            float* r_min, float* r_max,
+                            const float* support, const float* h,
+                            uint32_t* n_cells, uint32_t ihoc_capacity,
+                            uint32_t* status, void* workspace, void* stream);
+
+/* iCell kernel, LinkList.cl.in:54-85. */
+AQ_API int aq_linklist_icell(int dims, uint32_t* icell, const float* r,
+                             uint32_t N, const float* r_min,
+                             const float* support, const float* h,
+                             const uint32_t* n_cells, const uint32_t* status,
+                             void* stream);
+
+/* iHoc + linkList kernels, LinkList.cl.in:32-42 and 92-113, on sorted icell.
+ * Launches over ihoc_capacity cells; only cells < n_cells.w are written. */
+AQ_API int aq_linklist_ihoc(const uint32_t* icell, uint32_t* ihoc,
+                            uint32_t N, const uint32_t* n_cells,
+                            uint32_t ihoc_capacity, const uint32_t* status,
+                            void* stream);
+
+/* Whole LinkList tool (LinkList::_execute, LinkList.cpp:326-494) on the
+ * positions r: grid, iCell, stable radix sort of icell (id_unsorted = perm,
+ * id_sorted = inverse perm), iHoc, linkList. */
+AQ_API int aq_linklist(int dims, const float* r, uint32_t N, float* r_min,
+                       float* r_max, const float* support, const float* h,
+                       uint32_t* n_cells, uint32_t* icell,
+                       uint32_t* id_unsorted, uint32_t* id_sorted,
+                       uint32_t* ihoc, uint32_t ihoc_capacity,
+                       uint32_t* status, void* workspace, void* stream);
+/* Bytes of device workspace aq_linklist / aq_linklist_grid need for N. */
+AQ_API size_t aq_linklist_workspace(uint32_t N);
+
+/* ---- radix sort: aquagpusph/CalcServer/RadixSort.{cpp,cl.in} ---------- */
+
+/* Stable LSD radix sort of n uint32 keys on their low key_bits bits
+ * (RadixSort::_execute, RadixSort.cpp:129-303).  keys is sorted in place;
+ * perm[k] = original index of sorted key k (id_unsorted); inv_perm[perm[k]]
+ * = k (id_sorted, RadixSort.cl.in:313-323).  inv_perm may be NULL.
+ * workspace: aq_radix_sort_workspace(n). */
+AQ_API int aq_radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv_perm,
+                         uint32_t n, unsigned key_bits, void* workspace,
+                         void* stream);
+AQ_API size_t aq_radix_sort_workspace(uint32_t n);
+/* Key bits the reference processes for max_val (RadixSort.cpp:152-176). */
+AQ_API unsigned aq_radix_key_bits(uint64_t max_val);
+
+/* ---- reorder: resources/Scripts/basic/Sort.cl:57-124 ------------------ */
+
+/* dst[k] = src[perm[k]] for k < N, element size esize bytes (1..16, power of
+ * two).  With perm = id_unsorted this equals the reference scatter
+ * x[id_sorted[i]] = x_in[i]. */
+AQ_API int aq_permute(const void* src, void* dst, size_t esize,
+                      const uint32_t* perm, uint32_t N, void* stream);
+
+/* ---- streaming particle kernels --------------------------------------- */
+
+/* improved Euler predictor, basic/time_scheme/improved_euler.cl:75-103 */
+AQ_API int aq_predictor(int dims, const int32_t* imove, const float* r,
+                        const float* u, const float* dudt, const float* rho,
+                        const float* drhodt, float* r_in, float* u_in,
+                        float* dudt_in, float* rho_in, float* drhodt_in,
+                        uint32_t N, const float* dt, void* stream);
+
+/* improved Euler corrector, basic/time_scheme/improved_euler.cl:125-147 */
+AQ_API int aq_corrector(int dims, const int32_t* imove, float* r, float* u,
+                        const float* dudt, float* rho, const float* drhodt,
+                        const float* dudt_in, const float* drhodt_in,
+                        uint32_t N, const float* dt, void* stream);
+
+/* basic/Domain.cl:48-90 */
+AQ_API int aq_domain(int dims, int32_t* imove, float* r_in, float* u_in,
+                     float* dudt_in, float* m, uint32_t N,
+                     const float* domain_min, const float* domain_max,
+                     void* stream);
+
+/* basic/EOS.cl:57-73 */
+AQ_API int aq_eos(const uint32_t* iset, const int32_t* imove,
+                  const float* rho, float* p, const float* refd, uint32_t N,
+                  const float* cs, const float* p0, void* stream);
+
+/* basic/Binormal.cl:37-53 */
+AQ_API int aq_binormal(int dims, const float* normal, float* tangent,
+                       float* binormal, uint32_t N, void* stream);
+
+/* cfd/Rates.cl:55-77 */
+AQ_API int aq_rates(int dims, const uint32_t* iset, const int32_t* imove,
+                    const float* rho, const float* grad_p, const float* lap_u,
+                    const float* div_u, float* dudt, float* drhodt,
+                    const float* visc_dyn, uint32_t N, const float* g,
+                    void* stream);
+
+/* basic/deltaSPH.cl:330-350 (cfd/deltaSPH.cl: fluid only) */
+AQ_API int aq_delta_rate(const uint32_t* iset, const int32_t* imove,
+                         const float* rho, const float* lap_p, float* drhodt,
+                         const float* refd, const float* delta, uint32_t N,
+                         const float* dt, void* stream);
+
+/* cfd/TimeStep.cl:56-77 */
+AQ_API int aq_timestep(int dims, const int32_t* imove, const float* u,
+                       float* dt_var, uint32_t N, const float* dt,
+                       const float* dt_min, const float* courant,
+                       const float* dt_Ma, const float* h, void* stream);
+
+/* cfd/SensorsRenormalization.cl:42-67 */
+AQ_API int aq_sensors_renorm(int dims, const int32_t* imove,
+                             const float* shepard, float* u, float* rho,
+                             float* p, uint32_t N, void* stream);
+
+/* cfd/Boundary/BI/Shepard.cl:173-195 (apply) */
+AQ_API int aq_bi_shepard_apply(int dims, const int32_t* imove,
+                               const float* shepard, float* grad_p,
+                               float* lap_u, float* div_u, uint32_t N,
+                               void* stream);
+
+/* cfd/Boundary/BI/GradP.cl:55-83 (freeslip) */
+AQ_API int aq_bi_gradp(int dims, const uint32_t* iset, const int32_t* imove,
+                       const float* shepard, const float* rho,
+                       const float* lap_u, const float* dudt, float* grad_p,
+                       const float* visc_dyn, const float* refd, uint32_t N,
+                       const float* g, void* stream);
+
+/* cfd/Boundary/BI/InterpolationShepard.cl:48-76 */
+AQ_API int aq_bi_interp_shepard(const uint32_t* iset, const int32_t* imove,
+                                const float* shepard, float* rho, float* p,
+                                const float* refd, uint32_t N,
+                                const float* cs, const float* p0,
+                                void* stream);
+
+/* ---- pair (neighbour-walk) kernels ------------------------------------
+ * All take the link-list (icell, ihoc, n_cells) of the sorted particles and
+ * visit neighbours in the reference order (resources/Scripts/types/3D.h:
+ * 197-219, 2D.h:174-193), so FP32 sums associate exactly as there. */
+
+/* basic/Shepard.cl:76-125.  cfd_variant != 0 selects cfd/Shepard.cl
+ * (EXCLUDED_PARTICLE imove != 1), else the basic default (imove >= 3). */
+AQ_API int aq_shepard(int dims, const int32_t* imove, const float* r,
+                      const float* rho, const float* m, float* shepard,
+                      uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                      const uint32_t* n_cells, aq_defs defs, int cfd_variant,
+                      void* stream);
+
+/* cfd/Interactions.cl:60-145 */
+AQ_API int aq_interactions(int dims, const int32_t* imove, const float* r,
+                           const float* u, const float* rho, const float* m,
+                           const float* p, float* grad_p, float* lap_u,
+                           float* div_u, uint32_t N, const uint32_t* icell,
+                           const uint32_t* ihoc, const uint32_t* n_cells,
+                           aq_defs defs, void* stream);
+
+/* basic/deltaSPH.cl:191-242 (lapp), cfd variant (fluid only) */
+AQ_API int aq_lapp(int dims, const int32_t* imove, const float* r,
+                   const float* rho, const float* m, const float* p,
+                   float* lap_p, uint32_t N, const uint32_t* icell,
+                   const uint32_t* ihoc, const uint32_t* n_cells,
+                   aq_defs defs, void* stream);
+
+/* cfd/Sensors.cl:57-130 */
+AQ_API int aq_sensors(int dims, const int32_t* imove, const float* r,
+                      const float* m, float* u, float* rho, float* p,
+                      uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
+                      const uint32_t* n_cells, aq_defs defs, void* stream);
+
+/* cfd/Boundary/BI/Shepard.cl:61-150 (compute) */
+AQ_API int aq_bi_shepard_compute(int dims, const int32_t* imove,
+                                 const float* r, const float* normal,
+                                 const float* tangent, const float* binormal,
+                                 const float* m, float* shepard, uint32_t N,
+                                 const uint32_t* icell, const uint32_t* ihoc,
+                                 const uint32_t* n_cells, aq_defs defs,
+                                 void* stream);
+
+/* cfd/Boundary/BI/LapU.cl:58-123 (freeslip) */
+AQ_API int aq_bi_lapu(int dims, const int32_t* imove, const float* r,
+                      const float* u, const float* rho, const float* m,
+                      float* lap_u, uint32_t N, const uint32_t* icell,
+                      const uint32_t* ihoc, const uint32_t* n_cells,
+                      aq_defs defs, void* stream);
+
+/* cfd/Boundary/BI/Interpolation.cl:54-107 */
+AQ_API int aq_bi_interp(int dims, const int32_t* imove, const float* r,
+                        const float* m, const float* rho,
+                        const float* grad_p, float* p, uint32_t N,
+                        const uint32_t* icell, const uint32_t* ihoc,
+                        const uint32_t* n_cells, aq_defs defs, void* stream);
+
+/* cfd/Boundary/BI/Interactions.cl:51-133 */
+AQ_API int aq_bi_interactions(int dims, const int32_t* imove, const float* r,
+                              const float* normal, const float* u,
+                              const float* rho, const float* m,
+                              const float* p, float* grad_p, float* div_u,
+                              uint32_t N, const uint32_t* icell,
+                              const uint32_t* ihoc, const uint32_t* n_cells,
+                              aq_defs defs, void* stream);
+
+/* basic/neighs.cl:52-91 */
+AQ_API int aq_neighs(int dims, const int32_t* imove, uint32_t* n_neighs,
+                     const uint32_t* neighs_limit, uint32_t N,
+                     const uint32_t* icell, const uint32_t* ihoc,
+                     const uint32_t* n_cells, void* stream);
+
+/* ---- reductions: aquagpusph/CalcServer/Reduction.{cpp,cl.in} ---------- */
+#define AQ_RED_MIN 0
+#define AQ_RED_MAX 1
+#define AQ_RED_SUM 2
+#define AQ_T_FLOAT 0 /* float */
+#define AQ_T_INT 1   /* int32 */
+#define AQ_T_UINT 2  /* uint32 */
+/* Reduce n elements of `ncomp` components each (1 for scalars, 2/4 for
+ * vec) component-wise with op into out[ncomp] (device).  identity is the
+ * reference's "null" value per component (host array of ncomp values, as
+ * float/int32/uint32 bits per `type`).  workspace: aq_reduce_workspace(). */
+AQ_API int aq_reduce(int op, int type, int ncomp, const void* in, uint32_t n,
+                     void* out, const void* identity, void* workspace,
+                     void* stream);
+AQ_API size_t aq_reduce_workspace(uint32_t n, int ncomp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQUAGPUSPH_AMD_H */

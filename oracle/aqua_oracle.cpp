@@ -1,0 +1,210 @@
+// C exports of the CPU oracle (test infrastructure only; see aqua_oracle.hpp).
+// Symbol pattern: orc_<kernel>_<f32|f64>_<2|3>d, arguments in the order of
+// the reference kernel signature followed by the preset constants.
+#include "aqua_oracle.hpp"
+
+using namespace orc;
+
+#define ORC_EXPORT extern "C" __attribute__((visibility("default")))
+
+// ---- dimension/precision independent ------------------------------------
+ORC_EXPORT unsigned orc_radix_key_bits(unsigned long long max_val)
+{
+    return radix_key_bits(max_val);
+}
+
+ORC_EXPORT void orc_stable_sort(usize* keys, usize* perm, usize* inv, usize n,
+                                unsigned key_bits)
+{
+    stable_sort(keys, perm, inv, n, key_bits);
+}
+
+ORC_EXPORT void orc_ihoc(const usize* icell_, usize* hoc, usize N,
+                         const usize* nc)
+{
+    ihoc(icell_, hoc, N, nc);
+}
+
+ORC_EXPORT void orc_scatter(const void* src, void* dst, size_t esize,
+                            const usize* id_sorted, usize N)
+{
+    scatter(src, dst, esize, id_sorted, N);
+}
+
+ORC_EXPORT void orc_neighs_2d(const int* imove, usize* n_neighs, usize limit,
+                              usize N, const usize* ic, const usize* hoc,
+                              const usize* nc)
+{
+    neighs<2>(imove, n_neighs, limit, N, LL{ ic, hoc, nc });
+}
+
+ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
+                              usize N, const usize* ic, const usize* hoc,
+                              const usize* nc)
+{
+    neighs<3>(imove, n_neighs, limit, N, LL{ ic, hoc, nc });
+}
+
+// ---- per (R, D) ----------------------------------------------------------
+#define ORC_DEFINE(R, SR, D)                                                   \
+    ORC_EXPORT void orc_minmax_vec_##SR##_##D##d(const R* r, usize N,         \
+                                                 R* rmin, R* rmax)            \
+    {                                                                          \
+        minmax_vec<D, R>(r, N, rmin, rmax);                                    \
+    }                                                                          \
+    ORC_EXPORT void orc_n_cells_##SR##_##D##d(const R* rmin, const R* rmax,   \
+                                              R cell_length, usize* nc)       \
+    {                                                                          \
+        n_cells<D, R>(rmin, rmax, cell_length, nc);                            \
+    }                                                                          \
+    ORC_EXPORT void orc_icell_##SR##_##D##d(usize* ic, const R* r, usize N,   \
+                                            const R* rmin, R support, R h,    \
+                                            const usize* nc)                  \
+    {                                                                          \
+        icell<D, R>(ic, r, N, rmin, support, h, nc);                           \
+    }                                                                          \
+    ORC_EXPORT void orc_predictor_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* u, const R* dudt, const R* rho, \
+        const R* drhodt, R* r_in, R* u_in, R* dudt_in, R* rho_in,             \
+        R* drhodt_in, usize N, R dt)                                           \
+    {                                                                          \
+        predictor<D, R>(imove, r, u, dudt, rho, drhodt, r_in, u_in, dudt_in,   \
+                        rho_in, drhodt_in, N, dt);                             \
+    }                                                                          \
+    ORC_EXPORT void orc_corrector_##SR##_##D##d(                               \
+        const int* imove, R* r, R* u, const R* dudt, R* rho, const R* drhodt, \
+        const R* dudt_in, const R* drhodt_in, usize N, R dt)                   \
+    {                                                                          \
+        corrector<D, R>(imove, r, u, dudt, rho, drhodt, dudt_in, drhodt_in, N, \
+                        dt);                                                   \
+    }                                                                          \
+    ORC_EXPORT void orc_domain_##SR##_##D##d(int* imove, R* r_in, R* u_in,    \
+                                             R* dudt_in, R* m, usize N,       \
+                                             const R* dmin, const R* dmax)    \
+    {                                                                          \
+        domain<D, R>(imove, r_in, u_in, dudt_in, m, N, dmin, dmax);            \
+    }                                                                          \
+    ORC_EXPORT void orc_eos_##SR##_##D##d(const unsigned* iset,               \
+                                          const int* imove, const R* rho,     \
+                                          R* p, const R* refd, usize N, R cs, \
+                                          R p0)                               \
+    {                                                                          \
+        eos<R>(iset, imove, rho, p, refd, N, cs, p0);                          \
+    }                                                                          \
+    ORC_EXPORT void orc_binormal_##SR##_##D##d(const R* normal, R* tangent,   \
+                                               R* binormal_, usize N)         \
+    {                                                                          \
+        binormal<D, R>(normal, tangent, binormal_, N);                         \
+    }                                                                          \
+    ORC_EXPORT void orc_rates_##SR##_##D##d(                                   \
+        const unsigned* iset, const int* imove, const R* rho,                  \
+        const R* grad_p, const R* lap_u, const R* div_u, R* dudt, R* drhodt,   \
+        const R* visc_dyn, usize N, const R* g)                                \
+    {                                                                          \
+        rates<D, R>(iset, imove, rho, grad_p, lap_u, div_u, dudt, drhodt,      \
+                    visc_dyn, N, g);                                           \
+    }                                                                          \
+    ORC_EXPORT void orc_delta_rate_##SR##_##D##d(                              \
+        const unsigned* iset, const int* imove, const R* rho, const R* lap_p,  \
+        R* drhodt, const R* refd, const R* delta, usize N, R dt)               \
+    {                                                                          \
+        delta_rate<R>(iset, imove, rho, lap_p, drhodt, refd, delta, N, dt);    \
+    }                                                                          \
+    ORC_EXPORT void orc_timestep_##SR##_##D##d(const int* imove, const R* u,  \
+                                               R* dt_var, usize N, R dt,      \
+                                               R dt_min, R courant, R dt_Ma,  \
+                                               R h)                           \
+    {                                                                          \
+        timestep<D, R>(imove, u, dt_var, N, dt, dt_min, courant, dt_Ma, h);    \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_shepard_apply_##SR##_##D##d(                        \
+        const int* imove, const R* shepard_, R* grad_p, R* lap_u, R* div_u,   \
+        usize N)                                                               \
+    {                                                                          \
+        bi_shepard_apply<D, R>(imove, shepard_, grad_p, lap_u, div_u, N);      \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_gradp_##SR##_##D##d(                                \
+        const unsigned* iset, const int* imove, const R* shepard_,             \
+        const R* rho, const R* lap_u, const R* dudt, R* grad_p,                \
+        const R* visc_dyn, const R* refd, usize N, const R* g)                 \
+    {                                                                          \
+        bi_gradp<D, R>(iset, imove, shepard_, rho, lap_u, dudt, grad_p,        \
+                       visc_dyn, refd, N, g);                                  \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_interp_shepard_##SR##_##D##d(                       \
+        const unsigned* iset, const int* imove, const R* shepard_, R* rho,     \
+        R* p, const R* refd, usize N, R cs, R p0)                              \
+    {                                                                          \
+        bi_interp_shepard<R>(iset, imove, shepard_, rho, p, refd, N, cs, p0);  \
+    }                                                                          \
+    ORC_EXPORT void orc_sensors_renorm_##SR##_##D##d(                          \
+        const int* imove, const R* shepard_, R* u, R* rho, R* p, usize N)      \
+    {                                                                          \
+        sensors_renorm<D, R>(imove, shepard_, u, rho, p, N);                   \
+    }                                                                          \
+    ORC_EXPORT void orc_shepard_##SR##_##D##d(                                 \
+        const int* imove, const R* r, const R* rho, const R* m, R* shep,       \
+        usize N, const usize* ic, const usize* hoc, const usize* nc, R H,      \
+        R CONW, int cfd_excl)                                                  \
+    {                                                                          \
+        shepard<D, R>(imove, r, rho, m, shep, N, LL{ ic, hoc, nc }, H, CONW,   \
+                      cfd_excl);                                               \
+    }                                                                          \
+    ORC_EXPORT void orc_interactions_##SR##_##D##d(                            \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        const R* p, R* grad_p, R* lap_u, R* div_u, usize N, const usize* ic,   \
+        const usize* hoc, const usize* nc, R H, R CONF, int lap_form)          \
+    {                                                                          \
+        interactions<D, R>(imove, r, u, rho, m, p, grad_p, lap_u, div_u, N,    \
+                           LL{ ic, hoc, nc }, H, CONF, lap_form);              \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_##SR##_##D##d(                                    \
+        const int* imove, const R* r, const R* rho, const R* m, const R* p,    \
+        R* lap_p, usize N, const usize* ic, const usize* hoc, const usize* nc, \
+        R H, R CONF)                                                           \
+    {                                                                          \
+        lapp<D, R>(imove, r, rho, m, p, lap_p, N, LL{ ic, hoc, nc }, H, CONF); \
+    }                                                                          \
+    ORC_EXPORT void orc_sensors_##SR##_##D##d(                                 \
+        const int* imove, const R* r, const R* m, R* u, R* rho, R* p, usize N, \
+        const usize* ic, const usize* hoc, const usize* nc, R H, R CONW)       \
+    {                                                                          \
+        sensors<D, R>(imove, r, m, u, rho, p, N, LL{ ic, hoc, nc }, H, CONW);  \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_shepard_compute_##SR##_##D##d(                      \
+        const int* imove, const R* r, const R* normal, const R* tangent,       \
+        const R* binormal_, const R* m, R* shep, usize N, const usize* ic,     \
+        const usize* hoc, const usize* nc, R H, R CONW, R DIMS)                \
+    {                                                                          \
+        bi_shepard_compute<D, R>(imove, r, normal, tangent, binormal_, m,      \
+                                 shep, N, LL{ ic, hoc, nc }, H, CONW, DIMS);   \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_lapu_##SR##_##D##d(                                 \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        R* lap_u, usize N, const usize* ic, const usize* hoc, const usize* nc, \
+        R H, R CONF, int lap_form)                                             \
+    {                                                                          \
+        bi_lapu<D, R>(imove, r, u, rho, m, lap_u, N, LL{ ic, hoc, nc }, H,     \
+                      CONF, lap_form);                                         \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_interp_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* m, const R* rho,                \
+        const R* grad_p, R* p, usize N, const usize* ic, const usize* hoc,     \
+        const usize* nc, R H, R CONW)                                          \
+    {                                                                          \
+        bi_interp<D, R>(imove, r, m, rho, grad_p, p, N, LL{ ic, hoc, nc }, H,  \
+                        CONW);                                                 \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_interactions_##SR##_##D##d(                         \
+        const int* imove, const R* r, const R* normal, const R* u,             \
+        const R* rho, const R* m, const R* p, R* grad_p, R* div_u, usize N,    \
+        const usize* ic, const usize* hoc, const usize* nc, R H, R CONW)       \
+    {                                                                          \
+        bi_interactions<D, R>(imove, r, normal, u, rho, m, p, grad_p, div_u,   \
+                              N, LL{ ic, hoc, nc }, H, CONW);                  \
+    }
+
+ORC_DEFINE(float, f32, 2)
+ORC_DEFINE(float, f32, 3)
+ORC_DEFINE(double, f64, 2)
+ORC_DEFINE(double, f64, 3)

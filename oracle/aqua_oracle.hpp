@@ -1,0 +1,888 @@
+// CPU oracle for the AQUAgpusph per-timestep SPH pipeline.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, kernel by kernel, the
+// OpenCL scripts and native tools of the reference (AQUAgpusph v5.0.4) so
+// that the HIP implementation can be checked against it.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+// product library never does.
+//
+// Every function is templated on the spatial dimension D (2 or 3) and on the
+// real type R (float for parity runs, double for the "FP64 oracle" used to
+// state the error of every FP32 result).  Arithmetic is written in the same
+// association order as the reference kernels and is compiled without FP
+// contraction, so that the FP32 instantiation reproduces the reference
+// expression trees operation by operation.
+//
+// Layout follows the reference variables: one array per field; `vec` fields
+// hold VS(D) reals per particle (2 in 2D, 4 in 3D with w carried along);
+// indices are 32-bit (`usize` = uint, the reference default addr_bits=32).
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <vector>
+
+namespace orc {
+
+typedef uint32_t usize;
+
+template <int D> struct Dim {
+    static constexpr int VS = (D == 2) ? 2 : 4;  // stored components per vec
+};
+
+// ---------------------------------------------------------------------------
+// Small vector helpers over the first D ("XYZ") components.
+// dot() is evaluated left to right: ((x*x' + y*y') + z*z').
+// ---------------------------------------------------------------------------
+template <int D, typename R> struct V {
+    R c[3];
+};
+
+template <int D, typename R>
+inline V<D, R> ld(const R* a, usize i)
+{
+    V<D, R> v;
+    for (int k = 0; k < D; k++) v.c[k] = a[(size_t)i * Dim<D>::VS + k];
+    if (D == 2) v.c[2] = R(0);
+    return v;
+}
+
+template <int D, typename R>
+inline V<D, R> sub(const V<D, R>& a, const V<D, R>& b)
+{
+    V<D, R> v;
+    for (int k = 0; k < D; k++) v.c[k] = a.c[k] - b.c[k];
+    v.c[2] = (D == 2) ? R(0) : v.c[2];
+    return v;
+}
+
+template <int D, typename R>
+inline R dot(const V<D, R>& a, const V<D, R>& b)
+{
+    R s = a.c[0] * b.c[0];
+    s = s + a.c[1] * b.c[1];
+    if (D == 3) s = s + a.c[2] * b.c[2];
+    return s;
+}
+
+template <int D, typename R> inline R length(const V<D, R>& a)
+{
+    return std::sqrt(dot<D, R>(a, a));
+}
+
+// acc[i].XYZ += s * v, component by component
+template <int D, typename R>
+inline void addsv(R* acc, usize i, R s, const V<D, R>& v)
+{
+    for (int k = 0; k < D; k++) acc[(size_t)i * Dim<D>::VS + k] += s * v.c[k];
+}
+
+// ---------------------------------------------------------------------------
+// Wendland C2 kernel (reference: resources/Scripts/KernelFunctions/
+// Wendland2D.hcl:44-66 and Wendland3D.hcl:44-66, boundary-integral parts at
+// Wendland2D.hcl:100-154 and Wendland3D.hcl:100-187).  Constants are the
+// single-precision literals of the reference; products are kept in their
+// left-to-right order.
+// ---------------------------------------------------------------------------
+static const float kIPI = 0.318309886f;  // Wendland*.hcl iM_PI
+
+template <int D, typename R> inline R kernelW(R q)
+{
+    const R wcon = (D == 2) ? R(0.109375f * kIPI) : R(0.08203125f * kIPI);
+    const R a = R(2) - q;
+    return wcon * (R(1) + R(2) * q) * a * a * a * a;
+}
+
+template <int D, typename R> inline R kernelF(R q)
+{
+    const R wcon = (D == 2) ? R(1.09375f * kIPI) : R(0.8203125f * kIPI);
+    const R a = R(2) - q;
+    return wcon * a * a * a;
+}
+
+template <int D, typename R> inline R kernelS_P(R q)
+{
+    const R q2 = q * q;
+    const R q3 = q2 * q;
+    if (D == 2) {
+        const R wcon = R(0.109375f * kIPI);
+        return wcon * (R(0.285714f) * q3 * q2 - R(2.5f) * q2 * q2 +
+                       R(8.f) * q3 - R(10.f) * q2 + R(8.f));
+    }
+    const R wcon = R(0.08203125f * kIPI);
+    return wcon * (R(0.25f) * q3 * q2 - R(2.142857f) * q2 * q2 +
+                   R(6.666667f) * q3 - R(8.f) * q2 + R(5.333333f));
+}
+
+// OpenCL sign(): 1, -1, or the (signed) zero / 0 for NaN
+template <typename R> inline R cl_sign(R x)
+{
+    if (x > R(0)) return R(1);
+    if (x < R(0)) return R(-1);
+    if (x != x) return R(0);
+    return x;
+}
+
+template <typename R> inline R cl_min(R x, R y) { return (y < x) ? y : x; }
+template <typename R> inline R cl_max(R x, R y) { return (x < y) ? y : x; }
+
+// acospi(x) = acos(x)/pi.  glibc has no acospi; the double evaluation
+// rounded once to R is within the OpenCL 5-ulp bound for float.
+template <typename R> inline R cl_acospi(R x)
+{
+    return R(std::acos((double)x) / 3.14159265358979323846);
+}
+
+template <typename R> inline R Omega3(R a, R b)
+{
+    const R a2 = a * a;
+    const R b2 = b * b;
+    return cl_acospi<R>(cl_min<R>(
+        std::sqrt((R(1) + a2 + b2) / ((R(1) + a2) * (R(1) + b2))), R(1)));
+}
+
+template <int D, typename R> inline R kernelS_D(R d, R t, R b, R s)
+{
+    if (D == 2) {
+        const R wcon = R(0.5f * kIPI);
+        const R dr = R(0.5f) * s;
+        return -wcon * (std::atan((t + dr) / d) - std::atan((t - dr) / d));
+    }
+    const R wcon = R(0.25f);
+    const R dr = R(0.5f) * std::sqrt(s);
+    const R t1 = (t - dr) / d, t2 = (t + dr) / d;
+    const R b1 = (b - dr) / d, b2 = (b + dr) / d;
+    const R st = cl_sign(t1), sb = cl_sign(b1);
+    return -wcon * (Omega3(t2, b2) - st * Omega3(t1, b2) - sb * Omega3(t2, b1) +
+                    st * sb * Omega3(t1, b1));
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour walk (reference: resources/Scripts/types/3D.h:197-219 and
+// types/2D.h:174-193): x offset outermost, then y, then z; within each cell
+// the particles in ascending sorted index while icell[j] == cell.  Cell ids
+// are computed in 32-bit unsigned arithmetic like the reference.
+// ---------------------------------------------------------------------------
+template <int D, class F>
+inline void walk(usize c_i, usize N, const usize* nc, const usize* icell,
+                 const usize* ihoc, F f)
+{
+    for (int ci = -1; ci <= 1; ci++) {
+        for (int cj = -1; cj <= 1; cj++) {
+            if (D == 2) {
+                const usize c_j = c_i + (usize)ci + (usize)cj * nc[0];
+                usize j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    f(j);
+                    j++;
+                }
+            } else {
+                for (int ck = -1; ck <= 1; ck++) {
+                    const usize c_j = c_i + (usize)ci + (usize)cj * nc[0] +
+                                      (usize)ck * nc[0] * nc[1];
+                    usize j = ihoc[c_j];
+                    while ((j < N) && (icell[j] == c_j)) {
+                        f(j);
+                        j++;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// float -> unsigned conversion with the AMDGPU v_cvt_u32_f32 saturation
+// (NaN and negatives -> 0, too large -> UINT_MAX); the OpenCL cast is
+// implementation-defined outside [0, 2^32).
+template <typename R> inline usize cvt_u32(R x)
+{
+    if (!(x > R(0))) return 0u;
+    if (x >= R(4294967296.0)) return 0xFFFFFFFFu;
+    return (usize)x;
+}
+
+// ---------------------------------------------------------------------------
+// Link-list (reference: aquagpusph/CalcServer/LinkList.cpp:185-232 for
+// n_cells, LinkList.cl.in:32-113 for the kernels, RadixSort.cpp:129-303 and
+// RadixSort.cl.in for the stable LSD sort).
+// ---------------------------------------------------------------------------
+
+// component-wise min/max of a vec array with the reference identities
+// (Reduction.hcl.in VEC_INFINITY / -VEC_INFINITY)
+template <int D, typename R>
+void minmax_vec(const R* r, usize N, R* rmin, R* rmax)
+{
+    const int VS = Dim<D>::VS;
+    for (int k = 0; k < VS; k++) {
+        const bool w = (k == 3);
+        rmin[k] = w ? R(0) : std::numeric_limits<R>::infinity();
+        rmax[k] = w ? -R(0) : -std::numeric_limits<R>::infinity();
+    }
+    for (usize i = 0; i < N; i++)
+        for (int k = 0; k < VS; k++) {
+            rmin[k] = cl_min(rmin[k], r[(size_t)i * VS + k]);
+            rmax[k] = cl_max(rmax[k], r[(size_t)i * VS + k]);
+        }
+}
+
+// LinkList::nCells: n = (ulong)((max-min)/cell_length) + 6, w = x*y*z.
+// cell_length = support*h computed in float at setup.
+template <int D, typename R>
+void n_cells(const R* rmin, const R* rmax, R cell_length, usize* nc)
+{
+    uint64_t n[3] = { 1, 1, 1 };
+    for (int k = 0; k < D; k++) {
+        const R v = (rmax[k] - rmin[k]) / cell_length;
+        n[k] = (uint64_t)cvt_u32(v) + 6;
+    }
+    nc[0] = (usize)n[0];
+    nc[1] = (usize)n[1];
+    nc[2] = (usize)n[2];
+    nc[3] = (usize)(n[0] * n[1] * n[2]);
+}
+
+// iCell kernel, LinkList.cl.in:54-85
+template <int D, typename R>
+void icell(usize* ic, const R* r, usize N, const R* rmin, R support, R h,
+           const usize* nc)
+{
+    const int VS = Dim<D>::VS;
+    const R idist = R(1) / (support * h);
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        const usize cx = cvt_u32((r[(size_t)i * VS + 0] - rmin[0]) * idist) + 3u;
+        const usize cy = cvt_u32((r[(size_t)i * VS + 1] - rmin[1]) * idist) + 3u;
+        if (D == 3) {
+            const usize cz =
+                cvt_u32((r[(size_t)i * VS + 2] - rmin[2]) * idist) + 3u;
+            ic[i] = cx - 1u + (cy - 1u) * nc[0] + (cz - 1u) * nc[0] * nc[1];
+        } else {
+            ic[i] = cx - 1u + (cy - 1u) * nc[0];
+        }
+    }
+}
+
+// Number of key bits the reference radix sort processes
+// (RadixSort.cpp:152-176): bits of nextPowerOf2(max_val), rounded up to the
+// 4-bit digit.  max_val = n_cells.w for the link-list, UINT_MAX>>1 otherwise.
+inline unsigned radix_key_bits(uint64_t max_val)
+{
+    uint64_t p = 1;
+    while (p < max_val) p <<= 1;  // AuxiliarMethods nextPowerOf2
+    unsigned bits = 1;
+    for (; (p & 1) == 0; p >>= 1, bits++)
+        ;
+    return (bits + 3u) / 4u * 4u;
+}
+
+// Stable LSD radix sort result (RadixSort.cl.in:35-323): keys sorted on their
+// low `key_bits` bits, perm[k] = original index of the k-th sorted key
+// (id_unsorted) and inv[perm[k]] = k (id_sorted).  A stable LSD sort over
+// contiguous per-thread chunks with a radix-major exclusive scan equals a
+// stable sort by the masked key.
+inline void stable_sort(usize* keys, usize* perm, usize* inv, usize n,
+                        unsigned key_bits)
+{
+    const uint64_t mask =
+        (key_bits >= 32) ? 0xFFFFFFFFull : ((1ull << key_bits) - 1ull);
+    std::vector<usize> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::vector<usize> k(keys, keys + n);
+    std::stable_sort(idx.begin(), idx.end(), [&](usize a, usize b) {
+        return (k[a] & mask) < (k[b] & mask);
+    });
+    for (usize s = 0; s < n; s++) {
+        keys[s] = k[idx[s]];
+        perm[s] = idx[s];
+        inv[idx[s]] = s;
+    }
+}
+
+// iHoc + linkList kernels, LinkList.cl.in:32-42 and 92-113
+inline void ihoc(const usize* ic, usize* hoc, usize N, const usize* nc)
+{
+    for (usize c = 0; c < nc[3]; c++) hoc[c] = N;
+    if (N < 1) return;
+    for (usize i = 0; i + 1 < N; i++) {
+        const usize c = ic[i], c2 = ic[i + 1];
+        if (i == 0) hoc[c] = 0;
+        if (c2 != c) hoc[c2] = i + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reorder, resources/Scripts/basic/Sort.cl:57-124 (stage1 and stage2 after
+// the "Backup *" copies of Presets/src/basic.xml:141-146): x[id_sorted[i]] =
+// x_in[i].  Generic element-size scatter.
+// ---------------------------------------------------------------------------
+inline void scatter(const void* src, void* dst, size_t esize,
+                    const usize* id_sorted, usize N)
+{
+    const char* s = (const char*)src;
+    char* d = (char*)dst;
+    for (usize i = 0; i < N; i++)
+        std::memcpy(d + (size_t)id_sorted[i] * esize, s + (size_t)i * esize,
+                    esize);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming kernels
+// ---------------------------------------------------------------------------
+
+// improved Euler predictor, basic/time_scheme/improved_euler.cl:75-103
+template <int D, typename R>
+void predictor(const int* imove, const R* r, const R* u, const R* dudt,
+               const R* rho, const R* drhodt, R* r_in, R* u_in, R* dudt_in,
+               R* rho_in, R* drhodt_in, usize N, R dt)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        R DT = dt;
+        if (imove[i] <= 0) DT = R(0);
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            dudt_in[a] = dudt[a];
+            u_in[a] = u[a] + DT * dudt[a];
+            r_in[a] = r[a] + DT * u[a] + R(0.5f) * DT * DT * dudt[a];
+        }
+        drhodt_in[i] = drhodt[i];
+        rho_in[i] = rho[i] + DT * drhodt[i];
+    }
+}
+
+// improved Euler corrector, basic/time_scheme/improved_euler.cl:125-147
+template <int D, typename R>
+void corrector(const int* imove, R* r, R* u, const R* dudt, R* rho,
+               const R* drhodt, const R* dudt_in, const R* drhodt_in, usize N,
+               R dt)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] > 0) {
+            const R DT = R(0.5f) * dt;
+            for (int k = 0; k < VS; k++) {
+                const size_t a = i * VS + k;
+                u[a] += DT * (dudt[a] - dudt_in[a]);
+            }
+            for (int k = 0; k < VS; k++) {
+                const size_t a = i * VS + k;
+                r[a] += DT * DT * (dudt[a] - dudt_in[a]);
+            }
+            rho[i] += DT * (drhodt[i] - drhodt_in[i]);
+        }
+    }
+}
+
+// basic/Domain.cl:48-90
+template <int D, typename R>
+void domain(int* imove, R* r_in, R* u_in, R* dudt_in, R* m, usize N,
+            const R* dmin, const R* dmax)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] <= -255) continue;
+        const R* c = r_in + (size_t)i * VS;
+        bool out = false;
+        for (int k = 0; k < D; k++) {
+            if (std::isnan(c[k]) || std::isinf(c[k]) || c[k] < dmin[k] ||
+                c[k] > dmax[k])
+                out = true;
+        }
+        if (out) {
+            imove[i] = -256;
+            m[i] = R(0);
+            for (int k = 0; k < VS; k++) {
+                u_in[(size_t)i * VS + k] = R(0);
+                dudt_in[(size_t)i * VS + k] = R(0);
+                r_in[(size_t)i * VS + k] = dmax[k];
+            }
+        }
+    }
+}
+
+// basic/EOS.cl:57-73 (excluded: imove <= 0 && imove != -1)
+template <typename R>
+void eos(const unsigned* iset, const int* imove, const R* rho, R* p,
+         const R* refd, usize N, R cs, R p0)
+{
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if ((imove[i] <= 0) && (imove[i] != -1)) continue;
+        p[i] = p0 + cs * cs * (rho[i] - refd[iset[i]]);
+    }
+}
+
+// basic/Binormal.cl:37-53
+template <int D, typename R>
+void binormal(const R* normal, R* tangent, R* binormal, usize N)
+{
+    for (usize i = 0; i < N; i++) {
+        if (D == 2) {
+            binormal[2 * i] = R(0);
+            binormal[2 * i + 1] = R(0);
+            tangent[2 * i] = normal[2 * i + 1];
+            tangent[2 * i + 1] = -normal[2 * i];
+        } else {
+            const R* a = normal + 4 * (size_t)i;
+            const R* b = tangent + 4 * (size_t)i;
+            R* c = binormal + 4 * (size_t)i;
+            const R x = a[1] * b[2] - a[2] * b[1];
+            const R y = a[2] * b[0] - a[0] * b[2];
+            const R z = a[0] * b[1] - a[1] * b[0];
+            c[0] = x;
+            c[1] = y;
+            c[2] = z;
+            c[3] = R(0);
+        }
+    }
+}
+
+// cfd/Rates.cl:55-77
+template <int D, typename R>
+void rates(const unsigned* iset, const int* imove, const R* rho,
+           const R* grad_p, const R* lap_u, const R* div_u, R* dudt,
+           R* drhodt, const R* visc_dyn, usize N, const R* g)
+{
+    (void)rho;
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) continue;
+        const R mu = visc_dyn[iset[i]];
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            dudt[a] = -grad_p[a] + mu * lap_u[a] + g[k];
+        }
+        drhodt[i] = -div_u[i];
+    }
+}
+
+// basic/deltaSPH.cl:330-350 with cfd/deltaSPH.cl:29 (fluid only)
+template <typename R>
+void delta_rate(const unsigned* iset, const int* imove, const R* rho,
+                const R* lap_p, R* drhodt, const R* refd, const R* delta,
+                usize N, R dt)
+{
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) continue;
+        const unsigned s = iset[i];
+        const R delta_f = delta[s] * dt * rho[i] / refd[s];
+        drhodt[i] += delta_f * lap_p[i];
+    }
+}
+
+// cfd/TimeStep.cl:56-77; length() of the full vec (w included)
+template <int D, typename R>
+void timestep(const int* imove, const R* u, R* dt_var, usize N, R dt,
+              R dt_min, R courant, R dt_Ma, R h)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] <= 0) {
+            dt_var[i] = dt;
+            continue;
+        }
+        const R dr_max = dt_Ma * h;
+        R s = u[i * VS] * u[i * VS];
+        for (int k = 1; k < VS; k++) s = s + u[i * VS + k] * u[i * VS + k];
+        const R dt_u = courant * dr_max / std::sqrt(s);
+        dt_var[i] = cl_max(cl_min(dt, dt_u), dt_min);
+    }
+}
+
+// cfd/Boundary/BI/Shepard.cl:173-195 (apply)
+template <int D, typename R>
+void bi_shepard_apply(const int* imove, const R* shepard, R* grad_p,
+                      R* lap_u, R* div_u, usize N)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) continue;
+        const R s = shepard[i];
+        for (int k = 0; k < VS; k++) grad_p[i * VS + k] /= s;
+        for (int k = 0; k < VS; k++) lap_u[i * VS + k] /= s;
+        div_u[i] /= s;
+    }
+}
+
+// cfd/Boundary/BI/GradP.cl:55-83 (freeslip)
+template <int D, typename R>
+void bi_gradp(const unsigned* iset, const int* imove, const R* shepard,
+              const R* rho, const R* lap_u, const R* dudt, R* grad_p,
+              const R* visc_dyn, const R* refd, usize N, const R* g)
+{
+    (void)rho;
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -3) continue;
+        R s = shepard[i];
+        if (s < R(1.0E-6f)) s = R(1);
+        const R f = visc_dyn[iset[i]] / refd[iset[i]];
+        for (int k = 0; k < VS; k++) {
+            const size_t a = (size_t)i * VS + k;
+            grad_p[a] = f * lap_u[a] / s + g[k] - dudt[a];
+        }
+    }
+}
+
+// cfd/Boundary/BI/InterpolationShepard.cl:48-76
+template <typename R>
+void bi_interp_shepard(const unsigned* iset, const int* imove,
+                       const R* shepard, R* rho, R* p, const R* refd, usize N,
+                       R cs, R p0)
+{
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -3) continue;
+        R s = shepard[i];
+        if (s < R(1.0E-6f)) s = R(1);
+        p[i] /= s;
+        rho[i] = refd[iset[i]] + (p[i] - p0) / (cs * cs);
+    }
+}
+
+// cfd/SensorsRenormalization.cl:42-67
+template <int D, typename R>
+void sensors_renorm(const int* imove, const R* shepard, R* u, R* rho, R* p,
+                    usize N)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 0) continue;
+        R s = shepard[i];
+        if (s < R(1.0E-6f)) s = R(1);
+        for (int k = 0; k < VS; k++) u[(size_t)i * VS + k] /= s;
+        rho[i] /= s;
+        p[i] /= s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernels.  H, CONW, CONF are the preset definitions after the
+// reference's "%#G" formatting (CalcServer.cpp:241-260).
+// ---------------------------------------------------------------------------
+struct LL {
+    const usize* icell;
+    const usize* ihoc;
+    const usize* nc;
+};
+
+// basic/Shepard.cl:76-125; `cfd_excl` selects the cfd variant
+// (cfd/Shepard.cl:29: EXCLUDED_PARTICLE = imove != 1) over the basic default
+// (imove >= 3).
+template <int D, typename R>
+void shepard(const int* imove, const R* r, const R* rho, const R* m,
+             R* shep, usize N, LL ll, R H, R CONW, int cfd_excl)
+{
+    auto excl = [&](usize k) {
+        return cfd_excl ? (imove[k] != 1) : (imove[k] >= 3);
+    };
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if ((imove[i] < -3) || ((imove[i] > 0) && excl(i))) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R acc = shep[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (excl(j)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            acc += kernelW<D, R>(q) * CONW * m[j] / rho[j];
+        });
+        shep[i] = acc;
+    }
+}
+
+// cfd/Interactions.cl:60-145 (Monaghan-Cleary Laplacian when lap_form==1,
+// Morris when lap_form==2)
+template <int D, typename R>
+void interactions(const int* imove, const R* r, const R* u, const R* rho,
+                  const R* m, const R* p, R* grad_p, R* lap_u, R* div_u,
+                  usize N, LL ll, R H, R CONF, int lap_form)
+{
+    const int VS = Dim<D>::VS;
+    const R CLEARY = (D == 2) ? R(8.f) : R(10.f);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> u_i = ld<D, R>(u, i);
+        const R p_i = p[i];
+        const R rho_i = rho[i];
+        V<D, R> gp = ld<D, R>(grad_p, i), lu = ld<D, R>(lap_u, i);
+        R du = div_u[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (i == j) return;
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R p_j = p[j];
+            const V<D, R> u_ij = sub<D, R>(ld<D, R>(u, j), u_i);
+            const R udr = dot<D, R>(u_ij, r_ij);
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j];
+            const R sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
+            for (int k = 0; k < D; k++) gp.c[k] += sg * r_ij.c[k];
+            if (lap_form == 1) {
+                const R r2 = (q * q + R(0.01f)) * H * H;
+                const R sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
+                for (int k = 0; k < D; k++) lu.c[k] += sl * r_ij.c[k];
+            } else {
+                const R sl = f_ij * R(2.f) / (rho_i * rho_j);
+                for (int k = 0; k < D; k++) lu.c[k] += sl * u_ij.c[k];
+            }
+            du += udr * f_ij * rho_i / rho_j;
+        });
+        for (int k = 0; k < D; k++) {
+            grad_p[(size_t)i * VS + k] = gp.c[k];
+            lap_u[(size_t)i * VS + k] = lu.c[k];
+        }
+        div_u[i] = du;
+    }
+}
+
+// basic/deltaSPH.cl:191-242 (lapp) with cfd/deltaSPH.cl:29 (fluid only)
+template <int D, typename R>
+void lapp(const int* imove, const R* r, const R* rho, const R* m, const R* p,
+          R* lap_p, usize N, LL ll, R H, R CONF)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const R p_i = p[i];
+        R acc = lap_p[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((i == j) || (imove[j] != 1)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            acc += (p[j] - p_i) * f_ij;
+        });
+        lap_p[i] = acc;
+    }
+}
+
+// cfd/Sensors.cl:57-130
+template <int D, typename R>
+void sensors(const int* imove, const R* r, const R* m, R* u, R* rho, R* p,
+             usize N, LL ll, R H, R CONW)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 0) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R uu[3] = { 0, 0, 0 }, rr = 0, pp = 0;
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (i == j) return;
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R w_ij = kernelW<D, R>(q) * CONW * m[j] / rho_j;
+            for (int k = 0; k < D; k++) uu[k] += u[(size_t)j * VS + k] * w_ij;
+            rr += rho_j * w_ij;
+            pp += p[j] * w_ij;
+        });
+        for (int k = 0; k < D; k++) u[(size_t)i * VS + k] = uu[k];
+        rho[i] = rr;
+        p[i] = pp;
+    }
+}
+
+// cfd/Boundary/BI/Shepard.cl:61-150 (compute)
+template <int D, typename R>
+void bi_shepard_compute(const int* imove, const R* r, const R* normal,
+                        const R* tangent, const R* binormal, const R* m,
+                        R* shep, usize N, LL ll, R H, R CONW, R DIMS)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if ((imove[i] < -3) || (imove[i] > 1)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R acc = R(1);
+        bool self_added = false;
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -3) return;
+            if (i == j) {
+                if (!self_added) {
+                    self_added = true;
+                    acc -= R(0.5f);
+                }
+                return;
+            }
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const V<D, R> t_j = ld<D, R>(tangent, j);
+            const V<D, R> b_j = ld<D, R>(binormal, j);
+            const R r_n = dot<D, R>(r_ij, n_j);
+            const R area_j = m[j];
+            if ((r_n > R(-1E-8f) * H) && (r_n < R(1E-8f) * H)) {
+                if (!self_added) {
+                    const R r_t = std::fabs(dot<D, R>(r_ij, t_j));
+                    const R r_b = std::fabs(dot<D, R>(r_ij, b_j));
+                    const R dr = R(0.55f) *
+                                 std::pow(area_j, R(1.f) / (DIMS - R(1.f)));
+                    if ((r_t <= dr) && (r_b <= dr)) {
+                        self_added = true;
+                        acc -= R(0.5f);
+                    }
+                }
+                return;
+            }
+            const R r_t = std::fabs(dot<D, R>(r_ij, t_j));
+            const R r_b = std::fabs(dot<D, R>(r_ij, b_j));
+            acc += r_n * CONW * kernelS_P<D, R>(q) * area_j +
+                   kernelS_D<D, R>(std::fabs(r_n), r_t, r_b, area_j);
+        });
+        shep[i] = acc;
+    }
+}
+
+// cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
+template <int D, typename R>
+void bi_lapu(const int* imove, const R* r, const R* u, const R* rho,
+             const R* m, R* lap_u, usize N, LL ll, R H, R CONF, int lap_form)
+{
+    const int VS = Dim<D>::VS;
+    const R CLEARY = (D == 2) ? R(8.f) : R(10.f);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != -3) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> u_i = ld<D, R>(u, i);
+        V<D, R> lu = ld<D, R>(lap_u, i);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (i == j) return;
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j];
+            const V<D, R> u_ij = sub<D, R>(ld<D, R>(u, j), u_i);
+            if (lap_form == 1) {
+                const R udr = dot<D, R>(u_ij, r_ij);
+                const R r2 = (q * q + R(0.01f)) * H * H;
+                const R sl = f_ij * CLEARY * udr / (r2 * rho_j);
+                for (int k = 0; k < D; k++) lu.c[k] += sl * r_ij.c[k];
+            } else {
+                const R sl = f_ij * R(2.f) / rho_j;
+                for (int k = 0; k < D; k++) lu.c[k] += sl * u_ij.c[k];
+            }
+        });
+        for (int k = 0; k < D; k++) lap_u[(size_t)i * VS + k] = lu.c[k];
+    }
+}
+
+// cfd/Boundary/BI/Interpolation.cl:54-107
+template <int D, typename R>
+void bi_interp(const int* imove, const R* r, const R* m, const R* rho,
+               const R* grad_p, R* p, usize N, LL ll, R H, R CONW)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != -3) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        V<D, R> gp_i = ld<D, R>(grad_p, i);
+        for (int k = 0; k < D; k++) gp_i.c[k] = rho[i] * gp_i.c[k];
+        R acc = R(0);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R w_ij = kernelW<D, R>(q) * CONW * m[j] / rho[j];
+            acc += (p[j] - dot<D, R>(gp_i, r_ij)) * w_ij;
+        });
+        p[i] = acc;
+    }
+}
+
+// cfd/Boundary/BI/Interactions.cl:51-133
+template <int D, typename R>
+void bi_interactions(const int* imove, const R* r, const R* normal,
+                     const R* u, const R* rho, const R* m, const R* p,
+                     R* grad_p, R* div_u, usize N, LL ll, R H, R CONW)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> u_i = ld<D, R>(u, i);
+        const R p_i = p[i];
+        const R rho_i = rho[i];
+        V<D, R> gp = ld<D, R>(grad_p, i);
+        R dv = div_u[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -3) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const R area_j = m[j];
+            const R p_j = p[j];
+            const V<D, R> du = sub<D, R>(ld<D, R>(u, j), u_i);
+            const R w_ij = kernelW<D, R>(q) * CONW * area_j;
+            const R sg = (p_i + p_j) / rho_i * w_ij;
+            for (int k = 0; k < D; k++) gp.c[k] += sg * n_j.c[k];
+            dv += rho_i * dot<D, R>(du, n_j) * w_ij;
+        });
+        for (int k = 0; k < D; k++) grad_p[(size_t)i * VS + k] = gp.c[k];
+        div_u[i] = dv;
+    }
+}
+
+// basic/neighs.cl:52-91: every candidate of the 3^d cells (self included,
+// no distance or type filter); the walk stops as soon as the count reaches
+// neighs_limit; buffer particles (imove <= -255) get 0.
+template <int D>
+void neighs(const int* imove, usize* n_neighs, usize limit, usize N, LL ll)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] <= -255) {
+            n_neighs[i] = 0;
+            continue;
+        }
+        usize n = 0;
+        bool stop = false;
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize) {
+            if (stop) return;
+            n += 1;
+            if (n >= limit) stop = true;
+        });
+        n_neighs[i] = n;
+    }
+}
+
+}  // namespace orc

@@ -1,0 +1,274 @@
+"""CPU oracle pipeline: one AQUAgpusph time step replayed on the CPU.
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use this module, never the product library.
+
+The kernels are the C restatements in aqua_oracle.hpp; this module replays
+them in the tool order that the reference presets resolve to
+(resources/Presets/src/basic.xml:139-175 with the insert semantics of
+aquagpusph/InputOutput/State.cpp:803-1033), for the preset stack
+
+    basic + improved_euler + [domain] + cfd + [deltaSPH] + [BI] + [variableTimeStep]
+
+Scalars follow the reference's host semantics: set_scalar expressions are
+evaluated in double and narrowed to the variable type; preset definitions
+(H, CONW, CONF, DIMS) are evaluated, narrowed to float and re-parsed from
+their "%#G" text (aquagpusph/CalcServer/CalcServer.cpp:241-260).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    """Load (building if needed) the oracle shared library."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle_aqua.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def definition(value):
+    """Round a preset definition like the reference: float, then %#G."""
+    return float(np.float32(float("%#G" % float(np.float32(value)))))
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class OracleSim:
+    """Holds every pipeline variable as a numpy array and steps it."""
+
+    VEC_FIELDS = ("r", "normal", "tangent", "binormal", "u", "dudt")
+    SCA_FIELDS = ("rho", "drhodt", "m", "p")
+
+    def __init__(self, case, dtype=np.float32, domain=True, bi=True,
+                 delta=True, variable_dt=True, lap_form=1):
+        self.dims = case["dims"]
+        self.vs = 2 if self.dims == 2 else 4
+        self.dtype = np.dtype(dtype)
+        self.sfx = ("f32" if self.dtype == np.float32 else "f64") + \
+            "_%dd" % self.dims
+        self.opts = dict(domain=domain, bi=bi, delta=delta,
+                         variable_dt=variable_dt, lap_form=lap_form)
+        N = self.N = int(case["N"])
+        R = self.dtype
+        f = {}
+        for k in self.VEC_FIELDS:
+            v = case.get(k)
+            f[k] = np.zeros((N, self.vs), R) if v is None else \
+                np.ascontiguousarray(v, dtype=R).reshape(N, self.vs).copy()
+        for k in self.SCA_FIELDS:
+            v = case.get(k)
+            f[k] = np.zeros(N, R) if v is None else \
+                np.ascontiguousarray(v, dtype=R).copy()
+        f["imove"] = np.ascontiguousarray(case["imove"], np.int32).copy()
+        f["iset"] = np.ascontiguousarray(
+            case.get("iset", np.zeros(N)), np.uint32).copy()
+        f["id"] = np.arange(N, dtype=np.uint32)
+        for k in ("r", "u", "dudt", "normal", "tangent"):
+            f[k + "_in"] = np.zeros_like(f[k])
+        for k in ("rho", "drhodt", "m"):
+            f[k + "_in"] = np.zeros_like(f[k])
+        for k in ("id", "iset", "imove"):
+            f[k + "_in"] = np.zeros_like(f[k])
+        f["grad_p"] = np.zeros((N, self.vs), R)
+        f["lap_u"] = np.zeros((N, self.vs), R)
+        for k in ("div_u", "shepard", "lap_p", "dt_var"):
+            f[k] = np.zeros(N, R)
+        for k in ("icell", "id_sorted", "id_unsorted"):
+            f[k] = np.zeros(N, np.uint32)
+        self.f = f
+        self.ihoc = np.zeros(0, np.uint32)
+        self.n_cells = np.zeros(4, np.uint32)
+        self.r_min = np.zeros(self.vs, R)
+        self.r_max = np.zeros(self.vs, R)
+
+        p = case["params"]
+        self.sets = {k: np.ascontiguousarray(p[k], R)
+                     for k in ("refd", "visc_dyn", "delta")}
+        g = np.zeros(self.vs, R)
+        g[:self.dims] = p["g"][:self.dims]
+        self.g = g
+        self.h = np.float32(p["h"])
+        self.cs = np.float32(p["cs"])
+        self.p0 = np.float32(p.get("p0", 0.0))
+        self.courant = np.float32(p["courant"])
+        self.dt_Ma = np.float32(p.get("dt_Ma", 0.1))
+        self.dt_min = np.float32(p.get("dt_min", 0.0))
+        self.support = np.float32(2.0)
+        dmin = np.zeros(self.vs, R)
+        dmax = np.zeros(self.vs, R)
+        if domain:
+            dmin[:self.dims] = p["domain_min"][:self.dims]
+            dmax[:self.dims] = p["domain_max"][:self.dims]
+        self.domain_min, self.domain_max = dmin, dmax
+        # preset definitions (basic.xml Definitions), float -> %#G -> float
+        self.H = definition(self.h)
+        self.CONW = definition(1.0 / float(self.h) ** self.dims)
+        self.CONF = definition(1.0 / float(self.h) ** (self.dims + 2))
+        self.DIMS = definition(self.dims)
+        self.t = self.dtype.type(0.0)
+        self.dt = self.dtype.type(p.get("dt", 0.0))
+        self.iter = 0
+        self.cell_length = np.float32(self.support * self.h)
+        L = lib()
+        self._k = lambda name: getattr(L, "orc_%s_%s" % (name, self.sfx))
+
+    # ---- helpers -----------------------------------------------------
+    def _R(self, x):
+        return self.dtype.type(x)
+
+    def _r(self, x):
+        """ctypes scalar of the oracle's real type."""
+        return (ctypes.c_float if self.dtype == np.float32 else
+                ctypes.c_double)(float(x))
+
+    def _ll(self):
+        f = self.f
+        return (_p(f["icell"]), _p(self.ihoc), _p(self.n_cells))
+
+    # ---- link-list (LinkList.cpp:326-494) ----------------------------
+    def link_list(self, r_name="r_in"):
+        f, L, N = self.f, lib(), self.N
+        self._k("minmax_vec")(_p(f[r_name]), ctypes.c_uint(N),
+                              _p(self.r_min), _p(self.r_max))
+        self._k("n_cells")(_p(self.r_min), _p(self.r_max),
+                           self._r(self.cell_length), _p(self.n_cells))
+        nw = int(self.n_cells[3])
+        if self.ihoc.size < nw:
+            self.ihoc = np.zeros(nw, np.uint32)
+        self._k("icell")(_p(f["icell"]), _p(f[r_name]), ctypes.c_uint(N),
+                         _p(self.r_min), self._r(self.support),
+                         self._r(self.h), _p(self.n_cells))
+        bits = L.orc_radix_key_bits(ctypes.c_ulonglong(nw))
+        L.orc_stable_sort(_p(f["icell"]), _p(f["id_unsorted"]),
+                          _p(f["id_sorted"]), ctypes.c_uint(N),
+                          ctypes.c_uint(bits))
+        L.orc_ihoc(_p(f["icell"]), _p(self.ihoc), ctypes.c_uint(N),
+                   _p(self.n_cells))
+
+    def reorder(self):
+        """Backup copies + sort stage1/stage2 (basic.xml:141-148)."""
+        f = self.f
+        for k in ("id", "iset", "imove", "normal", "tangent", "m"):
+            f[k + "_in"][...] = f[k]
+        ids = f["id_sorted"]
+
+        def sc(src, dst):
+            dst[ids] = src
+        sc(f["id_in"], f["id"])
+        sc(f["iset_in"], f["iset"])
+        sc(f["imove_in"], f["imove"])
+        sc(f["r_in"], f["r"])
+        sc(f["normal_in"], f["normal"])
+        sc(f["tangent_in"], f["tangent"])
+        sc(f["rho_in"], f["rho"])
+        sc(f["m_in"], f["m"])
+        sc(f["u_in"], f["u"])
+        sc(f["dudt"], f["dudt_in"])
+        sc(f["drhodt"], f["drhodt_in"])
+
+    # ---- one time step ----------------------------------------------
+    def step(self):
+        f, N, o = self.f, self.N, self.opts
+        k, R, ll = self._k, self._r, None
+        n = ctypes.c_uint(N)
+        k("predictor")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
+                       _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
+                       _p(f["u_in"]), _p(f["dudt_in"]), _p(f["rho_in"]),
+                       _p(f["drhodt_in"]), n, R(self.dt))
+        if o["domain"]:
+            k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
+                        _p(f["dudt_in"]), _p(f["m"]), n,
+                        _p(self.domain_min), _p(self.domain_max))
+        self.link_list("r_in")
+        self.reorder()
+        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
+                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
+        for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
+            f[key][...] = 0
+        ll = self._ll()
+        H, CONW, CONF = R(self.H), R(self.CONW), R(self.CONF)
+        if o["bi"]:
+            k("bi_shepard_compute")(
+                _p(f["imove"]), _p(f["r"]), _p(f["normal"]), _p(f["tangent"]),
+                _p(f["binormal"]), _p(f["m"]), _p(f["shepard"]), n, *ll,
+                H, CONW, R(self.DIMS))
+        else:
+            k("shepard")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                         _p(f["shepard"]), n, *ll, H, CONW, ctypes.c_int(1))
+        k("interactions")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                          _p(f["rho"]), _p(f["m"]), _p(f["p"]),
+                          _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]), n,
+                          *ll, H, CONF, ctypes.c_int(o["lap_form"]))
+        k("sensors")(_p(f["imove"]), _p(f["r"]), _p(f["m"]), _p(f["u"]),
+                     _p(f["rho"]), _p(f["p"]), n, *ll, H, CONW)
+        k("sensors_renorm")(_p(f["imove"]), _p(f["shepard"]), _p(f["u"]),
+                            _p(f["rho"]), _p(f["p"]), n)
+        if o["delta"]:
+            k("lapp")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                      _p(f["p"]), _p(f["lap_p"]), n, *ll, H, CONF)
+        if o["bi"]:
+            k("bi_lapu")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["rho"]),
+                         _p(f["m"]), _p(f["lap_u"]), n, *ll, H, CONF,
+                         ctypes.c_int(o["lap_form"]))
+            k("bi_gradp")(_p(f["iset"]), _p(f["imove"]), _p(f["shepard"]),
+                          _p(f["rho"]), _p(f["lap_u"]), _p(f["dudt"]),
+                          _p(f["grad_p"]), _p(self.sets["visc_dyn"]),
+                          _p(self.sets["refd"]), n, _p(self.g))
+            k("bi_interp")(_p(f["imove"]), _p(f["r"]), _p(f["m"]),
+                           _p(f["rho"]), _p(f["grad_p"]), _p(f["p"]), n, *ll,
+                           H, CONW)
+            k("bi_interp_shepard")(_p(f["iset"]), _p(f["imove"]),
+                                   _p(f["shepard"]), _p(f["rho"]), _p(f["p"]),
+                                   _p(self.sets["refd"]), n, R(self.cs),
+                                   R(self.p0))
+            k("bi_interactions")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
+                                 _p(f["u"]), _p(f["rho"]), _p(f["m"]),
+                                 _p(f["p"]), _p(f["grad_p"]), _p(f["div_u"]),
+                                 n, *ll, H, CONW)
+            k("bi_shepard_apply")(_p(f["imove"]), _p(f["shepard"]),
+                                  _p(f["grad_p"]), _p(f["lap_u"]),
+                                  _p(f["div_u"]), n)
+        k("rates")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
+                   _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]),
+                   _p(f["dudt"]), _p(f["drhodt"]), _p(self.sets["visc_dyn"]),
+                   n, _p(self.g))
+        if o["delta"]:
+            k("delta_rate")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
+                            _p(f["lap_p"]), _p(f["drhodt"]),
+                            _p(self.sets["refd"]), _p(self.sets["delta"]), n,
+                            R(self.dt))
+        k("corrector")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
+                       _p(f["rho"]), _p(f["drhodt"]), _p(f["dudt_in"]),
+                       _p(f["drhodt_in"]), n, R(self.dt))
+        # "fixed time step" set_scalar (basic.xml:164), double -> float
+        self.dt = self.dtype.type(float(self.courant) * float(self.h) /
+                                  float(self.cs))
+        if o["variable_dt"]:
+            k("timestep")(_p(f["imove"]), _p(f["u"]), _p(f["dt_var"]), n,
+                          R(self.dt), R(self.dt_min), R(self.courant),
+                          R(self.dt_Ma), R(self.h))
+            self.dt = self.dtype.type(np.min(f["dt_var"]))
+            assert self.dt > 0.0, "dt > 0.0 assertion failed"
+        self.t = self.dtype.type(float(self.t) + float(self.dt))
+        self.iter += 1
+
+    def run(self, nsteps):
+        for _ in range(nsteps):
+            self.step()
