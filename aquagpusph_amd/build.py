@@ -30,7 +30,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
 
 
 def sources():
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "**", "*.hip"),
+                            recursive=True))
     srcs += sorted(glob.glob(os.path.join(CSRC, "**", "*.cpp"),
                              recursive=True))
     return srcs

@@ -1,0 +1,122 @@
+"""Write a case (from aquagpusph_amd.cases) as reference-format XML files.
+
+The layout is the one of the reference examples (examples/*/templates):
+Main.xml includes the presets, declares the case variables and one
+ParticlesSet per particle group, loading a FastASCII particle file.  Values
+are written with 9 significant digits so that float32 data round-trips
+exactly.
+"""
+import os
+
+import numpy as np
+
+PBI_PRESETS = ("resources/Presets/basic.xml",
+               "resources/Presets/basic/time_scheme/improved_euler.xml",
+               "resources/Presets/basic/domain.xml",
+               "resources/Presets/cfd.xml",
+               "resources/Presets/cfd/deltaSPH.xml",
+               "resources/Presets/cfd/BI.xml",
+               "resources/Presets/cfd/variableTimeStep.xml")
+
+FIELDS = ("r", "normal", "tangent", "u", "dudt", "rho", "drhodt", "m",
+          "imove")
+
+
+def _f(x):
+    return "%.9g" % float(np.float32(x))
+
+
+def _vec(v, dims):
+    vs = 2 if dims == 2 else 4
+    out = [0.0] * vs
+    for k in range(min(len(v), vs)):
+        out[k] = float(v[k])
+    return ", ".join(_f(x) for x in out)
+
+
+def write_particles(case, path):
+    dims = case["dims"]
+    vs = 2 if dims == 2 else 4
+    N = int(case["N"])
+    cols = []
+    for k in FIELDS:
+        v = case.get(k)
+        if k in ("r", "normal", "tangent", "u", "dudt"):
+            a = np.zeros((N, vs), np.float32) if v is None else \
+                np.asarray(v, np.float32).reshape(N, vs)
+            cols.append(a)
+        elif k == "imove":
+            cols.append(np.asarray(v, np.int64).reshape(N, 1))
+        else:
+            a = np.zeros(N, np.float32) if v is None else \
+                np.asarray(v, np.float32)
+            cols.append(a.reshape(N, 1))
+    fmt = []
+    for k, c in zip(FIELDS, cols):
+        fmt.append(" ".join(["%d" if k == "imove" else "%.9g"] * c.shape[1]))
+    data = np.concatenate([c.astype(np.float64) for c in cols], axis=1)
+    np.savetxt(path, data, fmt=", ".join(fmt),
+               header="aquagpusph_amd particles: " + ", ".join(FIELDS))
+
+
+def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml=""):
+    """Write Main.xml (+ Fluid.dat when load is True) and return its path.
+
+    With load=False the ParticlesSet has no Load element and the arrays are
+    uploaded afterwards through Simulation.set (large cases)."""
+    os.makedirs(folder, exist_ok=True)
+    dims = case["dims"]
+    p = case["params"]
+    lines = ['<?xml version="1.0" ?>', "<sphInput>"]
+    for pr in presets:
+        lines.append('    <Include file="%s" />' % pr)
+    lines.append("    <Variables>")
+    var = [("g", "vec", _vec(p["g"], dims)),
+           ("h", "float", _f(p["h"])),
+           ("cs", "float", _f(p["cs"])),
+           ("courant", "float", _f(p["courant"])),
+           ("dt_min", "float", _f(p.get("dt_min", 0.0))),
+           ("dt_Ma", "float", _f(p.get("dt_Ma", 0.1))),
+           ("p0", "float", _f(p.get("p0", 0.0)))]
+    if "hfac" in p:
+        var.append(("hfac", "float", _f(p["hfac"])))
+    if "domain_min" in p:
+        var.append(("domain_min", "vec", _vec(p["domain_min"], dims)))
+        var.append(("domain_max", "vec", _vec(p["domain_max"], dims)))
+    for name, t, v in var:
+        lines.append('        <Variable name="%s" type="%s" value="%s" />'
+                     % (name, t, v))
+    lines.append("    </Variables>")
+    lines.append('    <ParticlesSet n="%d">' % int(case["N"]))
+    for k in ("refd", "visc_dyn", "delta"):
+        lines.append('        <Scalar name="%s" value="%s" />'
+                     % (k, _f(p[k][0])))
+    if load:
+        write_particles(case, os.path.join(folder, "Fluid.dat"))
+        lines.append('        <Load format="FastASCII" file="Fluid.dat" '
+                     'fields="%s" />' % ", ".join(FIELDS))
+    lines.append("    </ParticlesSet>")
+    if extra_xml:
+        lines.append(extra_xml)
+    lines.append("</sphInput>")
+    path = os.path.join(folder, "Main.xml")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def upload(sim, case):
+    """Set the particle arrays of a case on a Simulation built with
+    write_case(load=False)."""
+    dims = case["dims"]
+    vs = 2 if dims == 2 else 4
+    N = int(case["N"])
+    for k in ("r", "normal", "tangent", "u", "dudt"):
+        v = case.get(k)
+        if v is not None:
+            sim.set(k, np.asarray(v, np.float32).reshape(N, vs))
+    for k in ("rho", "drhodt", "m"):
+        v = case.get(k)
+        if v is not None:
+            sim.set(k, np.asarray(v, np.float32))
+    sim.set("imove", np.asarray(case["imove"], np.int32))

@@ -40,10 +40,11 @@ class Error : public std::runtime_error {
     } while (0)
 
 // Wrap a C ABI body: translate exceptions into error codes.
-template <class F> inline int guard(F f)
+template <class F> inline int guard(F f, bool check_launch = true)
 {
     try {
         f();
+        if (!check_launch) return AQ_OK;
         hipError_t e = hipGetLastError();
         if (e != hipSuccess)
             return set_error(AQ_EHIP, std::string("kernel launch failed: ") +

@@ -1,0 +1,324 @@
+// Kernel-tool registry: (script path under Scripts/, entry point) -> HIP
+// launcher with the reference kernel's arguments bound by variable name
+// (aquagpusph/CalcServer/Kernel.cpp:464-556 binds OpenCL kernel arguments by
+// name the same way).  A preset keeps its `type="kernel" path=... entry_point=`
+// lines; only the implementation behind the key changes.
+#include <algorithm>
+#include <filesystem>
+#include <map>
+
+#include "sim.hpp"
+
+namespace aq {
+
+void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
+             uint32_t n, hipStream_t s);
+void launch_predictor(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, const float*, float*, float*,
+                      float*, float*, float*, uint32_t, const float*,
+                      hipStream_t);
+void launch_corrector(int, const int32_t*, float*, float*, const float*,
+                      float*, const float*, const float*, const float*,
+                      uint32_t, const float*, hipStream_t);
+void launch_domain(int, int32_t*, float*, float*, float*, float*, uint32_t,
+                   const float*, const float*, hipStream_t);
+void launch_eos(const uint32_t*, const int32_t*, const float*, float*,
+                const float*, uint32_t, const float*, const float*,
+                hipStream_t);
+void launch_binormal(int, const float*, float*, float*, uint32_t, hipStream_t);
+void launch_rates(int, const uint32_t*, const int32_t*, const float*,
+                  const float*, const float*, float*, float*, const float*,
+                  uint32_t, const float*, hipStream_t);
+void launch_delta_rate(const uint32_t*, const int32_t*, const float*,
+                       const float*, float*, const float*, const float*,
+                       uint32_t, const float*, hipStream_t);
+void launch_timestep(int, const int32_t*, const float*, float*, uint32_t,
+                     const float*, const float*, const float*, const float*,
+                     const float*, hipStream_t);
+void launch_sensors_renorm(int, const int32_t*, const float*, float*, float*,
+                           float*, uint32_t, hipStream_t);
+void launch_bi_shepard_apply(int, const int32_t*, const float*, float*,
+                             float*, float*, uint32_t, hipStream_t);
+void launch_bi_gradp(int, const uint32_t*, const int32_t*, const float*,
+                     const float*, const float*, float*, const float*,
+                     const float*, uint32_t, const float*, hipStream_t);
+void launch_bi_interp_shepard(const uint32_t*, const int32_t*, const float*,
+                              float*, float*, const float*, uint32_t,
+                              const float*, const float*, hipStream_t);
+void launch_shepard(int, const int32_t*, const float*, const float*,
+                    const float*, float*, uint32_t, const uint32_t*,
+                    const uint32_t*, const uint32_t*, const uint32_t*,
+                    aq_defs, int, hipStream_t);
+void launch_interactions(int, const int32_t*, const float*, const float*,
+                         const float*, const float*, const float*, float*,
+                         float*, float*, uint32_t, const uint32_t*,
+                         const uint32_t*, const uint32_t*, const uint32_t*,
+                         aq_defs, hipStream_t);
+void launch_lapp(int, const int32_t*, const float*, const float*,
+                 const float*, const float*, float*, uint32_t,
+                 const uint32_t*, const uint32_t*, const uint32_t*,
+                 const uint32_t*, aq_defs, hipStream_t);
+void launch_sensors(int, const int32_t*, const float*, const float*, float*,
+                    float*, float*, uint32_t, const uint32_t*,
+                    const uint32_t*, const uint32_t*, const uint32_t*,
+                    aq_defs, hipStream_t);
+void launch_bi_shepard_compute(int, const int32_t*, const float*,
+                               const float*, const float*, const float*,
+                               const float*, float*, uint32_t,
+                               const uint32_t*, const uint32_t*,
+                               const uint32_t*, const uint32_t*, aq_defs,
+                               hipStream_t);
+void launch_bi_lapu(int, const int32_t*, const float*, const float*,
+                    const float*, const float*, float*, uint32_t,
+                    const uint32_t*, const uint32_t*, const uint32_t*,
+                    const uint32_t*, aq_defs, hipStream_t);
+void launch_bi_interp(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, float*, uint32_t,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      const uint32_t*, aq_defs, hipStream_t);
+void launch_bi_interactions(int, const int32_t*, const float*, const float*,
+                            const float*, const float*, const float*,
+                            const float*, float*, float*, uint32_t,
+                            const uint32_t*, const uint32_t*,
+                            const uint32_t*, const uint32_t*, aq_defs,
+                            hipStream_t);
+void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
+                   const uint32_t*, const uint32_t*, const uint32_t*,
+                   const uint32_t*, hipStream_t);
+void launch_euler_predictor(int, const float*, const float*, const float*,
+                            const float*, const float*, float*, float*,
+                            float*, float*, float*, uint32_t, hipStream_t);
+void launch_euler_corrector(int, const int32_t*, float*, float*,
+                            const float*, float*, const float*, uint32_t,
+                            const float*, hipStream_t);
+
+namespace rt {
+
+#define F(k) ((float*)S.ptr(a[k]))
+#define I(k) ((int32_t*)S.ptr(a[k]))
+#define U(k) ((uint32_t*)S.ptr(a[k]))
+
+static void gather(Sim& S, Var* src, Var* dst, const uint32_t* perm,
+                   hipStream_t s)
+{
+    permute(src->dev, dst->dev, src->type.elem_bytes(), perm, S.N, s);
+}
+
+static const std::map<std::string, KernelSpec>& table()
+{
+    static const std::map<std::string, KernelSpec> t = {
+        // ---- time integration ------------------------------------------
+        { "basic/time_scheme/improved_euler.cl:predictor",
+          { { "imove", "r", "u", "dudt", "rho", "drhodt", "r_in", "u_in",
+              "dudt_in", "rho_in", "drhodt_in", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_predictor(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 F(6), F(7), F(8), F(9), F(10), S.N, F(11), s);
+            } } },
+        { "basic/time_scheme/improved_euler.cl:corrector",
+          { { "imove", "iset", "r", "u", "dudt", "rho", "drhodt", "dudt_in",
+              "drhodt_in", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_corrector(S.dims, I(0), F(2), F(3), F(4), F(5), F(6),
+                                 F(7), F(8), S.N, F(9), s);
+            } } },
+        { "basic/time_scheme/euler.cl:predictor",
+          { { "r", "u", "dudt", "rho", "drhodt", "r_in", "u_in", "dudt_in",
+              "rho_in", "drhodt_in" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_euler_predictor(S.dims, F(0), F(1), F(2), F(3), F(4),
+                                       F(5), F(6), F(7), F(8), F(9), S.N, s);
+            } } },
+        { "basic/time_scheme/euler.cl:corrector",
+          { { "imove", "iset", "r", "u", "dudt", "rho", "drhodt", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_euler_corrector(S.dims, I(0), F(2), F(3), F(4), F(5),
+                                       F(6), S.N, F(7), s);
+            } } },
+        { "basic/Domain.cl:entry",
+          { { "imove", "r_in", "u_in", "dudt_in", "m", "domain_min",
+              "domain_max" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_domain(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, F(5),
+                              F(6), s);
+            } } },
+        // ---- reorder (Sort.cl stage1/stage2 as gathers by id_unsorted) --
+        { "basic/Sort.cl:stage1",
+          { { "id_in", "id", "iset_in", "iset", "imove_in", "imove", "r_in",
+              "r", "normal_in", "normal", "tangent_in", "tangent",
+              "id_unsorted" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                const uint32_t* perm = U(12);
+                for (int k = 0; k < 12; k += 2)
+                    gather(S, a[k], a[k + 1], perm, s);
+            } } },
+        { "basic/Sort.cl:stage2",
+          { { "rho_in", "rho", "m_in", "m", "u_in", "u", "dudt", "dudt_in",
+              "drhodt", "drhodt_in", "id_unsorted" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                const uint32_t* perm = U(10);
+                for (int k = 0; k < 10; k += 2)
+                    gather(S, a[k], a[k + 1], perm, s);
+            } } },
+        // ---- basic ----------------------------------------------------------
+        { "basic/EOS.cl:entry",
+          { { "iset", "imove", "rho", "p", "refd", "cs", "p0" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_eos(U(0), I(1), F(2), F(3), F(4), S.N, F(5), F(6), s);
+            } } },
+        { "basic/Binormal.cl:entry",
+          { { "normal", "tangent", "binormal" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_binormal(S.dims, F(0), F(1), F(2), S.N, s);
+            } } },
+        { "basic/Shepard.cl:entry",
+          { { "imove", "r", "rho", "m", "shepard", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_shepard(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, U(5),
+                               U(6), U(7), S.d_status, S.kdefs, 0, s);
+            } } },
+        { "basic/neighs.cl:entry",
+          { { "imove", "n_neighs", "neighs_limit", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_neighs(S.dims, I(0), U(1), U(2), S.N, U(3), U(4), U(5),
+                              S.d_status, s);
+            } } },
+        // ---- cfd --------------------------------------------------------------
+        { "cfd/Shepard.cl:entry",
+          { { "imove", "r", "rho", "m", "shepard", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_shepard(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, U(5),
+                               U(6), U(7), S.d_status, S.kdefs, 1, s);
+            } } },
+        { "cfd/Interactions.cl:entry",
+          { { "imove", "r", "u", "rho", "m", "p", "grad_p", "lap_u", "div_u",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                    F(5), F(6), F(7), F(8), S.N, U(9), U(10),
+                                    U(11), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Sensors.cl:entry",
+          { { "imove", "r", "m", "u", "rho", "p", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_sensors(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                               S.N, U(6), U(7), U(8), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/SensorsRenormalization.cl:entry",
+          { { "imove", "shepard", "u", "rho", "p" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_sensors_renorm(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                      S.N, s);
+            } } },
+        { "cfd/deltaSPH.cl:lapp",
+          { { "imove", "r", "rho", "m", "p", "lap_p", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp(S.dims, I(0), F(1), F(2), F(3), F(4), F(5), S.N,
+                            U(6), U(7), U(8), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/deltaSPH.cl:deltaSPH",
+          { { "iset", "imove", "rho", "lap_p", "drhodt", "refd", "delta",
+              "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_delta_rate(U(0), I(1), F(2), F(3), F(4), F(5), F(6),
+                                  S.N, F(7), s);
+            } } },
+        { "cfd/Rates.cl:entry",
+          { { "iset", "imove", "grad_p", "lap_u", "div_u", "dudt", "drhodt",
+              "visc_dyn", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_rates(S.dims, U(0), I(1), F(2), F(3), F(4), F(5), F(6),
+                             F(7), S.N, F(8), s);
+            } } },
+        { "cfd/TimeStep.cl:entry",
+          { { "imove", "u", "dt_var", "dt", "dt_min", "courant", "dt_Ma",
+              "h" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_timestep(S.dims, I(0), F(1), F(2), S.N, F(3), F(4),
+                                F(5), F(6), F(7), s);
+            } } },
+        // ---- boundary integrals --------------------------------------------
+        { "cfd/Boundary/BI/Shepard.cl:compute",
+          { { "imove", "r", "normal", "tangent", "binormal", "m", "shepard",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_shepard_compute(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                          F(5), F(6), S.N, U(7), U(8), U(9),
+                                          S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Boundary/BI/Shepard.cl:apply",
+          { { "imove", "shepard", "grad_p", "lap_u", "div_u" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_shepard_apply(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                        S.N, s);
+            } } },
+        { "cfd/Boundary/BI/LapU.cl:freeslip",
+          { { "imove", "r", "u", "rho", "m", "lap_u", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_lapu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5), S.N,
+                               U(6), U(7), U(8), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Boundary/BI/GradP.cl:freeslip",
+          { { "iset", "imove", "shepard", "lap_u", "dudt", "grad_p",
+              "visc_dyn", "refd", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_gradp(S.dims, U(0), I(1), F(2), F(3), F(4), F(5),
+                                F(6), F(7), S.N, F(8), s);
+            } } },
+        { "cfd/Boundary/BI/Interpolation.cl:entry",
+          { { "imove", "r", "m", "rho", "grad_p", "p", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_interp(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                 s);
+            } } },
+        { "cfd/Boundary/BI/InterpolationShepard.cl:entry",
+          { { "iset", "imove", "shepard", "rho", "p", "refd", "cs", "p0" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_interp_shepard(U(0), I(1), F(2), F(3), F(4), F(5),
+                                         S.N, F(6), F(7), s);
+            } } },
+        { "cfd/Boundary/BI/Interactions.cl:entry",
+          { { "imove", "r", "normal", "u", "rho", "m", "p", "grad_p", "div_u",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                       F(5), F(6), F(7), F(8), S.N, U(9),
+                                       U(10), U(11), S.d_status, S.kdefs, s);
+            } } },
+    };
+    return t;
+}
+
+const KernelSpec* find_kernel(const std::string& key)
+{
+    auto it = table().find(key);
+    return it == table().end() ? nullptr : &it->second;
+}
+
+std::vector<std::string> registered_kernels()
+{
+    std::vector<std::string> v;
+    for (auto& kv : table()) v.push_back(kv.first);
+    return v;
+}
+
+std::string kernel_key(const std::string& path, const std::string& entry)
+{
+    std::string p =
+        std::filesystem::path(path).lexically_normal().generic_string();
+    const std::string tag = "Scripts/";
+    const size_t k = p.rfind(tag);
+    if (k != std::string::npos) p = p.substr(k + tag.size());
+    return p + ":" + (entry.empty() ? "entry" : entry);
+}
+
+}  // namespace rt
+}  // namespace aq

@@ -1,0 +1,331 @@
+// The runtime: variables + definitions + tool pipeline (CalcServer.cpp).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <dlfcn.h>
+#include <filesystem>
+#include <limits>
+
+#include "sim.hpp"
+
+namespace fs = std::filesystem;
+
+namespace aq {
+namespace rt {
+
+static std::string trim(const std::string& s)
+{
+    size_t b = s.find_first_not_of(" \t\r\n");
+    size_t e = s.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? "" : s.substr(b, e - b + 1);
+}
+
+// folder holding resources/Presets (the package folder of this library)
+static std::string package_dir()
+{
+    Dl_info info;
+    if (dladdr((void*)&package_dir, &info) && info.dli_fname)
+        return fs::path(info.dli_fname).parent_path().string();
+    return fs::current_path().string();
+}
+
+Sim::Sim(int d) : dims(d), vars(d)
+{
+    if (d != 2 && d != 3) throw Error(AQ_EINVAL, "dims must be 2 or 3");
+    AQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    AQ_HIP(hipMalloc(&d_status, sizeof(uint32_t)));
+    AQ_HIP(hipMalloc(&d_fail_id, sizeof(int32_t)));
+    AQ_HIP(hipMemset(d_status, 0, sizeof(uint32_t)));
+    AQ_HIP(hipMemset(d_fail_id, 0xff, sizeof(int32_t)));
+}
+
+Sim::~Sim()
+{
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    tools.clear();
+    if (d_status) (void)hipFree(d_status);
+    if (d_fail_id) (void)hipFree(d_fail_id);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void* Sim::ptr(Var* v)
+{
+    return v->is_array() ? v->dev : vars.scalar_dev(*v);
+}
+
+void* Sim::ptr(const std::string& name) { return ptr(&vars.at(name)); }
+
+int Sim::assert_id(const std::string& name)
+{
+    assert_names.push_back(name);
+    return (int)assert_names.size() - 1;
+}
+
+std::string Sim::def(const std::string& n) const
+{
+    auto it = def_values.find(n);
+    if (it == def_values.end())
+        throw Error(AQ_ENOTFOUND, "undefined preset definition '" + n + "'");
+    return it->second;
+}
+
+float Sim::def_float(const std::string& n) const
+{
+    std::string v = def(n);
+    if (!v.empty() && (v.back() == 'f' || v.back() == 'F')) v.pop_back();
+    return strtof(v.c_str(), nullptr);
+}
+
+void Sim::load(const std::string& xml_path)
+{
+    Problem P;
+    P.dims = dims;
+    P.base_path = package_dir();
+    parse_problem(xml_path, P, "");
+    build(P);
+}
+
+static unsigned long next_pow2(unsigned long n)
+{
+    unsigned long p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+void Sim::build(Problem& P)
+{
+    problem = P;
+    size_t n = 0;
+    for (auto& s : P.sets) n += s.n;
+    if (n == 0) throw Error(AQ_EINVAL, "the case defines no particles");
+    if (n > 0xFFFFFFF0ull) throw Error(AQ_EINVAL, "too many particles");
+    N = (uint32_t)n;
+    // built-in variables, CalcServer.cpp:178-230
+    const std::string sN = std::to_string(N);
+    vars.reg("mpi_rank", "unsigned int", "", "0");
+    vars.reg("mpi_size", "unsigned int", "", "1");
+    vars.reg("dims", "unsigned int", "", std::to_string(dims));
+    vars.reg("t", "float", "", "0");
+    vars.reg("dt", "float", "", "0");
+    vars.reg("iter", "unsigned int", "", "0");
+    vars.reg("frame", "unsigned int", "", "0");
+    {
+        char b[64];
+        snprintf(b, sizeof(b), "%.9g", (double)(FLT_MAX / 2.f));
+        vars.reg("end_t", "float", "", b);
+    }
+    vars.reg("end_iter", "unsigned int", "",
+             std::to_string(std::numeric_limits<int32_t>::max()));
+    vars.reg("end_frame", "unsigned int", "",
+             std::to_string(std::numeric_limits<int32_t>::max()));
+    vars.reg("N", "size_t", "", sN);
+    vars.reg("n_sets", "unsigned int", "", std::to_string(P.sets.size()));
+    vars.reg("n_radix", "size_t", "",
+             std::to_string(next_pow2((N + 1023) / 1024 * 1024)));
+    vars.reg("n_cells", "svec4", "", "1, 1, 1, 1");
+    vars.reg("support", "float", "", "2");
+    vars.reg("id", "size_t*", sN, "");
+    vars.reg("r", "vec*", sN, "");
+    vars.reg("iset", "unsigned int*", sN, "");
+    vars.reg("id_sorted", "size_t*", sN, "");
+    vars.reg("id_unsorted", "size_t*", sN, "");
+    vars.reg("icell", "size_t*", sN, "");
+    vars.reg("ihoc", "size_t*", "n_cells_w", "");
+    // user variables, in registration order
+    for (auto& v : P.variables)
+        vars.reg(std::get<0>(v), std::get<1>(v), std::get<2>(v),
+                 std::get<3>(v));
+    // end criteria
+    if (P.timing.end_t >= 0) vars.set_host(vars.at("end_t"), { P.timing.end_t });
+    if (P.timing.end_iter >= 0)
+        vars.set_host(vars.at("end_iter"), { (double)P.timing.end_iter });
+    if (P.timing.end_frame >= 0)
+        vars.set_host(vars.at("end_frame"), { (double)P.timing.end_frame });
+    definitions = P.definitions;
+    make_defs();
+    // tools: one instance per description (a description inserted at
+    // several places runs at each of them)
+    std::map<const ToolDesc*, std::shared_ptr<Tool>> made;
+    for (auto& d : P.tools) {
+        auto it = made.find(d.get());
+        if (it == made.end()) it = made.emplace(d.get(), make_tool(*this, *d)).first;
+        tools.push_back(it->second);
+    }
+    load_sets();
+}
+
+// CalcServer.cpp:241-260: evaluated definitions become "%#G" + "f"
+void Sim::make_defs()
+{
+    def_values.clear();
+    for (auto& d : definitions) {
+        std::string v = d.value;
+        if (d.evaluate && !d.value.empty()) {
+            const float f = (float)vars.eval(d.value);
+            char b[128];
+            snprintf(b, sizeof(b), "%#G", (double)f);
+            v = std::string(b) + "f";
+        }
+        def_values[d.name] = v;
+    }
+    auto num = [&](const std::string& n, float dflt) {
+        return defined(n) && !def(n).empty() ? def_float(n) : dflt;
+    };
+    // the Wendland kernel is the only kernel family built
+    if (defined("KERNEL_NAME") && trim(def("KERNEL_NAME")) != "Wendland")
+        throw Error(AQ_EUNSUP, "KERNEL_NAME=" + def("KERNEL_NAME") +
+                                   " is not supported (Wendland only)");
+    const float h = (float)vars.get_host(vars.at(vars.get("h") ? "h" : "support"))[0];
+    kdefs.H = num("H", h);
+    kdefs.CONW = num("CONW", 1.f / std::pow(kdefs.H, (float)dims));
+    kdefs.CONF = num("CONF", 1.f / std::pow(kdefs.H, (float)dims + 2.f));
+    kdefs.DIMS = num("DIMS", (float)dims);
+    kdefs.lap_formulation = AQ_LAP_MONAGHAN;
+    if (defined("__LAP_FORMULATION__")) {
+        std::string v = trim(def("__LAP_FORMULATION__"));
+        for (int k = 0; k < 4 && defined(v); k++) v = trim(def(v));
+        kdefs.lap_formulation = atoi(v.c_str());
+        if (kdefs.lap_formulation != AQ_LAP_MONAGHAN &&
+            kdefs.lap_formulation != AQ_LAP_MORRIS)
+            throw Error(AQ_EUNSUP, "unknown __LAP_FORMULATION__ " + v);
+    }
+}
+
+void Sim::load_sets()
+{
+    // iset / id of every particle, per-set scalars, then the particle files
+    std::vector<uint32_t> iset(N), id(N);
+    size_t off = 0;
+    for (size_t k = 0; k < problem.sets.size(); k++) {
+        for (size_t i = 0; i < problem.sets[k].n; i++) {
+            iset[off + i] = (uint32_t)k;
+            id[off + i] = (uint32_t)(off + i);
+        }
+        off += problem.sets[k].n;
+    }
+    AQ_HIP(hipMemcpy(vars.at("iset").dev, iset.data(), N * 4,
+                     hipMemcpyHostToDevice));
+    AQ_HIP(hipMemcpy(vars.at("id").dev, id.data(), N * 4,
+                     hipMemcpyHostToDevice));
+    for (size_t k = 0; k < problem.sets.size(); k++)
+        for (auto& sc : problem.sets[k].scalars) {
+            Var& v = vars.at(sc.first);
+            if (!v.is_array() || v.length <= k)
+                throw Error(AQ_EINVAL, "set scalar '" + sc.first +
+                                           "' must be an array of length "
+                                           "n_sets");
+            std::vector<double> vals;
+            for (auto& c : split_components(sc.second))
+                vals.push_back(vars.eval(c));
+            std::vector<uint32_t> w(v.type.stored, 0);
+            for (int c = 0; c < v.type.ncomp && c < (int)vals.size(); c++) {
+                if (v.type.base == 'f') {
+                    float f = (float)vals[c];
+                    memcpy(&w[c], &f, 4);
+                } else if (v.type.base == 'i') {
+                    int32_t x = (int32_t)vals[c];
+                    memcpy(&w[c], &x, 4);
+                } else
+                    w[c] = (uint32_t)(int64_t)vals[c];
+            }
+            AQ_HIP(hipMemcpy((char*)v.dev + k * v.type.elem_bytes(), w.data(),
+                             v.type.elem_bytes(), hipMemcpyHostToDevice));
+        }
+    off = 0;
+    for (auto& s : problem.sets) {
+        for (auto& l : s.loads) {
+            std::string f = l.format;
+            for (auto& c : f) c = (char)tolower((unsigned char)c);
+            if (f != "fastascii" && f != "ascii")
+                throw Error(AQ_EUNSUP, "particles format '" + l.format +
+                                           "' is not supported");
+            load_fastascii(*this, l.file, off, s.n, l.fields);
+        }
+        off += s.n;
+    }
+}
+
+void Sim::setup()
+{
+    for (auto& t : tools) t->setup(*this);
+    vars.push(stream);
+    AQ_HIP(hipStreamSynchronize(stream));
+    is_setup = true;
+}
+
+void Sim::run_tools(hipStream_t s)
+{
+    std::set<Tool*> ran;
+    for (auto& t : tools) {
+        if (t->once() && t->done) continue;
+        last_tool = t->name();
+        t->execute(*this, s);
+        ran.insert(t.get());
+    }
+    for (auto* t : ran) t->done = true;
+}
+
+void Sim::sync() { AQ_HIP(hipStreamSynchronize(stream)); }
+
+void Sim::check_status()
+{
+    uint32_t st = 0;
+    int32_t id = -1;
+    AQ_HIP(hipMemcpyAsync(&st, d_status, 4, hipMemcpyDeviceToHost, stream));
+    AQ_HIP(hipMemcpyAsync(&id, d_fail_id, 4, hipMemcpyDeviceToHost, stream));
+    AQ_HIP(hipStreamSynchronize(stream));
+    if (st & 1u)
+        throw Error(AQ_ENOMEM,
+                    "link-list: the cell grid outgrew the ihoc capacity "
+                    "(particles left the domain box); results of this step "
+                    "are invalid");
+    if (st & 2u)
+        throw Error(AQ_EASSERT,
+                    "assertion failed in tool " +
+                        (id >= 0 && id < (int)assert_names.size()
+                             ? assert_names[id]
+                             : std::string("?")));
+}
+
+void Sim::step(int nsteps)
+{
+    if (!is_setup) throw Error(AQ_EINVAL, "setup() has not been called");
+    if (vars.host_dirty()) vars.push(stream);
+    for (int k = 0; k < nsteps; k++) run_tools(stream);
+    check_status();
+}
+
+void Sim::step_graph(int nsteps)
+{
+    if (!is_setup) throw Error(AQ_EINVAL, "setup() has not been called");
+    if (nsteps <= 0) return;
+    if (vars.host_dirty()) vars.push(stream);
+    if (!graph_ready) {
+        for (auto& t : tools)
+            if (!t->graph_safe())
+                throw Error(AQ_EUNSUP, "tool \"" + t->name() +
+                                           "\" cannot be captured in a "
+                                           "graph");
+        // the first step runs eagerly so that `once` tools are done
+        run_tools(stream);
+        nsteps--;
+        AQ_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_tools(stream);
+        } catch (...) {
+            hipGraph_t g;
+            (void)hipStreamEndCapture(stream, &g);
+            throw;
+        }
+        AQ_HIP(hipStreamEndCapture(stream, &graph));
+        AQ_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        graph_ready = true;
+    }
+    for (int k = 0; k < nsteps; k++)
+        AQ_HIP(hipGraphLaunch(graph_exec, stream));
+    check_status();
+}
+
+}  // namespace rt
+}  // namespace aq

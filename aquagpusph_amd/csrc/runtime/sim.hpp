@@ -1,0 +1,255 @@
+// Host runtime: the counterpart of the reference's CalcServer
+// (aquagpusph/CalcServer/CalcServer.{hpp,cpp}), its variable registry
+// (aquagpusph/Variable.{hpp,cpp}), tool plugin API (CalcServer/Tool.hpp)
+// and XML problem setup (InputOutput/State.cpp, ProblemSetup.hpp).
+//
+// MI355X-first differences from the reference (same XML surface):
+//   * every scalar variable lives in one device "scalar arena" with a host
+//     mirror; kernels read scalars through device pointers and set_scalar /
+//     assert / reduction / link-list tools write them on the device, so a
+//     step never waits for the host and can be replayed from a HIP graph;
+//   * array variables are device buffers in the reference layout (one
+//     buffer per field, vec = float2/float4);
+//   * kernel tools resolve (script path, entry point) to hand-written HIP
+//     kernels through a registry instead of compiling OpenCL at run time.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../aq_internal.hpp"
+#include "expr.hpp"
+#include "xml.hpp"
+
+namespace aq {
+namespace rt {
+
+// ---- types ------------------------------------------------------------------
+struct Type {
+    char base = 'f';    // 'f' float, 'i' int32, 'u' uint32
+    int ncomp = 1;      // logical components
+    int stored = 1;     // stored components (vec3 is padded to 4)
+    bool array = false;
+    std::string canon;  // canonical reference type name
+    size_t elem_bytes() const { return 4 * (size_t)stored; }
+};
+// Resolve a reference type string ("vec*", "size_t", "svec4", "float", ...)
+// with the aliasing of aquagpusph/Variable.cpp:1547-1625 (addr_bits = 32).
+Type parse_type(const std::string& t, int dims);
+
+struct Var {
+    std::string name;
+    Type type;
+    uint32_t slot = 0;  // scalars: byte offset in the scalar arena
+    std::string length_expr;
+    size_t length = 0;  // arrays: number of elements
+    void* dev = nullptr;
+    size_t bytes = 0;
+    bool is_array() const { return type.array; }
+};
+
+class Variables {
+  public:
+    static constexpr size_t ARENA_SLOTS = 4096;
+    static constexpr size_t SLOT_BYTES = 16;
+
+    explicit Variables(int dims);
+    ~Variables();
+    Variables(const Variables&) = delete;
+
+    // Variables::registerVariable: replaces an existing variable of the same
+    // name, evaluates the value/length with the variables registered so far.
+    Var* reg(const std::string& name, const std::string& type,
+             const std::string& length, const std::string& value);
+    Var* get(const std::string& name);
+    Var& at(const std::string& name);
+    const std::vector<std::unique_ptr<Var>>& all() const { return vars_; }
+
+    // expressions over the host mirror of the scalars ("g_x", "n_cells_w")
+    bool symbol(const std::string& name, double& v) const;
+    bool slot(const std::string& name, SlotRef& r) const;
+    double eval(const std::string& expr) const;
+
+    // host mirror access (component-wise, as double)
+    void set_host(Var& v, const std::vector<double>& vals);
+    std::vector<double> get_host(const Var& v) const;
+    void* scalar_dev(const Var& v) const { return dev_arena_ + v.slot; }
+    void* scalar_host(const Var& v) { return host_arena_.data() + v.slot; }
+
+    // whole-arena transfers (synchronous w.r.t. the stream)
+    void push(hipStream_t s);
+    void pull(hipStream_t s);
+    bool host_dirty() const { return dirty_; }
+
+    // (re)allocate an array variable's device buffer, zero-filled
+    void allocate(Var& v, size_t length);
+    uint8_t* dev_arena() const { return dev_arena_; }
+
+  private:
+    int dims_;
+    std::vector<std::unique_ptr<Var>> vars_;
+    std::map<std::string, Var*> by_name_;
+    std::vector<uint8_t> host_arena_;
+    uint8_t* dev_arena_ = nullptr;
+    uint32_t next_slot_ = 0;
+    std::vector<uint32_t> free_slots_;
+    bool dirty_ = true;
+};
+
+class Sim;
+
+// ---- tools (Tool.hpp plugin API) -------------------------------------------
+class Tool {
+  public:
+    Tool(const std::string& name, const std::string& type, bool once)
+        : name_(name), type_(type), once_(once)
+    {
+    }
+    virtual ~Tool() {}
+    virtual void setup(Sim& S) { (void)S; }
+    // enqueue the tool's work on the stream; must not synchronise
+    virtual void execute(Sim& S, hipStream_t s) = 0;
+    // false when execute() needs the host between launches (eager only)
+    virtual bool graph_safe() const { return true; }
+    const std::string& name() const { return name_; }
+    const std::string& type() const { return type_; }
+    bool once() const { return once_; }
+    bool done = false;
+
+  protected:
+    std::string name_, type_;
+    bool once_;
+};
+
+// Tool description collected by the XML parser (ProblemSetup::sphTool)
+struct ToolDesc {
+    std::map<std::string, std::string> attrs;  // name, type, path, ...
+    std::string text;                          // reduction operation body
+    std::string xml_dir;                       // folder of the declaring file
+    std::string get(const std::string& k, const std::string& d = "") const
+    {
+        auto it = attrs.find(k);
+        return it == attrs.end() ? d : it->second;
+    }
+};
+
+struct Definition {
+    std::string name, value;
+    bool evaluate = false;
+};
+
+struct LoadDesc {
+    std::string format, file;
+    std::vector<std::string> fields;
+};
+
+struct SetDesc {
+    size_t n = 0;
+    std::vector<std::pair<std::string, std::string>> scalars;
+    std::vector<LoadDesc> loads, saves;
+};
+
+struct TimingDesc {
+    double end_t = -1;       // Time
+    long end_iter = -1;      // Steps
+    long end_frame = -1;     // Frames
+    double output_fps = -1;  // FPS
+    long output_ipf = -1;    // IPF
+};
+
+// Everything parsed from the XML files before the runtime is built.
+struct Problem {
+    int dims = 3;
+    std::vector<std::tuple<std::string, std::string, std::string, std::string>>
+        variables;  // name, type, length, value
+    std::vector<Definition> definitions;
+    std::vector<std::shared_ptr<ToolDesc>> tools;
+    std::vector<SetDesc> sets;
+    TimingDesc timing;
+    std::string base_path;
+    std::vector<std::string> xml_stack;
+    bool is_defined(const std::string& n) const;
+};
+
+void parse_problem(const std::string& path, Problem& P,
+                   const std::string& prefix);
+
+// ---- kernel registry ----------------------------------------------------------
+struct KernelBinding;
+struct KernelSpec {
+    std::vector<std::string> args;  // variable names bound by name
+    // launch with the bound variables (same order as args)
+    void (*launch)(Sim& S, Var* const* a, hipStream_t s);
+};
+const KernelSpec* find_kernel(const std::string& key);
+std::vector<std::string> registered_kernels();
+// "…/Scripts/cfd/Interactions.cl" + "entry" -> "cfd/Interactions.cl:entry"
+std::string kernel_key(const std::string& path, const std::string& entry);
+
+// ---- the runtime ------------------------------------------------------------
+class Sim {
+  public:
+    Sim(int dims);
+    ~Sim();
+
+    void load(const std::string& xml_path);  // parse + build
+    void build(Problem& P);                  // from a parsed problem
+    void setup();                            // tool setup + initial upload
+    void step(int nsteps);                   // eager execution
+    void step_graph(int nsteps);             // HIP-graph replay
+    void check_status();                     // raise on device failures
+    void sync();
+
+    int dims;
+    Variables vars;
+    std::vector<std::shared_ptr<Tool>> tools;
+    std::vector<Definition> definitions;
+    std::map<std::string, std::string> def_values;  // formatted (-D values)
+    aq_defs kdefs{};
+    uint32_t N = 0;
+    hipStream_t stream = nullptr;
+    uint32_t* d_status = nullptr;   // bit0 ihoc overflow, bit1 assert failed
+    int32_t* d_fail_id = nullptr;
+    std::vector<std::string> assert_names;
+    Problem problem;
+    std::string last_tool;
+    bool graph_ready = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool is_setup = false;
+
+    bool defined(const std::string& n) const
+    {
+        return def_values.count(n) > 0;
+    }
+    std::string def(const std::string& n) const;
+    float def_float(const std::string& n) const;
+    // device pointer of a variable (array buffer or scalar slot)
+    void* ptr(const std::string& name);
+    void* ptr(Var* v);
+    int assert_id(const std::string& name);
+
+  private:
+    void run_tools(hipStream_t s);
+    void make_defs();
+    void load_sets();
+};
+
+std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d);
+
+// particle file I/O (FastASCII / ASCII rows, comma or blank separated)
+void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
+                    const std::vector<std::string>& fields);
+void save_ascii(Sim& S, const std::string& path, size_t offset, size_t n,
+                const std::vector<std::string>& fields);
+
+// scalar bytecode on the device
+void vm_launch(const VmOp* ops, uint8_t* arena, uint32_t* status,
+               int32_t* fail_id, hipStream_t s);
+
+}  // namespace rt
+}  // namespace aq

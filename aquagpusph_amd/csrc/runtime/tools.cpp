@@ -1,0 +1,619 @@
+// Built-in tool types (the reference's CalcServer/*.cpp tools).
+#include <cmath>
+#include <cstring>
+#include <dlfcn.h>
+#include <filesystem>
+#include <limits>
+
+#include "sim.hpp"
+
+namespace aq {
+
+void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
+                  float* r_max, const float* support, const float* h,
+                  uint32_t* n_cells, uint32_t* icell, uint32_t* id_unsorted,
+                  uint32_t* id_sorted, uint32_t* ihoc, uint32_t cap,
+                  uint32_t* status, void* ws, hipStream_t s);
+size_t linklist_workspace(uint32_t N);
+void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
+                unsigned key_bits, void* workspace, hipStream_t s);
+size_t radix_sort_workspace(uint32_t n);
+unsigned radix_key_bits(uint64_t max_val);
+size_t reduce_workspace(uint32_t n, int ncomp);
+void reduce(int op, int type, int nc, const void* in, uint32_t n, void* out,
+            const void* ident_dev, void* ws, hipStream_t s);
+
+namespace rt {
+
+void fill_launch(void* dst, const uint32_t* pattern_dev, int stored, size_t n,
+                 hipStream_t s);
+
+static std::string trim(const std::string& s)
+{
+    size_t b = s.find_first_not_of(" \t\r\n");
+    size_t e = s.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? "" : s.substr(b, e - b + 1);
+}
+
+static std::string squeeze(const std::string& s)
+{
+    std::string o;
+    for (char c : s)
+        if (!isspace((unsigned char)c)) o += c;
+    return o;
+}
+
+// device buffer owned by a tool (freed with it)
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t bytes)
+    {
+        if (bytes <= n) return;
+        if (p) AQ_HIP(hipFree(p));
+        AQ_HIP(hipMalloc(&p, bytes));
+        n = bytes;
+    }
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// "VEC_ZERO", "-VEC_INFINITY", "INFINITY", "0.f", constant expressions
+static std::vector<double> macro_value(const std::string& v0, const Type& t,
+                                       int dims, const Variables& vars)
+{
+    const std::string v = squeeze(v0);
+    const double inf = std::numeric_limits<double>::infinity();
+    const int n = t.ncomp;
+    std::vector<double> out(n, 0.0);
+    auto vec = [&](double x, bool all) {
+        for (int k = 0; k < n; k++) out[k] = (k < dims || all) ? x : 0.0;
+    };
+    if (v == "VEC_ZERO") vec(0.0, true);
+    else if (v == "VEC_ONE") vec(1.0, false);
+    else if (v == "VEC_ALL_ONE") vec(1.0, true);
+    else if (v == "VEC_INFINITY") vec(inf, false);
+    else if (v == "VEC_ALL_INFINITY") vec(inf, true);
+    else if (v == "VEC_NEG_INFINITY" || v == "-VEC_INFINITY") {
+        vec(-inf, false);
+        if (n == 4 && dims == 3) out[3] = -0.0;
+    } else if (v == "VEC_ALL_NEG_INFINITY" || v == "-VEC_ALL_INFINITY")
+        vec(-inf, true);
+    else if (v == "INFINITY") vec(inf, true);
+    else if (v == "-INFINITY") vec(-inf, true);
+    else {
+        // one value fills every component; a list fills them in order
+        auto comps = split_components(v0);
+        if (comps.size() == 1) {
+            const double x = vars.eval(comps[0]);
+            for (int k = 0; k < n; k++) out[k] = x;
+        } else {
+            for (int k = 0; k < n && k < (int)comps.size(); k++)
+                out[k] = vars.eval(comps[k]);
+        }
+    }
+    return out;
+}
+
+static std::vector<uint32_t> words(const std::vector<double>& v,
+                                   const Type& t)
+{
+    std::vector<uint32_t> w(t.stored, 0u);
+    for (int k = 0; k < t.ncomp && k < (int)v.size(); k++) {
+        if (t.base == 'f') {
+            float f = (float)v[k];
+            std::memcpy(&w[k], &f, 4);
+        } else if (t.base == 'i') {
+            int32_t x = (int32_t)v[k];
+            std::memcpy(&w[k], &x, 4);
+        } else {
+            w[k] = (uint32_t)(int64_t)v[k];
+        }
+    }
+    return w;
+}
+
+// ---- dummy / reports ------------------------------------------------------
+class DummyTool : public Tool {
+  public:
+    using Tool::Tool;
+    void execute(Sim&, hipStream_t) override {}
+};
+
+// ---- kernel: registry of HIP implementations --------------------------------
+class KernelTool : public Tool {
+  public:
+    KernelTool(const ToolDesc& d)
+        : Tool(d.get("name"), "kernel", d.get("once") == "true"),
+          path_(d.get("path")), entry_(d.get("entry_point", "entry"))
+    {
+    }
+    void setup(Sim& S) override
+    {
+        key_ = kernel_key(path_, entry_);
+        spec_ = find_kernel(key_);
+        if (!spec_)
+            throw Error(AQ_EUNSUP,
+                        "tool \"" + name_ + "\": no HIP implementation for "
+                        "script \"" + path_ + "\" entry point \"" + entry_ +
+                        "\" (registry key \"" + key_ + "\"); "
+                        "provide one through an installable tool");
+        bound_.clear();
+        for (auto& a : spec_->args) {
+            Var* v = S.vars.get(a);
+            if (!v)
+                throw Error(AQ_ENOTFOUND,
+                            "tool \"" + name_ + "\" (" + key_ +
+                                ") requires the undeclared variable \"" + a +
+                                "\"");
+            bound_.push_back(v);
+        }
+    }
+    void execute(Sim& S, hipStream_t s) override
+    {
+        spec_->launch(S, bound_.data(), s);
+    }
+
+  private:
+    std::string path_, entry_, key_;
+    const KernelSpec* spec_ = nullptr;
+    std::vector<Var*> bound_;
+};
+
+// ---- link-list: LinkList.cpp ------------------------------------------------
+class LinkListTool : public Tool {
+  public:
+    LinkListTool(const ToolDesc& d)
+        : Tool(d.get("name"), "link-list", d.get("once") == "true"), d_(d)
+    {
+        const std::string so = d.get("sorter", "radix-sort");
+        if (so != "radix-sort")
+            throw Error(AQ_EUNSUP, "link-list \"" + name_ + "\": sorter '" +
+                                       so + "' is not supported (only the "
+                                       "stable radix-sort)");
+        if (d.get("recompute_grid", "true") != "true")
+            throw Error(AQ_EUNSUP, "link-list \"" + name_ +
+                                       "\": recompute_grid=false is not "
+                                       "supported");
+    }
+    void setup(Sim& S) override
+    {
+        in_ = &S.vars.at(d_.get("in"));
+        mn_ = &S.vars.at(d_.get("min"));
+        mx_ = &S.vars.at(d_.get("max"));
+        ihoc_ = &S.vars.at(d_.get("ihoc"));
+        icell_ = &S.vars.at(d_.get("icell"));
+        nc_ = &S.vars.at(d_.get("n_cells"));
+        perm_ = &S.vars.at(d_.get("perm"));
+        inv_ = &S.vars.at(d_.get("inv_perm"));
+        sup_ = &S.vars.at("support");
+        h_ = &S.vars.at("h");
+        if (in_->type.base != 'f' || !in_->is_array())
+            throw Error(AQ_EINVAL, "link-list input must be a vec array");
+        ws_.alloc(linklist_workspace(S.N));
+    }
+    // ihoc capacity: the grid of the domain box when domain_min/domain_max
+    // exist (Domain.cl keeps every particle inside it), else twice the
+    // extent of the current positions in every direction
+    size_t capacity(Sim& S)
+    {
+        if (cap_) return cap_;
+        const float cl = (float)S.vars.get_host(*sup_)[0] *
+                         (float)S.vars.get_host(*h_)[0];
+        if (!(cl > 0.f))
+            throw Error(AQ_EINVAL, "zero cell length (support * h)");
+        std::vector<double> lo(3, 0.0), hi(3, 0.0);
+        Var* dmn = S.vars.get("domain_min");
+        Var* dmx = S.vars.get("domain_max");
+        bool dom = false;
+        if (dmn && dmx && !dmn->is_array() && !dmx->is_array()) {
+            auto a = S.vars.get_host(*dmn), b = S.vars.get_host(*dmx);
+            dom = true;
+            for (int k = 0; k < S.dims; k++) {
+                lo[k] = a[k];
+                hi[k] = b[k];
+                if (!(hi[k] > lo[k])) dom = false;
+            }
+        }
+        Var& r = S.vars.at("r");
+        if (!dom && r.length) {
+            std::vector<float> h(r.length * r.type.stored);
+            AQ_HIP(hipMemcpy(h.data(), r.dev, r.bytes,
+                             hipMemcpyDeviceToHost));
+            for (int k = 0; k < S.dims; k++) {
+                lo[k] = INFINITY;
+                hi[k] = -INFINITY;
+            }
+            for (size_t i = 0; i < r.length; i++)
+                for (int k = 0; k < S.dims; k++) {
+                    const double x = h[i * r.type.stored + k];
+                    if (std::isfinite(x)) {
+                        lo[k] = std::min(lo[k], x);
+                        hi[k] = std::max(hi[k], x);
+                    }
+                }
+            for (int k = 0; k < S.dims; k++) {
+                const double e = std::max(hi[k] - lo[k], (double)cl);
+                lo[k] -= 0.5 * e;
+                hi[k] += 0.5 * e;
+            }
+        }
+        double cells = 1.0;
+        for (int k = 0; k < S.dims; k++)
+            cells *= std::floor((hi[k] - lo[k]) / cl) + 8.0;
+        if (cells > 4.0e9)
+            throw Error(AQ_ENOMEM, "link-list grid too large");
+        cap_ = (size_t)cells;
+        return cap_;
+    }
+    void execute(Sim& S, hipStream_t s) override
+    {
+        // the capacity is sized on the first (always eager) execution, from
+        // the particles as loaded or set by the caller
+        if (!cap_) {
+            AQ_HIP(hipStreamSynchronize(s));
+            const size_t c = capacity(S);
+            if (ihoc_->length < c) S.vars.allocate(*ihoc_, c);
+        }
+        linklist_all(S.dims, (const float*)in_->dev, S.N,
+                     (float*)S.ptr(mn_), (float*)S.ptr(mx_),
+                     (const float*)S.ptr(sup_), (const float*)S.ptr(h_),
+                     (uint32_t*)S.ptr(nc_), (uint32_t*)icell_->dev,
+                     (uint32_t*)perm_->dev, (uint32_t*)inv_->dev,
+                     (uint32_t*)ihoc_->dev, (uint32_t)ihoc_->length,
+                     S.d_status, ws_.p, s);
+    }
+
+  private:
+    ToolDesc d_;
+    Var *in_, *mn_, *mx_, *ihoc_, *icell_, *nc_, *perm_, *inv_, *sup_, *h_;
+    DevBuf ws_;
+    size_t cap_ = 0;
+};
+
+// ---- radix-sort: RadixSort.cpp ----------------------------------------------
+class RadixSortTool : public Tool {
+  public:
+    RadixSortTool(const ToolDesc& d)
+        : Tool(d.get("name"), "radix-sort", d.get("once") == "true"), d_(d)
+    {
+    }
+    void setup(Sim& S) override
+    {
+        in_ = &S.vars.at(d_.get("in"));
+        perm_ = &S.vars.at(d_.get("perm"));
+        inv_ = &S.vars.at(d_.get("inv_perm"));
+        for (Var* v : { in_, perm_, inv_ })
+            if (!v->is_array() || v->type.base != 'u' || v->type.ncomp != 1)
+                throw Error(AQ_EINVAL, "radix-sort \"" + name_ +
+                                           "\": variables must be "
+                                           "unsigned int arrays");
+        if (perm_->length != in_->length || inv_->length != in_->length)
+            throw Error(AQ_EINVAL, "radix-sort \"" + name_ +
+                                       "\": lengths mismatch");
+        ws_.alloc(radix_sort_workspace((uint32_t)in_->length));
+    }
+    void execute(Sim&, hipStream_t s) override
+    {
+        radix_sort((uint32_t*)in_->dev, (uint32_t*)perm_->dev,
+                   (uint32_t*)inv_->dev, (uint32_t)in_->length,
+                   radix_key_bits(0xFFFFFFFFull >> 1), ws_.p, s);
+    }
+
+  private:
+    ToolDesc d_;
+    Var *in_, *perm_, *inv_;
+    DevBuf ws_;
+};
+
+// ---- reduction: Reduction.cpp -----------------------------------------------
+class ReductionTool : public Tool {
+  public:
+    ReductionTool(const ToolDesc& d)
+        : Tool(d.get("name"), "reduction", d.get("once") == "true"), d_(d)
+    {
+        const std::string op = squeeze(d.text);
+        if (op == "c=min(a,b);" || op == "c=(a<b)?a:b;" ||
+            op == "c=(a<=b)?a:b;" || op == "c=(b<a)?b:a;")
+            op_ = AQ_RED_MIN;
+        else if (op == "c=max(a,b);" || op == "c=(a>b)?a:b;" ||
+                 op == "c=(a>=b)?a:b;" || op == "c=(b>a)?b:a;")
+            op_ = AQ_RED_MAX;
+        else if (op == "c=a+b;" || op == "c=b+a;")
+            op_ = AQ_RED_SUM;
+        else
+            throw Error(AQ_EUNSUP, "reduction \"" + name_ +
+                                       "\": unsupported operation \"" +
+                                       d.text + "\" (min, max or sum)");
+    }
+    void setup(Sim& S) override
+    {
+        in_ = &S.vars.at(d_.get("in"));
+        out_ = &S.vars.at(d_.get("out"));
+        if (!in_->is_array() || out_->is_array())
+            throw Error(AQ_EINVAL, "reduction \"" + name_ +
+                                       "\": in must be an array, out a "
+                                       "scalar");
+        if (in_->type.base != out_->type.base ||
+            in_->type.stored != out_->type.stored)
+            throw Error(AQ_EINVAL, "reduction \"" + name_ +
+                                       "\": in and out types differ");
+        nc_ = in_->type.stored;
+        auto id = words(macro_value(d_.get("null"), out_->type, S.dims, S.vars),
+                        out_->type);
+        ident_.alloc(id.size() * 4);
+        AQ_HIP(hipMemcpy(ident_.p, id.data(), id.size() * 4,
+                         hipMemcpyHostToDevice));
+        ws_.alloc(reduce_workspace((uint32_t)in_->length, nc_));
+    }
+    void execute(Sim& S, hipStream_t s) override
+    {
+        const int type = in_->type.base == 'f'   ? AQ_T_FLOAT
+                         : in_->type.base == 'i' ? AQ_T_INT
+                                                 : AQ_T_UINT;
+        reduce(op_, type, nc_, in_->dev, (uint32_t)in_->length, S.ptr(out_),
+               ident_.p, ws_.p, s);
+    }
+
+  private:
+    ToolDesc d_;
+    int op_ = 0, nc_ = 1;
+    Var *in_, *out_;
+    DevBuf ident_, ws_;
+};
+
+// ---- set: Set.cpp -----------------------------------------------------------
+class SetTool : public Tool {
+  public:
+    SetTool(const ToolDesc& d)
+        : Tool(d.get("name"), "set", d.get("once") == "true"), d_(d)
+    {
+    }
+    void setup(Sim& S) override
+    {
+        v_ = &S.vars.at(d_.get("in"));
+        if (!v_->is_array())
+            throw Error(AQ_EINVAL, "set \"" + name_ + "\": \"" + v_->name +
+                                       "\" is not an array");
+        auto w = words(macro_value(d_.get("value"), v_->type, S.dims, S.vars),
+                       v_->type);
+        pat_.alloc(w.size() * 4);
+        AQ_HIP(hipMemcpy(pat_.p, w.data(), w.size() * 4,
+                         hipMemcpyHostToDevice));
+    }
+    void execute(Sim&, hipStream_t s) override
+    {
+        fill_launch(v_->dev, (const uint32_t*)pat_.p, v_->type.stored,
+                    v_->length, s);
+    }
+
+  private:
+    ToolDesc d_;
+    Var* v_;
+    DevBuf pat_;
+};
+
+// ---- set_scalar / assert: SetScalar.cpp, Assert.cpp -------------------------
+static int vm_depth(const std::vector<VmOp>& ops)
+{
+    int d = 0, mx = 0;
+    for (auto& o : ops) {
+        switch (o.code) {
+        case VM_PUSH: case VM_LOADF: case VM_LOADI: case VM_LOADU: d++; break;
+        case VM_STOREF: case VM_STOREI: case VM_STOREU: case VM_ASSERT: d--; break;
+        case VM_NEG: case VM_NOT: case VM_F1: case VM_END: break;
+        case VM_SEL: d -= 2; break;
+        default: d--; break;
+        }
+        mx = std::max(mx, d);
+    }
+    return mx;
+}
+
+class VmTool : public Tool {
+  public:
+    using Tool::Tool;
+    void upload(const VmProgram& p)
+    {
+        if (vm_depth(p.ops) > 30)
+            throw Error(AQ_EUNSUP, "expression of tool \"" + name_ +
+                                       "\" is too deep");
+        prog_.alloc(p.ops.size() * sizeof(VmOp));
+        AQ_HIP(hipMemcpy(prog_.p, p.ops.data(), p.ops.size() * sizeof(VmOp),
+                         hipMemcpyHostToDevice));
+    }
+    void execute(Sim& S, hipStream_t s) override
+    {
+        vm_launch((const VmOp*)prog_.p, S.vars.dev_arena(), S.d_status,
+                  S.d_fail_id, s);
+    }
+
+  protected:
+    DevBuf prog_;
+};
+
+class SetScalarTool : public VmTool {
+  public:
+    SetScalarTool(const ToolDesc& d)
+        : VmTool(d.get("name"), "set_scalar", d.get("once") == "true"), d_(d)
+    {
+    }
+    void setup(Sim& S) override
+    {
+        Var& v = S.vars.at(d_.get("in"));
+        if (v.is_array())
+            throw Error(AQ_EINVAL, "set_scalar \"" + name_ + "\": \"" +
+                                       v.name + "\" is an array");
+        auto comps = split_components(d_.get("value"));
+        auto slots = [&](const std::string& n, SlotRef& r) {
+            return S.vars.slot(n, r);
+        };
+        VmProgram p;
+        for (int k = 0; k < v.type.ncomp && k < (int)comps.size(); k++) {
+            p.compile(parse_expr(comps[k]), slots);
+            p.store(SlotRef{ v.slot + 4u * k, v.type.base });
+        }
+        p.end();
+        upload(p);
+    }
+
+  private:
+    ToolDesc d_;
+};
+
+class AssertTool : public VmTool {
+  public:
+    AssertTool(const ToolDesc& d)
+        : VmTool(d.get("name"), "assert", d.get("once") == "true"), d_(d)
+    {
+    }
+    void setup(Sim& S) override
+    {
+        VmProgram p;
+        p.compile(parse_expr(d_.get("condition")),
+                  [&](const std::string& n, SlotRef& r) {
+                      return S.vars.slot(n, r);
+                  });
+        p.assert_true(S.assert_id(name_ + " (" + d_.get("condition") + ")"));
+        p.end();
+        upload(p);
+    }
+
+  private:
+    ToolDesc d_;
+};
+
+// ---- copy: Copy.cpp -----------------------------------------------------------
+class CopyTool : public Tool {
+  public:
+    CopyTool(const ToolDesc& d)
+        : Tool(d.get("name"), "copy", d.get("once") == "true"), d_(d)
+    {
+    }
+    void setup(Sim& S) override
+    {
+        in_ = &S.vars.at(d_.get("in"));
+        out_ = &S.vars.at(d_.get("out"));
+        if (!in_->is_array() || !out_->is_array())
+            throw Error(AQ_EINVAL, "copy \"" + name_ + "\": arrays expected");
+        if (in_->type.elem_bytes() != out_->type.elem_bytes())
+            throw Error(AQ_EINVAL, "copy \"" + name_ + "\": types differ");
+    }
+    void execute(Sim&, hipStream_t s) override
+    {
+        const size_t b = std::min(in_->bytes, out_->bytes);
+        if (b)
+            AQ_HIP(hipMemcpyAsync(out_->dev, in_->dev, b,
+                                  hipMemcpyDeviceToDevice, s));
+    }
+
+  private:
+    ToolDesc d_;
+    Var *in_, *out_;
+};
+
+// ---- installable: out-of-tree tools through a pure C ABI --------------------
+// The plugin exports
+//   int  aq_tool_create(const char* name, const char* const* kv, void** tool)
+//   int  aq_tool_setup(void* tool, const aq_tool_ctx* ctx)
+//   int  aq_tool_enqueue(void* tool, void* stream)
+//   void aq_tool_destroy(void* tool)
+// (replaces the C++ create_object() of tests/ExternalTool/tool.cpp:27-32).
+struct aq_tool_ctx_t {
+    void* sim;  // opaque: pass to aq_tool_var
+    void* (*var)(void* sim, const char* name, size_t* bytes);
+};
+
+static void* plugin_var(void* sim, const char* name, size_t* bytes)
+{
+    Sim* S = (Sim*)sim;
+    Var* v = S->vars.get(name);
+    if (!v) return nullptr;
+    if (bytes) *bytes = v->is_array() ? v->bytes : v->type.elem_bytes();
+    return S->ptr(v);
+}
+
+class InstallableTool : public Tool {
+  public:
+    InstallableTool(const ToolDesc& d)
+        : Tool(d.get("name"), "installable", d.get("once") == "true"), d_(d)
+    {
+    }
+    ~InstallableTool() override
+    {
+        if (obj_ && destroy_) destroy_(obj_);
+        if (h_) dlclose(h_);
+    }
+    void setup(Sim& S) override
+    {
+        h_ = dlopen(d_.get("path").c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h_)
+            throw Error(AQ_ENOTFOUND, "installable \"" + name_ + "\": " +
+                                          dlerror());
+        auto create = (int (*)(const char*, const char* const*, void**))dlsym(
+            h_, "aq_tool_create");
+        setup_ = (int (*)(void*, const aq_tool_ctx_t*))dlsym(h_,
+                                                            "aq_tool_setup");
+        enqueue_ = (int (*)(void*, void*))dlsym(h_, "aq_tool_enqueue");
+        destroy_ = (void (*)(void*))dlsym(h_, "aq_tool_destroy");
+        if (!create || !setup_ || !enqueue_ || !destroy_)
+            throw Error(AQ_ENOTFOUND, "installable \"" + name_ +
+                                          "\": missing aq_tool_* symbols");
+        std::vector<std::string> kv;
+        for (auto& a : d_.attrs) {
+            kv.push_back(a.first);
+            kv.push_back(a.second);
+        }
+        std::vector<const char*> ckv;
+        for (auto& s : kv) ckv.push_back(s.c_str());
+        ckv.push_back(nullptr);
+        if (create(name_.c_str(), ckv.data(), &obj_) != 0 || !obj_)
+            throw Error(AQ_EINVAL, "installable \"" + name_ +
+                                       "\": aq_tool_create failed");
+        ctx_.sim = &S;
+        ctx_.var = plugin_var;
+        if (setup_(obj_, &ctx_) != 0)
+            throw Error(AQ_EINVAL, "installable \"" + name_ +
+                                       "\": aq_tool_setup failed");
+    }
+    void execute(Sim&, hipStream_t s) override
+    {
+        if (enqueue_(obj_, (void*)s) != 0)
+            throw Error(AQ_EINVAL, "installable \"" + name_ +
+                                       "\": aq_tool_enqueue failed");
+    }
+
+  private:
+    ToolDesc d_;
+    void* h_ = nullptr;
+    void* obj_ = nullptr;
+    int (*setup_)(void*, const aq_tool_ctx_t*) = nullptr;
+    int (*enqueue_)(void*, void*) = nullptr;
+    void (*destroy_)(void*) = nullptr;
+    aq_tool_ctx_t ctx_{};
+};
+
+std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d)
+{
+    (void)S;
+    const std::string t = d.get("type");
+    const bool once = d.get("once") == "true";
+    if (t == "kernel") return std::make_shared<KernelTool>(d);
+    if (t == "link-list") return std::make_shared<LinkListTool>(d);
+    if (t == "radix-sort") return std::make_shared<RadixSortTool>(d);
+    if (t == "reduction") return std::make_shared<ReductionTool>(d);
+    if (t == "set") return std::make_shared<SetTool>(d);
+    if (t == "set_scalar") return std::make_shared<SetScalarTool>(d);
+    if (t == "assert") return std::make_shared<AssertTool>(d);
+    if (t == "copy") return std::make_shared<CopyTool>(d);
+    if (t == "installable") return std::make_shared<InstallableTool>(d);
+    if (t == "dummy" || t.compare(0, 7, "report_") == 0)
+        return std::make_shared<DummyTool>(d.get("name"), t, once);
+    throw Error(AQ_EUNSUP, "tool \"" + d.get("name") + "\": type '" + t +
+                               "' is not supported by this build");
+}
+
+}  // namespace rt
+}  // namespace aq

@@ -125,6 +125,59 @@ __global__ void corrector_k(const int* __restrict__ imove,
     }
 }
 
+// basic/time_scheme/euler.cl (predictor): back up the state
+template <int D>
+__global__ void euler_predictor_k(const float* __restrict__ r,
+                                  const float* __restrict__ u,
+                                  const float* __restrict__ dudt,
+                                  const float* __restrict__ rho,
+                                  const float* __restrict__ drhodt,
+                                  float* __restrict__ r_in,
+                                  float* __restrict__ u_in,
+                                  float* __restrict__ dudt_in,
+                                  float* __restrict__ rho_in,
+                                  float* __restrict__ drhodt_in, uint32_t N)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        dudt_in[a] = dudt[a];
+        u_in[a] = u[a];
+        r_in[a] = r[a];
+    }
+    drhodt_in[i] = drhodt[i];
+    rho_in[i] = rho[i];
+}
+
+// basic/time_scheme/euler.cl (corrector)
+template <int D>
+__global__ void euler_corrector_k(const int* __restrict__ imove,
+                                  float* __restrict__ r, float* __restrict__ u,
+                                  const float* __restrict__ dudt,
+                                  float* __restrict__ rho,
+                                  const float* __restrict__ drhodt, uint32_t N,
+                                  const float* dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] > 0) {
+        const float dt = *dtp;
+        const float hdt2 = 0.5f * dt * dt;
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            const size_t a = (size_t)i * VS + k;
+            const float u0 = u[a];
+            r[a] += dt * u0 + hdt2 * dudt[a];
+            u[a] = u0 + dt * dudt[a];
+        }
+        rho[i] += dt * drhodt[i];
+    }
+}
+
 // basic/Domain.cl:48-90
 template <int D>
 __global__ void domain_k(int* __restrict__ imove, float* __restrict__ r_in,
@@ -733,6 +786,23 @@ void launch_corrector(int dims, const int32_t* imove, float* r, float* u,
 {
     AQ_DISPATCH(dims, corrector_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
                 dudt_in, drhodt_in, N, dt);
+}
+void launch_euler_predictor(int dims, const float* r, const float* u,
+                            const float* dudt, const float* rho,
+                            const float* drhodt, float* r_in, float* u_in,
+                            float* dudt_in, float* rho_in, float* drhodt_in,
+                            uint32_t N, hipStream_t s)
+{
+    AQ_DISPATCH(dims, euler_predictor_k, AQ_GRID(N), r, u, dudt, rho, drhodt,
+                r_in, u_in, dudt_in, rho_in, drhodt_in, N);
+}
+void launch_euler_corrector(int dims, const int32_t* imove, float* r,
+                            float* u, const float* dudt, float* rho,
+                            const float* drhodt, uint32_t N, const float* dt,
+                            hipStream_t s)
+{
+    AQ_DISPATCH(dims, euler_corrector_k, AQ_GRID(N), imove, r, u, dudt, rho,
+                drhodt, N, dt);
 }
 void launch_domain(int dims, int32_t* imove, float* r_in, float* u_in,
                    float* dudt_in, float* m, uint32_t N, const float* dmin,

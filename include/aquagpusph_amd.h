@@ -289,6 +289,54 @@ AQ_API int aq_reduce(int op, int type, int ncomp, const void* in, uint32_t n,
                      void* stream);
 AQ_API size_t aq_reduce_workspace(uint32_t n, int ncomp);
 
+/* ---- simulation runtime: CalcServer + XML problem setup --------------
+ * The counterpart of the reference executable's CalcServer
+ * (aquagpusph/CalcServer/CalcServer.cpp:139-621) behind the same XML case
+ * and preset files (aquagpusph/InputOutput/State.cpp): variables, preset
+ * definitions, the ordered tool list and the particle sets.  Kernel tools
+ * resolve their `path` + `entry_point` to the HIP kernels above. */
+typedef struct aq_sim aq_sim;
+
+/* Parse the case (dims 2 or 3, like the reference's -d option), register
+ * the variables, build and set up every tool and load the particle files. */
+AQ_API int aq_sim_create(const char* xml_path, int dims, aq_sim** out);
+AQ_API void aq_sim_destroy(aq_sim* sim);
+/* Run nsteps time steps by walking the tool list (CalcServer::update). */
+AQ_API int aq_sim_step(aq_sim* sim, int nsteps);
+/* Same, replaying one captured HIP graph per step after a first eager step. */
+AQ_API int aq_sim_step_graph(aq_sim* sim, int nsteps);
+AQ_API int aq_sim_sync(aq_sim* sim);
+AQ_API void* aq_sim_stream(aq_sim* sim);
+AQ_API uint32_t aq_sim_n_particles(aq_sim* sim);
+/* Type (canonical reference name), length and element size of a variable. */
+AQ_API int aq_sim_var_info(aq_sim* sim, const char* name, char* type_buf,
+                           size_t type_len, size_t* length,
+                           size_t* elem_bytes);
+/* Copy a variable (array or scalar) to/from host memory. */
+AQ_API int aq_sim_get(aq_sim* sim, const char* name, void* host,
+                      size_t bytes);
+AQ_API int aq_sim_set(aq_sim* sim, const char* name, const void* host,
+                      size_t bytes);
+/* Device address of an array variable or of a scalar's slot. */
+AQ_API void* aq_sim_device_ptr(aq_sim* sim, const char* name);
+AQ_API int aq_sim_n_tools(aq_sim* sim);
+AQ_API const char* aq_sim_tool_name(aq_sim* sim, int i);
+AQ_API const char* aq_sim_tool_type(aq_sim* sim, int i);
+/* Formatted value of a preset definition (as the kernels receive it). */
+AQ_API int aq_sim_definition(aq_sim* sim, const char* name, char* buf,
+                             size_t len);
+/* Mean GPU milliseconds per tool over nsteps eager steps, into ms[n_tools]
+ * (the report_performance counterpart, CalcServer/Reports/Performance.cpp). */
+AQ_API int aq_sim_profile_tools(aq_sim* sim, int nsteps, double* ms);
+/* Parse a case without using the GPU and describe the resolved pipeline:
+ * "name<TAB>type<TAB>key" per tool in execution order, then "#var", "#def"
+ * and "#N" lines.  *needed receives the buffer size required. */
+AQ_API int aq_xml_resolve(const char* xml_path, int dims, char* out,
+                          size_t len, size_t* needed);
+/* Registry keys ("cfd/Interactions.cl:entry", ...) of the HIP kernels. */
+AQ_API int aq_sim_n_registered_kernels(void);
+AQ_API const char* aq_sim_registered_kernel(int i);
+
 #ifdef __cplusplus
 }
 #endif

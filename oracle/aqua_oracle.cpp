@@ -208,3 +208,23 @@ ORC_DEFINE(float, f32, 2)
 ORC_DEFINE(float, f32, 3)
 ORC_DEFINE(double, f64, 2)
 ORC_DEFINE(double, f64, 3)
+
+// ---- kernel-function probes (FP64 instantiation of the FP32 constants) ---
+ORC_EXPORT double orc_kernel_w(int dims, double q)
+{
+    return dims == 2 ? kernelW<2, double>(q) : kernelW<3, double>(q);
+}
+ORC_EXPORT double orc_kernel_f(int dims, double q)
+{
+    return dims == 2 ? kernelF<2, double>(q) : kernelF<3, double>(q);
+}
+ORC_EXPORT double orc_kernel_sp(int dims, double q)
+{
+    return dims == 2 ? kernelS_P<2, double>(q) : kernelS_P<3, double>(q);
+}
+ORC_EXPORT double orc_kernel_sd(int dims, double d, double t, double b,
+                                double s)
+{
+    return dims == 2 ? kernelS_D<2, double>(d, t, b, s)
+                     : kernelS_D<3, double>(d, t, b, s);
+}

@@ -98,10 +98,13 @@ class OracleSim:
         self.r_max = np.zeros(self.vs, R)
 
         p = case["params"]
-        self.sets = {k: np.ascontiguousarray(p[k], R)
+        # inputs are float variables in the reference: narrow them to float
+        # first so that the FP64 run starts from the same values
+        f32 = np.float32
+        self.sets = {k: np.asarray(p[k], f32).astype(R)
                      for k in ("refd", "visc_dyn", "delta")}
         g = np.zeros(self.vs, R)
-        g[:self.dims] = p["g"][:self.dims]
+        g[:self.dims] = np.asarray(p["g"][:self.dims], f32)
         self.g = g
         self.h = np.float32(p["h"])
         self.cs = np.float32(p["cs"])
@@ -113,8 +116,8 @@ class OracleSim:
         dmin = np.zeros(self.vs, R)
         dmax = np.zeros(self.vs, R)
         if domain:
-            dmin[:self.dims] = p["domain_min"][:self.dims]
-            dmax[:self.dims] = p["domain_max"][:self.dims]
+            dmin[:self.dims] = np.asarray(p["domain_min"][:self.dims], f32)
+            dmax[:self.dims] = np.asarray(p["domain_max"][:self.dims], f32)
         self.domain_min, self.domain_max = dmin, dmax
         # preset definitions (basic.xml Definitions), float -> %#G -> float
         self.H = definition(self.h)

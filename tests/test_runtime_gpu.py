@@ -1,0 +1,190 @@
+"""GPU tests of the XML-driven runtime (the product path): case files built
+from the presets run through aq_sim_* and are compared with the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from aquagpusph_amd import _lib, casefiles, cases
+from aquagpusph_amd.sim import Simulation
+from oracle import pipeline as op
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / s) if s else float(np.abs(a).max())
+
+
+FIELDS = ("icell", "id_sorted", "id_unsorted", "id", "imove", "r", "u", "rho",
+          "p", "dudt", "drhodt", "shepard", "grad_p", "lap_u", "div_u",
+          "lap_p", "dt_var")
+
+
+def _compare(sim, o, tol):
+    errs = {}
+    for k in FIELDS:
+        a = sim.get(k)
+        b = o.f[k]
+        if a.dtype.kind in "iu":
+            np.testing.assert_array_equal(a.view(b.dtype), b, err_msg=k)
+            errs[k] = 0.0
+        else:
+            errs[k] = relerr(a, b)
+    assert max(errs.values()) <= tol, errs
+    return errs
+
+
+@pytest.mark.parametrize("dims,nl,steps", [(2, 25, 2), (3, 8, 2),
+                                           (2, 25, 30), (3, 8, 15)])
+def test_xml_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
+    case = cases.dam_break(dims, 0.3 / nl)
+    xml = casefiles.write_case(case, str(tmp_path))
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4)
+    print(dims, steps, errs)
+    assert abs(float(sim.get("t")) - float(o.t)) <= 1e-6 * float(o.t)
+    assert int(sim.get("iter")) == steps
+    assert float(sim.get("dt")) == pytest.approx(float(o.dt), rel=1e-5)
+    sim.close()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_graph_replay_equals_eager(cuda, tmp_path, dims):
+    """A captured step replayed from a HIP graph gives bit-identical state,
+    replay after replay."""
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    xml = casefiles.write_case(case, str(tmp_path))
+    a = Simulation(xml, dims)
+    b = Simulation(xml, dims)
+    for k in range(3):
+        a.step(4)
+        b.step(4, graph=True)
+        for f in ("r", "u", "rho", "p", "id", "icell", "shepard"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+        assert float(a.get("t")) == float(b.get("t"))
+    a.close()
+    b.close()
+
+
+def test_upload_path_equals_file_path(cuda, tmp_path):
+    """ParticlesSet without Load + Simulation.set == FastASCII Load."""
+    case = cases.dam_break(3, 0.3 / 6)
+    x1 = casefiles.write_case(case, str(tmp_path / "a"))
+    x2 = casefiles.write_case(case, str(tmp_path / "b"), load=False)
+    a = Simulation(x1, 3)
+    b = Simulation(x2, 3)
+    casefiles.upload(b, case)
+    for f in ("r", "normal", "u", "rho", "m", "imove"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    a.step(3)
+    b.step(3)
+    np.testing.assert_array_equal(a.get("r"), b.get("r"))
+
+
+def _fixture_case(tmp_path, body, dims, data, fields):
+    np.savetxt(tmp_path / "particles.dat", data, fmt="%.17g")
+    (tmp_path / "main.xml").write_text("""<?xml version="1.0" ?>
+<sphInput>
+%s
+    <ParticlesSet n="%d">
+        <Load fields="%s" file="particles.dat" format="FastASCII"/>
+    </ParticlesSet>
+</sphInput>""" % (body, data.shape[0], fields))
+    return str(tmp_path / "main.xml")
+
+
+def test_reference_radixsort_case(cuda, tmp_path):
+    """The reference tests/3D/RadixSort main.xml tool set on its fixture."""
+    d = np.loadtxt(os.path.join(GOLDEN, "radixsort_3d_particles.dat"))
+    xml = _fixture_case(tmp_path, """
+    <Variables>
+        <Variable length="N" name="f" type="size_t*" />
+        <Variable length="N" name="f_orig" type="size_t*" />
+        <Variable length="N" name="perms" type="size_t*" />
+        <Variable length="N" name="inv_perms" type="size_t*" />
+        <Variable length="1" name="h" type="float" value="0.1"/>
+    </Variables>
+    <Tools>
+        <Tool action="add" name="copy f" once="true" type="copy" in="f" out="f_orig" />
+        <Tool action="add" name="sort f" once="true" type="radix-sort" in="f" perm="perms" inv_perm="inv_perms" />
+        <Tool action="add" name="iter += 1" type="set_scalar" in="iter" value="iter + 1"/>
+    </Tools>""", 3, d, "r, f")
+    sim = Simulation(xml, 3)
+    sim.step(2)  # once-tools run only in the first step
+    f, f_orig = sim.get("f"), sim.get("f_orig")
+    perms, inv = sim.get("perms"), sim.get("inv_perms")
+    assert list(f) == sorted(f_orig)
+    assert list(f_orig) != sorted(f_orig)
+    np.testing.assert_array_equal(f, f_orig[perms])
+    np.testing.assert_array_equal(f[inv], f_orig)
+    assert int(sim.get("iter")) == 2
+
+
+def test_reference_reduction_case(cuda, tmp_path):
+    """The reference tests/3D/Reduction tool set: vec sum (tol 1e-2) and max
+    (exact) of its fixture."""
+    d = np.loadtxt(os.path.join(GOLDEN, "reduction_3d_particles.dat"))
+    xml = _fixture_case(tmp_path, """
+    <Variables>
+        <Variable length="1" name="r_sum" type="vec" />
+        <Variable length="1" name="r_max" type="vec" />
+        <Variable length="1" name="h" type="float" value="0.1"/>
+    </Variables>
+    <Tools>
+        <Tool action="add" name="sum up" once="true" type="reduction" in="r" out="r_sum" null="VEC_ZERO">
+            c = a + b;
+        </Tool>
+        <Tool action="add" name="get max" once="true" type="reduction" in="r" out="r_max" null="VEC_NEG_INFINITY">
+            c = max(a, b);
+        </Tool>
+    </Tools>""", 3, d, "r")
+    sim = Simulation(xml, 3)
+    sim.step(1)
+    r32 = d.astype(np.float32)
+    np.testing.assert_allclose(sim.get("r_sum")[:3], d[:, :3].sum(0),
+                               atol=1e-2)
+    np.testing.assert_array_equal(sim.get("r_max")[:3], r32[:, :3].max(0))
+
+
+def test_assert_tool_failure_is_reported(cuda, tmp_path):
+    case = cases.dam_break(2, 0.3 / 10)
+    xml = casefiles.write_case(case, str(tmp_path), extra_xml="""
+    <Tools>
+        <Tool action="insert" before="End" name="never" type="assert" condition="iter &lt; 2"/>
+    </Tools>""")
+    sim = Simulation(xml, 2)
+    sim.step(1)
+    with pytest.raises(_lib.AquaError, match="never"):
+        sim.step(2)
+
+
+def test_neighbour_count_preset(cuda, tmp_path):
+    """basic/neighs_limit.xml: n_neighs bit-exact vs the oracle count, and the
+    guard trips when the limit is too small."""
+    import ctypes
+    case = cases.dam_break(3, 0.3 / 6)
+    pre = casefiles.PBI_PRESETS + ("resources/Presets/basic/neighs_limit.xml",)
+    xml = casefiles.write_case(case, str(tmp_path), presets=pre)
+    sim = Simulation(xml, 3)
+    sim.step(1)
+    o = op.OracleSim(case)
+    o.step()
+    n = np.zeros(o.N, np.uint32)
+    lim = int(sim.get("neighs_limit"))
+    op.lib().orc_neighs_3d(op._p(o.f["imove"]), op._p(n), ctypes.c_uint(lim),
+                           ctypes.c_uint(o.N), *o._ll())
+    np.testing.assert_array_equal(sim.get("n_neighs"), n)
+    assert int(sim.get("max_neighs")) == n.max()
+    sim.set("neighs_limit", np.uint32(n.max()))
+    with pytest.raises(_lib.AquaError, match="check number of neighbours"):
+        sim.step(1)
