@@ -173,8 +173,11 @@ def test_neighbour_count_preset(cuda, tmp_path):
     guard trips when the limit is too small."""
     import ctypes
     case = cases.dam_break(3, 0.3 / 6)
-    pre = casefiles.PBI_PRESETS + ("resources/Presets/basic/neighs_limit.xml",)
-    xml = casefiles.write_case(case, str(tmp_path), presets=pre)
+    # neighs_limit is evaluated from hfac: include the preset after the
+    # case variables are registered
+    xml = casefiles.write_case(case, str(tmp_path), extra_xml=(
+        '    <Include file="resources/Presets/basic/neighs_limit.xml" '
+        'when="end"/>'))
     sim = Simulation(xml, 3)
     sim.step(1)
     o = op.OracleSim(case)
