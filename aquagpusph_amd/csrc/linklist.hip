@@ -205,6 +205,63 @@ __global__ void ihoc_heads(const uint32_t* __restrict__ icell,
     if (c2 != c) ihoc[c2] = i + 1;
 }
 
+// ---- cell ranges for the cell-parallel walks --------------------------------
+// head[i] = 1 when i starts a run of equal icell; cell_end[c] = one past the
+// last particle of cell c (only written for non-empty cells).
+__global__ void cell_heads_k(const uint32_t* __restrict__ icell, uint32_t N,
+                             uint32_t* __restrict__ head,
+                             uint32_t* __restrict__ cell_end,
+                             const uint32_t* status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (status && (*status & 1u)) {
+        head[i] = 0;
+        return;
+    }
+    const uint32_t c = icell[i];
+    head[i] = (i == 0 || icell[i - 1] != c) ? 1u : 0u;
+    if (i == N - 1 || icell[i + 1] != c) cell_end[c] = i + 1;
+}
+
+__global__ void cell_list_k(const uint32_t* __restrict__ head,
+                            const uint32_t* __restrict__ pos, uint32_t N,
+                            uint32_t* __restrict__ heads,
+                            uint32_t* __restrict__ n_heads)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (head[i]) heads[pos[i]] = i;
+    if (i == N - 1) *n_heads = pos[i] + head[i];
+}
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
+                    uint32_t* sums, hipStream_t s);
+size_t scan_workspace(uint32_t m);
+
+size_t cell_list_workspace(uint32_t N)
+{
+    return (((size_t)N * 4 + 255) & ~(size_t)255) * 2 + scan_workspace(N) +
+           256;
+}
+
+// Non-empty cells of a sorted icell: heads[k] = first particle of the k-th
+// non-empty cell, *n_heads their number, cell_end[c] its end.
+void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
+               uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
+               void* ws, hipStream_t s)
+{
+    const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
+    uint32_t* head = (uint32_t*)ws;
+    uint32_t* pos = (uint32_t*)((char*)ws + a);
+    uint32_t* sums = (uint32_t*)((char*)ws + 2 * a);
+    const dim3 g(blocks_for(N, 256)), b(256);
+    hipLaunchKernelGGL(cell_heads_k, g, b, 0, s, icell, N, head, cell_end,
+                       status);
+    exclusive_scan(head, pos, N, sums, s);
+    hipLaunchKernelGGL(cell_list_k, g, b, 0, s, head, pos, N, heads, n_heads);
+}
+
 // ---- host drivers ---------------------------------------------------------
 size_t linklist_workspace(uint32_t N)
 {

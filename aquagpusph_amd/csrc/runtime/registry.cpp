@@ -67,7 +67,7 @@ void launch_bi_shepard_compute(int, const int32_t*, const float*,
                                const float*, float*, uint32_t,
                                const uint32_t*, const uint32_t*,
                                const uint32_t*, const uint32_t*, aq_defs,
-                               hipStream_t);
+                               hipStream_t, int);
 void launch_bi_lapu(int, const int32_t*, const float*, const float*,
                     const float*, const float*, float*, uint32_t,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -81,7 +81,7 @@ void launch_bi_interactions(int, const int32_t*, const float*, const float*,
                             const float*, float*, float*, uint32_t,
                             const uint32_t*, const uint32_t*,
                             const uint32_t*, const uint32_t*, aq_defs,
-                            hipStream_t);
+                            hipStream_t, const uint8_t*);
 void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
                    const uint32_t*, const uint32_t*, const uint32_t*,
                    const uint32_t*, hipStream_t);
@@ -249,7 +249,7 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_bi_shepard_compute(S.dims, I(0), F(1), F(2), F(3), F(4),
                                           F(5), F(6), S.N, U(7), U(8), U(9),
-                                          S.d_status, S.kdefs, s);
+                                          S.d_status, S.kdefs, s, 0);
             } } },
         { "cfd/Boundary/BI/Shepard.cl:apply",
           { { "imove", "shepard", "grad_p", "lap_u", "div_u" },
@@ -289,9 +289,11 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "normal", "u", "rho", "m", "p", "grad_p", "div_u",
               "icell", "ihoc", "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_bi_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
-                                       F(5), F(6), F(7), F(8), S.N, U(9),
-                                       U(10), U(11), S.d_status, S.kdefs, s);
+                launch_bi_interactions(
+                    S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7),
+                    F(8), S.N, U(9), U(10), U(11), S.d_status, S.kdefs, s,
+                    (S.fusion.active && S.fusion.near) ? S.fusion.near_buf
+                                                        : nullptr);
             } } },
     };
     return t;

@@ -44,6 +44,10 @@ Sim::~Sim()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     tools.clear();
+    for (void* b : { (void*)fusion.near_buf, (void*)fusion.cell_end,
+                     (void*)fusion.heads, (void*)fusion.n_heads,
+                     fusion.cells_ws })
+        if (b) (void)hipFree(b);
     if (d_status) (void)hipFree(d_status);
     if (d_fail_id) (void)hipFree(d_fail_id);
     if (stream) (void)hipStreamDestroy(stream);
@@ -249,9 +253,195 @@ void Sim::load_sets()
 void Sim::setup()
 {
     for (auto& t : tools) t->setup(*this);
+    plan_fusion();
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
     is_setup = true;
+}
+
+// Kernel tools that neither move particles nor change imove, r, u, rho, m, p
+// of fluid particles: they may sit between fused tools.
+static bool fusion_transparent(const Tool& t)
+{
+    if (t.type() == "dummy" || t.type().compare(0, 7, "report_") == 0)
+        return !t.once();
+    static const std::set<std::string> ok = {
+        "cfd/Sensors.cl:entry",
+        "cfd/SensorsRenormalization.cl:entry",
+        "cfd/deltaSPH.cl:lapp",
+        "cfd/Boundary/BI/LapU.cl:freeslip",
+        "cfd/Boundary/BI/GradP.cl:freeslip",
+        "cfd/Boundary/BI/Interpolation.cl:entry",
+        "cfd/Boundary/BI/InterpolationShepard.cl:entry",
+        "cfd/Interactions.cl:entry",
+    };
+    return t.type() == "kernel" && !t.once() && ok.count(t.key());
+}
+
+void Sim::plan_fusion()
+{
+    fusion = FusionPlan();
+    const char* env = getenv("AQ_NO_FUSION");
+    if (env && *env && strcmp(env, "0") != 0) return;
+    auto find = [&](const std::string& k) -> int {
+        int at = -1;
+        for (size_t i = 0; i < tools.size(); i++)
+            if (tools[i]->key() == k) {
+                if (at >= 0) return -2;  // several: do not fuse
+                at = (int)i;
+            }
+        return at;
+    };
+    const int iI = find("cfd/Interactions.cl:entry");
+    if (iI < 0 || tools[iI]->once()) return;
+    auto between = [&](int a, int b) {
+        for (int k = a + 1; k < b; k++)
+            if (!fusion_transparent(*tools[k])) return false;
+        return true;
+    };
+    const int iS2 = find("cfd/Boundary/BI/Shepard.cl:compute");
+    const int iS1 = find("cfd/Shepard.cl:entry");
+    const int iL = find("cfd/deltaSPH.cl:lapp");
+    const int iB = find("cfd/Boundary/BI/Interactions.cl:entry");
+    int anchor = iI;
+    if (iS2 >= 0 && iS1 < 0 && iS2 < iI && !tools[iS2]->once() &&
+        between(iS2, iI)) {
+        fusion.shep = 2;
+        anchor = iS2;
+    } else if (iS1 >= 0 && iS2 < 0 && iS1 < iI && !tools[iS1]->once() &&
+               between(iS1, iI)) {
+        fusion.shep = 1;
+        anchor = iS1;
+    }
+    if (iL > iI && !tools[iL]->once() && between(iI, iL) && vars.get("lap_p"))
+        fusion.lapp = true;
+    if (iB > anchor && !tools[iB]->once() && between(anchor, iB))
+        fusion.near = true;
+    auto V = [&](const char* n) { return vars.get(n); };
+    fusion.imove = V("imove");
+    fusion.r = V("r");
+    fusion.u = V("u");
+    fusion.rho = V("rho");
+    fusion.m = V("m");
+    fusion.p = V("p");
+    fusion.normal = V("normal");
+    fusion.tangent = V("tangent");
+    fusion.binormal = V("binormal");
+    fusion.shepard = V("shepard");
+    fusion.grad_p = V("grad_p");
+    fusion.lap_u = V("lap_u");
+    fusion.div_u = V("div_u");
+    fusion.lap_p = V("lap_p");
+    fusion.icell = V("icell");
+    fusion.ihoc = V("ihoc");
+    fusion.n_cells = V("n_cells");
+    for (Var* v : { fusion.imove, fusion.r, fusion.u, fusion.rho, fusion.m,
+                    fusion.p, fusion.grad_p, fusion.lap_u, fusion.div_u,
+                    fusion.icell, fusion.ihoc, fusion.n_cells })
+        if (!v) return;
+    if (fusion.shep && !fusion.shepard) return;
+    if (fusion.shep == 2 &&
+        !(fusion.normal && fusion.tangent && fusion.binormal))
+        return;
+    fusion.anchor = tools[anchor].get();
+    if (anchor != iI) fusion.absorbed.insert(tools[iI].get());
+    if (fusion.lapp) fusion.absorbed.insert(tools[iL].get());
+    if (fusion.near) {
+        AQ_HIP(hipMalloc(&fusion.near_buf, N));
+        AQ_HIP(hipMemset(fusion.near_buf, 1, N));
+    }
+    // per-particle walk unless AQ_WALK=cells asks for the cell-parallel one
+    const char* walk = getenv("AQ_WALK");
+    fusion.cells = walk && strcmp(walk, "cells") == 0;
+    if (const char* wb = getenv("AQ_WALK_BLOCKS"))
+        fusion.walk_blocks = std::max(1, atoi(wb));
+    fusion.active = true;
+}
+
+}  // namespace rt
+
+void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
+                       const float* r, const float* u, const float* rho,
+                       const float* m, const float* p, const float* normal,
+                       const float* tangent, const float* binormal,
+                       float* shepard, float* grad_p, float* lap_u,
+                       float* div_u, float* lap_p, uint8_t* near, uint32_t N,
+                       const uint32_t* icell, const uint32_t* ihoc,
+                       const uint32_t* n_cells, const uint32_t* status,
+                       aq_defs d, hipStream_t s);
+void launch_fluid_pass_cells(int dims, int shep, bool lapp,
+                             const int32_t* imove, const float* r,
+                             const float* u, const float* rho, const float* m,
+                             const float* p, const float* normal,
+                             const float* tangent, const float* binormal,
+                             float* shepard, float* grad_p, float* lap_u,
+                             float* div_u, float* lap_p, uint8_t* near,
+                             uint32_t N, const uint32_t* icell,
+                             const uint32_t* ihoc, const uint32_t* n_cells,
+                             const uint32_t* cell_end, const uint32_t* heads,
+                             const uint32_t* n_heads, const uint32_t* status,
+                             aq_defs d, float smax, unsigned nblocks,
+                             hipStream_t s);
+void launch_shepard(int, const int32_t*, const float*, const float*,
+                    const float*, float*, uint32_t, const uint32_t*,
+                    const uint32_t*, const uint32_t*, const uint32_t*,
+                    aq_defs, int, hipStream_t);
+void launch_bi_shepard_compute(int, const int32_t*, const float*,
+                               const float*, const float*, const float*,
+                               const float*, float*, uint32_t,
+                               const uint32_t*, const uint32_t*,
+                               const uint32_t*, const uint32_t*, aq_defs,
+                               hipStream_t, int);
+
+namespace rt {
+
+void Sim::run_fused(hipStream_t s)
+{
+    FusionPlan& f = fusion;
+    auto P = [&](Var* v) { return v ? ptr(v) : nullptr; };
+    const float* fr = (const float*)P(f.r);
+    // non-fluid part of the Shepard tool first (sensors, elements)
+    if (f.shep == 2)
+        launch_bi_shepard_compute(dims, (const int32_t*)P(f.imove), fr,
+                                  (const float*)P(f.normal),
+                                  (const float*)P(f.tangent),
+                                  (const float*)P(f.binormal),
+                                  (const float*)P(f.m), (float*)P(f.shepard),
+                                  N, (const uint32_t*)P(f.icell),
+                                  (const uint32_t*)P(f.ihoc),
+                                  (const uint32_t*)P(f.n_cells), d_status,
+                                  kdefs, s, 1);
+    else if (f.shep == 1)
+        launch_shepard(dims, (const int32_t*)P(f.imove), fr,
+                       (const float*)P(f.rho), (const float*)P(f.m),
+                       (float*)P(f.shepard), N, (const uint32_t*)P(f.icell),
+                       (const uint32_t*)P(f.ihoc),
+                       (const uint32_t*)P(f.n_cells), d_status, kdefs, 1 | 2,
+                       s);
+    if (f.cells && f.cell_end) {
+        launch_fluid_pass_cells(
+            dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
+            (const float*)P(f.u), (const float*)P(f.rho),
+            (const float*)P(f.m), (const float*)P(f.p),
+            (const float*)P(f.normal), (const float*)P(f.tangent),
+            (const float*)P(f.binormal), (float*)P(f.shepard),
+            (float*)P(f.grad_p), (float*)P(f.lap_u), (float*)P(f.div_u),
+            (float*)P(f.lap_p), f.near ? f.near_buf : nullptr, N,
+            (const uint32_t*)P(f.icell), (const uint32_t*)P(f.ihoc),
+            (const uint32_t*)P(f.n_cells), f.cell_end, f.heads, f.n_heads,
+            d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks, s);
+        return;
+    }
+    launch_fluid_pass(dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
+                      (const float*)P(f.u), (const float*)P(f.rho),
+                      (const float*)P(f.m), (const float*)P(f.p),
+                      (const float*)P(f.normal), (const float*)P(f.tangent),
+                      (const float*)P(f.binormal), (float*)P(f.shepard),
+                      (float*)P(f.grad_p), (float*)P(f.lap_u),
+                      (float*)P(f.div_u), (float*)P(f.lap_p),
+                      f.near ? f.near_buf : nullptr, N,
+                      (const uint32_t*)P(f.icell), (const uint32_t*)P(f.ihoc),
+                      (const uint32_t*)P(f.n_cells), d_status, kdefs, s);
 }
 
 void Sim::run_tools(hipStream_t s)

@@ -115,6 +115,8 @@ class Tool {
     virtual void execute(Sim& S, hipStream_t s) = 0;
     // false when execute() needs the host between launches (eager only)
     virtual bool graph_safe() const { return true; }
+    // registry key of kernel tools ("" for other tools)
+    virtual std::string key() const { return ""; }
     const std::string& name() const { return name_; }
     const std::string& type() const { return type_; }
     bool once() const { return once_; }
@@ -190,6 +192,36 @@ std::vector<std::string> registered_kernels();
 // "…/Scripts/cfd/Interactions.cl" + "entry" -> "cfd/Interactions.cl:entry"
 std::string kernel_key(const std::string& path, const std::string& entry);
 
+// ---- fusion of neighbour walks ------------------------------------------------
+// When the tool list holds the Shepard, cfd-interactions and lapp tools with
+// nothing in between that changes what they read, one walk per fluid
+// particle evaluates all of them (each sum in the reference order, so the
+// results are bit-identical) at the position of the first; the others
+// become no-ops.  The walk also flags fluid particles with a wall element
+// inside the support so the boundary-integral interactions skip the rest.
+struct FusionPlan {
+    bool active = false;
+    int shep = 0;  // 0 none, 1 cfd/Shepard.cl, 2 BI compute
+    bool lapp = false;
+    bool near = false;
+    const Tool* anchor = nullptr;
+    std::set<const Tool*> absorbed;
+    Var *imove = nullptr, *r = nullptr, *u = nullptr, *rho = nullptr,
+        *m = nullptr, *p = nullptr, *normal = nullptr, *tangent = nullptr,
+        *binormal = nullptr, *shepard = nullptr, *grad_p = nullptr,
+        *lap_u = nullptr, *div_u = nullptr, *lap_p = nullptr,
+        *icell = nullptr, *ihoc = nullptr, *n_cells = nullptr;
+    uint8_t* near_buf = nullptr;
+    // non-empty cell list for the cell-parallel walk, filled by the link-list
+    // tool after the sort (cell_end over the ihoc capacity)
+    bool cells = false;
+    uint32_t* cell_end = nullptr;
+    uint32_t* heads = nullptr;
+    uint32_t* n_heads = nullptr;
+    void* cells_ws = nullptr;
+    unsigned walk_blocks = 2048;
+};
+
 // ---- the runtime ------------------------------------------------------------
 class Sim {
   public:
@@ -221,6 +253,8 @@ class Sim {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool is_setup = false;
+    FusionPlan fusion;
+    void run_fused(hipStream_t s);
 
     bool defined(const std::string& n) const
     {
@@ -235,6 +269,7 @@ class Sim {
 
   private:
     void run_tools(hipStream_t s);
+    void plan_fusion();
     void make_defs();
     void load_sets();
 };

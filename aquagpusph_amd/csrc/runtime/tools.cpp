@@ -15,6 +15,10 @@ void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
                   uint32_t* id_sorted, uint32_t* ihoc, uint32_t cap,
                   uint32_t* status, void* ws, hipStream_t s);
 size_t linklist_workspace(uint32_t N);
+size_t cell_list_workspace(uint32_t N);
+void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
+               uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
+               void* ws, hipStream_t s);
 void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                 unsigned key_bits, void* workspace, hipStream_t s);
 size_t radix_sort_workspace(uint32_t n);
@@ -153,8 +157,16 @@ class KernelTool : public Tool {
     }
     void execute(Sim& S, hipStream_t s) override
     {
+        if (S.fusion.active) {
+            if (S.fusion.anchor == this) {
+                S.run_fused(s);
+                return;
+            }
+            if (S.fusion.absorbed.count(this)) return;
+        }
         spec_->launch(S, bound_.data(), s);
     }
+    std::string key() const override { return key_; }
 
   private:
     std::string path_, entry_, key_;
@@ -264,6 +276,20 @@ class LinkListTool : public Tool {
                      (uint32_t*)perm_->dev, (uint32_t*)inv_->dev,
                      (uint32_t*)ihoc_->dev, (uint32_t)ihoc_->length,
                      S.d_status, ws_.p, s);
+        // the cell list of the cell-parallel walk (when that walk reads this
+        // link-list)
+        FusionPlan& f = S.fusion;
+        if (f.active && f.cells && f.icell == icell_ && f.ihoc == ihoc_) {
+            if (!f.cell_end) {
+                AQ_HIP(hipMalloc(&f.cell_end, ihoc_->length * 4));
+                AQ_HIP(hipMalloc(&f.heads, (size_t)S.N * 4));
+                AQ_HIP(hipMalloc(&f.n_heads, 256));
+                AQ_HIP(hipMalloc(&f.cells_ws, cell_list_workspace(S.N)));
+                AQ_HIP(hipMemset(f.n_heads, 0, 256));
+            }
+            cell_list((const uint32_t*)icell_->dev, S.N, f.cell_end, f.heads,
+                      f.n_heads, S.d_status, f.cells_ws, s);
+        }
     }
 
   private:

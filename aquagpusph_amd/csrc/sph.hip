@@ -420,7 +420,8 @@ __global__ void __launch_bounds__(BS)
     if (i >= ll.N) return;
     AQ_STATUS_SKIP;
     const int im = imove[i];
-    auto excl = [&](int v) { return cfd ? (v != 1) : (v >= 3); };
+    if ((cfd & 2) && im == 1) return;  // fluid done by the fused walk
+    auto excl = [&](int v) { return (cfd & 1) ? (v != 1) : (v >= 3); };
     if ((im < -3) || ((im > 0) && excl(im))) return;
     const f3 r_i = ldv<D>(r, i);
     float acc = shepard[i];
@@ -558,13 +559,14 @@ __global__ void __launch_bounds__(BS)
                          const float* __restrict__ binormal,
                          const float* __restrict__ m,
                          float* __restrict__ shepard, LLArgs ll, float H,
-                         float CONW, float DIMS, float smax)
+                         float CONW, float DIMS, float smax, int skip_fluid)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ll.N) return;
     AQ_STATUS_SKIP;
     const int im = imove[i];
     if ((im < -3) || (im > 1)) return;
+    if (skip_fluid && im == 1) return;  // fluid done by the fused walk
     const f3 r_i = ldv<D>(r, i);
     float acc = 1.f;
     bool self_added = false;
@@ -690,12 +692,14 @@ __global__ void __launch_bounds__(BS)
                       const float* __restrict__ m,
                       const float* __restrict__ p, float* __restrict__ grad_p,
                       float* __restrict__ div_u, LLArgs ll, float H,
-                      float CONW, float smax)
+                      float CONW, float smax, const uint8_t* __restrict__ near)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ll.N) return;
     AQ_STATUS_SKIP;
     if (imove[i] != 1) return;
+    // no element inside the support: nothing is added (bit-identical skip)
+    if (near && !near[i]) return;
     const f3 r_i = ldv<D>(r, i);
     const f3 u_i = ldv<D>(u, i);
     const float p_i = p[i];
@@ -719,6 +723,114 @@ __global__ void __launch_bounds__(BS)
             });
     stxyz<D>(grad_p, i, gp);
     div_u[i] = dv;
+}
+
+// ---------------------------------------------------------------------------
+// Fused fluid walk.  For a fluid particle i one traversal of the neighbour
+// cells evaluates, each in its own accumulator and in the reference order,
+//   * the Shepard factor: cfd/Shepard.cl (SHEP == 1, fluid j incl. self) or
+//     the boundary-integral compute of cfd/Boundary/BI/Shepard.cl:61-150
+//     (SHEP == 2, element j),
+//   * cfd/Interactions.cl:60-145 (fluid j != i),
+//   * basic/deltaSPH.cl:191-242 lapp (LAPP, fluid j != i),
+// and flags particles with an element inside the support (the only ones
+// cfd/Boundary/BI/Interactions.cl adds anything to).  Every sum sees the
+// same terms in the same order as in the separate kernels, so the results
+// are bit-identical; the walk and the candidate tests are done once.
+// ---------------------------------------------------------------------------
+template <int D, int SHEP, bool LAPP>
+__global__ void __launch_bounds__(BS)
+    fluid_pass_k(const int* __restrict__ imove, const float* __restrict__ r,
+                 const float* __restrict__ u, const float* __restrict__ rho,
+                 const float* __restrict__ m, const float* __restrict__ p,
+                 const float* __restrict__ normal,
+                 const float* __restrict__ tangent,
+                 const float* __restrict__ binormal,
+                 float* __restrict__ shepard, float* __restrict__ grad_p,
+                 float* __restrict__ lap_u, float* __restrict__ div_u,
+                 float* __restrict__ lap_p, uint8_t* __restrict__ near,
+                 LLArgs ll, float H, float CONW, float CONF, float DIMS,
+                 float smax, int lap)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ll.N) return;
+    AQ_STATUS_SKIP;
+    if (imove[i] != 1) return;
+    constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
+    const f3 r_i = ldv<D>(r, i);
+    const f3 u_i = ldv<D>(u, i);
+    const float p_i = p[i];
+    const float rho_i = rho[i];
+    f3 gp = ldv<D>(grad_p, i), lu = ldv<D>(lap_u, i);
+    float dv = div_u[i];
+    float lp = LAPP ? lap_p[i] : 0.f;
+    float sh = (SHEP == 2) ? 1.f : (SHEP == 1 ? shepard[i] : 0.f);
+    bool self_added = false;
+    bool nearw = false;
+    const float eps = 1E-8f * H;
+    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+            ll.ihoc, [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                if (s > smax) return;
+                const int im_j = imove[j];
+                if (im_j == 1) {
+                    const float q = sqrtf(s) / H;
+                    const float m_j = m[j];
+                    const float rho_j = rho[j];
+                    if (SHEP == 1) sh += kernelW<D>(q) * CONW * m_j / rho_j;
+                    if (i == j) return;
+                    const float p_j = p[j];
+                    const f3 u_ij = sub<D>(ldv<D>(u, j), u_i);
+                    const float udr = dot<D>(u_ij, r_ij);
+                    const float fk = kernelF<D>(q) * CONF;
+                    const float f_ij = fk * m_j;
+                    axpy<D>(gp, (p_i + p_j) / (rho_i * rho_j) * f_ij, r_ij);
+                    if (lap == AQ_LAP_MONAGHAN) {
+                        const float r2 = (q * q + 0.01f) * H * H;
+                        axpy<D>(lu,
+                                f_ij * CLEARY * udr / (r2 * rho_i * rho_j),
+                                r_ij);
+                    } else {
+                        axpy<D>(lu, f_ij * 2.f / (rho_i * rho_j), u_ij);
+                    }
+                    dv += udr * f_ij * rho_i / rho_j;
+                    if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
+                } else if (im_j == -3) {
+                    nearw = true;
+                    if (SHEP == 2) {
+                        const float q = sqrtf(s) / H;
+                        const f3 n_j = ldv<D>(normal, j);
+                        const f3 t_j = ldv<D>(tangent, j);
+                        const f3 b_j = ldv<D>(binormal, j);
+                        const float r_n = dot<D>(r_ij, n_j);
+                        const float area_j = m[j];
+                        if ((r_n > -eps) && (r_n < eps)) {
+                            if (!self_added) {
+                                const float r_t = fabsf(dot<D>(r_ij, t_j));
+                                const float r_b = fabsf(dot<D>(r_ij, b_j));
+                                const float dr =
+                                    0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                                if ((r_t <= dr) && (r_b <= dr)) {
+                                    self_added = true;
+                                    sh -= 0.5f;
+                                }
+                            }
+                            return;
+                        }
+                        const float r_t = fabsf(dot<D>(r_ij, t_j));
+                        const float r_b = fabsf(dot<D>(r_ij, b_j));
+                        sh += r_n * CONW * kernelS_P<D>(q) * area_j +
+                              kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
+                    }
+                }
+            });
+    stxyz<D>(grad_p, i, gp);
+    stxyz<D>(lap_u, i, lu);
+    div_u[i] = dv;
+    if (LAPP) lap_p[i] = lp;
+    if (SHEP) shepard[i] = sh;
+    if (near) near[i] = nearw ? 1 : 0;
 }
 
 // basic/neighs.cl:52-91
@@ -928,12 +1040,12 @@ void launch_bi_shepard_compute(int dims, const int32_t* imove, const float* r,
                                const uint32_t* icell, const uint32_t* ihoc,
                                const uint32_t* n_cells,
                                const uint32_t* status, aq_defs d,
-                               hipStream_t s)
+                               hipStream_t s, int skip_fluid)
 {
     AQ_DISPATCH(dims, bi_shepard_compute_k, AQ_GRID(N), imove, r, normal,
                 tangent, binormal, m, shepard,
                 mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW, d.DIMS,
-                support_threshold(d.H));
+                support_threshold(d.H), skip_fluid);
 }
 void launch_bi_lapu(int dims, const int32_t* imove, const float* r,
                     const float* u, const float* rho, const float* m,
@@ -961,11 +1073,43 @@ void launch_bi_interactions(int dims, const int32_t* imove, const float* r,
                             float* grad_p, float* div_u, uint32_t N,
                             const uint32_t* icell, const uint32_t* ihoc,
                             const uint32_t* n_cells, const uint32_t* status,
-                            aq_defs d, hipStream_t s)
+                            aq_defs d, hipStream_t s, const uint8_t* near)
 {
     AQ_DISPATCH(dims, bi_interactions_k, AQ_GRID(N), imove, r, normal, u, rho,
                 m, p, grad_p, div_u, mkll(icell, ihoc, n_cells, status, N),
-                d.H, d.CONW, support_threshold(d.H));
+                d.H, d.CONW, support_threshold(d.H), near);
+}
+
+void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
+                       const float* r, const float* u, const float* rho,
+                       const float* m, const float* p, const float* normal,
+                       const float* tangent, const float* binormal,
+                       float* shepard, float* grad_p, float* lap_u,
+                       float* div_u, float* lap_p, uint8_t* near, uint32_t N,
+                       const uint32_t* icell, const uint32_t* ihoc,
+                       const uint32_t* n_cells, const uint32_t* status,
+                       aq_defs d, hipStream_t s)
+{
+    const LLArgs ll = mkll(icell, ihoc, n_cells, status, N);
+    const float smax = support_threshold(d.H);
+#define AQ_FP(DD, SH, LP)                                                      \
+    hipLaunchKernelGGL((fluid_pass_k<DD, SH, LP>), AQ_GRID(N), imove, r, u,    \
+                       rho, m, p, normal, tangent, binormal, shepard, grad_p,  \
+                       lap_u, div_u, lap_p, near, ll, d.H, d.CONW, d.CONF,     \
+                       d.DIMS, smax, d.lap_formulation)
+#define AQ_FP_D(SH, LP)                                                        \
+    do {                                                                       \
+        if (dims == 2) AQ_FP(2, SH, LP);                                       \
+        else AQ_FP(3, SH, LP);                                                 \
+    } while (0)
+    if (shep == 0 && !lapp) AQ_FP_D(0, false);
+    else if (shep == 0 && lapp) AQ_FP_D(0, true);
+    else if (shep == 1 && !lapp) AQ_FP_D(1, false);
+    else if (shep == 1 && lapp) AQ_FP_D(1, true);
+    else if (shep == 2 && !lapp) AQ_FP_D(2, false);
+    else AQ_FP_D(2, true);
+#undef AQ_FP_D
+#undef AQ_FP
 }
 void launch_neighs(int dims, const int32_t* imove, uint32_t* n_neighs,
                    const uint32_t* limit, uint32_t N, const uint32_t* icell,
@@ -1248,7 +1392,7 @@ AQ_API int aq_bi_shepard_compute(int dims, const int32_t* imove,
                 n_cells);
         launch_bi_shepard_compute(dims, imove, r, normal, tangent, binormal, m,
                                   shepard, N, icell, ihoc, n_cells, nullptr,
-                                  defs, S(stream));
+                                  defs, S(stream), 0);
     });
 }
 
@@ -1300,7 +1444,7 @@ AQ_API int aq_bi_interactions(int dims, const int32_t* imove, const float* r,
                 n_cells);
         launch_bi_interactions(dims, imove, r, normal, u, rho, m, p, grad_p,
                                div_u, N, icell, ihoc, n_cells, nullptr, defs,
-                               S(stream));
+                               S(stream), nullptr);
     });
 }
 

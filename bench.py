@@ -67,11 +67,88 @@ def cpu_baseline(case, budget_s=25.0):
                        "%.1f s" % (n, case["N"], threads, el))
 
 
-# compulsory (algorithmic) HBM bytes per particle for the cfd interactions
-# kernel in 3D: read imove 4, r 16, u 16, rho 4, m 4, p 4, icell 4 and write
-# grad_p 16, lap_u 16, div_u 4 (read-modify-write of the zeroed outputs
-# counts once), i.e. 88 B/particle (92 with the ihoc reads amortised)
-INTERACTIONS_BYTES = 88
+# Algorithmic figures of the fused fluid walk (one launch per step, at the
+# position of the Shepard tool; it evaluates Shepard, the cfd interactions,
+# lap p and the near-wall flags, see FusionPlan in csrc/runtime/sim.hpp).
+# Compulsory HBM bytes per particle (3D, float4 vec): reads imove 4, r 16,
+# u 16, rho 4, m 4, p 4, icell 4, grad_p/lap_u/div_u/lap_p 40 (the zeroed
+# accumulators), normal/tangent/binormal 48 of the wall elements (counted
+# for every particle: an upper bound), writes shepard 4, grad_p 16, lap_u 16,
+# div_u 4, lap_p 4, near 1.
+WALK_BYTES = 4 + 16 + 16 + 4 + 4 + 4 + 4 + 40 + 48 + 4 + 16 + 16 + 4 + 4 + 1
+# FP32 operations (sqrt and divide counted as one) per candidate distance
+# test (r_ij 3, |r_ij|^2 5, compare 1) and per accepted fluid pair (q 2,
+# F(q) 6, u_ij 3, u.r 5, grad_p 10, Monaghan lap_u 15, div_u 4, lap_p 4);
+# a wall element in the support costs about 100 (S_P, S_D with 4 acospi).
+FLOP_CAND = 9
+FLOP_PAIR = 49
+FLOP_ELEM = 100
+
+PROFILE_DIR = os.path.join(HERE, "profiles")
+
+
+def pair_counts(case, sample=200000):
+    """Candidates (particles in the 3^d cells around a fluid particle's cell,
+    what the walk tests) exactly from the cell occupancy, and accepted
+    fluid-fluid pairs (q <= 2, j != i) and fluid-element pairs from a
+    strided sample of the fluid particles (k-d tree counts)."""
+    from scipy.spatial import cKDTree
+    dims = case["dims"]
+    r = np.asarray(case["r"], np.float64)[:, :dims]
+    im = np.asarray(case["imove"])
+    h = float(case["params"]["h"])
+    cell = 2.0 * h
+    rmin = r.min(0)
+    ijk = np.floor((r - rmin) / cell).astype(np.int64) + 1
+    dim = ijk.max(0) + 2
+    flat = np.ravel_multi_index(ijk.T, dim)
+    occ = np.bincount(flat, minlength=int(np.prod(dim)))
+    fl = np.nonzero(im == 1)[0]
+    cand = np.zeros(len(fl), np.int64)
+    offs = np.stack(np.meshgrid(*[[-1, 0, 1]] * dims, indexing="ij"),
+                    -1).reshape(-1, dims)
+    for o in offs:
+        cand += occ[np.ravel_multi_index((ijk[fl] + o).T, dim)]
+    step = max(1, len(fl) // sample)
+    q = r[fl[::step]]
+    rad = 2.0 * h * (1 + 1e-6)
+    t_f = cKDTree(r[im == 1])
+    n_f = t_f.query_ball_point(q, rad, return_length=True, workers=-1) - 1
+    n_e = np.zeros_like(n_f)
+    if (im == -3).any():
+        t_e = cKDTree(r[im == -3])
+        n_e = t_e.query_ball_point(q, rad, return_length=True, workers=-1)
+    return dict(n_fluid=len(fl), cand=int(cand.sum()),
+                pairs=float(n_f.mean()) * len(fl),
+                elem_pairs=float(n_e.mean()) * len(fl),
+                sample_stride=step)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor),
+    or None when no summary is committed."""
+    path = os.path.join(PROFILE_DIR, kernel + "_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device-to-device copy of known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                      "reps": reps}))
 
 
 def main():
@@ -82,12 +159,18 @@ def main():
     ap.add_argument("--dr", type=float, default=0.003)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB copy kernel (for the --pmc "
+                         "counter scale) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
               "supported (see DESIGN.md for how the path would shard)"
               % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
 
     sim, case = build_sim(a.dr)
     N = sim.N
@@ -102,16 +185,49 @@ def main():
     ms = 1e3 * el / a.steps
     value = N * a.steps / el
 
-    # per-tool GPU time (HIP events on the runtime's stream)
+    # per-tool GPU time: HIP events recorded on the runtime's stream around
+    # every tool, over eager steps run right after the timed region
     prof = sim.profile(a.profile_steps)
-    by_name = {n: (t, m) for n, t, m in prof}
-    t_int = by_name["cfd interactions"][1] * 1e-3
     step_gpu_ms = sum(m for _, _, m in prof)
-    achieved = INTERACTIONS_BYTES * N / t_int / 1e9
-    # FP32 work of the interactions launch (accepted pairs ~ 60 flop,
-    # rejected candidates ~ 10 flop; 268 / 1728 per particle on the lattice)
-    flops = N * (268 * 60 + (1728 - 268) * 10)
-    top = sorted(prof, key=lambda x: -x[2])[:8]
+    top = sorted(prof, key=lambda x: -x[2])
+    dom_name, _, dom_ms = top[0]
+    by_name = {n: m for n, _, m in prof}
+    fused = (dom_name == "cfd Shepard"
+             and by_name.get("cfd interactions", 0.0) < 0.01 * dom_ms)
+    pc = pair_counts(case)
+    t = dom_ms * 1e-3
+    if fused:
+        kname = "fused fluid walk (cfd Shepard + cfd interactions + lap p)"
+        flops = (pc["cand"] * FLOP_CAND + pc["pairs"] * FLOP_PAIR
+                 + pc["elem_pairs"] * FLOP_ELEM)
+        hbm = WALK_BYTES * N
+    else:
+        kname = dom_name
+        flops = None
+        hbm = None
+    traffic = measured_traffic("fused_walk_dambreak3d") if fused else None
+    roof = {
+        "kernel": kname,
+        "bound": "mfma",
+        "achieved": flops / t / 1e12 if flops else None,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": flops / t / 1e12 / PEAK_FP32_TFLOPS if flops else None,
+        "traffic": traffic,
+        "note": "FP32 vector ALU work (no matrix instructions); the MI355X "
+                "FP32 vector and matrix peaks are both 157.3 TFLOP/s. The "
+                "walk is compute-bound: %.0f FLOP per %d compulsory bytes"
+                % (flops / N if flops else 0, WALK_BYTES),
+        "avg_launch_ms": dom_ms,
+        "flop_per_launch": flops,
+        "candidates_per_fluid_particle": pc["cand"] / pc["n_fluid"],
+        "pairs_per_fluid_particle": pc["pairs"] / pc["n_fluid"],
+        "elements_per_fluid_particle": pc["elem_pairs"] / pc["n_fluid"],
+        "hbm_algorithmic_bytes": hbm,
+        "hbm_achieved_gbs": hbm / t / 1e9 if hbm else None,
+        "hbm_frac": hbm / t / 1e9 / PEAK_HBM_GBS if hbm else None,
+        "pair_interactions_per_s": pc["pairs"] / t,
+    }
 
     out = {
         "metric": "particle-steps/s",
@@ -135,20 +251,8 @@ def main():
                        "variableTimeStep",
             "parallelism": "single GPU, HIP graph per step"},
         "fluid_particle_steps_per_s": nf * a.steps / el,
-        "roofline": {
-            "kernel": "cfd interactions (cfd/Interactions.cl:entry)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS,
-            "traffic": None,
-            "avg_launch_ms": t_int * 1e3,
-            "bytes_per_particle": INTERACTIONS_BYTES,
-            "fp32_tflops_est": flops / t_int / 1e12,
-            "fp32_peak_tflops": PEAK_FP32_TFLOPS,
-        },
-        "tools_ms": {n: round(m, 4) for n, _, m in top},
+        "roofline": roof,
+        "tools_ms": {n: round(m, 4) for n, _, m in top[:8]},
         "step_gpu_ms_sum": step_gpu_ms,
     }
     sim.close()

@@ -76,6 +76,69 @@ def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     b.close()
 
 
+@pytest.mark.parametrize("dims,bi", [(2, True), (3, True), (3, False)])
+def test_fused_walks_are_bit_identical(cuda, tmp_path, dims, bi, monkeypatch):
+    """The fused neighbour walk (Shepard + interactions + lap p + near-wall
+    flags) reproduces the separate reference-order kernels bit for bit."""
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    presets = casefiles.PBI_PRESETS if bi else tuple(
+        p for p in casefiles.PBI_PRESETS if not p.endswith("BI.xml"))
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets)
+    monkeypatch.setenv("AQ_NO_FUSION", "1")
+    a = Simulation(xml, dims)
+    monkeypatch.setenv("AQ_NO_FUSION", "0")
+    b = Simulation(xml, dims)
+    for _ in range(3):
+        a.step(3)
+        b.step(3, graph=True)
+        for f in ("r", "u", "rho", "p", "shepard", "grad_p", "lap_u",
+                  "div_u", "lap_p", "dudt", "drhodt"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+
+
+def _dense_case(dims, n, seed):
+    """Random fluid cloud with ~150 particles per link-list cell (more than
+    a wavefront per cell, several candidate segments per neighbourhood)."""
+    rng = np.random.default_rng(seed)
+    vs = 2 if dims == 2 else 4
+    r = np.zeros((n, vs))
+    r[:, :dims] = rng.uniform(0, 1, (n, dims))
+    per_cell = 150.0
+    h = 0.5 * (per_cell / n) ** (1.0 / dims)
+    u = np.zeros((n, vs))
+    u[:, :dims] = rng.normal(0, 0.05, (n, dims))
+    g = [0.0] * dims
+    g[-1] = -9.81
+    params = dict(h=h, cs=20.0, courant=0.1, g=g, refd=[1000.0],
+                  visc_dyn=[1e-3], delta=[0.1], dt_min=0.0, dt_Ma=0.1,
+                  domain_min=[-1.0] * dims, domain_max=[2.0] * dims)
+    return dict(dims=dims, N=n, r=r, u=u,
+                rho=1000.0 + rng.normal(0, 2.0, n),
+                m=np.full(n, 1000.0 / n), imove=np.ones(n, np.int32),
+                params=params)
+
+
+@pytest.mark.parametrize("dims,n", [(2, 6000), (3, 20000)])
+def test_cell_walk_dense_cells(cuda, tmp_path, dims, n, monkeypatch):
+    """Cells holding more particles than a wavefront and neighbourhoods of
+    several candidate segments: the cell-parallel walk, the per-particle
+    fused walk and the separate kernels agree bit for bit."""
+    case = _dense_case(dims, n, 11)
+    presets = tuple(p for p in casefiles.PBI_PRESETS
+                    if not p.endswith("BI.xml"))
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets)
+    sims = []
+    for fuse, walk in (("1", "cells"), ("0", "cells"), ("0", "thread")):
+        monkeypatch.setenv("AQ_NO_FUSION", fuse)
+        monkeypatch.setenv("AQ_WALK", walk)
+        sims.append(Simulation(xml, dims))
+    for s in sims:
+        s.step(2)
+    for f in ("shepard", "grad_p", "lap_u", "div_u", "lap_p", "r", "u"):
+        np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f), f)
+        np.testing.assert_array_equal(sims[0].get(f), sims[2].get(f), f)
+
+
 def test_upload_path_equals_file_path(cuda, tmp_path):
     """ParticlesSet without Load + Simulation.set == FastASCII Load."""
     case = cases.dam_break(3, 0.3 / 6)
