@@ -40,6 +40,38 @@ template <int D> __device__ __forceinline__ f3 ldv(const float* a, uint32_t i)
     return v;
 }
 
+// Loads at a 32-bit byte offset from a wave-uniform base pointer: the
+// address stays "SGPR base + VGPR offset" (no 64-bit address arithmetic per
+// load).  Callers guarantee that the offsets fit in 32 bits.
+__device__ __forceinline__ float ldf_o(const float* a, uint32_t j)
+{
+    return *reinterpret_cast<const float*>(
+        reinterpret_cast<const char*>(a) + (j << 2));
+}
+__device__ __forceinline__ int ldi_o(const int* a, uint32_t j)
+{
+    return *reinterpret_cast<const int*>(
+        reinterpret_cast<const char*>(a) + (j << 2));
+}
+template <int D>
+__device__ __forceinline__ f3 ldv_o(const float* a, uint32_t j)
+{
+    f3 v;
+    const char* b = reinterpret_cast<const char*>(a);
+    if constexpr (D == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(b + (j << 3));
+        v.x = t.x;
+        v.y = t.y;
+        v.z = 0.f;
+    } else {
+        const float4 t = *reinterpret_cast<const float4*>(b + (j << 4));
+        v.x = t.x;
+        v.y = t.y;
+        v.z = t.z;
+    }
+    return v;
+}
+
 // store the XYZ part, leaving w untouched (the reference writes .XYZ)
 template <int D>
 __device__ __forceinline__ void stxyz(float* a, uint32_t i, const f3& v)

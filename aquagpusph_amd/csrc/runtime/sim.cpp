@@ -350,11 +350,14 @@ void Sim::plan_fusion()
         AQ_HIP(hipMalloc(&fusion.near_buf, N));
         AQ_HIP(hipMemset(fusion.near_buf, 1, N));
     }
-    // per-particle walk unless AQ_WALK=cells asks for the cell-parallel one
+    // cell-parallel walk unless AQ_WALK=thread asks for the per-particle
+    // one (which also serves more than 2^28 particles)
     const char* walk = getenv("AQ_WALK");
-    fusion.cells = walk && strcmp(walk, "cells") == 0;
+    fusion.cells = !(walk && strcmp(walk, "thread") == 0) && N < (1u << 28);
     if (const char* wb = getenv("AQ_WALK_BLOCKS"))
         fusion.walk_blocks = std::max(1, atoi(wb));
+    if (const char* ws = getenv("AQ_WALK_SEGW"))
+        fusion.walk_segw = atoi(ws) == 16 ? 16 : 8;
     fusion.active = true;
 }
 
@@ -381,7 +384,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
-                             hipStream_t s);
+                             int segw, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -429,7 +432,8 @@ void Sim::run_fused(hipStream_t s)
             (float*)P(f.lap_p), f.near ? f.near_buf : nullptr, N,
             (const uint32_t*)P(f.icell), (const uint32_t*)P(f.ihoc),
             (const uint32_t*)P(f.n_cells), f.cell_end, f.heads, f.n_heads,
-            d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks, s);
+            d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
+            f.walk_segw, s);
         return;
     }
     launch_fluid_pass(dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,

@@ -220,6 +220,7 @@ struct FusionPlan {
     uint32_t* n_heads = nullptr;
     void* cells_ws = nullptr;
     unsigned walk_blocks = 2048;
+    int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
 };
 
 // ---- the runtime ------------------------------------------------------------

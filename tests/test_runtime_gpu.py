@@ -76,8 +76,13 @@ def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     b.close()
 
 
-@pytest.mark.parametrize("dims,bi", [(2, True), (3, True), (3, False)])
-def test_fused_walks_are_bit_identical(cuda, tmp_path, dims, bi, monkeypatch):
+@pytest.mark.parametrize("dims,bi,walk", [(2, True, "thread"),
+                                          (3, True, "thread"),
+                                          (3, False, "thread"),
+                                          (2, True, "cells"),
+                                          (3, True, "cells")])
+def test_fused_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk,
+                                       monkeypatch):
     """The fused neighbour walk (Shepard + interactions + lap p + near-wall
     flags) reproduces the separate reference-order kernels bit for bit."""
     case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
@@ -87,6 +92,7 @@ def test_fused_walks_are_bit_identical(cuda, tmp_path, dims, bi, monkeypatch):
     monkeypatch.setenv("AQ_NO_FUSION", "1")
     a = Simulation(xml, dims)
     monkeypatch.setenv("AQ_NO_FUSION", "0")
+    monkeypatch.setenv("AQ_WALK", walk)
     b = Simulation(xml, dims)
     for _ in range(3):
         a.step(3)
@@ -128,15 +134,17 @@ def test_cell_walk_dense_cells(cuda, tmp_path, dims, n, monkeypatch):
                     if not p.endswith("BI.xml"))
     xml = casefiles.write_case(case, str(tmp_path), presets=presets)
     sims = []
-    for fuse, walk in (("1", "cells"), ("0", "cells"), ("0", "thread")):
+    for fuse, walk, segw in (("1", "cells", "16"), ("0", "cells", "16"),
+                             ("0", "cells", "8"), ("0", "thread", "16")):
         monkeypatch.setenv("AQ_NO_FUSION", fuse)
         monkeypatch.setenv("AQ_WALK", walk)
+        monkeypatch.setenv("AQ_WALK_SEGW", segw)
         sims.append(Simulation(xml, dims))
     for s in sims:
         s.step(2)
     for f in ("shepard", "grad_p", "lap_u", "div_u", "lap_p", "r", "u"):
-        np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f), f)
-        np.testing.assert_array_equal(sims[0].get(f), sims[2].get(f), f)
+        for k in (1, 2, 3):
+            np.testing.assert_array_equal(sims[0].get(f), sims[k].get(f), f)
 
 
 def test_upload_path_equals_file_path(cuda, tmp_path):
