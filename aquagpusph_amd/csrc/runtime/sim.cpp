@@ -355,7 +355,7 @@ void Sim::plan_fusion()
     const char* walk = getenv("AQ_WALK");
     fusion.cells = !(walk && strcmp(walk, "thread") == 0) && N < (1u << 28);
     if (const char* wb = getenv("AQ_WALK_BLOCKS"))
-        fusion.walk_blocks = std::max(1, atoi(wb));
+        fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
         fusion.walk_segw = atoi(ws) == 16 ? 16 : 8;
     fusion.active = true;

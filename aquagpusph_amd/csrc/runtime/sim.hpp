@@ -219,7 +219,7 @@ struct FusionPlan {
     uint32_t* heads = nullptr;
     uint32_t* n_heads = nullptr;
     void* cells_ws = nullptr;
-    unsigned walk_blocks = 2048;
+    unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
 };
 
