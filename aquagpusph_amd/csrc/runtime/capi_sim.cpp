@@ -175,8 +175,16 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
         std::vector<hipEvent_t> ev(nt + 1);
         for (auto& e : ev) AQ_HIP(hipEventCreate(&e));
         std::vector<double> acc(nt, 0.0);
+        FusionPlan& F = S.fusion;
+        double walk_acc = 0.0;
+        int walk_n = 0;
+        if (F.active) {
+            if (!F.ev0) AQ_HIP(hipEventCreate(&F.ev0));
+            if (!F.ev1) AQ_HIP(hipEventCreate(&F.ev1));
+        }
         if (S.vars.host_dirty()) S.vars.push(S.stream);
         for (int k = 0; k < nsteps; k++) {
+            F.timing = F.active;
             std::set<Tool*> ran;
             AQ_HIP(hipEventRecord(ev[0], S.stream));
             for (size_t t = 0; t < nt; t++) {
@@ -188,7 +196,15 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
                 AQ_HIP(hipEventRecord(ev[t + 1], S.stream));
             }
             for (auto* tl : ran) tl->done = true;
+            F.timing = false;
             AQ_HIP(hipEventSynchronize(ev[nt]));
+            if (F.active) {
+                float x = 0.f;
+                if (hipEventElapsedTime(&x, F.ev0, F.ev1) == hipSuccess) {
+                    walk_acc += x;
+                    walk_n++;
+                }
+            }
             for (size_t t = 0; t < nt; t++) {
                 float x = 0.f;
                 AQ_HIP(hipEventElapsedTime(&x, ev[t], ev[t + 1]));
@@ -197,8 +213,19 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         for (size_t t = 0; t < nt; t++) ms[t] = nsteps ? acc[t] / nsteps : 0.0;
+        F.walk_ms = walk_n ? walk_acc / walk_n : -1.0;
         S.check_status();
     });
+}
+
+AQ_API int aq_sim_walk_ms(aq_sim* s, double* ms)
+{
+    return guard(
+        [&] {
+            AQ_REQUIRE(ms, "aq_sim_walk_ms: null output");
+            *ms = SIM(s).fusion.walk_ms;
+        },
+        false);
 }
 
 // Parse a case without touching the GPU and describe what the runtime would

@@ -48,6 +48,8 @@ Sim::~Sim()
                      (void*)fusion.heads, (void*)fusion.n_heads,
                      fusion.cells_ws })
         if (b) (void)hipFree(b);
+    if (fusion.ev0) (void)hipEventDestroy(fusion.ev0);
+    if (fusion.ev1) (void)hipEventDestroy(fusion.ev1);
     if (d_status) (void)hipFree(d_status);
     if (d_fail_id) (void)hipFree(d_fail_id);
     if (stream) (void)hipStreamDestroy(stream);
@@ -421,6 +423,7 @@ void Sim::run_fused(hipStream_t s)
                        (const uint32_t*)P(f.ihoc),
                        (const uint32_t*)P(f.n_cells), d_status, kdefs, 1 | 2,
                        s);
+    if (f.timing) AQ_HIP(hipEventRecord(f.ev0, s));
     if (f.cells && f.cell_end) {
         launch_fluid_pass_cells(
             dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
@@ -434,6 +437,7 @@ void Sim::run_fused(hipStream_t s)
             (const uint32_t*)P(f.n_cells), f.cell_end, f.heads, f.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
             f.walk_segw, s);
+        if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }
     launch_fluid_pass(dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
@@ -446,6 +450,7 @@ void Sim::run_fused(hipStream_t s)
                       f.near ? f.near_buf : nullptr, N,
                       (const uint32_t*)P(f.icell), (const uint32_t*)P(f.ihoc),
                       (const uint32_t*)P(f.n_cells), d_status, kdefs, s);
+    if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
 }
 
 void Sim::run_tools(hipStream_t s)

@@ -221,6 +221,10 @@ struct FusionPlan {
     void* cells_ws = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
+    // profiling: events around the walk launch while `timing` is set
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double walk_ms = -1.0;  // mean of the last profile
 };
 
 // ---- the runtime ------------------------------------------------------------

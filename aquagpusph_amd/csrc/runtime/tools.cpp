@@ -5,6 +5,7 @@
 #include <filesystem>
 #include <limits>
 
+#include "aquagpusph_amd_tool.h"
 #include "sim.hpp"
 
 namespace aq {
@@ -541,16 +542,9 @@ class CopyTool : public Tool {
 };
 
 // ---- installable: out-of-tree tools through a pure C ABI --------------------
-// The plugin exports
-//   int  aq_tool_create(const char* name, const char* const* kv, void** tool)
-//   int  aq_tool_setup(void* tool, const aq_tool_ctx* ctx)
-//   int  aq_tool_enqueue(void* tool, void* stream)
-//   void aq_tool_destroy(void* tool)
-// (replaces the C++ create_object() of tests/ExternalTool/tool.cpp:27-32).
-struct aq_tool_ctx_t {
-    void* sim;  // opaque: pass to aq_tool_var
-    void* (*var)(void* sim, const char* name, size_t* bytes);
-};
+// The plugin exports the aq_tool_* functions of aquagpusph_amd_tool.h
+// (replacing the C++ create_object() of tests/ExternalTool/tool.cpp:27-32).
+typedef aq_tool_ctx aq_tool_ctx_t;
 
 static void* plugin_var(void* sim, const char* name, size_t* bytes)
 {

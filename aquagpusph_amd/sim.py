@@ -59,6 +59,8 @@ class Simulation:
                                         ctypes.c_char_p, ctypes.c_size_t]
         L.aq_sim_profile_tools.argtypes = [ctypes.c_void_p, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_double)]
+        L.aq_sim_walk_ms.argtypes = [ctypes.c_void_p,
+                                     ctypes.POINTER(ctypes.c_double)]
         h = ctypes.c_void_p()
         _lib.check(L.aq_sim_create(os.fsencode(os.path.abspath(xml_path)),
                                    dims, ctypes.byref(h)))
@@ -140,3 +142,10 @@ class Simulation:
         ms = (ctypes.c_double * len(names))()
         _lib.check(self._L.aq_sim_profile_tools(self._h, int(nsteps), ms))
         return [(n, t, ms[i]) for i, (n, t) in enumerate(names)]
+
+    def walk_ms(self):
+        """Mean GPU ms of the fused neighbour-walk kernel alone over the last
+        profile() call (None without a fused walk)."""
+        v = ctypes.c_double()
+        _lib.check(self._L.aq_sim_walk_ms(self._h, ctypes.byref(v)))
+        return v.value if v.value >= 0 else None

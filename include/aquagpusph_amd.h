@@ -328,6 +328,11 @@ AQ_API int aq_sim_definition(aq_sim* sim, const char* name, char* buf,
 /* Mean GPU milliseconds per tool over nsteps eager steps, into ms[n_tools]
  * (the report_performance counterpart, CalcServer/Reports/Performance.cpp). */
 AQ_API int aq_sim_profile_tools(aq_sim* sim, int nsteps, double* ms);
+/* Mean GPU milliseconds of the fused fluid neighbour walk kernel alone
+ * (HIP events around its launch) over the last aq_sim_profile_tools call;
+ * -1 when the pipeline has no fused walk.  No counterpart in the reference:
+ * the fused walk replaces three of its kernel tools. */
+AQ_API int aq_sim_walk_ms(aq_sim* sim, double* ms);
 /* Parse a case without using the GPU and describe the resolved pipeline:
  * "name<TAB>type<TAB>key" per tool in execution order, then "#var", "#def"
  * and "#N" lines.  *needed receives the buffer size required. */
