@@ -83,8 +83,11 @@ def build(jobs=None, verbose=False, force=False):
     objs = [_obj(s) for s in srcs]
     if force or todo or not os.path.exists(LIB):
         tmp = LIB + ".tmp"
-        cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp]
-        cmd += objs
+        # -z defs: a declaration that matches no definition fails the build
+        # instead of the library load
+        cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950",
+               "-Wl,-z,defs", "-o", tmp]
+        cmd += objs + ["-ldl"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)

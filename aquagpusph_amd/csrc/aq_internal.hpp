@@ -39,6 +39,26 @@ class Error : public std::runtime_error {
         if (!(cond)) throw ::aq::Error(AQ_EINVAL, (msg));                      \
     } while (0)
 
+// Link-list data of the cell-parallel walks: the sorted icell, ihoc and
+// n_cells, plus the non-empty cell list (heads[k] = first particle of the
+// k-th non-empty cell, n_heads[0] = their number, n_heads[1] = work counter,
+// cell_end[c] = one past the last particle of cell c, AQ_CELL_END bits,
+// with the AQ_CELL_FLUID / AQ_CELL_ELEM kind flags above them).
+#define AQ_CELL_END 0x0FFFFFFFu   // cell_end bits holding the end index
+#define AQ_CELL_FLUID 0x40000000u // the cell holds fluid (imove == 1)
+#define AQ_CELL_ELEM 0x80000000u  // the cell holds wall elements (imove == -3)
+struct CellArgs {
+    const uint32_t* icell;
+    const uint32_t* ihoc;
+    const uint32_t* n_cells;
+    const uint32_t* cell_end;
+    const uint32_t* heads;
+    const uint32_t* n_heads;
+    const uint32_t* status;
+    uint32_t N;
+    uint32_t flagged;  // cell_end carries the kind flags
+};
+
 // Wrap a C ABI body: translate exceptions into error codes.
 template <class F> inline int guard(F f, bool check_launch = true)
 {

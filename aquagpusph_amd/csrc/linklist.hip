@@ -235,6 +235,33 @@ __global__ void cell_list_k(const uint32_t* __restrict__ head,
     if (i == N - 1) *n_heads = pos[i] + head[i];
 }
 
+// Particle kinds present in each non-empty cell, in the top bits of
+// cell_end (AQ_CELL_FLUID: imove == 1, AQ_CELL_ELEM: imove == -3; cell ends
+// stay below 2^28).  imove is read through perm (id_unsorted): the cell list
+// is built right after the sort, before the fields are reordered.
+__global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
+                             const uint32_t* __restrict__ n_heads,
+                             const uint32_t* __restrict__ icell,
+                             const uint32_t* __restrict__ perm,
+                             const int* __restrict__ imove,
+                             uint32_t* __restrict__ cell_end,
+                             const uint32_t* status)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *n_heads) return;
+    if (status && (*status & 1u)) return;
+    const uint32_t start = heads[k];
+    const uint32_t c = icell[start];
+    const uint32_t end = cell_end[c];
+    uint32_t f = 0;
+    for (uint32_t i = start; i < end; i++) {
+        const int im = imove[perm[i]];
+        f |= (im == 1) ? AQ_CELL_FLUID : 0u;
+        f |= (im == -3) ? AQ_CELL_ELEM : 0u;
+    }
+    cell_end[c] = end | f;
+}
+
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
                     uint32_t* sums, hipStream_t s);
 size_t scan_workspace(uint32_t m);
@@ -246,10 +273,12 @@ size_t cell_list_workspace(uint32_t N)
 }
 
 // Non-empty cells of a sorted icell: heads[k] = first particle of the k-th
-// non-empty cell, *n_heads their number, cell_end[c] its end.
+// non-empty cell, *n_heads their number, cell_end[c] its end and, when
+// imove and perm are given, the kinds of particle it holds.
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
-               void* ws, hipStream_t s)
+               void* ws, const uint32_t* perm, const int* imove,
+               hipStream_t s)
 {
     const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
     uint32_t* head = (uint32_t*)ws;
@@ -260,6 +289,9 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                        status);
     exclusive_scan(head, pos, N, sums, s);
     hipLaunchKernelGGL(cell_list_k, g, b, 0, s, head, pos, N, heads, n_heads);
+    if (perm && imove)
+        hipLaunchKernelGGL(cell_kinds_k, g, b, 0, s, heads, n_heads, icell,
+                           perm, imove, cell_end, status);
 }
 
 // ---- host drivers ---------------------------------------------------------

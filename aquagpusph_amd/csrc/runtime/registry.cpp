@@ -82,6 +82,23 @@ void launch_bi_interactions(int, const int32_t*, const float*, const float*,
                             const uint32_t*, const uint32_t*,
                             const uint32_t*, const uint32_t*, aq_defs,
                             hipStream_t, const uint8_t*);
+void launch_bi_shepard_compute_cells(int, const int32_t*, const float*,
+                                     const float*, const float*,
+                                     const float*, const float*, float*,
+                                     const CellArgs&, aq_defs, float, int,
+                                     hipStream_t);
+void launch_bi_lapu_cells(int, const int32_t*, const float*, const float*,
+                          const float*, const float*, float*,
+                          const CellArgs&, aq_defs, float, hipStream_t);
+void launch_bi_interp_cells(int, const int32_t*, const float*, const float*,
+                            const float*, const float*, float*,
+                            const CellArgs&, aq_defs, float, hipStream_t);
+void launch_bi_interactions_cells(int, const int32_t*, const float*,
+                                  const float*, const float*, const float*,
+                                  const float*, const float*, float*, float*,
+                                  const uint8_t*, const CellArgs&, aq_defs,
+                                  float, hipStream_t);
+float support_threshold(float H);
 void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
                    const uint32_t*, const uint32_t*, const uint32_t*,
                    const uint32_t*, hipStream_t);
@@ -247,9 +264,17 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "normal", "tangent", "binormal", "m", "shepard",
               "icell", "ihoc", "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_bi_shepard_compute(S.dims, I(0), F(1), F(2), F(3), F(4),
-                                          F(5), F(6), S.N, U(7), U(8), U(9),
-                                          S.d_status, S.kdefs, s, 0);
+                if (S.bi_cells && S.clist.usable(a[7], a[8]) &&
+                    a[9] == S.clist.n_cells)
+                    launch_bi_shepard_compute_cells(
+                        S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6),
+                        S.cell_args(), S.kdefs, support_threshold(S.kdefs.H),
+                        0, s);
+                else
+                    launch_bi_shepard_compute(S.dims, I(0), F(1), F(2), F(3),
+                                              F(4), F(5), F(6), S.N, U(7),
+                                              U(8), U(9), S.d_status, S.kdefs,
+                                              s, 0);
             } } },
         { "cfd/Boundary/BI/Shepard.cl:apply",
           { { "imove", "shepard", "grad_p", "lap_u", "div_u" },
@@ -261,8 +286,15 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "u", "rho", "m", "lap_u", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_bi_lapu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5), S.N,
-                               U(6), U(7), U(8), S.d_status, S.kdefs, s);
+                if (S.bi_cells && S.clist.usable(a[6], a[7]) &&
+                    a[8] == S.clist.n_cells)
+                    launch_bi_lapu_cells(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                         F(5), S.cell_args(), S.kdefs,
+                                         support_threshold(S.kdefs.H), s);
+                else
+                    launch_bi_lapu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                   S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                   s);
             } } },
         { "cfd/Boundary/BI/GradP.cl:freeslip",
           { { "iset", "imove", "shepard", "lap_u", "dudt", "grad_p",
@@ -275,9 +307,15 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "m", "rho", "grad_p", "p", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_bi_interp(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
-                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
-                                 s);
+                if (S.bi_cells && S.clist.usable(a[6], a[7]) &&
+                    a[8] == S.clist.n_cells)
+                    launch_bi_interp_cells(S.dims, I(0), F(1), F(2), F(3),
+                                           F(4), F(5), S.cell_args(), S.kdefs,
+                                           support_threshold(S.kdefs.H), s);
+                else
+                    launch_bi_interp(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                     F(5), S.N, U(6), U(7), U(8), S.d_status,
+                                     S.kdefs, s);
             } } },
         { "cfd/Boundary/BI/InterpolationShepard.cl:entry",
           { { "iset", "imove", "shepard", "rho", "p", "refd", "cs", "p0" },
@@ -289,11 +327,20 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "normal", "u", "rho", "m", "p", "grad_p", "div_u",
               "icell", "ihoc", "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_bi_interactions(
-                    S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7),
-                    F(8), S.N, U(9), U(10), U(11), S.d_status, S.kdefs, s,
-                    (S.fusion.active && S.fusion.near) ? S.fusion.near_buf
-                                                        : nullptr);
+                const uint8_t* near = (S.fusion.active && S.fusion.near)
+                                          ? S.fusion.near_buf
+                                          : nullptr;
+                if (S.bi_cells && S.clist.usable(a[9], a[10]) &&
+                    a[11] == S.clist.n_cells)
+                    launch_bi_interactions_cells(
+                        S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7),
+                        F(8), near, S.cell_args(), S.kdefs,
+                        support_threshold(S.kdefs.H), s);
+                else
+                    launch_bi_interactions(S.dims, I(0), F(1), F(2), F(3),
+                                           F(4), F(5), F(6), F(7), F(8), S.N,
+                                           U(9), U(10), U(11), S.d_status,
+                                           S.kdefs, s, near);
             } } },
     };
     return t;

@@ -44,9 +44,8 @@ Sim::~Sim()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     tools.clear();
-    for (void* b : { (void*)fusion.near_buf, (void*)fusion.cell_end,
-                     (void*)fusion.heads, (void*)fusion.n_heads,
-                     fusion.cells_ws })
+    for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
+                     (void*)clist.heads, (void*)clist.n_heads, clist.ws })
         if (b) (void)hipFree(b);
     if (fusion.ev0) (void)hipEventDestroy(fusion.ev0);
     if (fusion.ev1) (void)hipEventDestroy(fusion.ev1);
@@ -255,6 +254,19 @@ void Sim::load_sets()
 void Sim::setup()
 {
     for (auto& t : tools) t->setup(*this);
+    // cell-parallel walks unless AQ_WALK=thread asks for the per-particle
+    // ones (which also serve more than 2^28 particles)
+    const char* walk = getenv("AQ_WALK");
+    clist = CellList();
+    clist.enabled = !(walk && strcmp(walk, "thread") == 0) && N < (1u << 28);
+    clist.icell = vars.get("icell");
+    clist.ihoc = vars.get("ihoc");
+    clist.n_cells = vars.get("n_cells");
+    // boundary-integral element kernels: per-particle walks unless
+    // AQ_BI_WALK=cells (the cell-parallel forms are slower on wall cells,
+    // which hold few elements each)
+    const char* bw = getenv("AQ_BI_WALK");
+    bi_cells = bw && strcmp(bw, "cells") == 0;
     plan_fusion();
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
@@ -352,10 +364,6 @@ void Sim::plan_fusion()
         AQ_HIP(hipMalloc(&fusion.near_buf, N));
         AQ_HIP(hipMemset(fusion.near_buf, 1, N));
     }
-    // cell-parallel walk unless AQ_WALK=thread asks for the per-particle
-    // one (which also serves more than 2^28 particles)
-    const char* walk = getenv("AQ_WALK");
-    fusion.cells = !(walk && strcmp(walk, "thread") == 0) && N < (1u << 28);
     if (const char* wb = getenv("AQ_WALK_BLOCKS"))
         fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
@@ -374,6 +382,11 @@ void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
                        const uint32_t* icell, const uint32_t* ihoc,
                        const uint32_t* n_cells, const uint32_t* status,
                        aq_defs d, hipStream_t s);
+void launch_bi_shepard_compute_cells(int, const int32_t*, const float*,
+                                     const float*, const float*,
+                                     const float*, const float*, float*,
+                                     const CellArgs&, aq_defs, float, int,
+                                     hipStream_t);
 void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const int32_t* imove, const float* r,
                              const float* u, const float* rho, const float* m,
@@ -386,7 +399,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
-                             int segw, hipStream_t s);
+                             int segw, int flagged, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -400,13 +413,32 @@ void launch_bi_shepard_compute(int, const int32_t*, const float*,
 
 namespace rt {
 
+CellArgs Sim::cell_args()
+{
+    return CellArgs{ (const uint32_t*)clist.icell->dev,
+                     (const uint32_t*)clist.ihoc->dev,
+                     (const uint32_t*)ptr(clist.n_cells),
+                     clist.cell_end,
+                     clist.heads,
+                     clist.n_heads,
+                     d_status,
+                     N,
+                     clist.kinds ? 1u : 0u };
+}
+
 void Sim::run_fused(hipStream_t s)
 {
     FusionPlan& f = fusion;
     auto P = [&](Var* v) { return v ? ptr(v) : nullptr; };
     const float* fr = (const float*)P(f.r);
     // non-fluid part of the Shepard tool first (sensors, elements)
-    if (f.shep == 2)
+    if (f.shep == 2 && bi_cells && clist.usable(f.icell, f.ihoc))
+        launch_bi_shepard_compute_cells(
+            dims, (const int32_t*)P(f.imove), fr, (const float*)P(f.normal),
+            (const float*)P(f.tangent), (const float*)P(f.binormal),
+            (const float*)P(f.m), (float*)P(f.shepard), cell_args(), kdefs,
+            support_threshold(kdefs.H), 1, s);
+    else if (f.shep == 2)
         launch_bi_shepard_compute(dims, (const int32_t*)P(f.imove), fr,
                                   (const float*)P(f.normal),
                                   (const float*)P(f.tangent),
@@ -424,7 +456,7 @@ void Sim::run_fused(hipStream_t s)
                        (const uint32_t*)P(f.n_cells), d_status, kdefs, 1 | 2,
                        s);
     if (f.timing) AQ_HIP(hipEventRecord(f.ev0, s));
-    if (f.cells && f.cell_end) {
+    if (clist.usable(f.icell, f.ihoc)) {
         launch_fluid_pass_cells(
             dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
             (const float*)P(f.u), (const float*)P(f.rho),
@@ -434,9 +466,10 @@ void Sim::run_fused(hipStream_t s)
             (float*)P(f.grad_p), (float*)P(f.lap_u), (float*)P(f.div_u),
             (float*)P(f.lap_p), f.near ? f.near_buf : nullptr, N,
             (const uint32_t*)P(f.icell), (const uint32_t*)P(f.ihoc),
-            (const uint32_t*)P(f.n_cells), f.cell_end, f.heads, f.n_heads,
+            (const uint32_t*)P(f.n_cells), clist.cell_end, clist.heads,
+            clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
-            f.walk_segw, s);
+            f.walk_segw, clist.kinds ? 1 : 0, s);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

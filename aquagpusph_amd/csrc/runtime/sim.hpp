@@ -212,19 +212,32 @@ struct FusionPlan {
         *lap_u = nullptr, *div_u = nullptr, *lap_p = nullptr,
         *icell = nullptr, *ihoc = nullptr, *n_cells = nullptr;
     uint8_t* near_buf = nullptr;
-    // non-empty cell list for the cell-parallel walk, filled by the link-list
-    // tool after the sort (cell_end over the ihoc capacity)
-    bool cells = false;
-    uint32_t* cell_end = nullptr;
-    uint32_t* heads = nullptr;
-    uint32_t* n_heads = nullptr;
-    void* cells_ws = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
     // profiling: events around the walk launch while `timing` is set
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double walk_ms = -1.0;  // mean of the last profile
+};
+
+// ---- non-empty cell list of the cell-parallel walks ---------------------------
+// Filled by the link-list tool that writes the "icell"/"ihoc" variables,
+// right after its sort; pair kernels bound to those variables then run the
+// cell-parallel walk instead of the per-particle one.
+struct CellList {
+    bool enabled = false;  // AQ_WALK != thread and N < 2^28
+    bool filled = false;   // the link-list tool has run once
+    bool kinds = false;    // cell_end carries the particle-kind flags
+    Var *icell = nullptr, *ihoc = nullptr, *n_cells = nullptr;
+    uint32_t* cell_end = nullptr;
+    size_t cap = 0;  // cells cell_end holds
+    uint32_t* heads = nullptr;
+    uint32_t* n_heads = nullptr;  // [0] count, [1] work counter
+    void* ws = nullptr;
+    bool usable(const Var* ic, const Var* ih) const
+    {
+        return enabled && filled && ic == icell && ih == ihoc;
+    }
 };
 
 // ---- the runtime ------------------------------------------------------------
@@ -259,7 +272,10 @@ class Sim {
     hipGraphExec_t graph_exec = nullptr;
     bool is_setup = false;
     FusionPlan fusion;
+    CellList clist;
+    bool bi_cells = false;  // boundary-integral element kernels cell-parallel
     void run_fused(hipStream_t s);
+    CellArgs cell_args();
 
     bool defined(const std::string& n) const
     {

@@ -19,7 +19,8 @@ size_t linklist_workspace(uint32_t N);
 size_t cell_list_workspace(uint32_t N);
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
-               void* ws, hipStream_t s);
+               void* ws, const uint32_t* perm, const int* imove,
+               hipStream_t s);
 void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                 unsigned key_bits, void* workspace, hipStream_t s);
 size_t radix_sort_workspace(uint32_t n);
@@ -277,19 +278,31 @@ class LinkListTool : public Tool {
                      (uint32_t*)perm_->dev, (uint32_t*)inv_->dev,
                      (uint32_t*)ihoc_->dev, (uint32_t)ihoc_->length,
                      S.d_status, ws_.p, s);
-        // the cell list of the cell-parallel walk (when that walk reads this
-        // link-list)
-        FusionPlan& f = S.fusion;
-        if (f.active && f.cells && f.icell == icell_ && f.ihoc == ihoc_) {
-            if (!f.cell_end) {
-                AQ_HIP(hipMalloc(&f.cell_end, ihoc_->length * 4));
-                AQ_HIP(hipMalloc(&f.heads, (size_t)S.N * 4));
-                AQ_HIP(hipMalloc(&f.n_heads, 256));
-                AQ_HIP(hipMalloc(&f.cells_ws, cell_list_workspace(S.N)));
-                AQ_HIP(hipMemset(f.n_heads, 0, 256));
+        // the non-empty cell list of the cell-parallel walks
+        CellList& c = S.clist;
+        if (c.enabled && c.icell == icell_ && c.ihoc == ihoc_) {
+            if (c.cap < ihoc_->length) {
+                // first step (eager) or a grown ihoc: never under capture
+                if (c.cell_end) AQ_HIP(hipFree(c.cell_end));
+                c.cell_end = nullptr;
+                AQ_HIP(hipMalloc(&c.cell_end, ihoc_->length * 4));
+                c.cap = ihoc_->length;
             }
-            cell_list((const uint32_t*)icell_->dev, S.N, f.cell_end, f.heads,
-                      f.n_heads, S.d_status, f.cells_ws, s);
+            if (!c.heads) {
+                AQ_HIP(hipMalloc(&c.heads, (size_t)S.N * 4));
+                AQ_HIP(hipMalloc(&c.n_heads, 256));
+                AQ_HIP(hipMalloc(&c.ws, cell_list_workspace(S.N)));
+                AQ_HIP(hipMemset(c.n_heads, 0, 256));
+            }
+            Var* im = S.vars.get("imove");
+            const bool kinds = im && im->is_array() && im->length >= S.N &&
+                               im->type.base == 'i';
+            cell_list((const uint32_t*)icell_->dev, S.N, c.cell_end, c.heads,
+                      c.n_heads, S.d_status, c.ws,
+                      kinds ? (const uint32_t*)perm_->dev : nullptr,
+                      kinds ? (const int*)im->dev : nullptr, s);
+            c.kinds = kinds;
+            c.filled = true;
         }
     }
 

@@ -53,6 +53,7 @@ struct WalkArgs {
     uint32_t* next;  // work counter: next cell to hand out (zero at launch)
     const uint32_t* status;
     uint32_t N;
+    uint32_t flagged;  // cell_end carries AQ_CELL_FLUID / AQ_CELL_ELEM flags
 };
 
 // Shared state of one wavefront: neighbour-cell table, the candidate chunk
@@ -66,10 +67,14 @@ template <int SEGW> struct WaveLds {
     uint32_t bits[2][SEGW][64];
 };
 
+// Candidate table of cell c_i: start and prefix count of each neighbour
+// cell in walk order.  Cells known to hold none of the `want` kinds
+// (AQ_CELL_FLUID / AQ_CELL_ELEM) count as empty: they have no candidate the
+// caller accepts, so the sums are unchanged.
 template <int D, class Lds>
 __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
                                                     uint32_t c_i, Lds& L,
-                                                    int lane)
+                                                    int lane, uint32_t want)
 {
     // lane k < ncell fills entry k; the prefix sum runs on lane 0
     constexpr int NC = NCell<D>::n;
@@ -88,8 +93,13 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
         const uint32_t c = c_i + (uint32_t)ci + (uint32_t)cj * nx +
                            (uint32_t)ck * nx * ny;
         const uint32_t s = w.ihoc[c];
+        uint32_t n = 0;
+        if (s < w.N) {
+            const uint32_t ce = w.cell_end[c];
+            if (!w.flagged || (ce & want)) n = (ce & AQ_CELL_END) - s;
+        }
         L.start[lane] = s;
-        L.cum[lane + 1] = (s < w.N) ? w.cell_end[c] - s : 0u;
+        L.cum[lane + 1] = n;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_s_waitcnt(0);
@@ -253,17 +263,16 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
         if (hk >= n_heads) break;
         const uint32_t cstart = uni(w.heads[hk]);
         const uint32_t c_i = uni(w.icell[cstart]);
-        const uint32_t cend = uni(w.cell_end[c_i]);
-        bool table = false;
-        uint32_t total = 0;
+        const uint32_t ce = uni(w.cell_end[c_i]);
+        const uint32_t cend = ce & AQ_CELL_END;
+        if (w.flagged && !(ce & AQ_CELL_FLUID)) continue;  // no fluid here
         for (uint32_t i0 = cstart; i0 < cend; i0 += 64) {
             const uint32_t i = i0 + lane;
             const bool act = (i < cend) && (imove[i] == 1);
             if (__ballot(act) == 0) continue;  // no fluid particle here
-            if (!table) {
-                total = neighbour_table<D>(w, c_i, L, lane);
-                table = true;
-            }
+            const uint32_t total = neighbour_table<D>(
+                w, c_i, L, lane,
+                AQ_CELL_FLUID | (flag_el ? AQ_CELL_ELEM : 0u));
             f3 r_i = { 0.f, 0.f, 0.f }, u_i = { 0.f, 0.f, 0.f };
             float p_i = 0.f, rho_i = 1.f;
             f3 gp = { 0.f, 0.f, 0.f }, lu = { 0.f, 0.f, 0.f };
@@ -312,8 +321,10 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             if (SHEP == 2 && __ballot(nearw) != 0) {
                 bool self_added = false;
                 bool unused = false;
+                const uint32_t total_el =
+                    neighbour_table<D>(w, c_i, L, lane, AQ_CELL_ELEM);
                 ring_walk<D, SEGW>(
-                    L, lane, total, r_i, act && nearw, smax, r, imove, 1,
+                    L, lane, total_el, r_i, act && nearw, smax, r, imove, 1,
                     false, unused, [&](uint32_t j) {
                         const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
                         const float s = dot<D>(r_ij, r_ij);
@@ -355,6 +366,257 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Generic cell-parallel pair kernel: one policy per reference kernel.
+//   Pol::KIND           candidates walked (0 fluid, 1 wall elements)
+//   pol.owner(i, im_i)  the particles that run the kernel (the reference
+//                       kernel's early exits)
+//   Pol::State          per-particle accumulators
+//   pol.init(i, r_i, st), pol.pair(i, j, r_i, st), pol.finish(i, st)
+// pair() is called for the accepted candidates of each particle in the
+// reference walk order, so every sum is bit-identical to the per-particle
+// kernel of sph.hip.
+// ---------------------------------------------------------------------------
+template <int D, int SEGW, class Pol>
+__global__ void __launch_bounds__(64 * WC_WAVES)
+    cell_kernel_k(Pol pol, const int* __restrict__ imove,
+                  const float* __restrict__ r, WalkArgs w, float smax)
+{
+    __shared__ WaveLds<SEGW> lds[WC_WAVES];
+    const int lane = threadIdx.x & 63;
+    WaveLds<SEGW>& L = lds[threadIdx.x >> 6];
+    if (w.status && (*w.status & 1u)) return;
+    const uint32_t n_heads = uni(*w.n_heads);
+    while (true) {
+        uint32_t hk = 0;
+        if (lane == 0) hk = atomicAdd(w.next, 1u);
+        hk = uni(hk);
+        if (hk >= n_heads) break;
+        const uint32_t cstart = uni(w.heads[hk]);
+        const uint32_t c_i = uni(w.icell[cstart]);
+        const uint32_t ce = uni(w.cell_end[c_i]);
+        const uint32_t cend = ce & AQ_CELL_END;
+        if (w.flagged && Pol::OWNERS && !(ce & Pol::OWNERS)) continue;
+        bool table = false;
+        uint32_t total = 0;
+        for (uint32_t i0 = cstart; i0 < cend; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const bool act = (i < cend) && pol.owner(i, imove[i]);
+            if (__ballot(act) == 0) continue;
+            if (!table) {
+                total = neighbour_table<D>(
+                    w, c_i, L, lane,
+                    Pol::KIND ? AQ_CELL_ELEM : AQ_CELL_FLUID);
+                table = true;
+            }
+            typename Pol::State st{};
+            f3 r_i = { 0.f, 0.f, 0.f };
+            if (act) {
+                r_i = ldv<D>(r, i);
+                pol.init(i, r_i, st);
+            }
+            bool unused = false;
+            ring_walk<D, SEGW>(L, lane, total, r_i, act, smax, r, imove,
+                               Pol::KIND, false, unused, [&](uint32_t j) {
+                                   pol.pair(i, j, r_i, st);
+                               });
+            if (act) pol.finish(i, st);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// cfd/Boundary/BI/Shepard.cl:61-150 (compute) for the non-fluid particles
+// (skip_fluid) or all of them
+template <int D> struct PolBiShepard {
+    static constexpr int KIND = 1;
+    static constexpr uint32_t OWNERS = 0;  // elements, sensors, ...
+    const int* imove;
+    const float *normal, *tangent, *binormal, *m;
+    float* shepard;
+    float H, CONW, DIMS;
+    int skip_fluid;
+    struct State {
+        float acc;
+        bool self_added;
+    };
+    __device__ bool owner(uint32_t, int im) const
+    {
+        return !((im < -3) || (im > 1)) && !(skip_fluid && im == 1);
+    }
+    __device__ void init(uint32_t, const f3&, State& st) const
+    {
+        st.acc = 1.f;
+        st.self_added = false;
+    }
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        if (i == j) {
+            if (!st.self_added) {
+                st.self_added = true;
+                st.acc -= 0.5f;
+            }
+            return;
+        }
+        const float eps = 1E-8f * H;
+        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        const float s = dot<D>(r_ij, r_ij);
+        const float q = sqrtf(s) / H;
+        const f3 n_j = ldv_o<D>(normal, j);
+        const f3 t_j = ldv_o<D>(tangent, j);
+        const f3 b_j = ldv_o<D>(binormal, j);
+        const float r_n = dot<D>(r_ij, n_j);
+        const float area_j = ldf_o(m, j);
+        if ((r_n > -eps) && (r_n < eps)) {
+            if (!st.self_added) {
+                const float r_t = fabsf(dot<D>(r_ij, t_j));
+                const float r_b = fabsf(dot<D>(r_ij, b_j));
+                const float dr = 0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                if ((r_t <= dr) && (r_b <= dr)) {
+                    st.self_added = true;
+                    st.acc -= 0.5f;
+                }
+            }
+            return;
+        }
+        const float r_t = fabsf(dot<D>(r_ij, t_j));
+        const float r_b = fabsf(dot<D>(r_ij, b_j));
+        st.acc += r_n * CONW * kernelS_P<D>(q) * area_j +
+                  kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        shepard[i] = st.acc;
+    }
+    const float* r_;
+};
+
+// cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
+template <int D> struct PolBiLapU {
+    static constexpr int KIND = 0;
+    static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
+    const float *r_, *u, *rho, *m;
+    float* lap_u;
+    float H, CONF;
+    int lap;
+    struct State {
+        f3 u_i, lu;
+    };
+    __device__ bool owner(uint32_t, int im) const { return im == -3; }
+    __device__ void init(uint32_t i, const f3&, State& st) const
+    {
+        st.u_i = ldv<D>(u, i);
+        st.lu = ldv<D>(lap_u, i);
+    }
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
+        if (i == j) return;
+        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        const float s = dot<D>(r_ij, r_ij);
+        const float q = sqrtf(s) / H;
+        const float rho_j = ldf_o(rho, j);
+        const float f_ij = kernelF<D>(q) * CONF * ldf_o(m, j);
+        const f3 u_ij = sub<D>(ldv_o<D>(u, j), st.u_i);
+        if (lap == AQ_LAP_MONAGHAN) {
+            const float udr = dot<D>(u_ij, r_ij);
+            const float r2 = (q * q + 0.01f) * H * H;
+            axpy<D>(st.lu, f_ij * CLEARY * udr / (r2 * rho_j), r_ij);
+        } else {
+            axpy<D>(st.lu, f_ij * 2.f / rho_j, u_ij);
+        }
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        stxyz<D>(lap_u, i, st.lu);
+    }
+};
+
+// cfd/Boundary/BI/Interpolation.cl:54-107
+template <int D> struct PolBiInterp {
+    static constexpr int KIND = 0;
+    static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
+    const float *r_, *m, *rho, *grad_p;
+    float* p;
+    float H, CONW;
+    struct State {
+        f3 gp_i;
+        float acc;
+    };
+    __device__ bool owner(uint32_t, int im) const { return im == -3; }
+    __device__ void init(uint32_t i, const f3&, State& st) const
+    {
+        const float rho_i = rho[i];
+        st.gp_i = ldv<D>(grad_p, i);
+        st.gp_i.x = rho_i * st.gp_i.x;
+        st.gp_i.y = rho_i * st.gp_i.y;
+        st.gp_i.z = rho_i * st.gp_i.z;
+        st.acc = 0.f;
+    }
+    __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        const float s = dot<D>(r_ij, r_ij);
+        const float q = sqrtf(s) / H;
+        const float w_ij =
+            kernelW<D>(q) * CONW * ldf_o(m, j) / ldf_o(rho, j);
+        st.acc += (ldf_o(p, j) - dot<D>(st.gp_i, r_ij)) * w_ij;
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        p[i] = st.acc;
+    }
+};
+
+// cfd/Boundary/BI/Interactions.cl:51-133, for the fluid particles with an
+// element inside the support (near flags of the fused walk) or all of them
+template <int D> struct PolBiInteractions {
+    static constexpr int KIND = 1;
+    static constexpr uint32_t OWNERS = AQ_CELL_FLUID;
+    const float *r_, *normal, *u, *rho, *m, *p;
+    float *grad_p, *div_u;
+    const uint8_t* near;
+    float H, CONW;
+    struct State {
+        f3 u_i, gp;
+        float p_i, rho_i, dv;
+    };
+    __device__ bool owner(uint32_t i, int im) const
+    {
+        return im == 1 && (!near || near[i]);
+    }
+    __device__ void init(uint32_t i, const f3&, State& st) const
+    {
+        st.u_i = ldv<D>(u, i);
+        st.p_i = p[i];
+        st.rho_i = rho[i];
+        st.gp = ldv<D>(grad_p, i);
+        st.dv = div_u[i];
+    }
+    __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        const float s = dot<D>(r_ij, r_ij);
+        const float q = sqrtf(s) / H;
+        const f3 n_j = ldv_o<D>(normal, j);
+        const float area_j = ldf_o(m, j);
+        const float p_j = ldf_o(p, j);
+        const f3 du = sub<D>(ldv_o<D>(u, j), st.u_i);
+        const float w_ij = kernelW<D>(q) * CONW * area_j;
+        axpy<D>(st.gp, (st.p_i + p_j) / st.rho_i * w_ij, n_j);
+        st.dv += st.rho_i * dot<D>(du, n_j) * w_ij;
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        stxyz<D>(grad_p, i, st.gp);
+        div_u[i] = st.dv;
+    }
+};
+
 __global__ void zero_u32_k(uint32_t* p)
 {
     if (threadIdx.x == 0) *p = 0u;
@@ -395,15 +657,15 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
-                             hipStream_t s)
+                             int flagged, hipStream_t s)
 {
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     // the work counter lives next to the non-empty cell count
     uint32_t* next = const_cast<uint32_t*>(n_heads) + 1;
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
-    WalkArgs w{ icell, ihoc,  n_cells, cell_end, heads,
-                n_heads, next, status,  N };
+    WalkArgs w{ icell,   ihoc, n_cells, cell_end, heads,
+                n_heads, next, status,  N,        (uint32_t)flagged };
 #define AQ_FC(DD, SH, LP, SW)                                                  \
     do {                                                                       \
         auto* ker = fluid_pass_cells_k<DD, SH, LP, SW>;                        \
@@ -429,6 +691,89 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     else AQ_FC_D(2, true);
 #undef AQ_FC_D
 #undef AQ_FC
+}
+
+
+// ---- launchers of the generic cell kernels ---------------------------------
+template <int D, class Pol>
+static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
+                         const CellArgs& c, float smax, hipStream_t s)
+{
+    AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
+    uint32_t* next = const_cast<uint32_t*>(c.n_heads) + 1;
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
+    WalkArgs w{ c.icell,   c.ihoc, c.n_cells, c.cell_end, c.heads,
+                c.n_heads, next,   c.status,  c.N,        c.flagged };
+    auto* ker = cell_kernel_k<D, 8, Pol>;
+    const unsigned nb = resident_blocks((const void*)ker, 64 * WC_WAVES);
+    hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
+                       r, w, smax);
+}
+
+void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
+                                     const float* r, const float* normal,
+                                     const float* tangent,
+                                     const float* binormal, const float* m,
+                                     float* shepard, const CellArgs& c,
+                                     aq_defs d, float smax, int skip_fluid,
+                                     hipStream_t s)
+{
+    if (dims == 2) {
+        PolBiShepard<2> p{ imove, normal, tangent, binormal, m, shepard,
+                           d.H,   d.CONW, d.DIMS,  skip_fluid, r };
+        launch_cells<2>(p, imove, r, c, smax, s);
+    } else {
+        PolBiShepard<3> p{ imove, normal, tangent, binormal, m, shepard,
+                           d.H,   d.CONW, d.DIMS,  skip_fluid, r };
+        launch_cells<3>(p, imove, r, c, smax, s);
+    }
+}
+
+void launch_bi_lapu_cells(int dims, const int32_t* imove, const float* r,
+                          const float* u, const float* rho, const float* m,
+                          float* lap_u, const CellArgs& c, aq_defs d,
+                          float smax, hipStream_t s)
+{
+    if (dims == 2) {
+        PolBiLapU<2> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
+        launch_cells<2>(p, imove, r, c, smax, s);
+    } else {
+        PolBiLapU<3> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
+        launch_cells<3>(p, imove, r, c, smax, s);
+    }
+}
+
+void launch_bi_interp_cells(int dims, const int32_t* imove, const float* r,
+                            const float* m, const float* rho,
+                            const float* grad_p, float* p, const CellArgs& c,
+                            aq_defs d, float smax, hipStream_t s)
+{
+    if (dims == 2) {
+        PolBiInterp<2> q{ r, m, rho, grad_p, p, d.H, d.CONW };
+        launch_cells<2>(q, imove, r, c, smax, s);
+    } else {
+        PolBiInterp<3> q{ r, m, rho, grad_p, p, d.H, d.CONW };
+        launch_cells<3>(q, imove, r, c, smax, s);
+    }
+}
+
+void launch_bi_interactions_cells(int dims, const int32_t* imove,
+                                  const float* r, const float* normal,
+                                  const float* u, const float* rho,
+                                  const float* m, const float* p,
+                                  float* grad_p, float* div_u,
+                                  const uint8_t* near, const CellArgs& c,
+                                  aq_defs d, float smax, hipStream_t s)
+{
+    if (dims == 2) {
+        PolBiInteractions<2> q{ r, normal, u, rho, m, p, grad_p, div_u,
+                                near, d.H, d.CONW };
+        launch_cells<2>(q, imove, r, c, smax, s);
+    } else {
+        PolBiInteractions<3> q{ r, normal, u, rho, m, p, grad_p, div_u,
+                                near, d.H, d.CONW };
+        launch_cells<3>(q, imove, r, c, smax, s);
+    }
 }
 
 }  // namespace aq
