@@ -18,8 +18,21 @@ PBI_PRESETS = ("resources/Presets/basic.xml",
                "resources/Presets/cfd/BI.xml",
                "resources/Presets/cfd/variableTimeStep.xml")
 
+PGP_PRESETS = ("resources/Presets/basic.xml",
+               "resources/Presets/basic/time_scheme/improved_euler.xml",
+               "resources/Presets/basic/domain.xml",
+               "resources/Presets/cfd.xml",
+               "resources/Presets/cfd/deltaSPH.xml",
+               "resources/Presets/cfd/GP.xml",
+               "resources/Presets/cfd/variableTimeStep.xml")
+
 FIELDS = ("r", "normal", "tangent", "u", "dudt", "rho", "drhodt", "m",
           "imove")
+
+
+def fields_of(case):
+    """Particle-file columns of a case (associations for ghost walls)."""
+    return FIELDS + (("associations",) if "associations" in case else ())
 
 
 def _f(x):
@@ -38,25 +51,27 @@ def write_particles(case, path):
     dims = case["dims"]
     vs = 2 if dims == 2 else 4
     N = int(case["N"])
+    fields = fields_of(case)
     cols = []
-    for k in FIELDS:
+    for k in fields:
         v = case.get(k)
         if k in ("r", "normal", "tangent", "u", "dudt"):
             a = np.zeros((N, vs), np.float32) if v is None else \
                 np.asarray(v, np.float32).reshape(N, vs)
             cols.append(a)
-        elif k == "imove":
+        elif k in ("imove", "associations"):
             cols.append(np.asarray(v, np.int64).reshape(N, 1))
         else:
             a = np.zeros(N, np.float32) if v is None else \
                 np.asarray(v, np.float32)
             cols.append(a.reshape(N, 1))
     fmt = []
-    for k, c in zip(FIELDS, cols):
-        fmt.append(" ".join(["%d" if k == "imove" else "%.9g"] * c.shape[1]))
+    for k, c in zip(fields, cols):
+        fmt.append(" ".join(["%d" if k in ("imove", "associations")
+                             else "%.9g"] * c.shape[1]))
     data = np.concatenate([c.astype(np.float64) for c in cols], axis=1)
     np.savetxt(path, data, fmt=", ".join(fmt),
-               header="aquagpusph_amd particles: " + ", ".join(FIELDS))
+               header="aquagpusph_amd particles: " + ", ".join(fields))
 
 
 def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml=""):
@@ -94,7 +109,7 @@ def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml=""):
     if load:
         write_particles(case, os.path.join(folder, "Fluid.dat"))
         lines.append('        <Load format="FastASCII" file="Fluid.dat" '
-                     'fields="%s" />' % ", ".join(FIELDS))
+                     'fields="%s" />' % ", ".join(fields_of(case)))
     lines.append("    </ParticlesSet>")
     if extra_xml:
         lines.append(extra_xml)
@@ -120,3 +135,5 @@ def upload(sim, case):
         if v is not None:
             sim.set(k, np.asarray(v, np.float32))
     sim.set("imove", np.asarray(case["imove"], np.int32))
+    if "associations" in case:
+        sim.set("associations", np.asarray(case["associations"], np.uint32))

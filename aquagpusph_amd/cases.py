@@ -31,12 +31,19 @@ def _lattice(*axes):
 
 def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
               refd=998.0, alpha=0.02, mu=8.9e-4, delta=0.1, courant=0.1,
-              walls=("bottom", "top", "left", "right", "front", "back")):
-    """Dam break with one layer of BI wall elements.
+              walls=("bottom", "top", "left", "right", "front", "back"),
+              bc="BI"):
+    """Dam break with one layer of wall elements.
 
     2D: tank [-B, L-B] x [0, D], column [-B, 0] x [0, H] (g along -y).
     3D: tank [-B, L-B] x [0, W] x [0, D], column [-B, 0] x [0, W] x [0, H]
         (g along -z).
+    bc="BI": boundary-integral elements (imove = -3).
+    bc="GP": mirroring elements (imove = -2) with ghost particles
+    (imove = -1) behind each, at 0.5 dr, 1.5 dr, ... < 2h along the outward
+    normal, associated with their element (the layout of the reference's
+    examples/2D/spheric_testcase6_movingsquare/src/Create.py:85-146);
+    "associations" holds N for every other particle.
     """
     if L is None:
         L = 1.61 if dims == 2 else 1.6
@@ -89,7 +96,19 @@ def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
         if "back" in walls:
             els.append((_lattice(xe, [Wt], de), (0, 1, 0), (1, 0, 0)))
     ne = sum(e[0].shape[0] for e in els)
-    N = nf + ne
+    ghosts = []  # (positions, normal, element index offset in els order)
+    if bc == "GP":
+        nl = int(math.ceil(2.0 * h / dr - 0.5))  # layers with d < 2h
+        layers = [(k + 0.5) * dr for k in range(nl) if (k + 0.5) * dr < 2.0 * h]
+        o = nf
+        for pos, n, t in els:
+            k = pos.shape[0]
+            nn = np.asarray(n[:dims], float)
+            for d in layers:
+                ghosts.append((pos + d * nn, n, t, np.arange(o, o + k)))
+            o += k
+    ng = sum(g[0].shape[0] for g in ghosts)
+    N = nf + ne + ng
 
     r = np.zeros((N, vs))
     normal = np.zeros((N, vs))
@@ -102,15 +121,25 @@ def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
         normal[o:o + k, :dims] = n[:dims]
         tangent[o:o + k, :dims] = t[:dims]
         o += k
-    imove = np.full(N, -3, np.int32)
+    assoc = np.full(N, N, np.uint32)
+    for pos, n, t, ref in ghosts:
+        k = pos.shape[0]
+        r[o:o + k, :dims] = pos
+        normal[o:o + k, :dims] = n[:dims]
+        tangent[o:o + k, :dims] = t[:dims]
+        assoc[o:o + k] = ref
+        o += k
+    imove = np.full(N, -3 if bc == "BI" else -2, np.int32)
     imove[:nf] = 1
+    imove[nf + ne:] = -1
     press = np.zeros(N)
     press[:nf] = refd * G * (H - fluid[:, vert])
     rho = refd + press / cs ** 2
     rho[nf:] = refd
     m = np.empty(N)
     m[:nf] = refd * dr ** dims
-    m[nf:] = dr ** (dims - 1)
+    m[nf:nf + ne] = dr ** (dims - 1)
+    m[nf + ne:] = refd * dr ** dims
 
     visc = max(alpha / (8.0 if dims == 2 else 10.0) * refd * h * cs, mu)
     g = [0.0] * dims
@@ -123,10 +152,38 @@ def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
                   visc_dyn=[visc], delta=[delta], p0=0.0,
                   dt_min=0.25 * courant * h / cs, dt_Ma=0.1,
                   domain_min=lo, domain_max=hi, dr=dr, hfac=hfac)
-    return dict(dims=dims, N=N, n_fluid=nf, r=r, normal=normal,
-                tangent=tangent, u=np.zeros((N, vs)), dudt=np.zeros((N, vs)),
-                rho=rho, drhodt=np.zeros(N), m=m, imove=imove,
-                iset=np.zeros(N, np.uint32), params=params)
+    out = dict(dims=dims, N=N, n_fluid=nf, r=r, normal=normal,
+               tangent=tangent, u=np.zeros((N, vs)), dudt=np.zeros((N, vs)),
+               rho=rho, drhodt=np.zeros(N), m=m, imove=imove,
+               iset=np.zeros(N, np.uint32), params=params, bc=bc)
+    if bc == "GP":
+        out["associations"] = assoc
+    return out
+
+
+def still_tank(dims, dr, L=0.3, H=0.15, W=0.2, hfac=2.0, refd=998.0,
+               alpha=0.02, mu=8.9e-4, delta=0.1, courant=0.1, bc="GP",
+               u0=0.0):
+    """Water at rest filling a tank [0, L] x [0, H] (x [0, W] in 3D) up to
+    its open top: walls on the bottom and on the sides up to the free
+    surface, hydrostatic pressure.  The fluid stays at rest, which checks
+    the wall pressure extension (dp/dn = rho g.n).  u0 gives the fluid a
+    uniform horizontal velocity (a tangential wall flow)."""
+    case = dam_break(dims, dr, L=L + 0.0, D=H, B=L, H=H, W=W, hfac=hfac,
+                     refd=refd, alpha=alpha, mu=mu, delta=delta,
+                     courant=courant,
+                     walls=("bottom", "left", "right", "front", "back"),
+                     bc=bc)
+    # shift so that the tank is [0, L]: dam_break puts the column at [-B, 0]
+    vs = 2 if dims == 2 else 4
+    case["r"][:, 0] += L
+    case["params"]["domain_min"][0] += L
+    case["params"]["domain_max"][0] += L
+    if u0:
+        fl = case["imove"] == 1
+        case["u"][fl, 0] = u0
+    case["params"]["still_tank"] = True
+    return case
 
 
 def dam_break_dr_for(n_fluid, dims=3, B=0.6, H=0.3, W=0.6):

@@ -98,6 +98,39 @@ void launch_bi_interactions_cells(int, const int32_t*, const float*,
                                   const float*, const float*, float*, float*,
                                   const uint8_t*, const CellArgs&, aq_defs,
                                   float, hipStream_t);
+void launch_gp_sort(const uint32_t*, uint32_t*, const uint32_t*, uint32_t,
+                    hipStream_t);
+void launch_gp_mirror(int, const int32_t*, const uint32_t*, const float*,
+                      float*, uint32_t*, uint32_t, const float*,
+                      const uint32_t*, float, float, hipStream_t);
+void launch_gp_interp(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, const float*, float*,
+                      float*, float*, float*, uint32_t, const uint32_t*,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      const uint32_t*, aq_defs, hipStream_t);
+void launch_gp_renorm(int, const int32_t*, const float*, float*, float*,
+                      float*, uint32_t, hipStream_t);
+void launch_gp_prelapu(int, const int32_t*, const uint32_t*, const float*,
+                       const float*, const float*, float*, float*, uint32_t,
+                       int, hipStream_t);
+void launch_gp_freeslip(int, const uint32_t*, const int32_t*,
+                        const uint32_t*, const float*, float*, uint32_t,
+                        const uint32_t*, hipStream_t);
+void launch_gp_lapu(int, const int32_t*, const float*, const float*,
+                    const float*, const float*, float*, uint32_t,
+                    const uint32_t*, const uint32_t*, const uint32_t*,
+                    const uint32_t*, aq_defs, hipStream_t);
+void launch_gp_preinteractions(int, const uint32_t*, const int32_t*,
+                               const uint32_t*, const float*, const float*,
+                               const float*, const float*, const float*,
+                               const float*, const float*, float*, float*,
+                               const float*, uint32_t, const float*,
+                               hipStream_t);
+void launch_gp_interactions(int, const int32_t*, const float*, const float*,
+                            const float*, const float*, const float*, float*,
+                            float*, uint32_t, const uint32_t*,
+                            const uint32_t*, const uint32_t*,
+                            const uint32_t*, aq_defs, hipStream_t);
 float support_threshold(float H);
 void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
                    const uint32_t*, const uint32_t*, const uint32_t*,
@@ -119,6 +152,19 @@ static void gather(Sim& S, Var* src, Var* dst, const uint32_t* perm,
                    hipStream_t s)
 {
     permute(src->dev, dst->dev, src->type.elem_bytes(), perm, S.N, s);
+}
+
+// __GP_LAPU_MODEL__ names another definition (__GP_LAPU_TAKEDA__ = 1 or
+// __GP_LAPU_U0M__ = 2, cfd/GP.xml): follow the chain to the number.
+static int gp_lapu_model(Sim& S)
+{
+    if (!S.defined("__GP_LAPU_MODEL__")) return 2;
+    std::string v = S.def("__GP_LAPU_MODEL__");
+    for (int k = 0; k < 4 && S.defined(v); k++) v = S.def(v);
+    const int m = atoi(v.c_str());
+    if (m != 1 && m != 2)
+        throw Error(AQ_EUNSUP, "unknown __GP_LAPU_MODEL__ " + v);
+    return m;
 }
 
 static const std::map<std::string, KernelSpec>& table()
@@ -322,6 +368,69 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_bi_interp_shepard(U(0), I(1), F(2), F(3), F(4), F(5),
                                          S.N, F(6), F(7), s);
+            } } },
+        // ---- ghost particles ---------------------------------------------
+        { "cfd/Boundary/GP/Sort.cl:entry",
+          { { "associations_in", "associations", "id_sorted" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_sort(U(0), U(1), U(2), S.N, s);
+            } } },
+        { "cfd/Boundary/GP/Mirror.cl:entry",
+          { { "imove", "associations", "normal", "r", "gp_icell", "r_min",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_mirror(S.dims, I(0), U(1), F(2), F(3), U(4), S.N,
+                                 F(5), U(6), S.kdefs.H, S.def_float("SUPPORT"),
+                                 s);
+            } } },
+        { "cfd/Boundary/GP/Interpolation.cl:entry",
+          { { "imove", "r", "m", "rho", "p", "u", "gp_rho", "gp_p", "gp_u",
+              "shepard", "gp_icell", "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_interp(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 F(6), F(7), F(8), F(9), S.N, U(10), U(11),
+                                 U(12), U(13), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Boundary/GP/Renormalization.cl:entry",
+          { { "imove", "shepard", "gp_rho", "gp_p", "gp_u" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_renorm(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, s);
+            } } },
+        { "cfd/Boundary/GP/PreLapU.cl:entry",
+          { { "imove", "associations", "normal", "gp_rho", "gp_u", "rho",
+              "u" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_prelapu(S.dims, I(0), U(1), F(2), F(3), F(4), F(5),
+                                  F(6), S.N, gp_lapu_model(S), s);
+            } } },
+        { "cfd/Boundary/GP/FreeSlip.cl:entry",
+          { { "iset", "imove", "associations", "gp_u", "u", "freeslip_iset" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_freeslip(S.dims, U(0), I(1), U(2), F(3), F(4), S.N,
+                                   U(5), s);
+            } } },
+        { "cfd/Boundary/GP/LapU.cl:entry",
+          { { "imove", "r", "u", "rho", "m", "lap_u", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_lapu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                               S.N, U(6), U(7), U(8), S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Boundary/GP/PreInteractions.cl:entry",
+          { { "iset", "imove", "associations", "r", "normal", "dudt",
+              "lap_u", "gp_p", "gp_u", "rho", "p", "u", "visc_dyn", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_preinteractions(S.dims, U(0), I(1), U(2), F(3),
+                                          F(4), F(5), F(6), F(7), F(8), F(9),
+                                          F(10), F(11), F(12), S.N, F(13), s);
+            } } },
+        { "cfd/Boundary/GP/Interactions.cl:entry",
+          { { "imove", "r", "u", "rho", "m", "p", "grad_p", "div_u", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_gp_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                       F(5), F(6), F(7), S.N, U(8), U(9),
+                                       U(10), S.d_status, S.kdefs, s);
             } } },
         { "cfd/Boundary/BI/Interactions.cl:entry",
           { { "imove", "r", "normal", "u", "rho", "m", "p", "grad_p", "div_u",

@@ -9,7 +9,7 @@
 //
 //   test     the lanes load 64 candidates at a time (coalesced) into LDS;
 //            the candidate loop reads them back by broadcast and every lane
-//            shifts in one accept bit per candidate (q <= 2 and the wanted
+//            shifts in one accept bit per candidate (q < 2 and the wanted
 //            particle type);
 //   consume  every lane walks its own accepted candidates in increasing
 //            order and evaluates the pair terms.  A lane that finishes a

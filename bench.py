@@ -90,7 +90,7 @@ PROFILE_DIR = os.path.join(HERE, "profiles")
 def pair_counts(case, sample=200000):
     """Candidates (particles in the 3^d cells around a fluid particle's cell,
     what the walk tests) exactly from the cell occupancy, and accepted
-    fluid-fluid pairs (q <= 2, j != i) and fluid-element pairs from a
+    fluid-fluid pairs (q < 2, j != i) and fluid-element pairs from a
     strided sample of the fluid particles (k-d tree counts)."""
     from scipy.spatial import cKDTree
     dims = case["dims"]
@@ -111,7 +111,7 @@ def pair_counts(case, sample=200000):
         cand += occ[np.ravel_multi_index((ijk[fl] + o).T, dim)]
     step = max(1, len(fl) // sample)
     q = r[fl[::step]]
-    rad = 2.0 * h * (1 + 1e-6)
+    rad = 2.0 * h * (1 - 1e-6)
     t_f = cKDTree(r[im == 1])
     n_f = t_f.query_ball_point(q, rad, return_length=True, workers=-1) - 1
     n_e = np.zeros_like(n_f)

@@ -31,6 +31,12 @@ ORC_EXPORT void orc_scatter(const void* src, void* dst, size_t esize,
     scatter(src, dst, esize, id_sorted, N);
 }
 
+ORC_EXPORT void orc_gp_sort(const usize* assoc_in, usize* assoc,
+                            const usize* id_sorted, usize N)
+{
+    gp_sort(assoc_in, assoc, id_sorted, N);
+}
+
 ORC_EXPORT void orc_neighs_2d(const int* imove, usize* n_neighs, usize limit,
                               usize N, const usize* ic, const usize* hoc,
                               const usize* nc)
@@ -202,6 +208,67 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         bi_interactions<D, R>(imove, r, normal, u, rho, m, p, grad_p, div_u,   \
                               N, LL{ ic, hoc, nc }, H, CONW);                  \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_mirror_##SR##_##D##d(                               \
+        const int* imove, const usize* assoc, const R* normal, R* r,           \
+        usize* gp_icell, usize N, const R* rmin, const usize* nc, R H,         \
+        R SUPPORT)                                                             \
+    {                                                                          \
+        gp_mirror<D, R>(imove, assoc, normal, r, gp_icell, N, rmin, nc, H,     \
+                        SUPPORT);                                              \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_interp_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* m, const R* rho, const R* p,    \
+        const R* u, R* gp_rho, R* gp_p, R* gp_u, R* shepard, usize N,          \
+        const usize* gp_icell, const usize* ic, const usize* hoc,              \
+        const usize* nc, R H, R CONW)                                          \
+    {                                                                          \
+        gp_interp<D, R>(imove, r, m, rho, p, u, gp_rho, gp_p, gp_u, shepard,   \
+                        N, gp_icell, LL{ ic, hoc, nc }, H, CONW);              \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_renorm_##SR##_##D##d(const int* imove,             \
+                                                const R* shepard, R* gp_rho,  \
+                                                R* gp_p, R* gp_u, usize N)    \
+    {                                                                          \
+        gp_renorm<D, R>(imove, shepard, gp_rho, gp_p, gp_u, N);                \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_prelapu_##SR##_##D##d(                              \
+        const int* imove, const usize* assoc, const R* normal,                 \
+        const R* gp_rho, const R* gp_u, R* rho, R* u, usize N, int model)      \
+    {                                                                          \
+        gp_prelapu<D, R>(imove, assoc, normal, gp_rho, gp_u, rho, u, N,        \
+                         model);                                               \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_freeslip_##SR##_##D##d(                             \
+        const unsigned* iset, const int* imove, const usize* assoc,            \
+        const R* gp_u, R* u, usize N, unsigned fs)                             \
+    {                                                                          \
+        gp_freeslip<D, R>(iset, imove, assoc, gp_u, u, N, fs);                 \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_lapu_##SR##_##D##d(                                 \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        R* lap_u, usize N, const usize* ic, const usize* hoc,                  \
+        const usize* nc, R H, R CONF, int lap_form)                            \
+    {                                                                          \
+        gp_lapu<D, R>(imove, r, u, rho, m, lap_u, N, LL{ ic, hoc, nc }, H,     \
+                      CONF, lap_form);                                         \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_preinteractions_##SR##_##D##d(                      \
+        const unsigned* iset, const int* imove, const usize* assoc,            \
+        const R* r, const R* normal, const R* dudt, const R* lap_u,            \
+        const R* gp_p, const R* gp_u, const R* rho, R* p, R* u,                \
+        const R* visc_dyn, usize N, const R* g)                                \
+    {                                                                          \
+        gp_preinteractions<D, R>(iset, imove, assoc, r, normal, dudt, lap_u,   \
+                                 gp_p, gp_u, rho, p, u, visc_dyn, N, g);       \
+    }                                                                          \
+    ORC_EXPORT void orc_gp_interactions_##SR##_##D##d(                         \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        const R* p, R* grad_p, R* div_u, usize N, const usize* ic,             \
+        const usize* hoc, const usize* nc, R H, R CONF)                        \
+    {                                                                          \
+        gp_interactions<D, R>(imove, r, u, rho, m, p, grad_p, div_u, N,        \
+                              LL{ ic, hoc, nc }, H, CONF);                     \
     }
 
 ORC_DEFINE(float, f32, 2)

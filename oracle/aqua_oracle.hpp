@@ -861,6 +861,279 @@ void bi_interactions(const int* imove, const R* r, const R* normal,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Ghost-particle walls (resources/Scripts/cfd/Boundary/GP/*.cl): ghosts are
+// imove == -1, their mirroring elements imove == -2 (unit outward normal,
+// area in m), associations[i] the element of ghost i (>= N: none).
+// ---------------------------------------------------------------------------
+
+// The walk of a particle whose cell is given explicitly (GP/Interpolation.cl
+// walks around the mirrored position, gp_icell).  Neighbour cells outside
+// the grid (c_j >= n_cells.w, e.g. a mirrored point beyond the particle
+// bounding box) are skipped; the reference reads ihoc out of range there.
+template <int D, class F>
+inline void walk_cell(usize c_i, usize N, const usize* nc, const usize* icell,
+                      const usize* ihoc, F f)
+{
+    for (int ci = -1; ci <= 1; ci++)
+        for (int cj = -1; cj <= 1; cj++)
+            for (int ck = (D == 3 ? -1 : 0); ck <= (D == 3 ? 1 : 0); ck++) {
+                const usize c_j = c_i + (usize)ci + (usize)cj * nc[0] +
+                                  (usize)ck * nc[0] * nc[1];
+                if (c_j >= nc[3]) continue;
+                usize j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    f(j);
+                    j++;
+                }
+            }
+}
+
+// GP/Sort.cl:35-53: associations into the new sorted order
+inline void gp_sort(const usize* assoc_in, usize* assoc, const usize* id_sorted,
+                    usize N)
+{
+    for (usize i = 0; i < N; i++) {
+        const usize i_out = id_sorted[i];
+        assoc[i_out] = (assoc_in[i] >= N) ? N : id_sorted[assoc_in[i]];
+    }
+}
+
+// GP/Mirror.cl:47-90: reflect each ghost across its element and hash the
+// mirrored point (iCell formula with the SUPPORT and H definitions)
+template <int D, typename R>
+void gp_mirror(const int* imove, const usize* assoc, const R* normal, R* r,
+               usize* gp_icell, usize N, const R* rmin, const usize* nc, R H,
+               R SUPPORT)
+{
+    const int VS = Dim<D>::VS;
+    const R idist = R(1) / (SUPPORT * H);
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -1) continue;
+        const usize iref = assoc[i];
+        if (iref >= N) continue;
+        const V<D, R> r_ref = ld<D, R>(r, iref);
+        const V<D, R> n_ref = ld<D, R>(normal, iref);
+        const R d = dot<D, R>(sub<D, R>(r_ref, ld<D, R>(r, i)), n_ref);
+        for (int k = 0; k < D; k++)
+            r[(size_t)i * VS + k] += R(2.f) * (d * n_ref.c[k]);
+        const usize cx = cvt_u32((r[(size_t)i * VS + 0] - rmin[0]) * idist) + 3u;
+        const usize cy = cvt_u32((r[(size_t)i * VS + 1] - rmin[1]) * idist) + 3u;
+        if (D == 3) {
+            const usize cz =
+                cvt_u32((r[(size_t)i * VS + 2] - rmin[2]) * idist) + 3u;
+            gp_icell[i] = cx - 1u + (cy - 1u) * nc[0] + (cz - 1u) * nc[0] * nc[1];
+        } else {
+            gp_icell[i] = cx - 1u + (cy - 1u) * nc[0];
+        }
+    }
+}
+
+// GP/Interpolation.cl:58-140: SPH interpolation of rho, p, u (and the
+// Shepard sum) at the mirrored ghost position
+template <int D, typename R>
+void gp_interp(const int* imove, const R* r, const R* m, const R* rho,
+               const R* p, const R* u, R* gp_rho, R* gp_p, R* gp_u,
+               R* shepard, usize N, const usize* gp_icell, LL ll, R H, R CONW)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != -1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R a_rho = gp_rho[i], a_p = gp_p[i], sh = R(0);
+        V<D, R> a_u = ld<D, R>(gp_u, i);
+        walk_cell<D>(gp_icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R w_ij = kernelW<D, R>(q) * CONW * m[j] / rho_j;
+            a_rho += w_ij * rho_j;
+            a_p += w_ij * p[j];
+            const V<D, R> u_j = ld<D, R>(u, j);
+            for (int k = 0; k < D; k++) a_u.c[k] += w_ij * u_j.c[k];
+            sh += w_ij;
+        });
+        gp_rho[i] = a_rho;
+        gp_p[i] = a_p;
+        for (int k = 0; k < D; k++) gp_u[(size_t)i * VS + k] = a_u.c[k];
+        shepard[i] = sh;
+    }
+}
+
+// GP/Renormalization.cl:44-69 (the whole vec, w included, is divided)
+template <int D, typename R>
+void gp_renorm(const int* imove, const R* shepard, R* gp_rho, R* gp_p,
+               R* gp_u, usize N)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -1) continue;
+        R s = shepard[i];
+        if (s < R(1.0E-6f)) s = R(1);
+        gp_p[i] /= s;
+        gp_rho[i] /= s;
+        for (int k = 0; k < VS; k++) gp_u[(size_t)i * VS + k] /= s;
+    }
+}
+
+// GP/PreLapU.cl:51-90: interpolated density; the velocity for the wall
+// Laplacian by the Takeda extension (model 1) or frozen (model 2)
+template <int D, typename R>
+void gp_prelapu(const int* imove, const usize* assoc, const R* normal,
+                const R* gp_rho, const R* gp_u, R* rho, R* u, usize N,
+                int model)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -1) continue;
+        const usize iref = assoc[i];
+        if (iref >= N) continue;
+        rho[i] = gp_rho[i];
+        if (model != 1) continue;
+        const V<D, R> n = ld<D, R>(normal, iref);
+        const V<D, R> ur = ld<D, R>(u, iref);
+        const R dr = dot<D, R>(ur, n);
+        const V<D, R> gu = ld<D, R>(gp_u, i);
+        const R dg = dot<D, R>(gu, n);
+        for (int k = 0; k < D; k++) {
+            const R u_ref = ur.c[k] - dr * n.c[k];
+            const R u_n = dg * n.c[k];
+            const R u_t = R(2.f) * u_ref - (gu.c[k] - u_n);
+            u[(size_t)i * VS + k] = u_n + u_t;
+        }
+    }
+}
+
+// GP/FreeSlip.cl:46-66
+template <int D, typename R>
+void gp_freeslip(const unsigned* iset, const int* imove, const usize* assoc,
+                 const R* gp_u, R* u, usize N, unsigned freeslip_iset)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if ((imove[i] != -1) || (iset[i] != freeslip_iset)) continue;
+        if (assoc[i] >= N) continue;
+        for (int k = 0; k < D; k++)
+            u[(size_t)i * VS + k] = gp_u[(size_t)i * VS + k];
+    }
+}
+
+// GP/LapU.cl:58-127: fluid (and elements) against ghosts (and fluid), the
+// fluid-fluid and ghost-ghost pairs excluded
+template <int D, typename R>
+void gp_lapu(const int* imove, const R* r, const R* u, const R* rho,
+             const R* m, R* lap_u, usize N, LL ll, R H, R CONF, int lap_form)
+{
+    const int VS = Dim<D>::VS;
+    const R CLEARY = (D == 2) ? R(8.f) : R(10.f);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        const int im_i = imove[i];
+        if ((im_i != 1) && (im_i != -2)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> u_i = ld<D, R>(u, i);
+        const R rho_i = rho[i];
+        V<D, R> lu = ld<D, R>(lap_u, i);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            const int im_j = imove[j];
+            if ((im_j != 1) && (im_j != -1)) return;
+            if (im_i == im_j) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const V<D, R> u_ij = sub<D, R>(ld<D, R>(u, j), u_i);
+            const R udr = dot<D, R>(u_ij, r_ij);
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j];
+            if (lap_form == 1) {
+                const R r2 = (q * q + R(0.01f)) * H * H;
+                const R sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
+                for (int k = 0; k < D; k++) lu.c[k] += sl * r_ij.c[k];
+            } else {
+                const R sl = f_ij * R(2.f) / (rho_i * rho_j);
+                for (int k = 0; k < D; k++) lu.c[k] += sl * u_ij.c[k];
+            }
+        });
+        for (int k = 0; k < D; k++) lap_u[(size_t)i * VS + k] = lu.c[k];
+    }
+}
+
+// GP/PreInteractions.cl:66-104: ghost velocity with the mirrored normal
+// component, ghost pressure extended with the element's normal gradient
+// dp/dn = rho (g - dudt + mu lap_u) . n
+template <int D, typename R>
+void gp_preinteractions(const unsigned* iset, const int* imove,
+                        const usize* assoc, const R* r, const R* normal,
+                        const R* dudt, const R* lap_u, const R* gp_p,
+                        const R* gp_u, const R* rho, R* p, R* u,
+                        const R* visc_dyn, usize N, const R* g)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -1) continue;
+        const usize iref = assoc[i];
+        if (iref >= N) continue;
+        const V<D, R> n = ld<D, R>(normal, iref);
+        const V<D, R> gu = ld<D, R>(gp_u, i);
+        const R dg = dot<D, R>(gu, n);
+        const R dr = dot<D, R>(ld<D, R>(u, iref), n);
+        for (int k = 0; k < D; k++) {
+            const R u_n = dg * n.c[k];
+            const R u_t = gu.c[k] - u_n;
+            u[(size_t)i * VS + k] = R(2.f) * dr * n.c[k] - u_n + u_t;
+        }
+        const R mu = visc_dyn[iset[iref]];
+        V<D, R> a;
+        for (int k = 0; k < D; k++)
+            a.c[k] = g[k] - dudt[(size_t)iref * VS + k] +
+                     mu * lap_u[(size_t)iref * VS + k];
+        if (D == 2) a.c[2] = R(0);
+        const R dpdn = rho[iref] * dot<D, R>(a, n);
+        const R dn = dot<D, R>(sub<D, R>(ld<D, R>(r, i), ld<D, R>(r, iref)), n);
+        p[i] = gp_p[i] + R(2.f) * dpdn * dn;
+    }
+}
+
+// GP/Interactions.cl:62-132: fluid against ghosts
+template <int D, typename R>
+void gp_interactions(const int* imove, const R* r, const R* u, const R* rho,
+                     const R* m, const R* p, R* grad_p, R* div_u, usize N,
+                     LL ll, R H, R CONF)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> u_i = ld<D, R>(u, i);
+        const R p_i = p[i];
+        const R rho_i = rho[i];
+        V<D, R> gp = ld<D, R>(grad_p, i);
+        R dv = div_u[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R p_j = p[j];
+            const R udr = dot<D, R>(sub<D, R>(ld<D, R>(u, j), u_i), r_ij);
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j];
+            const R sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
+            for (int k = 0; k < D; k++) gp.c[k] += sg * r_ij.c[k];
+            dv += udr * f_ij * rho_i / rho_j;
+        });
+        for (int k = 0; k < D; k++) grad_p[(size_t)i * VS + k] = gp.c[k];
+        div_u[i] = dv;
+    }
+}
+
 // basic/neighs.cl:52-91: every candidate of the 3^d cells (self included,
 // no distance or type filter); the walk stops as soon as the count reaches
 // neighs_limit; buffer particles (imove <= -255) get 0.

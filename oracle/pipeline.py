@@ -8,7 +8,11 @@ them in the tool order that the reference presets resolve to
 (resources/Presets/src/basic.xml:139-175 with the insert semantics of
 aquagpusph/InputOutput/State.cpp:803-1033), for the preset stack
 
-    basic + improved_euler + [domain] + cfd + [deltaSPH] + [BI] + [variableTimeStep]
+    basic + improved_euler + [domain] + cfd + [deltaSPH] + [BI | GP]
+          + [variableTimeStep]
+
+(the ghost-particle order of resources/Presets/src/cfd/GP.xml:50-80 when the
+case carries "associations").
 
 Scalars follow the reference's host semantics: set_scalar expressions are
 evaluated in double and narrowed to the variable type; preset definitions
@@ -55,15 +59,20 @@ class OracleSim:
     VEC_FIELDS = ("r", "normal", "tangent", "binormal", "u", "dudt")
     SCA_FIELDS = ("rho", "drhodt", "m", "p")
 
-    def __init__(self, case, dtype=np.float32, domain=True, bi=True,
-                 delta=True, variable_dt=True, lap_form=1):
+    def __init__(self, case, dtype=np.float32, domain=True, bi=None,
+                 delta=True, variable_dt=True, lap_form=1, gp_model=2,
+                 freeslip_iset=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
         self.sfx = ("f32" if self.dtype == np.float32 else "f64") + \
             "_%dd" % self.dims
-        self.opts = dict(domain=domain, bi=bi, delta=delta,
-                         variable_dt=variable_dt, lap_form=lap_form)
+        gp = "associations" in case
+        if bi is None:
+            bi = not gp
+        self.opts = dict(domain=domain, bi=bi, gp=gp, delta=delta,
+                         variable_dt=variable_dt, lap_form=lap_form,
+                         gp_model=gp_model, freeslip_iset=freeslip_iset)
         N = self.N = int(case["N"])
         R = self.dtype
         f = {}
@@ -91,6 +100,15 @@ class OracleSim:
             f[k] = np.zeros(N, R)
         for k in ("icell", "id_sorted", "id_unsorted"):
             f[k] = np.zeros(N, np.uint32)
+        if gp:
+            f["associations"] = np.ascontiguousarray(
+                case["associations"], np.uint32).copy()
+            f["associations_in"] = np.zeros(N, np.uint32)
+            f["gp_icell"] = np.zeros(N, np.uint32)
+            f["gp_rho"] = np.zeros(N, R)
+            f["gp_p"] = np.zeros(N, R)
+            for k in ("gp_u", "gp_r_in", "gp_u_in"):
+                f[k] = np.zeros((N, self.vs), R)
         self.f = f
         self.ihoc = np.zeros(0, np.uint32)
         self.n_cells = np.zeros(4, np.uint32)
@@ -203,6 +221,11 @@ class OracleSim:
         k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
                  _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
         k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
+        if o["gp"]:
+            # "Backup associations" + "Sort associations" (GP/Sort.cl)
+            f["associations_in"][...] = f["associations"]
+            lib().orc_gp_sort(_p(f["associations_in"]), _p(f["associations"]),
+                              _p(f["id_sorted"]), n)
         for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
             f[key][...] = 0
         ll = self._ll()
@@ -215,6 +238,24 @@ class OracleSim:
         else:
             k("shepard")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
                          _p(f["shepard"]), n, *ll, H, CONW, ctypes.c_int(1))
+        if o["gp"]:
+            # mirror, interpolate, renormalise, unmirror (GP.xml:58-70)
+            f["gp_r_in"][...] = f["r"]
+            k("gp_mirror")(_p(f["imove"]), _p(f["associations"]),
+                           _p(f["normal"]), _p(f["r"]), _p(f["gp_icell"]), n,
+                           _p(self.r_min), _p(self.n_cells), H,
+                           R(self.support))
+            for key in ("gp_rho", "gp_p", "gp_u"):
+                f[key][...] = 0
+            k("gp_interp")(_p(f["imove"]), _p(f["r"]), _p(f["m"]),
+                           _p(f["rho"]), _p(f["p"]), _p(f["u"]),
+                           _p(f["gp_rho"]), _p(f["gp_p"]), _p(f["gp_u"]),
+                           _p(f["shepard"]), n, _p(f["gp_icell"]), *ll, H,
+                           CONW)
+            k("gp_renorm")(_p(f["imove"]), _p(f["shepard"]), _p(f["gp_rho"]),
+                           _p(f["gp_p"]), _p(f["gp_u"]), n)
+            f["r"][...] = f["gp_r_in"]
+            f["gp_u_in"][...] = f["u"]
         k("interactions")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
                           _p(f["rho"]), _p(f["m"]), _p(f["p"]),
                           _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]), n,
@@ -226,6 +267,29 @@ class OracleSim:
         if o["delta"]:
             k("lapp")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
                       _p(f["p"]), _p(f["lap_p"]), n, *ll, H, CONF)
+        if o["gp"]:
+            k("gp_prelapu")(_p(f["imove"]), _p(f["associations"]),
+                            _p(f["normal"]), _p(f["gp_rho"]), _p(f["gp_u"]),
+                            _p(f["rho"]), _p(f["u"]), n,
+                            ctypes.c_int(o["gp_model"]))
+            if o["freeslip_iset"] is not None:
+                k("gp_freeslip")(_p(f["iset"]), _p(f["imove"]),
+                                 _p(f["associations"]), _p(f["gp_u"]),
+                                 _p(f["u"]), n,
+                                 ctypes.c_uint(o["freeslip_iset"]))
+            k("gp_lapu")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["rho"]),
+                         _p(f["m"]), _p(f["lap_u"]), n, *ll, H, CONF,
+                         ctypes.c_int(o["lap_form"]))
+            k("gp_preinteractions")(
+                _p(f["iset"]), _p(f["imove"]), _p(f["associations"]),
+                _p(f["r"]), _p(f["normal"]), _p(f["dudt"]), _p(f["lap_u"]),
+                _p(f["gp_p"]), _p(f["gp_u"]), _p(f["rho"]), _p(f["p"]),
+                _p(f["u"]), _p(self.sets["visc_dyn"]), n, _p(self.g))
+            k("gp_interactions")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                                 _p(f["rho"]), _p(f["m"]), _p(f["p"]),
+                                 _p(f["grad_p"]), _p(f["div_u"]), n, *ll, H,
+                                 CONF)
+            f["u"][...] = f["gp_u_in"]
         if o["bi"]:
             k("bi_lapu")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["rho"]),
                          _p(f["m"]), _p(f["lap_u"]), n, *ll, H, CONF,

@@ -163,3 +163,26 @@ def test_xml_entities_and_reduction_text(tmp_path):
                               "max number of neighbours",
                               "check number of neighbours"]
     assert names.index("Sort") < i
+
+
+def test_pgp_preset_stack_order(tmp_path):
+    """cfd/GP.xml inserted into the basic + cfd stack: the associations are
+    re-sorted before "Sort", the mirror/interpolate/unmirror chain follows
+    "cfd Shepard", the extensions and ghost interactions close the
+    interactions stage (resources/Presets/src/cfd/GP.xml:50-80)."""
+    case = cases.still_tank(2, 0.15 / 8, bc="GP")
+    xml = casefiles.write_case(case, str(tmp_path),
+                               presets=casefiles.PGP_PRESETS, load=False)
+    names = [t[0] for t in _resolve(xml, 2)[0]]
+    idx = {n: k for k, n in enumerate(names)}
+    chain = ["Backup associations", "Sort associations", "Sort",
+             "cfd Shepard", "cfd GP backup r", "cfd GP mirror",
+             "cfd reinit gp_rho", "cfd reinit gp_p", "cfd reinit gp_u",
+             "cfd GP interpolation", "cfd GP renormalization",
+             "cfd GP unmirror", "cfd GP backup u", "cfd interactions",
+             "cfd GP preLapU", "cfd GP LapU", "cfd GP preInteractions",
+             "cfd GP interactions", "cfd GP restore u", "Interactions",
+             "cfd rates"]
+    pos = [idx[n] for n in chain]
+    assert pos == sorted(pos), list(zip(chain, pos))
+    assert idx["sort stage2"] < idx["Backup associations"]

@@ -58,6 +58,35 @@ def test_xml_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
     sim.close()
 
 
+GP_FIELDS = FIELDS + ("associations", "gp_icell", "gp_rho", "gp_p", "gp_u")
+
+
+@pytest.mark.parametrize("dims,nl,steps", [(2, 15, 2), (3, 6, 2),
+                                           (2, 15, 25), (3, 6, 10)])
+def test_gp_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
+    """Ghost-particle walls (cfd/GP.xml) on water at rest in a tank: every
+    field, the ghost fields and the associations against the oracle."""
+    case = cases.still_tank(dims, 0.15 / nl, bc="GP")
+    xml = casefiles.write_case(case, str(tmp_path),
+                               presets=casefiles.PGP_PRESETS)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = {}
+    for k in GP_FIELDS:
+        a = sim.get(k)
+        b = o.f[k]
+        if a.dtype.kind in "iu":
+            np.testing.assert_array_equal(a.view(b.dtype), b, err_msg=k)
+        else:
+            errs[k] = relerr(a, b)
+    print(dims, steps, errs)
+    assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
+    sim.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     """A captured step replayed from a HIP graph gives bit-identical state,
