@@ -26,6 +26,14 @@ PGP_PRESETS = ("resources/Presets/basic.xml",
                "resources/Presets/cfd/GP.xml",
                "resources/Presets/cfd/variableTimeStep.xml")
 
+PBIE_PRESETS = ("resources/Presets/basic.xml",
+                "resources/Presets/basic/time_scheme/improved_euler.xml",
+                "resources/Presets/basic/domain.xml",
+                "resources/Presets/cfd.xml",
+                "resources/Presets/cfd/deltaSPH.xml",
+                "resources/Presets/cfd/BIe.xml",
+                "resources/Presets/cfd/variableTimeStep.xml")
+
 FIELDS = ("r", "normal", "tangent", "u", "dudt", "rho", "drhodt", "m",
           "imove")
 

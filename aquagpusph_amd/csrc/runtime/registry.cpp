@@ -131,6 +131,27 @@ void launch_gp_interactions(int, const int32_t*, const float*, const float*,
                             float*, uint32_t, const uint32_t*,
                             const uint32_t*, const uint32_t*,
                             const uint32_t*, aq_defs, hipStream_t);
+void launch_bie_interactions(int, const int32_t*, const float*, const float*,
+                             const float*, const float*, float*, float*,
+                             uint32_t, const uint32_t*, const uint32_t*,
+                             const uint32_t*, const uint32_t*, aq_defs,
+                             hipStream_t);
+void launch_bie_p_boundary(int, const int32_t*, const float*, const float*,
+                           const float*, float*, uint32_t, const uint32_t*,
+                           const uint32_t*, const uint32_t*, const uint32_t*,
+                           aq_defs, hipStream_t);
+void launch_bie_rates(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, const float*, float*,
+                      float*, uint32_t, hipStream_t);
+void launch_bie_bounce(int, const int32_t*, const float*, const float*,
+                       const float*, const float*, float*, uint32_t,
+                       const float*, const uint32_t*, const uint32_t*,
+                       const uint32_t*, const uint32_t*, float, float,
+                       hipStream_t);
+void launch_bie_pst(int, const int32_t*, float*, const float*, const float*,
+                    const float*, uint32_t, const uint32_t*, const uint32_t*,
+                    const uint32_t*, const uint32_t*, aq_defs, float,
+                    hipStream_t);
 float support_threshold(float H);
 void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
                    const uint32_t*, const uint32_t*, const uint32_t*,
@@ -165,6 +186,14 @@ static int gp_lapu_model(Sim& S)
     if (m != 1 && m != 2)
         throw Error(AQ_EUNSUP, "unknown __GP_LAPU_MODEL__ " + v);
     return m;
+}
+
+// A preset definition as a float, or a script's own default when no preset
+// defines it (the #ifndef defaults of the OpenCL sources).
+static float def_or(Sim& S, const char* name, float dflt)
+{
+    return (S.defined(name) && !S.def(name).empty()) ? S.def_float(name)
+                                                     : dflt;
 }
 
 static const std::map<std::string, KernelSpec>& table()
@@ -431,6 +460,46 @@ static const std::map<std::string, KernelSpec>& table()
                 launch_gp_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
                                        F(5), F(6), F(7), S.N, U(8), U(9),
                                        U(10), S.d_status, S.kdefs, s);
+            } } },
+        // ---- conservative boundary integrals ------------------------------
+        { "cfd/Boundary/BIe/Interactions.cl:entry",
+          { { "imove", "r", "normal", "u", "m", "grad_w_bi", "div_u_bi",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bie_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                        F(5), F(6), S.N, U(7), U(8), U(9),
+                                        S.d_status, S.kdefs, s);
+            } } },
+        { "cfd/Boundary/BIe/Interactions.cl:p_boundary",
+          { { "imove", "r", "m", "rho", "p", "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bie_p_boundary(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                      S.N, U(5), U(6), U(7), S.d_status,
+                                      S.kdefs, s);
+            } } },
+        { "cfd/Boundary/BIe/Rates.cl:entry",
+          { { "imove", "rho", "p", "u", "grad_w_bi", "div_u_bi", "grad_p",
+              "div_u" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bie_rates(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 F(6), F(7), S.N, s);
+            } } },
+        { "cfd/Boundary/BIe/ElasticBounce.cl:entry",
+          { { "imove", "r_in", "normal", "m", "u_in", "dudt", "dt", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bie_bounce(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                  S.N, F(6), U(7), U(8), U(9), S.d_status,
+                                  def_or(S, "__DR_FACTOR__", 0.5f),
+                                  def_or(S, "__MIN_BOUND_DIST__", 0.f), s);
+            } } },
+        { "cfd/Boundary/BIe/PST.cl:entry",
+          { { "imove", "r", "normal", "m", "rho", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bie_pst(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, U(5),
+                               U(6), U(7), S.d_status, S.kdefs,
+                               def_or(S, "__DR_FACTOR__", 0.5f), s);
             } } },
         { "cfd/Boundary/BI/Interactions.cl:entry",
           { { "imove", "r", "normal", "u", "rho", "m", "p", "grad_p", "div_u",

@@ -121,33 +121,49 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
 // imove == -3); with kind 0 and `flag_el`, wall elements in range are also
 // tested and reported through `el_near`.  pair(j) is called for the
 // accepted candidates of each lane in increasing candidate order.
-template <int D, int SEGW, class Pair>
+//
+// UNIFORM: pair(j, has) is called on every lane at every step, with
+// has = false and j = idle_j on the lanes that have no candidate at that
+// step (idle_j = the particle itself, whose terms are exact zeros, or any
+// valid index on lanes whose sums are discarded).  The pair arithmetic then
+// runs outside any branch: no exec-mask save/restore, no copies of the
+// accumulators.
+template <int D, int SEGW, bool UNIFORM = false, class Pair>
 __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
                                           uint32_t total, const f3& r_i,
                                           bool act, float smax,
                                           const float* __restrict__ r,
                                           const int* __restrict__ imove,
                                           int kind, bool flag_el,
-                                          bool& el_near, Pair pair)
+                                          bool& el_near, Pair pair,
+                                          uint32_t idle_j = 0)
 {
     constexpr uint32_t SEG = SEGW * 32;
     const uint32_t nsegs = (total + SEG - 1) / SEG;
-    const int nwords = (int)((total + 31) >> 5);
-    int kl = 0;        // per-lane cursor: neighbour cell of a staged candidate
-    int gw = -1;       // word being consumed (over the whole list)
-    uint32_t bw = 0;   // its remaining bits
+    const uint32_t nwords = (total + 31) >> 5;
+    // the two slots form a ring of 2 SEGW words: word w of the candidate
+    // list sits at ring position w % (2 SEGW)
+    constexpr uint32_t RING = 2 * SEGW;
+    const uint32_t* bits = &L.bits[0][0][0];
+    const uint32_t* cand = &L.cand[0][0];
+    int kl = 0;          // per-lane cursor: neighbour cell of a staged candidate
+    uint32_t nw = 0;     // next word to load (the current one is nw - 1)
+    uint32_t bw = 0;     // remaining bits of the current word
     uint32_t tested = 0;
     bool eln = false;
-    while (true) {
-        const int tw = min((int)(tested * SEGW), nwords);
-        while (bw == 0 && gw + 1 < tw) {
-            gw++;
-            bw = L.bits[(gw / SEGW) & 1][gw % SEGW][lane];
+    auto advance = [&]() {
+        const uint32_t tw = min(tested * SEGW, nwords);
+        while (bw == 0 && nw < tw) {
+            bw = bits[(nw % RING) * 64 + lane];
+            nw++;
         }
+    };
+    while (true) {
+        advance();
         // refill the oldest slot when no lane still reads from it
         if (tested < nsegs &&
             (tested < 2 ||
-             __ballot(bw != 0 && gw < (int)((tested - 1) * SEGW)) == 0)) {
+             __ballot(bw != 0 && nw <= (tested - 1) * SEGW) == 0)) {
             const uint32_t seg = tested * SEG;
             const uint32_t nseg = min(SEG, total - seg);
             const int slot = tested & 1;
@@ -210,13 +226,26 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_s_waitcnt(0);
             tested++;
-            continue;
+            // UNIFORM keeps a single path through the loop body (the pair
+            // arithmetic is never skipped, so its accumulators need no
+            // copies between paths)
+            if constexpr (!UNIFORM) continue;
+            advance();
         }
-        if (__ballot(bw != 0) == 0) break;
-        if (bw != 0) {
-            const int b = __builtin_clz(bw);
-            bw ^= 0x80000000u >> b;
-            pair(L.cand[(gw / SEGW) & 1][(gw % SEGW) * 32 + b]);
+        const bool has = bw != 0;
+        if (__ballot(has) == 0 && (!UNIFORM || tested >= nsegs)) break;
+        if constexpr (UNIFORM) {
+            // first set bit (31 on an empty word: read, never used)
+            const uint32_t b = __builtin_clz(bw | 1u);
+            bw = has ? (bw ^ (0x80000000u >> b)) : 0u;
+            const uint32_t jc = cand[((nw - 1) % RING) * 32 + b];
+            pair(has ? jc : idle_j, has);
+        } else {
+            if (has) {
+                const uint32_t b = __builtin_clz(bw);
+                bw ^= 0x80000000u >> b;
+                pair(cand[((nw - 1) % RING) * 32 + b]);
+            }
         }
     }
     el_near = eln;
@@ -226,7 +255,7 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
 // ---------------------------------------------------------------------------
 // Fused fluid walk (same terms as fluid_pass_k in sph.hip).
 // ---------------------------------------------------------------------------
-template <int D, int SHEP, bool LAPP, int SEGW>
+template <int D, int SHEP, bool LAPP, int SEGW, int LAPF>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     fluid_pass_cells_k(const int* __restrict__ imove,
                        const float* __restrict__ r,
@@ -241,7 +270,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                        float* __restrict__ lap_u, float* __restrict__ div_u,
                        float* __restrict__ lap_p, uint8_t* __restrict__ near,
                        WalkArgs w, float H, float CONW, float CONF, float DIMS,
-                       float smax, int lap)
+                       float smax)
 {
     constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
     __shared__ WaveLds<SEGW> lds[WC_WAVES];
@@ -290,23 +319,31 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             }
             // ---- fluid neighbours ------------------------------------------
             bool nearw = false;
-            ring_walk<D, SEGW>(
+            ring_walk<D, SEGW, true>(
                 L, lane, total, r_i, act, smax, r, imove, 0, flag_el, nearw,
-                [&](uint32_t j) {
+                [&](uint32_t j, bool has) {
                     const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
                     const float s = dot<D>(r_ij, r_ij);
                     const float q = sqrtf(s) / H;
                     const float m_j = ldf_o(m, j);
                     const float rho_j = ldf_o(rho, j);
-                    if (SHEP == 1) sh += kernelW<D>(q) * CONW * m_j / rho_j;
-                    if (i == j) return;
+                    // (+0 leaves the sum unchanged: it is never -0)
+                    if (SHEP == 1)
+                        sh += has ? kernelW<D>(q) * CONW * m_j / rho_j : 0.f;
+                    // The reference skips j == i here.  For i itself r_ij,
+                    // u_ij and p_j - p_i are exact zeros, so every term
+                    // below is +-0 and leaves the accumulators unchanged
+                    // (they are never -0: they start at +0 and an exact
+                    // cancellation rounds to +0); with finite fields the
+                    // sums are bit-identical without the test.  The same
+                    // holds for the idle steps, which evaluate j = i.
                     const float p_j = ldf_o(p, j);
                     const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
                     const float udr = dot<D>(u_ij, r_ij);
                     const float fk = kernelF<D>(q) * CONF;
                     const float f_ij = fk * m_j;
                     axpy<D>(gp, (p_i + p_j) / (rho_i * rho_j) * f_ij, r_ij);
-                    if (lap == AQ_LAP_MONAGHAN) {
+                    if constexpr (LAPF == AQ_LAP_MONAGHAN) {
                         const float r2 = (q * q + 0.01f) * H * H;
                         axpy<D>(lu,
                                 f_ij * CLEARY * udr / (r2 * rho_i * rho_j),
@@ -316,7 +353,8 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     }
                     dv += udr * f_ij * rho_i / rho_j;
                     if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
-                });
+                },
+                act ? i : cstart);
             // ---- wall elements (boundary-integral Shepard) ------------------
             if (SHEP == 2 && __ballot(nearw) != 0) {
                 bool self_added = false;
@@ -666,15 +704,22 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
     WalkArgs w{ icell,   ihoc, n_cells, cell_end, heads,
                 n_heads, next, status,  N,        (uint32_t)flagged };
-#define AQ_FC(DD, SH, LP, SW)                                                  \
+#define AQ_FC2(DD, SH, LP, SW, LF)                                             \
     do {                                                                       \
-        auto* ker = fluid_pass_cells_k<DD, SH, LP, SW>;                        \
+        auto* ker = fluid_pass_cells_k<DD, SH, LP, SW, LF>;                    \
         const unsigned nb = nblocks ? nblocks : resident_blocks((const void*)ker, \
                                                     64 * WC_WAVES);            \
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \
                            u, rho, m, p, normal, tangent, binormal, shepard,   \
                            grad_p, lap_u, div_u, lap_p, near, w, d.H, d.CONW,  \
-                           d.CONF, d.DIMS, smax, d.lap_formulation);           \
+                           d.CONF, d.DIMS, smax);                              \
+    } while (0)
+#define AQ_FC(DD, SH, LP, SW)                                                  \
+    do {                                                                       \
+        if (d.lap_formulation == AQ_LAP_MONAGHAN)                              \
+            AQ_FC2(DD, SH, LP, SW, AQ_LAP_MONAGHAN);                           \
+        else                                                                   \
+            AQ_FC2(DD, SH, LP, SW, AQ_LAP_MORRIS);                             \
     } while (0)
 #define AQ_FC_D(SH, LP)                                                        \
     do {                                                                       \
@@ -691,6 +736,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     else AQ_FC_D(2, true);
 #undef AQ_FC_D
 #undef AQ_FC
+#undef AQ_FC2
 }
 
 

@@ -269,6 +269,45 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         gp_interactions<D, R>(imove, r, u, rho, m, p, grad_p, div_u, N,        \
                               LL{ ic, hoc, nc }, H, CONF);                     \
+    }                                                                          \
+    ORC_EXPORT void orc_bie_interactions_##SR##_##D##d(                        \
+        const int* imove, const R* r, const R* normal, const R* u,             \
+        const R* m, R* grad_w_bi, R* div_u_bi, usize N, const usize* ic,       \
+        const usize* hoc, const usize* nc, R H, R CONW)                        \
+    {                                                                          \
+        bie_interactions<D, R>(imove, r, normal, u, m, grad_w_bi, div_u_bi, N, \
+                               LL{ ic, hoc, nc }, H, CONW);                    \
+    }                                                                          \
+    ORC_EXPORT void orc_bie_p_boundary_##SR##_##D##d(                          \
+        const int* imove, const R* r, const R* m, const R* rho, R* p, usize N, \
+        const usize* ic, const usize* hoc, const usize* nc, R H, R CONW)       \
+    {                                                                          \
+        bie_p_boundary<D, R>(imove, r, m, rho, p, N, LL{ ic, hoc, nc }, H,     \
+                             CONW);                                            \
+    }                                                                          \
+    ORC_EXPORT void orc_bie_rates_##SR##_##D##d(                               \
+        const int* imove, const R* rho, const R* p, const R* u,                \
+        const R* grad_w_bi, const R* div_u_bi, R* grad_p, R* div_u, usize N)   \
+    {                                                                          \
+        bie_rates<D, R>(imove, rho, p, u, grad_w_bi, div_u_bi, grad_p, div_u,  \
+                        N);                                                    \
+    }                                                                          \
+    ORC_EXPORT void orc_bie_elastic_bounce_##SR##_##D##d(                      \
+        const int* imove, const R* r_in, const R* normal, const R* m,          \
+        const R* u_in, R* dudt, usize N, R dt, const usize* ic,                \
+        const usize* hoc, const usize* nc, R DR_FACTOR, R MIN_BOUND_DIST)      \
+    {                                                                          \
+        bie_elastic_bounce<D, R>(imove, r_in, normal, m, u_in, dudt, N, dt,    \
+                                 LL{ ic, hoc, nc }, DR_FACTOR,                 \
+                                 MIN_BOUND_DIST);                              \
+    }                                                                          \
+    ORC_EXPORT void orc_bie_pst_##SR##_##D##d(                                 \
+        const int* imove, R* r, const R* normal, const R* m, const R* rho,     \
+        usize N, const usize* ic, const usize* hoc, const usize* nc, R DIMS,   \
+        R DR_FACTOR)                                                           \
+    {                                                                          \
+        bie_pst<D, R>(imove, r, normal, m, rho, N, LL{ ic, hoc, nc }, DIMS,    \
+                      DR_FACTOR);                                              \
     }
 
 ORC_DEFINE(float, f32, 2)

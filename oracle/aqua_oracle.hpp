@@ -1134,6 +1134,165 @@ void gp_interactions(const int* imove, const R* r, const R* u, const R* rho,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Conservative boundary integrals (resources/Scripts/cfd/Boundary/BIe/*.cl):
+// wall elements imove == -3 (area in m, outward normal).
+// ---------------------------------------------------------------------------
+
+// BIe/Interactions.cl:entry: grad_w_bi = sum n_j W CONW area_j,
+// div_u_bi -= u_j . (that term), over the elements in the support
+template <int D, typename R>
+void bie_interactions(const int* imove, const R* r, const R* normal,
+                      const R* u, const R* m, R* grad_w_bi, R* div_u_bi,
+                      usize N, LL ll, R H, R CONW)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        V<D, R> gw = ld<D, R>(grad_w_bi, i);
+        R dv = div_u_bi[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -3) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const V<D, R> u_j = ld<D, R>(u, j);
+            const R w = kernelW<D, R>(q) * CONW * m[j];
+            V<D, R> g;
+            for (int k = 0; k < 3; k++) g.c[k] = (k < D) ? n_j.c[k] * w : R(0);
+            for (int k = 0; k < D; k++) gw.c[k] += g.c[k];
+            dv -= dot<D, R>(u_j, g);
+        });
+        for (int k = 0; k < D; k++) grad_w_bi[(size_t)i * VS + k] = gw.c[k];
+        div_u_bi[i] = dv;
+    }
+}
+
+// BIe/Interactions.cl:p_boundary: element pressure 2 sum p_j W CONW m_j/rho_j
+template <int D, typename R>
+void bie_p_boundary(const int* imove, const R* r, const R* m, const R* rho,
+                    R* p, usize N, LL ll, R H, R CONW)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != -3) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R acc = R(0);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            acc += R(2.f) * p[j] * kernelW<D, R>(q) * CONW * m[j] / rho[j];
+        });
+        p[i] = acc;
+    }
+}
+
+// BIe/Rates.cl:entry (whole-vec operations: the w component takes part in
+// the 3D sum and dot product)
+template <int D, typename R>
+void bie_rates(const int* imove, const R* rho, const R* p, const R* u,
+               const R* grad_w_bi, const R* div_u_bi, R* grad_p, R* div_u,
+               usize N)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 1) continue;
+        const R s = R(2.f) * p[i] / rho[i];
+        R d = u[(size_t)i * VS] * grad_w_bi[(size_t)i * VS];
+        for (int k = 1; k < VS; k++)
+            d = d + u[(size_t)i * VS + k] * grad_w_bi[(size_t)i * VS + k];
+        for (int k = 0; k < VS; k++)
+            grad_p[(size_t)i * VS + k] += s * grad_w_bi[(size_t)i * VS + k];
+        div_u[i] -= R(2.f) * rho[i] * (d + div_u_bi[i]);
+    }
+}
+
+// BIe/ElasticBounce.cl:entry: reflect the velocity of fluid particles about
+// to cross an element (within DR_FACTOR dr of its centre, tangentially)
+template <int D, typename R>
+void bie_elastic_bounce(const int* imove, const R* r_in, const R* normal,
+                        const R* m, const R* u_in, R* dudt, usize N, R dt,
+                        LL ll, R DR_FACTOR, R MIN_BOUND_DIST)
+{
+    const int VS = Dim<D>::VS;
+    if (!(dt != R(0))) return;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r_in, i);
+        const V<D, R> ui = ld<D, R>(u_in, i);
+        V<D, R> a = ld<D, R>(dudt, i), uu;
+        for (int k = 0; k < 3; k++)
+            uu.c[k] = (k < D) ? ui.c[k] + R(0.5f) * dt * a.c[k] : R(0);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -3) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r_in, j), r_i);
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const R rn = dot<D, R>(r_ij, n_j);
+            if (rn < R(0)) return;
+            const R dr = (D == 3) ? std::sqrt(m[j]) : m[j];
+            const R Rr = DR_FACTOR * dr;
+            V<D, R> rt;
+            for (int k = 0; k < 3; k++)
+                rt.c[k] = (k < D) ? r_ij.c[k] - rn * n_j.c[k] : R(0);
+            if (dot<D, R>(rt, rt) >= Rr * Rr) return;
+            const R drn = dt * dot<D, R>(uu, n_j);
+            if (drn < R(0)) return;
+            if (rn - drn <= MIN_BOUND_DIST * dr) {
+                V<D, R> v, vr;
+                for (int k = 0; k < 3; k++)
+                    v.c[k] = (k < D) ? ui.c[k] + dt * a.c[k] : R(0);
+                const R vn = dot<D, R>(v, n_j);
+                for (int k = 0; k < D; k++) {
+                    vr.c[k] = v.c[k] - R(2.f) * vn * n_j.c[k];
+                    a.c[k] = (vr.c[k] - ui.c[k]) / dt;
+                    uu.c[k] = ui.c[k] + R(0.5f) * dt * a.c[k];
+                }
+            }
+        });
+        for (int k = 0; k < D; k++) dudt[(size_t)i * VS + k] = a.c[k];
+    }
+}
+
+// BIe/PST.cl:entry: push fluid particles that overlap an element back to
+// the distance Ri = 0.5 (m/rho)^(1/DIMS); r is updated in place and the
+// next elements see the moved particle
+template <int D, typename R>
+void bie_pst(const int* imove, R* r, const R* normal, const R* m,
+             const R* rho, usize N, LL ll, R DIMS, R DR_FACTOR)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const R Ri = R(0.5f) * std::pow(m[i] / rho[i], R(1.f) / DIMS);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imove[j] != -3) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), ld<D, R>(r, i));
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const R rn = dot<D, R>(r_ij, n_j);
+            if (std::fabs(rn) > Ri) return;
+            const R dr = (D == 3) ? std::sqrt(m[j]) : m[j];
+            const R Rj = DR_FACTOR * dr;
+            V<D, R> rt;
+            for (int k = 0; k < 3; k++)
+                rt.c[k] = (k < D) ? r_ij.c[k] - rn * n_j.c[k] : R(0);
+            if (dot<D, R>(rt, rt) >= Rj * Rj) return;
+            for (int k = 0; k < D; k++)
+                r[(size_t)i * VS + k] += (rn - Ri) * n_j.c[k];
+        });
+    }
+}
+
 // basic/neighs.cl:52-91: every candidate of the 3^d cells (self included,
 // no distance or type filter); the walk stops as soon as the count reaches
 // neighs_limit; buffer particles (imove <= -255) get 0.

@@ -61,7 +61,8 @@ class OracleSim:
 
     def __init__(self, case, dtype=np.float32, domain=True, bi=None,
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
-                 freeslip_iset=None):
+                 freeslip_iset=None, bie=False, dr_factor=0.5,
+                 min_bound_dist=0.0):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -69,10 +70,12 @@ class OracleSim:
             "_%dd" % self.dims
         gp = "associations" in case
         if bi is None:
-            bi = not gp
+            bi = not gp and not bie
         self.opts = dict(domain=domain, bi=bi, gp=gp, delta=delta,
                          variable_dt=variable_dt, lap_form=lap_form,
-                         gp_model=gp_model, freeslip_iset=freeslip_iset)
+                         gp_model=gp_model, freeslip_iset=freeslip_iset,
+                         bie=bie, dr_factor=dr_factor,
+                         min_bound_dist=min_bound_dist)
         N = self.N = int(case["N"])
         R = self.dtype
         f = {}
@@ -100,6 +103,9 @@ class OracleSim:
             f[k] = np.zeros(N, R)
         for k in ("icell", "id_sorted", "id_unsorted"):
             f[k] = np.zeros(N, np.uint32)
+        if bie:
+            f["grad_w_bi"] = np.zeros((N, self.vs), R)
+            f["div_u_bi"] = np.zeros(N, R)
         if gp:
             f["associations"] = np.ascontiguousarray(
                 case["associations"], np.uint32).copy()
@@ -267,6 +273,16 @@ class OracleSim:
         if o["delta"]:
             k("lapp")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
                       _p(f["p"]), _p(f["lap_p"]), n, *ll, H, CONF)
+        if o["bie"]:
+            # BIe.xml: boundary terms of the conservative integrals
+            f["grad_w_bi"][...] = 0
+            f["div_u_bi"][...] = 0
+            k("bie_interactions")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
+                                  _p(f["u"]), _p(f["m"]), _p(f["grad_w_bi"]),
+                                  _p(f["div_u_bi"]), n, *ll, H, CONW)
+            k("bie_rates")(_p(f["imove"]), _p(f["rho"]), _p(f["p"]),
+                           _p(f["u"]), _p(f["grad_w_bi"]), _p(f["div_u_bi"]),
+                           _p(f["grad_p"]), _p(f["div_u"]), n)
         if o["gp"]:
             k("gp_prelapu")(_p(f["imove"]), _p(f["associations"]),
                             _p(f["normal"]), _p(f["gp_rho"]), _p(f["gp_u"]),
@@ -321,9 +337,19 @@ class OracleSim:
                             _p(f["lap_p"]), _p(f["drhodt"]),
                             _p(self.sets["refd"]), _p(self.sets["delta"]), n,
                             R(self.dt))
+        if o["bie"]:
+            k("bie_elastic_bounce")(_p(f["imove"]), _p(f["r_in"]),
+                                    _p(f["normal"]), _p(f["m"]),
+                                    _p(f["u_in"]), _p(f["dudt"]), n,
+                                    R(self.dt), *ll, R(o["dr_factor"]),
+                                    R(o["min_bound_dist"]))
         k("corrector")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
                        _p(f["rho"]), _p(f["drhodt"]), _p(f["dudt_in"]),
                        _p(f["drhodt_in"]), n, R(self.dt))
+        if o["bie"]:
+            k("bie_pst")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
+                         _p(f["m"]), _p(f["rho"]), n, *ll, R(self.DIMS),
+                         R(o["dr_factor"]))
         # "fixed time step" set_scalar (basic.xml:164), double -> float
         self.dt = self.dtype.type(float(self.courant) * float(self.h) /
                                   float(self.cs))

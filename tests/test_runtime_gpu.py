@@ -87,6 +87,32 @@ def test_gp_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
     sim.close()
 
 
+@pytest.mark.parametrize("dims,nl,steps", [(2, 20, 2), (3, 6, 2),
+                                           (2, 20, 25), (3, 6, 10)])
+def test_bie_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
+    """Conservative boundary integrals (cfd/BIe.xml: wall terms, elastic
+    bounce, particle shifting) on the dam break against the oracle."""
+    case = cases.dam_break(dims, 0.3 / nl)
+    xml = casefiles.write_case(case, str(tmp_path),
+                               presets=casefiles.PBIE_PRESETS)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, bie=True)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = {}
+    for k in FIELDS + ("grad_w_bi", "div_u_bi"):
+        a = sim.get(k)
+        b = o.f[k]
+        if a.dtype.kind in "iu":
+            np.testing.assert_array_equal(a.view(b.dtype), b, err_msg=k)
+        else:
+            errs[k] = relerr(a, b)
+    print(dims, steps, errs)
+    assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
+    sim.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     """A captured step replayed from a HIP graph gives bit-identical state,
