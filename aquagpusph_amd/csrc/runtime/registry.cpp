@@ -152,6 +152,25 @@ void launch_bie_pst(int, const int32_t*, float*, const float*, const float*,
                     const float*, uint32_t, const uint32_t*, const uint32_t*,
                     const uint32_t*, const uint32_t*, aq_defs, float,
                     hipStream_t);
+void launch_mls_entry(int, const int32_t*, const float*, const float*,
+                      const float*, float*, const uint32_t*, uint32_t,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      const uint32_t*, aq_defs, hipStream_t);
+void launch_mls_inv(int, const int32_t*, float*, uint32_t, const uint32_t*,
+                    hipStream_t);
+void launch_lapp_simple(int, const uint32_t*, const int32_t*, float*,
+                        const float*, uint32_t, const float*, int,
+                        hipStream_t);
+void launch_lapp_full(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, float*, uint32_t,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      const uint32_t*, aq_defs, int, hipStream_t);
+void launch_lapp_full_mls(int, const int32_t*, const float*, float*,
+                          uint32_t, int, hipStream_t);
+void launch_lapp_corr(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, float*, uint32_t,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      const uint32_t*, aq_defs, int, hipStream_t);
 float support_threshold(float H);
 void launch_neighs(int, const int32_t*, uint32_t*, const uint32_t*, uint32_t,
                    const uint32_t*, const uint32_t*, const uint32_t*,
@@ -397,6 +416,73 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_bi_interp_shepard(U(0), I(1), F(2), F(3), F(4), F(5),
                                          S.N, F(6), F(7), s);
+            } } },
+        // ---- MLS and the delta-SPH Laplacian corrections ------------------
+        { "cfd/deltaSPH.cl:simple",
+          { { "iset", "imove", "lap_p_corr", "refd", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_simple(S.dims, U(0), I(1), F(2), F(3), S.N, F(4),
+                                   0, s);
+            } } },
+        { "cfd/deltaSPH.cl:full",
+          { { "imove", "r", "rho", "m", "p", "lap_p_corr", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_full(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                 0, s);
+            } } },
+        { "cfd/deltaSPH.cl:full_mls",
+          { { "imove", "mls", "lap_p_corr" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_full_mls(S.dims, I(0), F(1), F(2), S.N, 0, s);
+            } } },
+        { "cfd/deltaSPH.cl:lapp_corr",
+          { { "imove", "r", "rho", "m", "lap_p_corr", "lap_p", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_corr(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                 0, s);
+            } } },
+        { "basic/deltaSPH.cl:simple",
+          { { "iset", "imove", "lap_p_corr", "refd", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_simple(S.dims, U(0), I(1), F(2), F(3), S.N, F(4),
+                                   1, s);
+            } } },
+        { "basic/deltaSPH.cl:full",
+          { { "imove", "r", "rho", "m", "p", "lap_p_corr", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_full(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                 1, s);
+            } } },
+        { "basic/deltaSPH.cl:full_mls",
+          { { "imove", "mls", "lap_p_corr" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_full_mls(S.dims, I(0), F(1), F(2), S.N, 1, s);
+            } } },
+        { "basic/deltaSPH.cl:lapp_corr",
+          { { "imove", "r", "rho", "m", "lap_p_corr", "lap_p", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_lapp_corr(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
+                                 1, s);
+            } } },
+        { "basic/MLS.cl:entry",
+          { { "imove", "r", "rho", "m", "mls", "mls_imove", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mls_entry(S.dims, I(0), F(1), F(2), F(3), F(4), U(5),
+                                 S.N, U(6), U(7), U(8), S.d_status, S.kdefs, s);
+            } } },
+        { "basic/MLS.cl:mls_inv",
+          { { "imove", "mls", "mls_imove" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mls_inv(S.dims, I(0), F(1), S.N, U(2), s);
             } } },
         // ---- ghost particles ---------------------------------------------
         { "cfd/Boundary/GP/Sort.cl:entry",

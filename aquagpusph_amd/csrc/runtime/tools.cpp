@@ -77,7 +77,7 @@ static std::vector<double> macro_value(const std::string& v0, const Type& t,
     auto vec = [&](double x, bool all) {
         for (int k = 0; k < n; k++) out[k] = (k < dims || all) ? x : 0.0;
     };
-    if (v == "VEC_ZERO") vec(0.0, true);
+    if (v == "VEC_ZERO" || v == "MAT_ZERO") vec(0.0, true);
     else if (v == "VEC_ONE") vec(1.0, false);
     else if (v == "VEC_ALL_ONE") vec(1.0, true);
     else if (v == "VEC_INFINITY") vec(inf, false);

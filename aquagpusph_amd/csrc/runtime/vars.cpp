@@ -40,7 +40,16 @@ Type parse_type(const std::string& t0, int dims)
     const std::string dflt = dims == 3 ? "4" : "2";
     if (t == "vec" || t == "ivec" || t == "uivec") t += dflt;
 
-    if (t == "float") {
+    if (t == "matrix") {
+        // float4 (2x2) in 2D, float16 (4x4) in 3D, types/{2D,3D}.h
+        if (!T.array)
+            throw Error(AQ_EUNSUP, "scalar matrix variables are not supported");
+        T.base = 'f';
+        T.ncomp = dims == 3 ? 16 : 4;
+        T.stored = T.ncomp;
+        T.canon = "matrix*";
+        return T;
+    } else if (t == "float") {
         T.base = 'f';
     } else if (t == "int") {
         T.base = 'i';

@@ -308,6 +308,46 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         bie_pst<D, R>(imove, r, normal, m, rho, N, LL{ ic, hoc, nc }, DIMS,    \
                       DR_FACTOR);                                              \
+    }                                                                          \
+    ORC_EXPORT void orc_mls_entry_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* rho, const R* m, R* mls,        \
+        usize N, unsigned mls_imove, const usize* ic, const usize* hoc,        \
+        const usize* nc, R H, R CONF)                                          \
+    {                                                                          \
+        mls_entry<D, R>(imove, r, rho, m, mls, N, mls_imove,                   \
+                        LL{ ic, hoc, nc }, H, CONF);                           \
+    }                                                                          \
+    ORC_EXPORT void orc_mls_inv_##SR##_##D##d(const int* imove, R* mls,       \
+                                              usize N, unsigned mls_imove)    \
+    {                                                                          \
+        mls_inv<D, R>(imove, mls, N, mls_imove);                               \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_simple_##SR##_##D##d(                             \
+        const unsigned* iset, const int* imove, R* lap_p_corr, const R* refd,  \
+        usize N, const R* g)                                                   \
+    {                                                                          \
+        lapp_simple<D, R>(iset, imove, lap_p_corr, refd, N, g);                \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_full_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* rho, const R* m, const R* p,    \
+        R* lap_p_corr, usize N, const usize* ic, const usize* hoc,             \
+        const usize* nc, R H, R CONF)                                          \
+    {                                                                          \
+        lapp_full<D, R>(imove, r, rho, m, p, lap_p_corr, N, LL{ ic, hoc, nc }, \
+                        H, CONF);                                              \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_full_mls_##SR##_##D##d(                           \
+        const int* imove, const R* mls, R* lap_p_corr, usize N)                \
+    {                                                                          \
+        lapp_full_mls<D, R>(imove, mls, lap_p_corr, N);                        \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_corr_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* rho, const R* m,                \
+        const R* lap_p_corr, R* lap_p, usize N, const usize* ic,               \
+        const usize* hoc, const usize* nc, R H, R CONF)                        \
+    {                                                                          \
+        lapp_corr<D, R>(imove, r, rho, m, lap_p_corr, lap_p, N,                \
+                        LL{ ic, hoc, nc }, H, CONF);                           \
     }
 
 ORC_DEFINE(float, f32, 2)

@@ -1293,6 +1293,220 @@ void bie_pst(const int* imove, R* r, const R* normal, const R* m,
     }
 }
 
+// ---------------------------------------------------------------------------
+// MLS kernel-gradient correction and the full / simple delta-SPH Laplacian
+// corrections (resources/Scripts/basic/MLS.cl, basic/deltaSPH.cl:57-190 and
+// 261-313, cfd/deltaSPH.cl: EXCLUDED_PARTICLE = imove != 1).  A "matrix"
+// is float4 (2x2, row major) in 2D and float16 (4x4, row major, the 3x3
+// block used) in 3D, with the helpers of types/{2D,3D}.h.
+// ---------------------------------------------------------------------------
+template <int D> struct Mat {
+    static constexpr int S = (D == 2) ? 4 : 16;
+};
+
+// MATRIX_MUL (3x3 block in 3D, the rest zero); dot products left to right
+template <int D, typename R>
+inline void mat_mul(const R* a, const R* b, R* c)
+{
+    if (D == 2) {
+        c[0] = a[0] * b[0] + a[1] * b[2];
+        c[1] = a[0] * b[1] + a[1] * b[3];
+        c[2] = a[2] * b[0] + a[3] * b[2];
+        c[3] = a[2] * b[1] + a[3] * b[3];
+        return;
+    }
+    for (int k = 0; k < 16; k++) c[k] = R(0);
+    for (int r_ = 0; r_ < 3; r_++)
+        for (int c_ = 0; c_ < 3; c_++) {
+            R s = a[4 * r_ + 0] * b[c_];
+            s = s + a[4 * r_ + 1] * b[4 + c_];
+            s = s + a[4 * r_ + 2] * b[8 + c_];
+            c[4 * r_ + c_] = s;
+        }
+}
+
+template <int D, typename R> inline void mat_transpose(const R* a, R* t)
+{
+    const int n = (D == 2) ? 2 : 4;
+    for (int r_ = 0; r_ < n; r_++)
+        for (int c_ = 0; c_ < n; c_++) t[n * r_ + c_] = a[n * c_ + r_];
+}
+
+// inv(): adjugate times 1/det, the identity when |1/det| > 1e16
+template <int D, typename R> inline void mat_inv(const R* m, R* o)
+{
+    if (D == 2) {
+        const R d = R(1) / (m[0] * m[3] - m[1] * m[2]);
+        if (std::fabs(d) > R(1.e16f)) {
+            o[0] = R(1), o[1] = R(0), o[2] = R(0), o[3] = R(1);
+            return;
+        }
+        o[0] = m[3] * d;
+        o[1] = -m[1] * d;
+        o[2] = -m[2] * d;
+        o[3] = m[0] * d;
+        return;
+    }
+    const R det = m[0] * (m[5] * m[10] - m[6] * m[9]) +
+                  m[1] * (m[6] * m[8] - m[4] * m[10]) +
+                  m[2] * (m[4] * m[9] - m[5] * m[8]);
+    const R d = R(1) / det;
+    for (int k = 0; k < 16; k++) o[k] = (k % 5 == 0) ? R(1) : R(0);
+    if (std::fabs(d) > R(1.e16f)) return;
+    o[0] = (m[5] * m[10] - m[6] * m[9]) * d;
+    o[1] = (m[2] * m[9] - m[1] * m[10]) * d;
+    o[2] = (m[1] * m[6] - m[2] * m[5]) * d;
+    o[4] = (m[6] * m[8] - m[4] * m[10]) * d;
+    o[5] = (m[0] * m[10] - m[2] * m[8]) * d;
+    o[6] = (m[2] * m[4] - m[0] * m[6]) * d;
+    o[8] = (m[4] * m[9] - m[5] * m[8]) * d;
+    o[9] = (m[1] * m[8] - m[0] * m[9]) * d;
+    o[10] = (m[0] * m[5] - m[1] * m[4]) * d;
+}
+
+// MATRIX_INV: the pseudo-inverse inv(M^T M) M^T
+template <int D, typename R> inline void mat_pinv(const R* m, R* o)
+{
+    R t[16], mtm[16], iv[16];
+    mat_transpose<D, R>(m, t);
+    mat_mul<D, R>(t, m, mtm);
+    mat_inv<D, R>(mtm, iv);
+    mat_mul<D, R>(iv, t, o);
+}
+
+// basic/MLS.cl:entry: sum over the mls_imove neighbours of
+// outer(r_ij, F CONF m_j / rho_j r_ij)
+template <int D, typename R>
+void mls_entry(const int* imove, const R* r, const R* rho, const R* m,
+               R* mls, usize N, unsigned mls_imove, LL ll, R H, R CONF)
+{
+    const int S = Mat<D>::S;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != (int)mls_imove) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R acc[16];
+        for (int k = 0; k < S; k++) acc[k] = R(0);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (i == j) return;
+            if (imove[j] != (int)mls_imove) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R f = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            R v2[3];
+            for (int k = 0; k < D; k++) v2[k] = f * r_ij.c[k];
+            const int n = (D == 2) ? 2 : 4;
+            for (int a = 0; a < D; a++)
+                for (int b = 0; b < D; b++)
+                    acc[n * a + b] += r_ij.c[a] * v2[b];
+        });
+        for (int k = 0; k < S; k++) mls[(size_t)i * S + k] = acc[k];
+    }
+}
+
+// basic/MLS.cl:mls_inv
+template <int D, typename R>
+void mls_inv(const int* imove, R* mls, usize N, unsigned mls_imove)
+{
+    const int S = Mat<D>::S;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != (int)mls_imove) continue;
+        R o[16];
+        mat_pinv<D, R>(mls + (size_t)i * S, o);
+        for (int k = 0; k < S; k++) mls[(size_t)i * S + k] = o[k];
+    }
+}
+
+// deltaSPH.cl:simple: lap_p_corr = refd g (the whole vec)
+template <int D, typename R>
+void lapp_simple(const unsigned* iset, const int* imove, R* lap_p_corr,
+                 const R* refd, usize N, const R* g)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 1) continue;
+        for (int k = 0; k < VS; k++)
+            lap_p_corr[(size_t)i * VS + k] = refd[iset[i]] * g[k];
+    }
+}
+
+// deltaSPH.cl:full: lap_p_corr += sum (p_j - p_i) F CONF m_j/rho_j r_ij
+template <int D, typename R>
+void lapp_full(const int* imove, const R* r, const R* rho, const R* m,
+               const R* p, R* lap_p_corr, usize N, LL ll, R H, R CONF)
+{
+    const int VS = Dim<D>::VS;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const R p_i = p[i];
+        V<D, R> g = ld<D, R>(lap_p_corr, i);
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((i == j) || (imove[j] != 1)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R f = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            const R s = (p[j] - p_i) * f;
+            for (int k = 0; k < D; k++) g.c[k] += s * r_ij.c[k];
+        });
+        for (int k = 0; k < D; k++) lap_p_corr[(size_t)i * VS + k] = g.c[k];
+    }
+}
+
+// deltaSPH.cl:full_mls: lap_p_corr = MATRIX_DOT(mls, lap_p_corr) (w = 0)
+template <int D, typename R>
+void lapp_full_mls(const int* imove, const R* mls, R* lap_p_corr, usize N)
+{
+    const int VS = Dim<D>::VS, S = Mat<D>::S;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 1) continue;
+        const R* M = mls + (size_t)i * S;
+        R* v = lap_p_corr + (size_t)i * VS;
+        R o[4] = { R(0), R(0), R(0), R(0) };
+        const int n = (D == 2) ? 2 : 4;
+        for (int a = 0; a < D; a++) {
+            R s = M[n * a + 0] * v[0];
+            s = s + M[n * a + 1] * v[1];
+            if (D == 3) s = s + M[n * a + 2] * v[2];
+            o[a] = s;
+        }
+        for (int k = 0; k < VS; k++) v[k] = o[k];
+    }
+}
+
+// deltaSPH.cl:lapp_corr: lap_p -= 0.5 (corr_j + corr_i) . r_ij F CONF m_j/rho_j
+template <int D, typename R>
+void lapp_corr(const int* imove, const R* r, const R* rho, const R* m,
+               const R* lap_p_corr, R* lap_p, usize N, LL ll, R H, R CONF)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> c_i = ld<D, R>(lap_p_corr, i);
+        R lp = lap_p[i];
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((i == j) || (imove[j] != 1)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            V<D, R> gij;
+            const V<D, R> c_j = ld<D, R>(lap_p_corr, j);
+            for (int k = 0; k < 3; k++)
+                gij.c[k] = (k < D) ? c_j.c[k] + c_i.c[k] : R(0);
+            const R f = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            lp -= R(0.5f) * dot<D, R>(gij, r_ij) * f;
+        });
+        lap_p[i] = lp;
+    }
+}
+
 // basic/neighs.cl:52-91: every candidate of the 3^d cells (self included,
 // no distance or type filter); the walk stops as soon as the count reaches
 // neighs_limit; buffer particles (imove <= -255) get 0.

@@ -62,7 +62,7 @@ class OracleSim:
     def __init__(self, case, dtype=np.float32, domain=True, bi=None,
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
-                 min_bound_dist=0.0):
+                 min_bound_dist=0.0, delta_corr=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -75,7 +75,8 @@ class OracleSim:
                          variable_dt=variable_dt, lap_form=lap_form,
                          gp_model=gp_model, freeslip_iset=freeslip_iset,
                          bie=bie, dr_factor=dr_factor,
-                         min_bound_dist=min_bound_dist)
+                         min_bound_dist=min_bound_dist,
+                         delta_corr=delta_corr)
         N = self.N = int(case["N"])
         R = self.dtype
         f = {}
@@ -103,6 +104,12 @@ class OracleSim:
             f[k] = np.zeros(N, R)
         for k in ("icell", "id_sorted", "id_unsorted"):
             f[k] = np.zeros(N, np.uint32)
+        if delta_corr:
+            f["lap_p_corr"] = np.zeros((N, self.vs), R)
+        if delta_corr == "full":
+            ms = 4 if self.dims == 2 else 16
+            for k in ("mls", "mls_fluid"):
+                f[k] = np.zeros((N, ms), R)
         if bie:
             f["grad_w_bi"] = np.zeros((N, self.vs), R)
             f["div_u_bi"] = np.zeros(N, R)
@@ -232,10 +239,19 @@ class OracleSim:
             f["associations_in"][...] = f["associations"]
             lib().orc_gp_sort(_p(f["associations_in"]), _p(f["associations"]),
                               _p(f["id_sorted"]), n)
-        for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
-            f[key][...] = 0
         ll = self._ll()
         H, CONW, CONF = R(self.H), R(self.CONW), R(self.CONF)
+        if o["delta_corr"]:
+            f["lap_p_corr"][...] = 0
+        if o["delta_corr"] == "full":
+            # MLS.xml (prefix imove1_): matrices of the fluid particles
+            k("mls_entry")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
+                           _p(f["m"]), _p(f["mls"]), n, ctypes.c_uint(1), *ll,
+                           H, CONF)
+            k("mls_inv")(_p(f["imove"]), _p(f["mls"]), n, ctypes.c_uint(1))
+            f["mls_fluid"][...] = f["mls"]
+        for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
+            f[key][...] = 0
         if o["bi"]:
             k("bi_shepard_compute")(
                 _p(f["imove"]), _p(f["r"]), _p(f["normal"]), _p(f["tangent"]),
@@ -270,6 +286,15 @@ class OracleSim:
                      _p(f["rho"]), _p(f["p"]), n, *ll, H, CONW)
         k("sensors_renorm")(_p(f["imove"]), _p(f["shepard"]), _p(f["u"]),
                             _p(f["rho"]), _p(f["p"]), n)
+        if o["delta_corr"] == "full":
+            f["mls"][...] = f["mls_fluid"]
+            k("lapp_full")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
+                           _p(f["m"]), _p(f["p"]), _p(f["lap_p_corr"]), n,
+                           *ll, H, CONF)
+        elif o["delta_corr"] == "simple":
+            k("lapp_simple")(_p(f["iset"]), _p(f["imove"]),
+                             _p(f["lap_p_corr"]), _p(self.sets["refd"]), n,
+                             _p(self.g))
         if o["delta"]:
             k("lapp")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
                       _p(f["p"]), _p(f["lap_p"]), n, *ll, H, CONF)
@@ -328,6 +353,13 @@ class OracleSim:
             k("bi_shepard_apply")(_p(f["imove"]), _p(f["shepard"]),
                                   _p(f["grad_p"]), _p(f["lap_u"]),
                                   _p(f["div_u"]), n)
+        if o["delta_corr"] == "full":
+            k("lapp_full_mls")(_p(f["imove"]), _p(f["mls"]),
+                               _p(f["lap_p_corr"]), n)
+        if o["delta_corr"]:
+            k("lapp_corr")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
+                           _p(f["m"]), _p(f["lap_p_corr"]), _p(f["lap_p"]), n,
+                           *ll, H, CONF)
         k("rates")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
                    _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]),
                    _p(f["dudt"]), _p(f["drhodt"]), _p(self.sets["visc_dyn"]),

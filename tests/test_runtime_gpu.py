@@ -113,6 +113,37 @@ def test_bie_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
     sim.close()
 
 
+@pytest.mark.parametrize("dims,corr,steps", [(2, "full", 2), (3, "full", 2),
+                                            (2, "simple", 2), (2, "full", 20),
+                                            (3, "simple", 10)])
+def test_delta_sph_corrections_match_oracle(cuda, tmp_path, dims, corr,
+                                            steps):
+    """cfd/deltaSPH-full.xml (MLS matrices, full and MLS-renormalised
+    corrections) and cfd/deltaSPH-simple.xml against the oracle."""
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    pres = list(casefiles.PBI_PRESETS)
+    pres.insert(5, "resources/Presets/cfd/deltaSPH-%s.xml" % corr)
+    xml = casefiles.write_case(case, str(tmp_path), presets=tuple(pres))
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, delta_corr=corr)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    keys = FIELDS + ("lap_p_corr",) + (("mls_fluid",) if corr == "full"
+                                         else ())
+    errs = {}
+    for k in keys:
+        a = sim.get(k)
+        b = o.f[k]
+        if a.dtype.kind in "iu":
+            np.testing.assert_array_equal(a.view(b.dtype), b, err_msg=k)
+        else:
+            errs[k] = relerr(a, b)
+    print(dims, corr, steps, errs)
+    assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
+    sim.close()
+
+
 @pytest.mark.parametrize("dims", [2, 3])
 def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     """A captured step replayed from a HIP graph gives bit-identical state,
