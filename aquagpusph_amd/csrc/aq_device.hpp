@@ -96,6 +96,23 @@ template <int D> __device__ __forceinline__ f3 sub(const f3& a, const f3& b)
 }
 
 // dot over the first D components, left to right
+// Contracted forms (one rounding per multiply-add), for the kernels built
+// with the OpenCL C default floating-point rules.
+template <int D> __device__ __forceinline__ float fdot(const f3& a, const f3& b)
+{
+    float s = a.x * b.x;
+    s = fmaf(a.y, b.y, s);
+    if constexpr (D == 3) s = fmaf(a.z, b.z, s);
+    return s;
+}
+template <int D>
+__device__ __forceinline__ void faxpy(f3& acc, float s, const f3& v)
+{
+    acc.x = fmaf(s, v.x, acc.x);
+    acc.y = fmaf(s, v.y, acc.y);
+    if constexpr (D == 3) acc.z = fmaf(s, v.z, acc.z);
+}
+
 template <int D> __device__ __forceinline__ float dot(const f3& a, const f3& b)
 {
     float s = a.x * b.x;

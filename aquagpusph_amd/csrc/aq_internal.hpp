@@ -41,7 +41,8 @@ class Error : public std::runtime_error {
 
 // Link-list data of the cell-parallel walks: the sorted icell, ihoc and
 // n_cells, plus the non-empty cell list (heads[k] = first particle of the
-// k-th non-empty cell, n_heads[0] = their number, n_heads[1] = work counter,
+// k-th non-empty cell, n_heads[0] = their number, the walks' hand-out
+// counters from n_heads[32] on,
 // cell_end[c] = one past the last particle of cell c, AQ_CELL_END bits,
 // with the AQ_CELL_FLUID / AQ_CELL_ELEM kind flags above them).
 #define AQ_CELL_END 0x0FFFFFFFu   // cell_end bits holding the end index
@@ -58,6 +59,23 @@ struct CellArgs {
     uint32_t N;
     uint32_t flagged;  // cell_end carries the kind flags
 };
+
+// Optional aids of the per-particle walk kernels, supplied by the runtime:
+// `list` / `count` are scratch for the compacted indices of the particles
+// that run a kernel (N entries and one counter), and cell_end (carrying the
+// kind flags when `flagged`) lets a walk skip neighbour cells that hold no
+// candidate of the wanted kind.  Kernels given none run one thread per
+// particle and walk every cell.
+struct PartWalk {
+    uint32_t* list = nullptr;
+    uint32_t* count = nullptr;
+    const uint32_t* cell_end = nullptr;
+    uint32_t flagged = 0;
+};
+
+// Workgroups of `block` threads of kernel `ker` resident at once on the
+// current device (occupancy x compute units; cached per device).
+unsigned resident_blocks(const void* ker, int block);
 
 // Wrap a C ABI body: translate exceptions into error codes.
 template <class F> inline int guard(F f, bool check_launch = true)

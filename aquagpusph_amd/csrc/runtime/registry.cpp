@@ -48,7 +48,7 @@ void launch_bi_interp_shepard(const uint32_t*, const int32_t*, const float*,
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
-                    aq_defs, int, hipStream_t);
+                    aq_defs, int, hipStream_t, const PartWalk*);
 void launch_interactions(int, const int32_t*, const float*, const float*,
                          const float*, const float*, const float*, float*,
                          float*, float*, uint32_t, const uint32_t*,
@@ -67,21 +67,23 @@ void launch_bi_shepard_compute(int, const int32_t*, const float*,
                                const float*, float*, uint32_t,
                                const uint32_t*, const uint32_t*,
                                const uint32_t*, const uint32_t*, aq_defs,
-                               hipStream_t, int);
+                               hipStream_t, int, const PartWalk*);
 void launch_bi_lapu(int, const int32_t*, const float*, const float*,
                     const float*, const float*, float*, uint32_t,
                     const uint32_t*, const uint32_t*, const uint32_t*,
-                    const uint32_t*, aq_defs, hipStream_t);
+                    const uint32_t*, aq_defs, hipStream_t,
+                    const PartWalk*);
 void launch_bi_interp(int, const int32_t*, const float*, const float*,
                       const float*, const float*, float*, uint32_t,
                       const uint32_t*, const uint32_t*, const uint32_t*,
-                      const uint32_t*, aq_defs, hipStream_t);
+                      const uint32_t*, aq_defs, hipStream_t,
+                      const PartWalk*);
 void launch_bi_interactions(int, const int32_t*, const float*, const float*,
                             const float*, const float*, const float*,
                             const float*, float*, float*, uint32_t,
                             const uint32_t*, const uint32_t*,
                             const uint32_t*, const uint32_t*, aq_defs,
-                            hipStream_t, const uint8_t*);
+                            hipStream_t, const uint8_t*, const PartWalk*);
 void launch_bi_shepard_compute_cells(int, const int32_t*, const float*,
                                      const float*, const float*,
                                      const float*, const float*, float*,
@@ -286,8 +288,9 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "rho", "m", "shepard", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
+                const PartWalk pw = S.part_walk(a[5], a[6], a[7]);
                 launch_shepard(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, U(5),
-                               U(6), U(7), S.d_status, S.kdefs, 0, s);
+                               U(6), U(7), S.d_status, S.kdefs, 0, s, &pw);
             } } },
         { "basic/neighs.cl:entry",
           { { "imove", "n_neighs", "neighs_limit", "icell", "ihoc",
@@ -301,8 +304,9 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "rho", "m", "shepard", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
+                const PartWalk pw = S.part_walk(a[5], a[6], a[7]);
                 launch_shepard(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, U(5),
-                               U(6), U(7), S.d_status, S.kdefs, 1, s);
+                               U(6), U(7), S.d_status, S.kdefs, 1, s, &pw);
             } } },
         { "cfd/Interactions.cl:entry",
           { { "imove", "r", "u", "rho", "m", "p", "grad_p", "lap_u", "div_u",
@@ -364,11 +368,13 @@ static const std::map<std::string, KernelSpec>& table()
                         S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6),
                         S.cell_args(), S.kdefs, support_threshold(S.kdefs.H),
                         0, s);
-                else
+                else {
+                    const PartWalk pw = S.part_walk(a[7], a[8], a[9]);
                     launch_bi_shepard_compute(S.dims, I(0), F(1), F(2), F(3),
                                               F(4), F(5), F(6), S.N, U(7),
                                               U(8), U(9), S.d_status, S.kdefs,
-                                              s, 0);
+                                              s, 0, &pw);
+                }
             } } },
         { "cfd/Boundary/BI/Shepard.cl:apply",
           { { "imove", "shepard", "grad_p", "lap_u", "div_u" },
@@ -385,10 +391,12 @@ static const std::map<std::string, KernelSpec>& table()
                     launch_bi_lapu_cells(S.dims, I(0), F(1), F(2), F(3), F(4),
                                          F(5), S.cell_args(), S.kdefs,
                                          support_threshold(S.kdefs.H), s);
-                else
+                else {
+                    const PartWalk pw = S.part_walk(a[6], a[7], a[8]);
                     launch_bi_lapu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
                                    S.N, U(6), U(7), U(8), S.d_status, S.kdefs,
-                                   s);
+                                   s, &pw);
+                }
             } } },
         { "cfd/Boundary/BI/GradP.cl:freeslip",
           { { "iset", "imove", "shepard", "lap_u", "dudt", "grad_p",
@@ -406,10 +414,12 @@ static const std::map<std::string, KernelSpec>& table()
                     launch_bi_interp_cells(S.dims, I(0), F(1), F(2), F(3),
                                            F(4), F(5), S.cell_args(), S.kdefs,
                                            support_threshold(S.kdefs.H), s);
-                else
+                else {
+                    const PartWalk pw = S.part_walk(a[6], a[7], a[8]);
                     launch_bi_interp(S.dims, I(0), F(1), F(2), F(3), F(4),
                                      F(5), S.N, U(6), U(7), U(8), S.d_status,
-                                     S.kdefs, s);
+                                     S.kdefs, s, &pw);
+                }
             } } },
         { "cfd/Boundary/BI/InterpolationShepard.cl:entry",
           { { "iset", "imove", "shepard", "rho", "p", "refd", "cs", "p0" },
@@ -600,11 +610,13 @@ static const std::map<std::string, KernelSpec>& table()
                         S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6), F(7),
                         F(8), near, S.cell_args(), S.kdefs,
                         support_threshold(S.kdefs.H), s);
-                else
+                else {
+                    const PartWalk pw = S.part_walk(a[9], a[10], a[11]);
                     launch_bi_interactions(S.dims, I(0), F(1), F(2), F(3),
                                            F(4), F(5), F(6), F(7), F(8), S.N,
                                            U(9), U(10), U(11), S.d_status,
-                                           S.kdefs, s, near);
+                                           S.kdefs, s, near, &pw);
+                }
             } } },
     };
     return t;

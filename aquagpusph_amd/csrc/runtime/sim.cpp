@@ -45,7 +45,8 @@ Sim::~Sim()
     if (graph) (void)hipGraphDestroy(graph);
     tools.clear();
     for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
-                     (void*)clist.heads, (void*)clist.n_heads, clist.ws })
+                     (void*)clist.heads, (void*)clist.n_heads, clist.ws,
+                     (void*)sel_list, (void*)sel_count })
         if (b) (void)hipFree(b);
     if (fusion.ev0) (void)hipEventDestroy(fusion.ev0);
     if (fusion.ev1) (void)hipEventDestroy(fusion.ev1);
@@ -267,6 +268,10 @@ void Sim::setup()
     // which hold few elements each)
     const char* bw = getenv("AQ_BI_WALK");
     bi_cells = bw && strcmp(bw, "cells") == 0;
+    if (clist.enabled && N && !sel_list) {
+        AQ_HIP(hipMalloc(&sel_list, (size_t)N * sizeof(uint32_t)));
+        AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
+    }
     plan_fusion();
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
@@ -368,6 +373,8 @@ void Sim::plan_fusion()
         fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
         fusion.walk_segw = atoi(ws) == 16 ? 16 : 8;
+    if (const char* fm = getenv("AQ_MATH"))
+        fusion.fast_math = strcmp(fm, "relaxed") == 0;
     fusion.active = true;
 }
 
@@ -399,17 +406,17 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
-                             int segw, int flagged, hipStream_t s);
+                             int segw, int flagged, int fast, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
-                    aq_defs, int, hipStream_t);
+                    aq_defs, int, hipStream_t, const PartWalk*);
 void launch_bi_shepard_compute(int, const int32_t*, const float*,
                                const float*, const float*, const float*,
                                const float*, float*, uint32_t,
                                const uint32_t*, const uint32_t*,
                                const uint32_t*, const uint32_t*, aq_defs,
-                               hipStream_t, int);
+                               hipStream_t, int, const PartWalk*);
 
 namespace rt {
 
@@ -426,11 +433,25 @@ CellArgs Sim::cell_args()
                      clist.kinds ? 1u : 0u };
 }
 
+PartWalk Sim::part_walk(const Var* ic, const Var* ih, const Var* nc)
+{
+    PartWalk pw;
+    if (!clist.enabled || !sel_list) return pw;
+    pw.list = sel_list;
+    pw.count = sel_count;
+    if (clist.usable(ic, ih) && nc == clist.n_cells) {
+        pw.cell_end = clist.cell_end;
+        pw.flagged = clist.kinds ? 1u : 0u;
+    }
+    return pw;
+}
+
 void Sim::run_fused(hipStream_t s)
 {
     FusionPlan& f = fusion;
     auto P = [&](Var* v) { return v ? ptr(v) : nullptr; };
     const float* fr = (const float*)P(f.r);
+    const PartWalk pw = part_walk(f.icell, f.ihoc, f.n_cells);
     // non-fluid part of the Shepard tool first (sensors, elements)
     if (f.shep == 2 && bi_cells && clist.usable(f.icell, f.ihoc))
         launch_bi_shepard_compute_cells(
@@ -447,14 +468,14 @@ void Sim::run_fused(hipStream_t s)
                                   N, (const uint32_t*)P(f.icell),
                                   (const uint32_t*)P(f.ihoc),
                                   (const uint32_t*)P(f.n_cells), d_status,
-                                  kdefs, s, 1);
+                                  kdefs, s, 1, &pw);
     else if (f.shep == 1)
         launch_shepard(dims, (const int32_t*)P(f.imove), fr,
                        (const float*)P(f.rho), (const float*)P(f.m),
                        (float*)P(f.shepard), N, (const uint32_t*)P(f.icell),
                        (const uint32_t*)P(f.ihoc),
                        (const uint32_t*)P(f.n_cells), d_status, kdefs, 1 | 2,
-                       s);
+                       s, &pw);
     if (f.timing) AQ_HIP(hipEventRecord(f.ev0, s));
     if (clist.usable(f.icell, f.ihoc)) {
         launch_fluid_pass_cells(
@@ -469,7 +490,7 @@ void Sim::run_fused(hipStream_t s)
             (const uint32_t*)P(f.n_cells), clist.cell_end, clist.heads,
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
-            f.walk_segw, clist.kinds ? 1 : 0, s);
+            f.walk_segw, clist.kinds ? 1 : 0, f.fast_math ? 1 : 0, s);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

@@ -214,6 +214,7 @@ struct FusionPlan {
     uint8_t* near_buf = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
+    bool fast_math = false;  // pair arithmetic with the OpenCL default rules
     // profiling: events around the walk launch while `timing` is set
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -232,7 +233,7 @@ struct CellList {
     uint32_t* cell_end = nullptr;
     size_t cap = 0;  // cells cell_end holds
     uint32_t* heads = nullptr;
-    uint32_t* n_heads = nullptr;  // [0] count, [1] work counter
+    uint32_t* n_heads = nullptr;  // [0] count, [32..] hand-out counters
     void* ws = nullptr;
     bool usable(const Var* ic, const Var* ih) const
     {
@@ -274,8 +275,14 @@ class Sim {
     FusionPlan fusion;
     CellList clist;
     bool bi_cells = false;  // boundary-integral element kernels cell-parallel
+    // scratch of the per-particle walk kernels' compacted particle lists
+    // (with the cell-parallel walks on): N indices and one counter
+    uint32_t* sel_list = nullptr;
+    uint32_t* sel_count = nullptr;
     void run_fused(hipStream_t s);
     CellArgs cell_args();
+    // aids of a per-particle walk kernel bound to these link-list variables
+    PartWalk part_walk(const Var* icell, const Var* ihoc, const Var* n_cells);
 
     bool defined(const std::string& n) const
     {

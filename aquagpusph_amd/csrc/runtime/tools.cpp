@@ -290,9 +290,11 @@ class LinkListTool : public Tool {
             }
             if (!c.heads) {
                 AQ_HIP(hipMalloc(&c.heads, (size_t)S.N * 4));
-                AQ_HIP(hipMalloc(&c.n_heads, 256));
+                // [0] count, then the walks' hand-out counters (one
+                // 128-byte line each, from word 32)
+                AQ_HIP(hipMalloc(&c.n_heads, 2048));
                 AQ_HIP(hipMalloc(&c.ws, cell_list_workspace(S.N)));
-                AQ_HIP(hipMemset(c.n_heads, 0, 256));
+                AQ_HIP(hipMemset(c.n_heads, 0, 2048));
             }
             Var* im = S.vars.get("imove");
             const bool kinds = im && im->is_array() && im->length >= S.N &&

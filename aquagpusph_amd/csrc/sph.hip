@@ -406,7 +406,86 @@ struct LLArgs {
     const uint32_t* n_cells;
     const uint32_t* status;
     uint32_t N;
+    // optional cell list: cell ends (AQ_CELL_END bits) and, when `flagged`,
+    // the kinds each cell holds
+    const uint32_t* cell_end = nullptr;
+    uint32_t flagged = 0;
 };
+
+// walk() over the cell list when one is given: the same candidates in the
+// same order, the cell end read once per cell instead of icell once per
+// candidate, and cells holding none of the `want` kinds (AQ_CELL_FLUID /
+// AQ_CELL_ELEM) skipped, which only drops candidates the caller rejects.
+template <int D, class Body>
+__device__ __forceinline__ void walk_ll(const LLArgs& ll, uint32_t c_i,
+                                        uint32_t want, Body body)
+{
+    if (!ll.cell_end) {
+        walk<D>(c_i, ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell, ll.ihoc,
+                body);
+        return;
+    }
+    const uint32_t nx = ll.n_cells[0], ny = ll.n_cells[1];
+    if (!ll.flagged) want = 0;
+    constexpr int NC = (D == 2) ? 9 : 27;
+#pragma unroll 1
+    for (int k = 0; k < NC; k++) {
+        int ci, cj, ck;
+        if constexpr (D == 2) {
+            ci = k / 3 - 1;
+            cj = k % 3 - 1;
+            ck = 0;
+        } else {
+            ci = k / 9 - 1;
+            cj = (k / 3) % 3 - 1;
+            ck = k % 3 - 1;
+        }
+        const uint32_t c_j = c_i + (uint32_t)ci + (uint32_t)cj * nx +
+                             (uint32_t)ck * nx * ny;
+        const uint32_t j0 = ll.ihoc[c_j];
+        if (j0 >= ll.N) continue;
+        const uint32_t ce = ll.cell_end[c_j];
+        if (want && !(ce & want)) continue;
+        const uint32_t j1 = ce & AQ_CELL_END;
+        for (uint32_t j = j0; j < j1; j++) body(j);
+    }
+}
+
+// The particles a per-particle kernel runs on: every index below N, or the
+// compacted list[0 .. *count) (in any order: each particle's result is
+// independent of the others).
+struct Sel {
+    const uint32_t* list;
+    const uint32_t* count;
+    __device__ uint32_t n(uint32_t N) const { return list ? *count : N; }
+    __device__ uint32_t at(uint32_t k) const { return list ? list[k] : k; }
+};
+
+// Indices i < N with lo <= imove[i] <= hi (and near[i] != 0 when near is
+// given), appended to list in per-wave runs; *count is zero on entry and
+// ends as their number.
+__global__ void __launch_bounds__(BS)
+    select_k(const int* __restrict__ imove, const uint8_t* __restrict__ near,
+             int lo, int hi, uint32_t N, uint32_t* __restrict__ list,
+             uint32_t* __restrict__ count)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    int im = 0;
+    if (i < N) im = imove[i];
+    const bool take = (i < N) && (im >= lo) && (im <= hi) &&
+                      (!near || near[i]);
+    const uint64_t mk = __ballot(take);
+    if (mk == 0) return;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0)
+        base = atomicAdd(count, (uint32_t)__popcll(mk));
+    base = __builtin_amdgcn_readfirstlane(base);
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u));
+    if (take) list[base + below] = i;
+}
+
+__global__ void set_zero_k(uint32_t* __restrict__ x) { *x = 0u; }
 
 // basic/Shepard.cl:76-125 (cfd/Shepard.cl when cfd != 0)
 template <int D>
@@ -414,27 +493,31 @@ __global__ void __launch_bounds__(BS)
     shepard_k(const int* __restrict__ imove, const float* __restrict__ r,
               const float* __restrict__ rho, const float* __restrict__ m,
               float* __restrict__ shepard, LLArgs ll, float H, float CONW,
-              float smax, int cfd)
+              float smax, int cfd, Sel sel)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ll.N) return;
     AQ_STATUS_SKIP;
-    const int im = imove[i];
-    if ((cfd & 2) && im == 1) return;  // fluid done by the fused walk
     auto excl = [&](int v) { return (cfd & 1) ? (v != 1) : (v >= 3); };
-    if ((im < -3) || ((im > 0) && excl(im))) return;
-    const f3 r_i = ldv<D>(r, i);
-    float acc = shepard[i];
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                if (excl(imove[j])) return;
-                const float q = sqrtf(s) / H;
-                acc += kernelW<D>(q) * CONW * m[j] / rho[j];
-            });
-    shepard[i] = acc;
+    // cfd: fluid candidates only
+    const uint32_t want = (cfd & 1) ? AQ_CELL_FLUID : 0u;
+    const uint32_t n = sel.n(ll.N);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t i = sel.at(k);
+        const int im = imove[i];
+        if ((cfd & 2) && im == 1) continue;  // fluid done by the fused walk
+        if ((im < -3) || ((im > 0) && excl(im))) continue;
+        const f3 r_i = ldv<D>(r, i);
+        float acc = shepard[i];
+        walk_ll<D>(ll, ll.icell[i], want, [&](uint32_t j) {
+            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const float s = dot<D>(r_ij, r_ij);
+            if (s > smax) return;
+            if (excl(imove[j])) return;
+            const float q = sqrtf(s) / H;
+            acc += kernelW<D>(q) * CONW * m[j] / rho[j];
+        });
+        shepard[i] = acc;
+    }
 }
 
 // cfd/Interactions.cl:60-145
@@ -559,56 +642,59 @@ __global__ void __launch_bounds__(BS)
                          const float* __restrict__ binormal,
                          const float* __restrict__ m,
                          float* __restrict__ shepard, LLArgs ll, float H,
-                         float CONW, float DIMS, float smax, int skip_fluid)
+                         float CONW, float DIMS, float smax, int skip_fluid,
+                         Sel sel)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ll.N) return;
     AQ_STATUS_SKIP;
-    const int im = imove[i];
-    if ((im < -3) || (im > 1)) return;
-    if (skip_fluid && im == 1) return;  // fluid done by the fused walk
-    const f3 r_i = ldv<D>(r, i);
-    float acc = 1.f;
-    bool self_added = false;
     const float eps = 1E-8f * H;
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                if (imove[j] != -3) return;
-                if (i == j) {
-                    if (!self_added) {
+    const uint32_t n = sel.n(ll.N);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t i = sel.at(k);
+        const int im = imove[i];
+        if ((im < -3) || (im > 1)) continue;
+        if (skip_fluid && im == 1) continue;  // fluid done by the fused walk
+        const f3 r_i = ldv<D>(r, i);
+        float acc = 1.f;
+        bool self_added = false;
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, [&](uint32_t j) {
+            if (imove[j] != -3) return;
+            if (i == j) {
+                if (!self_added) {
+                    self_added = true;
+                    acc -= 0.5f;
+                }
+                return;
+            }
+            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const float s = dot<D>(r_ij, r_ij);
+            if (s > smax) return;
+            const float q = sqrtf(s) / H;
+            const f3 n_j = ldv<D>(normal, j);
+            const f3 t_j = ldv<D>(tangent, j);
+            const f3 b_j = ldv<D>(binormal, j);
+            const float r_n = dot<D>(r_ij, n_j);
+            const float area_j = m[j];
+            if ((r_n > -eps) && (r_n < eps)) {
+                if (!self_added) {
+                    const float r_t = fabsf(dot<D>(r_ij, t_j));
+                    const float r_b = fabsf(dot<D>(r_ij, b_j));
+                    const float dr =
+                        0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                    if ((r_t <= dr) && (r_b <= dr)) {
                         self_added = true;
                         acc -= 0.5f;
                     }
-                    return;
                 }
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                const float q = sqrtf(s) / H;
-                const f3 n_j = ldv<D>(normal, j);
-                const f3 t_j = ldv<D>(tangent, j);
-                const f3 b_j = ldv<D>(binormal, j);
-                const float r_n = dot<D>(r_ij, n_j);
-                const float area_j = m[j];
-                if ((r_n > -eps) && (r_n < eps)) {
-                    if (!self_added) {
-                        const float r_t = fabsf(dot<D>(r_ij, t_j));
-                        const float r_b = fabsf(dot<D>(r_ij, b_j));
-                        const float dr =
-                            0.55f * powf(area_j, 1.f / (DIMS - 1.f));
-                        if ((r_t <= dr) && (r_b <= dr)) {
-                            self_added = true;
-                            acc -= 0.5f;
-                        }
-                    }
-                    return;
-                }
-                const float r_t = fabsf(dot<D>(r_ij, t_j));
-                const float r_b = fabsf(dot<D>(r_ij, b_j));
-                acc += r_n * CONW * kernelS_P<D>(q) * area_j +
-                       kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
-            });
-    shepard[i] = acc;
+                return;
+            }
+            const float r_t = fabsf(dot<D>(r_ij, t_j));
+            const float r_b = fabsf(dot<D>(r_ij, b_j));
+            acc += r_n * CONW * kernelS_P<D>(q) * area_j +
+                   kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
+        });
+        shepard[i] = acc;
+    }
 }
 
 // cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
@@ -617,36 +703,38 @@ __global__ void __launch_bounds__(BS)
     bi_lapu_k(const int* __restrict__ imove, const float* __restrict__ r,
               const float* __restrict__ u, const float* __restrict__ rho,
               const float* __restrict__ m, float* __restrict__ lap_u,
-              LLArgs ll, float H, float CONF, float smax, int lap)
+              LLArgs ll, float H, float CONF, float smax, int lap, Sel sel)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ll.N) return;
     AQ_STATUS_SKIP;
-    if (imove[i] != -3) return;
-    constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
-    const f3 r_i = ldv<D>(r, i);
-    const f3 u_i = ldv<D>(u, i);
-    f3 lu = ldv<D>(lap_u, i);
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                if (i == j) return;
-                if (imove[j] != 1) return;
-                const float q = sqrtf(s) / H;
-                const float rho_j = rho[j];
-                const float f_ij = kernelF<D>(q) * CONF * m[j];
-                const f3 u_ij = sub<D>(ldv<D>(u, j), u_i);
-                if (lap == AQ_LAP_MONAGHAN) {
-                    const float udr = dot<D>(u_ij, r_ij);
-                    const float r2 = (q * q + 0.01f) * H * H;
-                    axpy<D>(lu, f_ij * CLEARY * udr / (r2 * rho_j), r_ij);
-                } else {
-                    axpy<D>(lu, f_ij * 2.f / rho_j, u_ij);
-                }
-            });
-    stxyz<D>(lap_u, i, lu);
+    const uint32_t n = sel.n(ll.N);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t i = sel.at(k);
+        if (imove[i] != -3) continue;
+        constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
+        const f3 r_i = ldv<D>(r, i);
+        const f3 u_i = ldv<D>(u, i);
+        f3 lu = ldv<D>(lap_u, i);
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, [&](uint32_t j) {
+            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const float s = dot<D>(r_ij, r_ij);
+            if (s > smax) return;
+            if (i == j) return;
+            if (imove[j] != 1) return;
+            const float q = sqrtf(s) / H;
+            const float rho_j = rho[j];
+            const float f_ij = kernelF<D>(q) * CONF * m[j];
+            const f3 u_ij = sub<D>(ldv<D>(u, j), u_i);
+            if (lap == AQ_LAP_MONAGHAN) {
+                const float udr = dot<D>(u_ij, r_ij);
+                const float r2 = (q * q + 0.01f) * H * H;
+                axpy<D>(lu, f_ij * CLEARY * udr / (r2 * rho_j), r_ij);
+            } else {
+                axpy<D>(lu, f_ij * 2.f / rho_j, u_ij);
+            }
+        });
+        stxyz<D>(lap_u, i, lu);
+    }
 }
 
 // cfd/Boundary/BI/Interpolation.cl:54-107
@@ -655,30 +743,32 @@ __global__ void __launch_bounds__(BS)
     bi_interp_k(const int* __restrict__ imove, const float* __restrict__ r,
                 const float* __restrict__ m, const float* __restrict__ rho,
                 const float* __restrict__ grad_p, float* __restrict__ p,
-                LLArgs ll, float H, float CONW, float smax)
+                LLArgs ll, float H, float CONW, float smax, Sel sel)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ll.N) return;
     AQ_STATUS_SKIP;
-    if (imove[i] != -3) return;
-    const f3 r_i = ldv<D>(r, i);
-    const float rho_i = rho[i];
-    f3 gp_i = ldv<D>(grad_p, i);
-    gp_i.x = rho_i * gp_i.x;
-    gp_i.y = rho_i * gp_i.y;
-    gp_i.z = rho_i * gp_i.z;
-    float acc = 0.f;
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                if (imove[j] != 1) return;
-                const float q = sqrtf(s) / H;
-                const float w_ij = kernelW<D>(q) * CONW * m[j] / rho[j];
-                acc += (p[j] - dot<D>(gp_i, r_ij)) * w_ij;
-            });
-    p[i] = acc;
+    const uint32_t n = sel.n(ll.N);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t i = sel.at(k);
+        if (imove[i] != -3) continue;
+        const f3 r_i = ldv<D>(r, i);
+        const float rho_i = rho[i];
+        f3 gp_i = ldv<D>(grad_p, i);
+        gp_i.x = rho_i * gp_i.x;
+        gp_i.y = rho_i * gp_i.y;
+        gp_i.z = rho_i * gp_i.z;
+        float acc = 0.f;
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, [&](uint32_t j) {
+            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const float s = dot<D>(r_ij, r_ij);
+            if (s > smax) return;
+            if (imove[j] != 1) return;
+            const float q = sqrtf(s) / H;
+            const float w_ij = kernelW<D>(q) * CONW * m[j] / rho[j];
+            acc += (p[j] - dot<D>(gp_i, r_ij)) * w_ij;
+        });
+        p[i] = acc;
+    }
 }
 
 // cfd/Boundary/BI/Interactions.cl:51-133
@@ -692,37 +782,40 @@ __global__ void __launch_bounds__(BS)
                       const float* __restrict__ m,
                       const float* __restrict__ p, float* __restrict__ grad_p,
                       float* __restrict__ div_u, LLArgs ll, float H,
-                      float CONW, float smax, const uint8_t* __restrict__ near)
+                      float CONW, float smax, const uint8_t* __restrict__ near,
+                      Sel sel)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ll.N) return;
     AQ_STATUS_SKIP;
-    if (imove[i] != 1) return;
-    // no element inside the support: nothing is added (bit-identical skip)
-    if (near && !near[i]) return;
-    const f3 r_i = ldv<D>(r, i);
-    const f3 u_i = ldv<D>(u, i);
-    const float p_i = p[i];
-    const float rho_i = rho[i];
-    f3 gp = ldv<D>(grad_p, i);
-    float dv = div_u[i];
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                if (imove[j] != -3) return;
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                const float q = sqrtf(s) / H;
-                const f3 n_j = ldv<D>(normal, j);
-                const float area_j = m[j];
-                const float p_j = p[j];
-                const f3 du = sub<D>(ldv<D>(u, j), u_i);
-                const float w_ij = kernelW<D>(q) * CONW * area_j;
-                axpy<D>(gp, (p_i + p_j) / rho_i * w_ij, n_j);
-                dv += rho_i * dot<D>(du, n_j) * w_ij;
-            });
-    stxyz<D>(grad_p, i, gp);
-    div_u[i] = dv;
+    const uint32_t n = sel.n(ll.N);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t i = sel.at(k);
+        if (imove[i] != 1) continue;
+        // no element inside the support: nothing is added (bit-identical skip)
+        if (near && !near[i]) continue;
+        const f3 r_i = ldv<D>(r, i);
+        const f3 u_i = ldv<D>(u, i);
+        const float p_i = p[i];
+        const float rho_i = rho[i];
+        f3 gp = ldv<D>(grad_p, i);
+        float dv = div_u[i];
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, [&](uint32_t j) {
+            if (imove[j] != -3) return;
+            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const float s = dot<D>(r_ij, r_ij);
+            if (s > smax) return;
+            const float q = sqrtf(s) / H;
+            const f3 n_j = ldv<D>(normal, j);
+            const float area_j = m[j];
+            const float p_j = p[j];
+            const f3 du = sub<D>(ldv<D>(u, j), u_i);
+            const float w_ij = kernelW<D>(q) * CONW * area_j;
+            axpy<D>(gp, (p_i + p_j) / rho_i * w_ij, n_j);
+            dv += rho_i * dot<D>(du, n_j) * w_ij;
+        });
+        stxyz<D>(grad_p, i, gp);
+        div_u[i] = dv;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -882,6 +975,39 @@ static LLArgs mkll(const uint32_t* icell, const uint32_t* ihoc,
     return a;
 }
 
+// Link-list arguments of a per-particle walk kernel, with the cell list of
+// `pw` when there is one.
+static LLArgs mkll_pw(const uint32_t* icell, const uint32_t* ihoc,
+                      const uint32_t* n_cells, const uint32_t* status,
+                      uint32_t N, const PartWalk* pw)
+{
+    LLArgs a = mkll(icell, ihoc, n_cells, status, N);
+    if (pw && pw->cell_end) {
+        a.cell_end = pw->cell_end;
+        a.flagged = pw->flagged;
+    }
+    return a;
+}
+
+// The particles a per-particle kernel runs on: all N, or (with the scratch
+// of `pw`) the compacted indices with lo <= imove <= hi (and near != 0).
+static Sel select_parts(const PartWalk* pw, const int32_t* imove,
+                        const uint8_t* near, int lo, int hi, uint32_t N,
+                        hipStream_t s)
+{
+    if (!pw || !pw->list || !pw->count) return Sel{ nullptr, nullptr };
+    hipLaunchKernelGGL(set_zero_k, dim3(1), dim3(1), 0, s, pw->count);
+    hipLaunchKernelGGL(select_k, AQ_GRID(N), imove, near, lo, hi, N,
+                       pw->list, pw->count);
+    return Sel{ pw->list, pw->count };
+}
+
+// One thread per particle, or a resident grid striding over a list.
+static dim3 sel_grid(const void* ker, const Sel& sel, uint32_t N)
+{
+    return sel.list ? dim3(resident_blocks(ker, BS)) : dim3(blocks_for(N, BS));
+}
+
 void launch_predictor(int dims, const int32_t* imove, const float* r,
                       const float* u, const float* dudt, const float* rho,
                       const float* drhodt, float* r_in, float* u_in,
@@ -996,11 +1122,16 @@ void launch_shepard(int dims, const int32_t* imove, const float* r,
                     const float* rho, const float* m, float* shepard,
                     uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
                     const uint32_t* n_cells, const uint32_t* status,
-                    aq_defs d, int cfd, hipStream_t s)
+                    aq_defs d, int cfd, hipStream_t s, const PartWalk* pw)
 {
-    AQ_DISPATCH(dims, shepard_k, AQ_GRID(N), imove, r, rho, m, shepard,
-                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
-                support_threshold(d.H), cfd);
+    // with the fluid done elsewhere (cfd & 2) it runs on -3 <= imove <= 0
+    const LLArgs ll = mkll_pw(icell, ihoc, n_cells, status, N, pw);
+    const Sel sel = select_parts((cfd & 2) ? pw : nullptr, imove, nullptr,
+                                 -3, 0, N, s);
+    auto* ker = (dims == 2) ? shepard_k<2> : shepard_k<3>;
+    hipLaunchKernelGGL(ker, sel_grid((const void*)ker, sel, N), dim3(BS), 0,
+                       s, imove, r, rho, m, shepard, ll, d.H, d.CONW,
+                       support_threshold(d.H), cfd, sel);
 }
 void launch_interactions(int dims, const int32_t* imove, const float* r,
                          const float* u, const float* rho, const float* m,
@@ -1040,32 +1171,46 @@ void launch_bi_shepard_compute(int dims, const int32_t* imove, const float* r,
                                const uint32_t* icell, const uint32_t* ihoc,
                                const uint32_t* n_cells,
                                const uint32_t* status, aq_defs d,
-                               hipStream_t s, int skip_fluid)
+                               hipStream_t s, int skip_fluid,
+                               const PartWalk* pw)
 {
-    AQ_DISPATCH(dims, bi_shepard_compute_k, AQ_GRID(N), imove, r, normal,
-                tangent, binormal, m, shepard,
-                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW, d.DIMS,
-                support_threshold(d.H), skip_fluid);
+    const LLArgs ll = mkll_pw(icell, ihoc, n_cells, status, N, pw);
+    const Sel sel = select_parts(pw, imove, nullptr, -3, skip_fluid ? 0 : 1,
+                                 N, s);
+    auto* ker =
+        (dims == 2) ? bi_shepard_compute_k<2> : bi_shepard_compute_k<3>;
+    hipLaunchKernelGGL(ker, sel_grid((const void*)ker, sel, N), dim3(BS), 0,
+                       s, imove, r, normal, tangent, binormal, m, shepard, ll,
+                       d.H, d.CONW, d.DIMS, support_threshold(d.H),
+                       skip_fluid, sel);
 }
 void launch_bi_lapu(int dims, const int32_t* imove, const float* r,
                     const float* u, const float* rho, const float* m,
                     float* lap_u, uint32_t N, const uint32_t* icell,
                     const uint32_t* ihoc, const uint32_t* n_cells,
-                    const uint32_t* status, aq_defs d, hipStream_t s)
+                    const uint32_t* status, aq_defs d, hipStream_t s,
+                    const PartWalk* pw)
 {
-    AQ_DISPATCH(dims, bi_lapu_k, AQ_GRID(N), imove, r, u, rho, m, lap_u,
-                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONF,
-                support_threshold(d.H), d.lap_formulation);
+    const LLArgs ll = mkll_pw(icell, ihoc, n_cells, status, N, pw);
+    const Sel sel = select_parts(pw, imove, nullptr, -3, -3, N, s);
+    auto* ker = (dims == 2) ? bi_lapu_k<2> : bi_lapu_k<3>;
+    hipLaunchKernelGGL(ker, sel_grid((const void*)ker, sel, N), dim3(BS), 0,
+                       s, imove, r, u, rho, m, lap_u, ll, d.H, d.CONF,
+                       support_threshold(d.H), d.lap_formulation, sel);
 }
 void launch_bi_interp(int dims, const int32_t* imove, const float* r,
                       const float* m, const float* rho, const float* grad_p,
                       float* p, uint32_t N, const uint32_t* icell,
                       const uint32_t* ihoc, const uint32_t* n_cells,
-                      const uint32_t* status, aq_defs d, hipStream_t s)
+                      const uint32_t* status, aq_defs d, hipStream_t s,
+                      const PartWalk* pw)
 {
-    AQ_DISPATCH(dims, bi_interp_k, AQ_GRID(N), imove, r, m, rho, grad_p, p,
-                mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
-                support_threshold(d.H));
+    const LLArgs ll = mkll_pw(icell, ihoc, n_cells, status, N, pw);
+    const Sel sel = select_parts(pw, imove, nullptr, -3, -3, N, s);
+    auto* ker = (dims == 2) ? bi_interp_k<2> : bi_interp_k<3>;
+    hipLaunchKernelGGL(ker, sel_grid((const void*)ker, sel, N), dim3(BS), 0,
+                       s, imove, r, m, rho, grad_p, p, ll, d.H, d.CONW,
+                       support_threshold(d.H), sel);
 }
 void launch_bi_interactions(int dims, const int32_t* imove, const float* r,
                             const float* normal, const float* u,
@@ -1073,11 +1218,15 @@ void launch_bi_interactions(int dims, const int32_t* imove, const float* r,
                             float* grad_p, float* div_u, uint32_t N,
                             const uint32_t* icell, const uint32_t* ihoc,
                             const uint32_t* n_cells, const uint32_t* status,
-                            aq_defs d, hipStream_t s, const uint8_t* near)
+                            aq_defs d, hipStream_t s, const uint8_t* near,
+                            const PartWalk* pw)
 {
-    AQ_DISPATCH(dims, bi_interactions_k, AQ_GRID(N), imove, r, normal, u, rho,
-                m, p, grad_p, div_u, mkll(icell, ihoc, n_cells, status, N),
-                d.H, d.CONW, support_threshold(d.H), near);
+    const LLArgs ll = mkll_pw(icell, ihoc, n_cells, status, N, pw);
+    const Sel sel = select_parts(pw, imove, near, 1, 1, N, s);
+    auto* ker = (dims == 2) ? bi_interactions_k<2> : bi_interactions_k<3>;
+    hipLaunchKernelGGL(ker, sel_grid((const void*)ker, sel, N), dim3(BS), 0,
+                       s, imove, r, normal, u, rho, m, p, grad_p, div_u, ll,
+                       d.H, d.CONW, support_threshold(d.H), near, sel);
 }
 
 void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
@@ -1322,7 +1471,7 @@ AQ_API int aq_shepard(int dims, const int32_t* imove, const float* r,
         AQ_PTRS(imove, r, rho, m, shepard, icell, ihoc, n_cells);
         launch_shepard(dims, imove, r, rho, m, shepard, N, icell, ihoc,
                        n_cells, nullptr, defs, cfd_variant ? 1 : 0,
-                       S(stream));
+                       S(stream), nullptr);
     });
 }
 
@@ -1392,7 +1541,7 @@ AQ_API int aq_bi_shepard_compute(int dims, const int32_t* imove,
                 n_cells);
         launch_bi_shepard_compute(dims, imove, r, normal, tangent, binormal, m,
                                   shepard, N, icell, ihoc, n_cells, nullptr,
-                                  defs, S(stream), 0);
+                                  defs, S(stream), 0, nullptr);
     });
 }
 
@@ -1408,7 +1557,7 @@ AQ_API int aq_bi_lapu(int dims, const int32_t* imove, const float* r,
         if (!N) return;
         AQ_PTRS(imove, r, u, rho, m, lap_u, icell, ihoc, n_cells);
         launch_bi_lapu(dims, imove, r, u, rho, m, lap_u, N, icell, ihoc,
-                       n_cells, nullptr, defs, S(stream));
+                       n_cells, nullptr, defs, S(stream), nullptr);
     });
 }
 
@@ -1424,7 +1573,7 @@ AQ_API int aq_bi_interp(int dims, const int32_t* imove, const float* r,
         if (!N) return;
         AQ_PTRS(imove, r, m, rho, grad_p, p, icell, ihoc, n_cells);
         launch_bi_interp(dims, imove, r, m, rho, grad_p, p, N, icell, ihoc,
-                         n_cells, nullptr, defs, S(stream));
+                         n_cells, nullptr, defs, S(stream), nullptr);
     });
 }
 
@@ -1444,7 +1593,7 @@ AQ_API int aq_bi_interactions(int dims, const int32_t* imove, const float* r,
                 n_cells);
         launch_bi_interactions(dims, imove, r, normal, u, rho, m, p, grad_p,
                                div_u, N, icell, ihoc, n_cells, nullptr, defs,
-                               S(stream), nullptr);
+                               S(stream), nullptr, nullptr);
     });
 }
 

@@ -43,6 +43,16 @@ __device__ __forceinline__ uint32_t uni(uint32_t x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
+// Cells are handed out in NRANGE contiguous ranges of the non-empty cell
+// list, one per XCD: workgroups are dispatched to the XCDs round-robin, so
+// workgroup b runs on XCD b % NRANGE and takes the cells of range b % NRANGE
+// first; the neighbourhoods in flight on one XCD then lie in one slab of
+// the domain and share that XCD's L2.  A workgroup whose range is done
+// helps with the next ones.  Each range has its own counter, RSTRIDE words
+// apart (one cache line each).
+constexpr uint32_t NRANGE = 8;
+constexpr uint32_t RSTRIDE = 32;
+
 struct WalkArgs {
     const uint32_t* icell;
     const uint32_t* ihoc;
@@ -50,11 +60,37 @@ struct WalkArgs {
     const uint32_t* cell_end;
     const uint32_t* heads;
     const uint32_t* n_heads;
-    uint32_t* next;  // work counter: next cell to hand out (zero at launch)
+    uint32_t* next;  // range counters (zero at launch), RSTRIDE words apart
     const uint32_t* status;
     uint32_t N;
     uint32_t flagged;  // cell_end carries AQ_CELL_FLUID / AQ_CELL_ELEM flags
+    uint32_t ranges;   // 1: one global counter; NRANGE: per-XCD ranges
 };
+
+// Next cell (index into heads) for the calling wave, n_heads when all are
+// taken.  `q` is the wave's current range (starts at its XCD's).
+__device__ __forceinline__ uint32_t next_cell(const WalkArgs& w,
+                                              uint32_t n_heads, int lane,
+                                              uint32_t& q)
+{
+    uint32_t hk = n_heads;
+    if (lane == 0) {
+        const uint32_t nr = w.ranges;
+        const uint32_t home = q;
+        while (q < home + nr) {
+            const uint32_t k = q % nr;
+            const uint32_t lo = (uint32_t)(((uint64_t)n_heads * k) / nr);
+            const uint32_t hi = (uint32_t)(((uint64_t)n_heads * (k + 1)) / nr);
+            const uint32_t t = atomicAdd(w.next + k * RSTRIDE, 1u);
+            if (t < hi - lo) {
+                hk = lo + t;
+                break;
+            }
+            q++;
+        }
+    }
+    return uni(hk);
+}
 
 // Shared state of one wavefront: neighbour-cell table, the candidate chunk
 // being tested and the two segment slots (candidate indices, accept bits;
@@ -122,21 +158,18 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
 // tested and reported through `el_near`.  pair(j) is called for the
 // accepted candidates of each lane in increasing candidate order.
 //
-// UNIFORM: pair(j, has) is called on every lane at every step, with
-// has = false and j = idle_j on the lanes that have no candidate at that
-// step (idle_j = the particle itself, whose terms are exact zeros, or any
-// valid index on lanes whose sums are discarded).  The pair arithmetic then
-// runs outside any branch: no exec-mask save/restore, no copies of the
-// accumulators.
-template <int D, int SEGW, bool UNIFORM = false, class Pair>
+// (A branch-free variant, calling pair() on every lane at every step with a
+// dummy index on the idle lanes, measured 25% slower on the 3D dam break:
+// the idle lanes' loads and the longer dependent chains cost more than the
+// exec-mask changes they save.)
+template <int D, int SEGW, class Pair>
 __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
                                           uint32_t total, const f3& r_i,
                                           bool act, float smax,
                                           const float* __restrict__ r,
                                           const int* __restrict__ imove,
                                           int kind, bool flag_el,
-                                          bool& el_near, Pair pair,
-                                          uint32_t idle_j = 0)
+                                          bool& el_near, Pair pair)
 {
     constexpr uint32_t SEG = SEGW * 32;
     const uint32_t nsegs = (total + SEG - 1) / SEG;
@@ -226,26 +259,13 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_s_waitcnt(0);
             tested++;
-            // UNIFORM keeps a single path through the loop body (the pair
-            // arithmetic is never skipped, so its accumulators need no
-            // copies between paths)
-            if constexpr (!UNIFORM) continue;
-            advance();
+            continue;
         }
-        const bool has = bw != 0;
-        if (__ballot(has) == 0 && (!UNIFORM || tested >= nsegs)) break;
-        if constexpr (UNIFORM) {
-            // first set bit (31 on an empty word: read, never used)
-            const uint32_t b = __builtin_clz(bw | 1u);
-            bw = has ? (bw ^ (0x80000000u >> b)) : 0u;
-            const uint32_t jc = cand[((nw - 1) % RING) * 32 + b];
-            pair(has ? jc : idle_j, has);
-        } else {
-            if (has) {
-                const uint32_t b = __builtin_clz(bw);
-                bw ^= 0x80000000u >> b;
-                pair(cand[((nw - 1) % RING) * 32 + b]);
-            }
+        if (__ballot(bw != 0) == 0) break;
+        if (bw != 0) {
+            const uint32_t b = __builtin_clz(bw);
+            bw ^= 0x80000000u >> b;
+            pair(cand[((nw - 1) % RING) * 32 + b]);
         }
     }
     el_near = eln;
@@ -255,7 +275,7 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
 // ---------------------------------------------------------------------------
 // Fused fluid walk (same terms as fluid_pass_k in sph.hip).
 // ---------------------------------------------------------------------------
-template <int D, int SHEP, bool LAPP, int SEGW, int LAPF>
+template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, bool FAST>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     fluid_pass_cells_k(const int* __restrict__ imove,
                        const float* __restrict__ r,
@@ -270,7 +290,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                        float* __restrict__ lap_u, float* __restrict__ div_u,
                        float* __restrict__ lap_p, uint8_t* __restrict__ near,
                        WalkArgs w, float H, float CONW, float CONF, float DIMS,
-                       float smax)
+                       float smax, float iH)
 {
     constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
     __shared__ WaveLds<SEGW> lds[WC_WAVES];
@@ -284,11 +304,10 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     // sum and for the near-wall flags
     const bool flag_el = (SHEP == 2) || (near != nullptr);
 
+    uint32_t range = blockIdx.x % w.ranges;
     while (true) {
         // cells are handed out one at a time (a vector atomic from lane 0)
-        uint32_t hk = 0;
-        if (lane == 0) hk = atomicAdd(w.next, 1u);
-        hk = uni(hk);
+        const uint32_t hk = next_cell(w, n_heads, lane, range);
         if (hk >= n_heads) break;
         const uint32_t cstart = uni(w.heads[hk]);
         const uint32_t c_i = uni(w.icell[cstart]);
@@ -317,26 +336,55 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 if (LAPP) lp = lap_p[i];
                 sh = (SHEP == 2) ? 1.f : (SHEP == 1 ? shepard[i] : 0.f);
             }
+            const float irho_i = FAST ? __builtin_amdgcn_rcpf(rho_i) : 0.f;
             // ---- fluid neighbours ------------------------------------------
             bool nearw = false;
-            ring_walk<D, SEGW, true>(
+            ring_walk<D, SEGW>(
                 L, lane, total, r_i, act, smax, r, imove, 0, flag_el, nearw,
-                [&](uint32_t j, bool has) {
+                [&](uint32_t j) {
+                    if constexpr (FAST) {
+                        // OpenCL C default accuracy: reciprocal-based
+                        // division, hardware square root, contracted
+                        // multiply-adds
+                        const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+                        const float s = fdot<D>(r_ij, r_ij);
+                        const float q = __builtin_amdgcn_sqrtf(s) * iH;
+                        const float m_j = ldf_o(m, j);
+                        const float irho_j =
+                            __builtin_amdgcn_rcpf(ldf_o(rho, j));
+                        if (SHEP == 1)
+                            sh = fmaf(kernelW<D>(q) * CONW * m_j, irho_j, sh);
+                        const float p_j = ldf_o(p, j);
+                        const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
+                        const float udr = fdot<D>(u_ij, r_ij);
+                        const float f_ij = kernelF<D>(q) * CONF * m_j;
+                        const float fr = f_ij * irho_i * irho_j;
+                        faxpy<D>(gp, (p_i + p_j) * fr, r_ij);
+                        if constexpr (LAPF == AQ_LAP_MONAGHAN) {
+                            const float r2 = fmaf(q, q, 0.01f) * (H * H);
+                            faxpy<D>(lu,
+                                     CLEARY * udr * fr *
+                                         __builtin_amdgcn_rcpf(r2),
+                                     r_ij);
+                        } else {
+                            faxpy<D>(lu, 2.f * fr, u_ij);
+                        }
+                        dv = fmaf(udr * f_ij * rho_i, irho_j, dv);
+                        if (LAPP) lp = fmaf(p_j - p_i, f_ij * irho_j, lp);
+                        return;
+                    }
                     const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
                     const float s = dot<D>(r_ij, r_ij);
                     const float q = sqrtf(s) / H;
                     const float m_j = ldf_o(m, j);
                     const float rho_j = ldf_o(rho, j);
-                    // (+0 leaves the sum unchanged: it is never -0)
-                    if (SHEP == 1)
-                        sh += has ? kernelW<D>(q) * CONW * m_j / rho_j : 0.f;
+                    if (SHEP == 1) sh += kernelW<D>(q) * CONW * m_j / rho_j;
                     // The reference skips j == i here.  For i itself r_ij,
                     // u_ij and p_j - p_i are exact zeros, so every term
                     // below is +-0 and leaves the accumulators unchanged
                     // (they are never -0: they start at +0 and an exact
                     // cancellation rounds to +0); with finite fields the
-                    // sums are bit-identical without the test.  The same
-                    // holds for the idle steps, which evaluate j = i.
+                    // sums are bit-identical without the test.
                     const float p_j = ldf_o(p, j);
                     const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
                     const float udr = dot<D>(u_ij, r_ij);
@@ -353,8 +401,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     }
                     dv += udr * f_ij * rho_i / rho_j;
                     if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
-                },
-                act ? i : cstart);
+                });
             // ---- wall elements (boundary-integral Shepard) ------------------
             if (SHEP == 2 && __ballot(nearw) != 0) {
                 bool self_added = false;
@@ -425,10 +472,9 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     WaveLds<SEGW>& L = lds[threadIdx.x >> 6];
     if (w.status && (*w.status & 1u)) return;
     const uint32_t n_heads = uni(*w.n_heads);
+    uint32_t range = blockIdx.x % w.ranges;
     while (true) {
-        uint32_t hk = 0;
-        if (lane == 0) hk = atomicAdd(w.next, 1u);
-        hk = uni(hk);
+        const uint32_t hk = next_cell(w, n_heads, lane, range);
         if (hk >= n_heads) break;
         const uint32_t cstart = uni(w.heads[hk]);
         const uint32_t c_i = uni(w.icell[cstart]);
@@ -655,15 +701,26 @@ template <int D> struct PolBiInteractions {
     }
 };
 
+// Clears the NRANGE range counters.
 __global__ void zero_u32_k(uint32_t* p)
 {
-    if (threadIdx.x == 0) *p = 0u;
+    if (threadIdx.x < NRANGE) p[threadIdx.x * RSTRIDE] = 0u;
+}
+
+// Cell hand-out ranges: NRANGE (one per XCD) unless AQ_WALK_RANGES=1.
+static uint32_t walk_ranges()
+{
+    static const uint32_t nr = [] {
+        const char* e = getenv("AQ_WALK_RANGES");
+        return (e && atoi(e) == 1) ? 1u : NRANGE;
+    }();
+    return nr;
 }
 
 // Workgroups of `ker` that fit on the device at once (all CUs), cached per
 // kernel: the walk hands cells out dynamically, so a grid of exactly that
 // many workgroups keeps every slot busy until the cells run out.
-static unsigned resident_blocks(const void* ker, int block)
+unsigned resident_blocks(const void* ker, int block)
 {
     static std::mutex mu;
     static std::map<std::pair<int, const void*>, unsigned> cache;
@@ -695,24 +752,26 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
-                             int flagged, hipStream_t s)
+                             int flagged, int fast, hipStream_t s)
 {
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
-    // the work counter lives next to the non-empty cell count
-    uint32_t* next = const_cast<uint32_t*>(n_heads) + 1;
+    // the hand-out counters live after the non-empty cell count
+    uint32_t* next = const_cast<uint32_t*>(n_heads) + RSTRIDE;
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
-    WalkArgs w{ icell,   ihoc, n_cells, cell_end, heads,
-                n_heads, next, status,  N,        (uint32_t)flagged };
+    WalkArgs w{ icell,  ihoc,   n_cells, cell_end, heads,
+                n_heads, next, status, N,       (uint32_t)flagged,
+                walk_ranges() };
 #define AQ_FC2(DD, SH, LP, SW, LF)                                             \
     do {                                                                       \
-        auto* ker = fluid_pass_cells_k<DD, SH, LP, SW, LF>;                    \
+        auto* ker = fast ? fluid_pass_cells_k<DD, SH, LP, SW, LF, true>        \
+                         : fluid_pass_cells_k<DD, SH, LP, SW, LF, false>;      \
         const unsigned nb = nblocks ? nblocks : resident_blocks((const void*)ker, \
                                                     64 * WC_WAVES);            \
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \
                            u, rho, m, p, normal, tangent, binormal, shepard,   \
                            grad_p, lap_u, div_u, lap_p, near, w, d.H, d.CONW,  \
-                           d.CONF, d.DIMS, smax);                              \
+                           d.CONF, d.DIMS, smax, 1.f / d.H);                   \
     } while (0)
 #define AQ_FC(DD, SH, LP, SW)                                                  \
     do {                                                                       \
@@ -746,10 +805,11 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
                          const CellArgs& c, float smax, hipStream_t s)
 {
     AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
-    uint32_t* next = const_cast<uint32_t*>(c.n_heads) + 1;
+    uint32_t* next = const_cast<uint32_t*>(c.n_heads) + RSTRIDE;
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
-    WalkArgs w{ c.icell,   c.ihoc, c.n_cells, c.cell_end, c.heads,
-                c.n_heads, next,   c.status,  c.N,        c.flagged };
+    WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
+                c.n_heads, next,     c.status,  c.N,        c.flagged,
+                walk_ranges() };
     auto* ker = cell_kernel_k<D, 8, Pol>;
     const unsigned nb = resident_blocks((const void*)ker, 64 * WC_WAVES);
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,

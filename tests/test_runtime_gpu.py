@@ -162,18 +162,23 @@ def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     b.close()
 
 
-@pytest.mark.parametrize("dims,bi,walk,no_fusion", [(2, True, "thread", "0"),
-                                               (3, True, "thread", "0"),
-                                               (3, False, "thread", "0"),
-                                               (2, True, "cells", "0"),
-                                               (3, True, "cells", "0"),
-                                               (3, True, "cells", "1")])
-def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, no_fusion,
-                                 monkeypatch):
+@pytest.mark.parametrize("dims,bi,walk,bi_walk,no_fusion", [
+    (2, True, "thread", "thread", "0"),
+    (3, True, "thread", "thread", "0"),
+    (3, False, "thread", "thread", "0"),
+    (2, True, "cells", "cells", "0"),
+    (3, True, "cells", "cells", "0"),
+    (3, True, "cells", "cells", "1"),
+    (2, True, "cells", "thread", "0"),
+    (3, True, "cells", "thread", "0"),
+    (3, True, "cells", "thread", "1")])
+def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, bi_walk,
+                                 no_fusion, monkeypatch):
     """The fused neighbour walk (Shepard + interactions + lap p + near-wall
-    flags) and the cell-parallel pair kernels (fused walk, boundary-integral
-    element kernels) reproduce the separate per-particle kernels, which
-    follow the reference walk order, bit for bit."""
+    flags), the cell-parallel pair kernels (fused walk, boundary-integral
+    element kernels) and the per-particle kernels run over compacted particle
+    lists with cell-kind skipping reproduce the separate per-particle
+    kernels, which follow the reference walk order, bit for bit."""
     case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
     presets = casefiles.PBI_PRESETS if bi else tuple(
         p for p in casefiles.PBI_PRESETS if not p.endswith("BI.xml"))
@@ -184,7 +189,7 @@ def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, no_fusion,
     a = Simulation(xml, dims)
     monkeypatch.setenv("AQ_NO_FUSION", no_fusion)
     monkeypatch.setenv("AQ_WALK", walk)
-    monkeypatch.setenv("AQ_BI_WALK", walk)
+    monkeypatch.setenv("AQ_BI_WALK", bi_walk)
     b = Simulation(xml, dims)
     for _ in range(3):
         a.step(3)
