@@ -263,11 +263,11 @@ void Sim::setup()
     clist.icell = vars.get("icell");
     clist.ihoc = vars.get("ihoc");
     clist.n_cells = vars.get("n_cells");
-    // boundary-integral element kernels: per-particle walks unless
-    // AQ_BI_WALK=cells (the cell-parallel forms are slower on wall cells,
-    // which hold few elements each)
+    // boundary-integral element kernels: cell-parallel unless
+    // AQ_BI_WALK=thread asks for the per-particle forms (over compacted
+    // particle lists while the cell list is on)
     const char* bw = getenv("AQ_BI_WALK");
-    bi_cells = bw && strcmp(bw, "cells") == 0;
+    bi_cells = clist.enabled && !(bw && strcmp(bw, "thread") == 0);
     if (clist.enabled && N && !sel_list) {
         AQ_HIP(hipMalloc(&sel_list, (size_t)N * sizeof(uint32_t)));
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));

@@ -414,17 +414,24 @@ struct LLArgs {
 
 // walk() over the cell list when one is given: the same candidates in the
 // same order, the cell end read once per cell instead of icell once per
-// candidate, and cells holding none of the `want` kinds (AQ_CELL_FLUID /
-// AQ_CELL_ELEM) skipped, which only drops candidates the caller rejects.
+// candidate, cells holding none of the `want` kinds (AQ_CELL_FLUID /
+// AQ_CELL_ELEM) skipped, which only drops candidates the caller rejects, and
+// the positions and imove of WB candidates loaded together before their
+// bodies run (one memory latency per batch instead of per candidate).
+// body(j, r_j, imove_j).
 template <int D, class Body>
 __device__ __forceinline__ void walk_ll(const LLArgs& ll, uint32_t c_i,
-                                        uint32_t want, Body body)
+                                        uint32_t want,
+                                        const float* __restrict__ r,
+                                        const int* __restrict__ imove,
+                                        Body body)
 {
     if (!ll.cell_end) {
         walk<D>(c_i, ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell, ll.ihoc,
-                body);
+                [&](uint32_t j) { body(j, ldv<D>(r, j), imove[j]); });
         return;
     }
+    constexpr uint32_t WB = 4;
     const uint32_t nx = ll.n_cells[0], ny = ll.n_cells[1];
     if (!ll.flagged) want = 0;
     constexpr int NC = (D == 2) ? 9 : 27;
@@ -447,7 +454,19 @@ __device__ __forceinline__ void walk_ll(const LLArgs& ll, uint32_t c_i,
         const uint32_t ce = ll.cell_end[c_j];
         if (want && !(ce & want)) continue;
         const uint32_t j1 = ce & AQ_CELL_END;
-        for (uint32_t j = j0; j < j1; j++) body(j);
+        uint32_t j = j0;
+        for (; j + WB <= j1; j += WB) {
+            f3 rb[WB];
+            int ib[WB];
+#pragma unroll
+            for (uint32_t t = 0; t < WB; t++) {
+                rb[t] = ldv<D>(r, j + t);
+                ib[t] = imove[j + t];
+            }
+#pragma unroll
+            for (uint32_t t = 0; t < WB; t++) body(j + t, rb[t], ib[t]);
+        }
+        for (; j < j1; j++) body(j, ldv<D>(r, j), imove[j]);
     }
 }
 
@@ -508,11 +527,12 @@ __global__ void __launch_bounds__(BS)
         if ((im < -3) || ((im > 0) && excl(im))) continue;
         const f3 r_i = ldv<D>(r, i);
         float acc = shepard[i];
-        walk_ll<D>(ll, ll.icell[i], want, [&](uint32_t j) {
-            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+        walk_ll<D>(ll, ll.icell[i], want, r, imove,
+                   [&](uint32_t j, const f3& r_j, int im_j) {
+            const f3 r_ij = sub<D>(r_j, r_i);
             const float s = dot<D>(r_ij, r_ij);
             if (s > smax) return;
-            if (excl(imove[j])) return;
+            if (excl(im_j)) return;
             const float q = sqrtf(s) / H;
             acc += kernelW<D>(q) * CONW * m[j] / rho[j];
         });
@@ -657,8 +677,9 @@ __global__ void __launch_bounds__(BS)
         const f3 r_i = ldv<D>(r, i);
         float acc = 1.f;
         bool self_added = false;
-        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, [&](uint32_t j) {
-            if (imove[j] != -3) return;
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, r, imove,
+                   [&](uint32_t j, const f3& r_j, int im_j) {
+            if (im_j != -3) return;
             if (i == j) {
                 if (!self_added) {
                     self_added = true;
@@ -666,7 +687,7 @@ __global__ void __launch_bounds__(BS)
                 }
                 return;
             }
-            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+            const f3 r_ij = sub<D>(r_j, r_i);
             const float s = dot<D>(r_ij, r_ij);
             if (s > smax) return;
             const float q = sqrtf(s) / H;
@@ -715,12 +736,13 @@ __global__ void __launch_bounds__(BS)
         const f3 r_i = ldv<D>(r, i);
         const f3 u_i = ldv<D>(u, i);
         f3 lu = ldv<D>(lap_u, i);
-        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, [&](uint32_t j) {
-            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, r, imove,
+                   [&](uint32_t j, const f3& r_j, int im_j) {
+            const f3 r_ij = sub<D>(r_j, r_i);
             const float s = dot<D>(r_ij, r_ij);
             if (s > smax) return;
             if (i == j) return;
-            if (imove[j] != 1) return;
+            if (im_j != 1) return;
             const float q = sqrtf(s) / H;
             const float rho_j = rho[j];
             const float f_ij = kernelF<D>(q) * CONF * m[j];
@@ -758,11 +780,12 @@ __global__ void __launch_bounds__(BS)
         gp_i.y = rho_i * gp_i.y;
         gp_i.z = rho_i * gp_i.z;
         float acc = 0.f;
-        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, [&](uint32_t j) {
-            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_FLUID, r, imove,
+                   [&](uint32_t j, const f3& r_j, int im_j) {
+            const f3 r_ij = sub<D>(r_j, r_i);
             const float s = dot<D>(r_ij, r_ij);
             if (s > smax) return;
-            if (imove[j] != 1) return;
+            if (im_j != 1) return;
             const float q = sqrtf(s) / H;
             const float w_ij = kernelW<D>(q) * CONW * m[j] / rho[j];
             acc += (p[j] - dot<D>(gp_i, r_ij)) * w_ij;
@@ -799,9 +822,10 @@ __global__ void __launch_bounds__(BS)
         const float rho_i = rho[i];
         f3 gp = ldv<D>(grad_p, i);
         float dv = div_u[i];
-        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, [&](uint32_t j) {
-            if (imove[j] != -3) return;
-            const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+        walk_ll<D>(ll, ll.icell[i], AQ_CELL_ELEM, r, imove,
+                   [&](uint32_t j, const f3& r_j, int im_j) {
+            if (im_j != -3) return;
+            const f3 r_ij = sub<D>(r_j, r_i);
             const float s = dot<D>(r_ij, r_ij);
             if (s > smax) return;
             const float q = sqrtf(s) / H;
