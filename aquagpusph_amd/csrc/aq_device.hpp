@@ -96,6 +96,16 @@ template <int D> __device__ __forceinline__ f3 sub(const f3& a, const f3& b)
 }
 
 // dot over the first D components, left to right
+// s * v and a + b per component (the z component only in 3D)
+template <int D> __device__ __forceinline__ f3 scl(float s, const f3& v)
+{
+    return f3{ s * v.x, s * v.y, (D == 3) ? s * v.z : 0.f };
+}
+template <int D> __device__ __forceinline__ f3 add3(const f3& a, const f3& b)
+{
+    return f3{ a.x + b.x, a.y + b.y, (D == 3) ? a.z + b.z : a.z };
+}
+
 // Contracted forms (one rounding per multiply-add), for the kernels built
 // with the OpenCL C default floating-point rules.
 template <int D> __device__ __forceinline__ float fdot(const f3& a, const f3& b)

@@ -191,82 +191,90 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             nw++;
         }
     };
-    while (true) {
+    // consume while no slot can be refilled (a lane still reads from the
+    // oldest) and some lane has a candidate left
+    auto go = [&]() {
         advance();
-        // refill the oldest slot when no lane still reads from it
         if (tested < nsegs &&
             (tested < 2 ||
-             __ballot(bw != 0 && nw <= (tested - 1) * SEGW) == 0)) {
-            const uint32_t seg = tested * SEG;
-            const uint32_t nseg = min(SEG, total - seg);
-            const int slot = tested & 1;
-            for (uint32_t cb = 0; cb < nseg; cb += 64) {
-                const uint32_t t = seg + cb + lane;
-                float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
-                bool take = false, is_el = false;
-                if (cb + lane < nseg) {
-                    while (t >= L.cum[kl + 1]) kl++;
-                    const uint32_t j = L.start[kl] + (t - L.cum[kl]);
-                    const f3 rj = ldv_o<D>(r, j);
-                    const int im_j = ldi_o(imove, j);
-                    is_el = (im_j == -3);
-                    take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
-                    st = make_float4(rj.x, rj.y, rj.z, 0.f);
-                    L.cand[slot][cb + lane] = j;
-                }
-                const uint64_t tm = __ballot(take);
-                const uint64_t em = __ballot(take && is_el);
-                uint32_t lo = 0, hi = 0;
-                if (tm != 0) {
-                    L.stage[lane] = st;
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_s_waitcnt(0);
-#pragma unroll 8
-                    for (int k = 0; k < 32; k++) {
-                        const float4 c = L.stage[k];
-                        const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
-                        const float s = dot<D>(r_ij, r_ij);
-                        lo = (lo << 1) | (!(s > smax) ? 1u : 0u);
-                    }
-                    if (nseg - cb > 32) {
-#pragma unroll 8
-                        for (int k = 32; k < 64; k++) {
-                            const float4 c = L.stage[k];
-                            const f3 r_ij =
-                                sub<D>(f3{ c.x, c.y, c.z }, r_i);
-                            const float s = dot<D>(r_ij, r_ij);
-                            hi = (hi << 1) | (!(s > smax) ? 1u : 0u);
-                        }
-                    }
-                    lo &= __builtin_bitreverse32((uint32_t)tm);
-                    hi &= __builtin_bitreverse32((uint32_t)(tm >> 32));
-                    if (!act) lo = hi = 0;
-                    if (!kind && em != 0) {
-                        // wall elements in range: flagged, then dropped
-                        const uint32_t el =
-                            __builtin_bitreverse32((uint32_t)em);
-                        const uint32_t eh =
-                            __builtin_bitreverse32((uint32_t)(em >> 32));
-                        eln = eln || ((lo & el) | (hi & eh)) != 0;
-                        lo &= ~el;
-                        hi &= ~eh;
-                    }
-                }
-                L.bits[slot][cb >> 5][lane] = lo;
-                if (nseg - cb > 32) L.bits[slot][(cb >> 5) + 1][lane] = hi;
-                __builtin_amdgcn_wave_barrier();
+             __ballot(bw != 0 && nw <= (tested - 1) * SEGW) == 0))
+            return false;
+        return __ballot(bw != 0) != 0;
+    };
+    while (true) {
+        // the test sits at the bottom of the consume loop, so the pair sums
+        // leave it from where they are updated (no copies at the back edge)
+        for (bool more = go(); more; more = go()) {
+            if (bw != 0) {
+                const uint32_t b = __builtin_clz(bw);
+                bw ^= 0x80000000u >> b;
+                pair(cand[((nw - 1) % RING) * 32 + b]);
             }
+        }
+        if (tested >= nsegs) break;  // and no lane has a candidate left
+        // refill the oldest slot
+        const uint32_t seg = tested * SEG;
+        const uint32_t nseg = min(SEG, total - seg);
+        const int slot = tested & 1;
+        for (uint32_t cb = 0; cb < nseg; cb += 64) {
+            const uint32_t t = seg + cb + lane;
+            float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
+            bool take = false, is_el = false;
+            if (cb + lane < nseg) {
+                while (t >= L.cum[kl + 1]) kl++;
+                const uint32_t j = L.start[kl] + (t - L.cum[kl]);
+                const f3 rj = ldv_o<D>(r, j);
+                const int im_j = ldi_o(imove, j);
+                is_el = (im_j == -3);
+                take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
+                st = make_float4(rj.x, rj.y, rj.z, 0.f);
+                L.cand[slot][cb + lane] = j;
+            }
+            const uint64_t tm = __ballot(take);
+            const uint64_t em = __ballot(take && is_el);
+            uint32_t lo = 0, hi = 0;
+            if (tm != 0) {
+                L.stage[lane] = st;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll 8
+                for (int k = 0; k < 32; k++) {
+                    const float4 c = L.stage[k];
+                    const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
+                    const float s = dot<D>(r_ij, r_ij);
+                    lo = (lo << 1) | (!(s > smax) ? 1u : 0u);
+                }
+                if (nseg - cb > 32) {
+#pragma unroll 8
+                    for (int k = 32; k < 64; k++) {
+                        const float4 c = L.stage[k];
+                        const f3 r_ij =
+                            sub<D>(f3{ c.x, c.y, c.z }, r_i);
+                        const float s = dot<D>(r_ij, r_ij);
+                        hi = (hi << 1) | (!(s > smax) ? 1u : 0u);
+                    }
+                }
+                lo &= __builtin_bitreverse32((uint32_t)tm);
+                hi &= __builtin_bitreverse32((uint32_t)(tm >> 32));
+                if (!act) lo = hi = 0;
+                if (!kind && em != 0) {
+                    // wall elements in range: flagged, then dropped
+                    const uint32_t el =
+                        __builtin_bitreverse32((uint32_t)em);
+                    const uint32_t eh =
+                        __builtin_bitreverse32((uint32_t)(em >> 32));
+                    eln = eln || ((lo & el) | (hi & eh)) != 0;
+                    lo &= ~el;
+                    hi &= ~eh;
+                }
+            }
+            L.bits[slot][cb >> 5][lane] = lo;
+            if (nseg - cb > 32) L.bits[slot][(cb >> 5) + 1][lane] = hi;
             __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_s_waitcnt(0);
-            tested++;
-            continue;
         }
-        if (__ballot(bw != 0) == 0) break;
-        if (bw != 0) {
-            const uint32_t b = __builtin_clz(bw);
-            bw ^= 0x80000000u >> b;
-            pair(cand[((nw - 1) % RING) * 32 + b]);
-        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0);
+        tested++;
     }
     el_near = eln;
     __builtin_amdgcn_wave_barrier();

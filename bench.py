@@ -45,6 +45,29 @@ def build_sim(dr, dims=3):
     return sim, case
 
 
+# water-column resolutions giving ~1M and ~16M fluid particles: the sizes
+# the particle-steps/s metric is quoted at
+METRIC_SIZES = ((0.00476, "1M"), (0.00189, "16M"))
+
+
+def size_point(dr, steps=10, warmup=2):
+    """particle-steps/s of the same pipeline at another resolution (HIP
+    graph steps, inputs resident; the same timing as the main line)."""
+    sim, case = build_sim(dr)
+    sim.step(warmup, graph=True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps, graph=True)
+    sim.sync()
+    el = time.perf_counter() - t0
+    out = {"dr": dr, "n_particles": sim.N, "n_fluid": int(case["n_fluid"]),
+           "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "particle_steps_per_s": sim.N * steps / el,
+           "fluid_particle_steps_per_s": int(case["n_fluid"]) * steps / el}
+    sim.close()
+    return out
+
+
 def cpu_baseline(case, budget_s=25.0):
     """FP32 oracle (OpenMP C++ restatement) on the same case: a bounded
     number of full steps, reported as particle-steps/s."""
@@ -159,6 +182,8 @@ def main():
     ap.add_argument("--dr", type=float, default=0.003)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-sizes", action="store_true",
+                    help="skip the ~1M and ~16M fluid-particle points")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (for the --pmc "
                          "counter scale) and exit")
@@ -258,6 +283,9 @@ def main():
         "step_gpu_ms_sum": step_gpu_ms,
     }
     sim.close()
+    if not a.no_sizes:
+        out["metric_sizes"] = {name: size_point(dr)
+                               for dr, name in METRIC_SIZES}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(out))
