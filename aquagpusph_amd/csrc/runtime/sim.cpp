@@ -297,6 +297,21 @@ static bool fusion_transparent(const Tool& t)
     return t.type() == "kernel" && !t.once() && ok.count(t.key());
 }
 
+// The pair kernels compute with IEEE-754 correctly rounded division and
+// square root and no contraction (results bit-identical to the oracle)
+// unless the case's Device compile_flags ask OpenCL for relaxed math
+// (-cl-fast-relaxed-math or -cl-unsafe-math-optimizations): then the fused
+// walk uses reciprocal-based division (within the OpenCL C default 2.5
+// ulp), the hardware square root (within 3 ulp), contracted multiply-adds
+// and products regrouped around 1/rho, as those flags allow.
+static bool relaxed_math(const std::string& flags)
+{
+    for (const char* f : { "-cl-fast-relaxed-math",
+                           "-cl-unsafe-math-optimizations" })
+        if (flags.find(f) != std::string::npos) return true;
+    return false;
+}
+
 void Sim::plan_fusion()
 {
     fusion = FusionPlan();
@@ -373,6 +388,7 @@ void Sim::plan_fusion()
         fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
         fusion.walk_segw = atoi(ws) == 16 ? 16 : 8;
+    fusion.fast_math = relaxed_math(problem.compile_flags);
     if (const char* fm = getenv("AQ_MATH"))
         fusion.fast_math = strcmp(fm, "relaxed") == 0;
     fusion.active = true;

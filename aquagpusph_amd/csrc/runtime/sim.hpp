@@ -173,6 +173,7 @@ struct Problem {
     std::vector<SetDesc> sets;
     TimingDesc timing;
     std::string base_path;
+    std::string compile_flags;  // of the <Device> settings
     std::vector<std::string> xml_stack;
     bool is_defined(const std::string& n) const;
 };
@@ -214,7 +215,10 @@ struct FusionPlan {
     uint8_t* near_buf = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
-    bool fast_math = false;  // pair arithmetic with the OpenCL default rules
+    // fused walk arithmetic with the OpenCL C default accuracy (reciprocal
+    // division, hardware square root, contracted multiply-adds) instead of
+    // IEEE-754 correctly rounded division and square root
+    bool fast_math = false;
     // profiling: events around the walk launch while `timing` is set
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -102,11 +102,17 @@ static std::vector<size_t> places_glob(const std::string& name,
     return out;
 }
 
+// Settings: RootPath, and the compile_flags of the Device elements
+// (State.cpp:464-515), which select the arithmetic of the pair kernels.
 static void parse_settings(const XmlNode& root, Problem& P)
 {
-    for (auto* s : root.all("Settings"))
+    for (auto* s : root.all("Settings")) {
         for (auto* r : s->all("RootPath"))
             if (r->has("path")) P.base_path = find_path(r->attr("path"), P);
+        for (auto* d : s->all("Device"))
+            if (d->has("compile_flags"))
+                P.compile_flags += " " + d->attr("compile_flags");
+    }
 }
 
 static void parse_variables(const XmlNode& root, Problem& P)

@@ -58,6 +58,37 @@ def test_xml_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
     sim.close()
 
 
+RELAXED = ('    <Settings><Device platform="0" device="0" '
+           'compile_flags="-cl-fast-relaxed-math" /></Settings>')
+
+
+@pytest.mark.parametrize("dims,nl,steps", [(2, 25, 2), (3, 8, 2),
+                                           (2, 25, 30), (3, 8, 15)])
+def test_relaxed_math_case_matches_oracle(cuda, tmp_path, dims, nl, steps):
+    """A case whose Device compile_flags ask for relaxed math runs the
+    fused walk with reciprocal division, hardware square root and
+    contracted multiply-adds: not bit-identical to the IEEE kernels, within
+    the same tolerance of the FP32 oracle (1e-5 of each field's largest
+    magnitude after two steps, 1e-4 after 15 or 30)."""
+    case = cases.dam_break(dims, 0.3 / nl)
+    xml = casefiles.write_case(case, str(tmp_path / "relaxed"),
+                               extra_xml=RELAXED)
+    xml_ieee = casefiles.write_case(case, str(tmp_path / "ieee"))
+    sim = Simulation(xml, dims)
+    ieee = Simulation(xml_ieee, dims)
+    o = op.OracleSim(case)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    ieee.step(steps)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4)
+    print(dims, steps, errs)
+    assert any(not np.array_equal(sim.get(f), ieee.get(f))
+               for f in ("grad_p", "lap_u", "div_u", "lap_p"))
+    sim.close()
+    ieee.close()
+
+
 GP_FIELDS = FIELDS + ("associations", "gp_icell", "gp_rho", "gp_p", "gp_u")
 
 
