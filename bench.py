@@ -241,8 +241,10 @@ def main():
         "frac": flops / t / 1e12 / PEAK_FP32_TFLOPS if flops else None,
         "traffic": traffic,
         "note": "FP32 vector ALU work (no matrix instructions); the MI355X "
-                "FP32 vector and matrix peaks are both 157.3 TFLOP/s. The "
-                "walk is compute-bound: %.0f FLOP per %d compulsory bytes"
+                "FP32 vector and matrix peaks are both 157.3 TFLOP/s, a "
+                "figure that counts packed FP32, which this build does not "
+                "emit (half of it without). The walk is compute-bound: %.0f "
+                "FLOP per %d compulsory bytes, IEEE division and square root"
                 % (flops / N if flops else 0, WALK_BYTES),
         "avg_launch_ms": t * 1e3,
         "tool_ms": dom_ms,
