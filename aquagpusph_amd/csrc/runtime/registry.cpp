@@ -37,6 +37,16 @@ void launch_timestep(int, const int32_t*, const float*, float*, uint32_t,
                      const float*, hipStream_t);
 void launch_sensors_renorm(int, const int32_t*, const float*, float*, float*,
                            float*, uint32_t, hipStream_t);
+void launch_energy_kin(int, float*, const int32_t*, const float*,
+                       const float*, uint32_t, hipStream_t);
+void launch_energy_power(int, float*, float*, float*, const int32_t*,
+                         const float*, const float*, const float*,
+                         const float*, const float*, const float*, uint32_t,
+                         const float*, hipStream_t);
+void launch_energy_parts(int, float*, float*, float*, const uint32_t*,
+                         const int32_t*, const float*, const float*,
+                         const float*, const float*, const float*, uint32_t,
+                         const float*, const float*, hipStream_t);
 void launch_bi_shepard_apply(int, const int32_t*, const float*, float*,
                              float*, float*, uint32_t, hipStream_t);
 void launch_bi_gradp(int, const uint32_t*, const int32_t*, const float*,
@@ -356,6 +366,28 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_timestep(S.dims, I(0), F(1), F(2), S.N, F(3), F(4),
                                 F(5), F(6), F(7), s);
+            } } },
+        // ---- energy (cfd/energy_kin.xml, cfd/energy.xml) -------------------
+        { "cfd/Energy/EnergyKin.cl:entry",
+          { { "energy_kin", "imove", "u", "m" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_energy_kin(S.dims, F(0), I(1), F(2), F(3), S.N, s);
+            } } },
+        { "cfd/Energy/Energy.cl:power",
+          { { "energy_dekdt", "energy_depdt", "energy_decdt", "imove", "u",
+              "rho", "m", "p", "dudt", "drhodt", "g" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_energy_power(S.dims, F(0), F(1), F(2), I(3), F(4),
+                                    F(5), F(6), F(7), F(8), F(9), S.N, F(10),
+                                    s);
+            } } },
+        { "cfd/Energy/Energy.cl:energy",
+          { { "energy_ek", "energy_ep", "energy_ec", "iset", "imove", "r", "u",
+              "rho", "m", "refd", "g", "cs" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_energy_parts(S.dims, F(0), F(1), F(2), U(3), I(4), F(5),
+                                    F(6), F(7), F(8), F(9), S.N, F(10), F(11),
+                                    s);
             } } },
         // ---- boundary integrals --------------------------------------------
         { "cfd/Boundary/BI/Shepard.cl:compute",

@@ -267,6 +267,79 @@ __global__ void rates_k(const uint32_t* __restrict__ iset,
     drhodt[i] = -div_u[i];
 }
 
+// cfd/Energy/EnergyKin.cl:40-56
+template <int D>
+__global__ void energy_kin_k(float* __restrict__ ek,
+                             const int* __restrict__ imove,
+                             const float* __restrict__ u,
+                             const float* __restrict__ m, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) {
+        ek[i] = 0.f;
+        return;
+    }
+    const f3 u_i = ldv<D>(u, i);
+    ek[i] = 0.5f * m[i] * dot<D>(u_i, u_i);
+}
+
+// cfd/Energy/Energy.cl:59-86 (power)
+template <int D>
+__global__ void energy_power_k(float* __restrict__ dek,
+                               float* __restrict__ dep,
+                               float* __restrict__ dec,
+                               const int* __restrict__ imove,
+                               const float* __restrict__ u,
+                               const float* __restrict__ rho,
+                               const float* __restrict__ m,
+                               const float* __restrict__ p,
+                               const float* __restrict__ dudt,
+                               const float* __restrict__ drhodt, uint32_t N,
+                               const float* __restrict__ g)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) {
+        dek[i] = dep[i] = dec[i] = 0.f;
+        return;
+    }
+    const f3 gv = { g[0], g[1], (D == 3) ? g[2] : 0.f };
+    const f3 u_i = ldv<D>(u, i);
+    dep[i] = -m[i] * dot<D>(gv, u_i);
+    dek[i] = m[i] * dot<D>(u_i, ldv<D>(dudt, i));
+    dec[i] = m[i] * p[i] / (rho[i] * rho[i]) * drhodt[i];
+}
+
+// cfd/Energy/Energy.cl:113-142 (energy)
+template <int D>
+__global__ void energy_parts_k(float* __restrict__ ek, float* __restrict__ ep,
+                               float* __restrict__ ec,
+                               const uint32_t* __restrict__ iset,
+                               const int* __restrict__ imove,
+                               const float* __restrict__ r,
+                               const float* __restrict__ u,
+                               const float* __restrict__ rho,
+                               const float* __restrict__ m,
+                               const float* __restrict__ refd, uint32_t N,
+                               const float* __restrict__ g,
+                               const float* __restrict__ csp)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != 1) {
+        ek[i] = ep[i] = ec[i] = 0.f;
+        return;
+    }
+    const float cs = *csp;
+    const f3 gv = { g[0], g[1], (D == 3) ? g[2] : 0.f };
+    const f3 u_i = ldv<D>(u, i);
+    ek[i] = 0.5f * m[i] * dot<D>(u_i, u_i);
+    ep[i] = -m[i] * dot<D>(gv, ldv<D>(r, i));
+    const float rho0 = refd[iset[i]];
+    ec[i] = m[i] * cs * cs * (rho0 / rho[i] + logf(rho[i] / rho0) - 1.f);
+}
+
 // basic/deltaSPH.cl:330-350, cfd/deltaSPH.cl:29 (fluid only)
 __global__ void delta_rate_k(const uint32_t* __restrict__ iset,
                              const int* __restrict__ imove,
@@ -1093,6 +1166,30 @@ void launch_rates(int dims, const uint32_t* iset, const int32_t* imove,
     AQ_DISPATCH(dims, rates_k, AQ_GRID(N), iset, imove, grad_p, lap_u, div_u,
                 dudt, drhodt, visc_dyn, N, g);
 }
+void launch_energy_kin(int dims, float* ek, const int32_t* imove,
+                       const float* u, const float* m, uint32_t N,
+                       hipStream_t s)
+{
+    AQ_DISPATCH(dims, energy_kin_k, AQ_GRID(N), ek, imove, u, m, N);
+}
+void launch_energy_power(int dims, float* dek, float* dep, float* dec,
+                         const int32_t* imove, const float* u,
+                         const float* rho, const float* m, const float* p,
+                         const float* dudt, const float* drhodt, uint32_t N,
+                         const float* g, hipStream_t s)
+{
+    AQ_DISPATCH(dims, energy_power_k, AQ_GRID(N), dek, dep, dec, imove, u, rho,
+                m, p, dudt, drhodt, N, g);
+}
+void launch_energy_parts(int dims, float* ek, float* ep, float* ec,
+                         const uint32_t* iset, const int32_t* imove,
+                         const float* r, const float* u, const float* rho,
+                         const float* m, const float* refd, uint32_t N,
+                         const float* g, const float* cs, hipStream_t s)
+{
+    AQ_DISPATCH(dims, energy_parts_k, AQ_GRID(N), ek, ep, ec, iset, imove, r,
+                u, rho, m, refd, N, g, cs);
+}
 void launch_delta_rate(const uint32_t* iset, const int32_t* imove,
                        const float* rho, const float* lap_p, float* drhodt,
                        const float* refd, const float* delta, uint32_t N,
@@ -1396,6 +1493,53 @@ AQ_API int aq_rates(int dims, const uint32_t* iset, const int32_t* imove,
         AQ_PTRS(iset, imove, grad_p, lap_u, div_u, dudt, drhodt, visc_dyn, g);
         launch_rates(dims, iset, imove, grad_p, lap_u, div_u, dudt, drhodt,
                      visc_dyn, N, g, S(stream));
+    });
+}
+
+AQ_API int aq_energy_kin(int dims, float* energy_kin, const int32_t* imove,
+                         const float* u, const float* m, uint32_t N,
+                         void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(energy_kin, imove, u, m);
+        launch_energy_kin(dims, energy_kin, imove, u, m, N, S(stream));
+    });
+}
+
+AQ_API int aq_energy_power(int dims, float* energy_dekdt, float* energy_depdt,
+                           float* energy_decdt, const int32_t* imove,
+                           const float* u, const float* rho, const float* m,
+                           const float* p, const float* dudt,
+                           const float* drhodt, uint32_t N, const float* g,
+                           void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(energy_dekdt, energy_depdt, energy_decdt, imove, u, rho, m, p,
+                dudt, drhodt, g);
+        launch_energy_power(dims, energy_dekdt, energy_depdt, energy_decdt,
+                            imove, u, rho, m, p, dudt, drhodt, N, g,
+                            S(stream));
+    });
+}
+
+AQ_API int aq_energy(int dims, float* energy_ek, float* energy_ep,
+                     float* energy_ec, const uint32_t* iset,
+                     const int32_t* imove, const float* r, const float* u,
+                     const float* rho, const float* m, const float* refd,
+                     uint32_t N, const float* g, const float* cs,
+                     void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        if (!N) return;
+        AQ_PTRS(energy_ek, energy_ep, energy_ec, iset, imove, r, u, rho, m,
+                refd, g, cs);
+        launch_energy_parts(dims, energy_ek, energy_ep, energy_ec, iset,
+                            imove, r, u, rho, m, refd, N, g, cs, S(stream));
     });
 }
 

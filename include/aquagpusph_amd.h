@@ -164,6 +164,29 @@ AQ_API int aq_rates(int dims, const uint32_t* iset, const int32_t* imove,
                     const float* visc_dyn, uint32_t N, const float* g,
                     void* stream);
 
+/* cfd/Energy/EnergyKin.cl:40-56: energy_kin = m u.u / 2 (fluid; 0 else) */
+AQ_API int aq_energy_kin(int dims, float* energy_kin, const int32_t* imove,
+                         const float* u, const float* m, uint32_t N,
+                         void* stream);
+
+/* cfd/Energy/Energy.cl:59-86 (power): kinetic, potential and
+ * compressibility energy rates per fluid particle; g on the device */
+AQ_API int aq_energy_power(int dims, float* energy_dekdt, float* energy_depdt,
+                           float* energy_decdt, const int32_t* imove,
+                           const float* u, const float* rho, const float* m,
+                           const float* p, const float* dudt,
+                           const float* drhodt, uint32_t N, const float* g,
+                           void* stream);
+
+/* cfd/Energy/Energy.cl:113-142 (energy): kinetic, potential and
+ * compressibility energies per fluid particle; g and cs on the device */
+AQ_API int aq_energy(int dims, float* energy_ek, float* energy_ep,
+                     float* energy_ec, const uint32_t* iset,
+                     const int32_t* imove, const float* r, const float* u,
+                     const float* rho, const float* m, const float* refd,
+                     uint32_t N, const float* g, const float* cs,
+                     void* stream);
+
 /* basic/deltaSPH.cl:330-350 (cfd/deltaSPH.cl: fluid only) */
 AQ_API int aq_delta_rate(const uint32_t* iset, const int32_t* imove,
                          const float* rho, const float* lap_p, float* drhodt,

@@ -102,6 +102,27 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         binormal<D, R>(normal, tangent, binormal_, N);                         \
     }                                                                          \
+    ORC_EXPORT void orc_energy_kin_##SR##_##D##d(                              \
+        R* ek, const int* imove, const R* u, const R* m, usize N)              \
+    {                                                                          \
+        energy_kin<D, R>(ek, imove, u, m, N);                                  \
+    }                                                                          \
+    ORC_EXPORT void orc_energy_power_##SR##_##D##d(                            \
+        R* dek, R* dep, R* dec, const int* imove, const R* u, const R* rho,    \
+        const R* m, const R* p, const R* dudt, const R* drhodt, usize N,       \
+        const R* g)                                                            \
+    {                                                                          \
+        energy_power<D, R>(dek, dep, dec, imove, u, rho, m, p, dudt, drhodt,   \
+                           N, g);                                              \
+    }                                                                          \
+    ORC_EXPORT void orc_energy_parts_##SR##_##D##d(                            \
+        R* ek, R* ep, R* ec, const unsigned* iset, const int* imove,           \
+        const R* r, const R* u, const R* rho, const R* m, const R* refd,       \
+        usize N, const R* g, R cs)                                             \
+    {                                                                          \
+        energy_parts<D, R>(ek, ep, ec, iset, imove, r, u, rho, m, refd, N, g,  \
+                           cs);                                                \
+    }                                                                          \
     ORC_EXPORT void orc_rates_##SR##_##D##d(                                   \
         const unsigned* iset, const int* imove, const R* rho,                  \
         const R* grad_p, const R* lap_u, const R* div_u, R* dudt, R* drhodt,   \

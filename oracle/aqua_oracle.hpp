@@ -1531,4 +1531,72 @@ void neighs(const int* imove, usize* n_neighs, usize limit, usize N, LL ll)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Energy of the fluid (cfd/energy_kin.xml, cfd/energy.xml).  Non-fluid
+// particles get zeros.
+// ---------------------------------------------------------------------------
+// cfd/Energy/EnergyKin.cl:40-56: E_kin = m u.u / 2
+template <int D, typename R>
+void energy_kin(R* ek, const int* imove, const R* u, const R* m, usize N)
+{
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) {
+            ek[i] = R(0);
+            continue;
+        }
+        const V<D, R> u_i = ld<D, R>(u, i);
+        ek[i] = R(0.5f) * m[i] * dot<D, R>(u_i, u_i);
+    }
+}
+
+// cfd/Energy/Energy.cl:59-86 (power): dEk/dt = m u.du/dt,
+// dEp/dt = -m g.u, dEc/dt = m p / rho^2 drho/dt
+template <int D, typename R>
+void energy_power(R* dek, R* dep, R* dec, const int* imove, const R* u,
+                  const R* rho, const R* m, const R* p, const R* dudt,
+                  const R* drhodt, usize N, const R* g)
+{
+    V<D, R> gv;
+    for (int k = 0; k < 3; k++) gv.c[k] = k < D ? g[k] : R(0);
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) {
+            dek[i] = dep[i] = dec[i] = R(0);
+            continue;
+        }
+        const V<D, R> u_i = ld<D, R>(u, i);
+        dep[i] = -m[i] * dot<D, R>(gv, u_i);
+        dek[i] = m[i] * dot<D, R>(u_i, ld<D, R>(dudt, i));
+        dec[i] = m[i] * p[i] / (rho[i] * rho[i]) * drhodt[i];
+    }
+}
+
+// cfd/Energy/Energy.cl:113-142 (energy): E_k = m u.u / 2, E_p = -m g.r,
+// E_c = m cs^2 (rho0 / rho + log(rho / rho0) - 1)
+template <int D, typename R>
+void energy_parts(R* ek, R* ep, R* ec, const unsigned* iset,
+                  const int* imove, const R* r, const R* u, const R* rho,
+                  const R* m, const R* refd, usize N, const R* g, R cs)
+{
+    V<D, R> gv;
+    for (int k = 0; k < 3; k++) gv.c[k] = k < D ? g[k] : R(0);
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if (imove[i] != 1) {
+            ek[i] = ep[i] = ec[i] = R(0);
+            continue;
+        }
+        const V<D, R> u_i = ld<D, R>(u, i);
+        ek[i] = R(0.5f) * m[i] * dot<D, R>(u_i, u_i);
+        ep[i] = -m[i] * dot<D, R>(gv, ld<D, R>(r, i));
+        const R rho0 = refd[iset[i]];
+        ec[i] = m[i] * cs * cs *
+                (rho0 / rho[i] + std::log(rho[i] / rho0) - R(1));
+    }
+}
+
 }  // namespace orc

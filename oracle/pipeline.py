@@ -62,7 +62,7 @@ class OracleSim:
     def __init__(self, case, dtype=np.float32, domain=True, bi=None,
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
-                 min_bound_dist=0.0, delta_corr=None):
+                 min_bound_dist=0.0, delta_corr=None, energy=False):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -76,7 +76,7 @@ class OracleSim:
                          gp_model=gp_model, freeslip_iset=freeslip_iset,
                          bie=bie, dr_factor=dr_factor,
                          min_bound_dist=min_bound_dist,
-                         delta_corr=delta_corr)
+                         delta_corr=delta_corr, energy=energy)
         N = self.N = int(case["N"])
         R = self.dtype
         f = {}
@@ -110,6 +110,15 @@ class OracleSim:
             ms = 4 if self.dims == 2 else 16
             for k in ("mls", "mls_fluid"):
                 f[k] = np.zeros((N, ms), R)
+        if energy:
+            # cfd/energy_kin.xml and cfd/energy.xml arrays and scalars
+            for k in ("energy_kin", "energy_dekdt", "energy_depdt",
+                      "energy_decdt", "energy_ek", "energy_ep", "energy_ec"):
+                f[k] = np.zeros(N, R)
+            self.energy = {k: R.type(0) for k in (
+                "energy_Kin", "energy_dEkdt", "energy_dEpdt", "energy_dEcdt",
+                "energy_Ek", "energy_Ep", "energy_Ec", "energy_Ek_ref",
+                "energy_Ep_ref", "energy_Ec_ref")}
         if bie:
             f["grad_w_bi"] = np.zeros((N, self.vs), R)
             f["div_u_bi"] = np.zeros(N, R)
@@ -163,6 +172,12 @@ class OracleSim:
         self._k = lambda name: getattr(L, "orc_%s_%s" % (name, self.sfx))
 
     # ---- helpers -----------------------------------------------------
+    def _sum(self, a):
+        """A sum reduction of the oracle's real type (accumulated in
+        float64; the GPU's tree order differs, so sums agree to a tolerance
+        only)."""
+        return self.dtype.type(np.sum(a, dtype=np.float64))
+
     def _R(self, x):
         return self.dtype.type(x)
 
@@ -369,15 +384,46 @@ class OracleSim:
                             _p(f["lap_p"]), _p(f["drhodt"]),
                             _p(self.sets["refd"]), _p(self.sets["delta"]), n,
                             R(self.dt))
+        if o["energy"]:
+            # "cfd Energy kin" + its sum, after "Rates"
+            k("energy_kin")(_p(f["energy_kin"]), _p(f["imove"]), _p(f["u"]),
+                            _p(f["m"]), n)
+            E = self.energy
+            E["energy_Kin"] = self._sum(f["energy_kin"])
         if o["bie"]:
             k("bie_elastic_bounce")(_p(f["imove"]), _p(f["r_in"]),
                                     _p(f["normal"]), _p(f["m"]),
                                     _p(f["u_in"]), _p(f["dudt"]), n,
                                     R(self.dt), *ll, R(o["dr_factor"]),
                                     R(o["min_bound_dist"]))
+        if o["energy"]:
+            # "cfd Power", its sums and the time integration (set_scalar in
+            # double, narrowed), before the corrector
+            k("energy_power")(_p(f["energy_dekdt"]), _p(f["energy_depdt"]),
+                              _p(f["energy_decdt"]), _p(f["imove"]),
+                              _p(f["u"]), _p(f["rho"]), _p(f["m"]),
+                              _p(f["p"]), _p(f["dudt"]), _p(f["drhodt"]), n,
+                              _p(self.g))
+            E = self.energy
+            for c in ("k", "p", "c"):
+                E["energy_dE%sdt" % c] = self._sum(f["energy_de%sdt" % c])
+                E["energy_E%s" % c] = self.dtype.type(
+                    float(E["energy_E%s" % c]) +
+                    float(self.dt) * float(E["energy_dE%sdt" % c]))
         k("corrector")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
                        _p(f["rho"]), _p(f["drhodt"]), _p(f["dudt_in"]),
                        _p(f["drhodt_in"]), n, R(self.dt))
+        if o["energy"]:
+            # "cfd Energy" and its sums, after the corrector
+            k("energy_parts")(_p(f["energy_ek"]), _p(f["energy_ep"]),
+                              _p(f["energy_ec"]), _p(f["iset"]),
+                              _p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                              _p(f["rho"]), _p(f["m"]),
+                              _p(self.sets["refd"]), n, _p(self.g),
+                              R(self.cs))
+            E = self.energy
+            for c in ("k", "p", "c"):
+                E["energy_E%s_ref" % c] = self._sum(f["energy_e%s" % c])
         if o["bie"]:
             k("bie_pst")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
                          _p(f["m"]), _p(f["rho"]), n, *ll, R(self.DIMS),
