@@ -172,8 +172,10 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
         Sim& S = SIM(s);
         AQ_REQUIRE(ms, "aq_sim_profile_tools: null output");
         const size_t nt = S.tools.size();
-        std::vector<hipEvent_t> ev(nt + 1);
-        for (auto& e : ev) AQ_HIP(hipEventCreate(&e));
+        // a pair of events around every execution of a tool (tools inside
+        // while loops run several times per step), from a growing pool
+        std::vector<hipEvent_t> ev;
+        std::vector<size_t> which;
         std::vector<double> acc(nt, 0.0);
         FusionPlan& F = S.fusion;
         double walk_acc = 0.0;
@@ -185,19 +187,19 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
         if (S.vars.host_dirty()) S.vars.push(S.stream);
         for (int k = 0; k < nsteps; k++) {
             F.timing = F.active;
-            std::set<Tool*> ran;
-            AQ_HIP(hipEventRecord(ev[0], S.stream));
-            for (size_t t = 0; t < nt; t++) {
-                Tool* tl = S.tools[t].get();
-                if (!(tl->once() && tl->done)) {
-                    tl->execute(S, S.stream);
-                    ran.insert(tl);
+            size_t used = 0;
+            which.clear();
+            S.run_tools(S.stream, [&](size_t t, bool after) {
+                if (used == ev.size()) {
+                    hipEvent_t e;
+                    AQ_HIP(hipEventCreate(&e));
+                    ev.push_back(e);
                 }
-                AQ_HIP(hipEventRecord(ev[t + 1], S.stream));
-            }
-            for (auto* tl : ran) tl->done = true;
+                AQ_HIP(hipEventRecord(ev[used++], S.stream));
+                if (!after) which.push_back(t);
+            });
             F.timing = false;
-            AQ_HIP(hipEventSynchronize(ev[nt]));
+            if (used) AQ_HIP(hipEventSynchronize(ev[used - 1]));
             if (F.active) {
                 float x = 0.f;
                 if (hipEventElapsedTime(&x, F.ev0, F.ev1) == hipSuccess) {
@@ -205,10 +207,10 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
                     walk_n++;
                 }
             }
-            for (size_t t = 0; t < nt; t++) {
+            for (size_t e = 0; e < which.size(); e++) {
                 float x = 0.f;
-                AQ_HIP(hipEventElapsedTime(&x, ev[t], ev[t + 1]));
-                acc[t] += x;
+                AQ_HIP(hipEventElapsedTime(&x, ev[2 * e], ev[2 * e + 1]));
+                acc[which[e]] += x;
             }
         }
         for (auto& e : ev) (void)hipEventDestroy(e);

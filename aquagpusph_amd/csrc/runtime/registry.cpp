@@ -37,6 +37,24 @@ void launch_timestep(int, const int32_t*, const float*, float*, uint32_t,
                      const float*, hipStream_t);
 void launch_sensors_renorm(int, const int32_t*, const float*, float*, float*,
                            float*, uint32_t, hipStream_t);
+void launch_mp_predictor(int, const float*, const float*, const float*,
+                         const float*, const float*, float*, float*, float*,
+                         float*, float*, uint32_t, hipStream_t);
+void launch_mp_advance(int, const int32_t*, const float*, float*,
+                       const float*, const float*, float*, const float*,
+                       uint32_t, const float*, hipStream_t);
+void launch_mp_advance_r(int, const int32_t*, const float*, float*,
+                         const float*, uint32_t, const float*, hipStream_t);
+void launch_mp_relax(int, const int32_t*, const float*, float*, const float*,
+                     float*, uint32_t, const float*, hipStream_t);
+void launch_mp_residuals(int, const int32_t*, const float*, const float*,
+                         const float*, const float*, const float*,
+                         const float*, const float*, const float*, float*,
+                         uint32_t, hipStream_t);
+void launch_mp_corrector(int, const int32_t*, const float*, float*,
+                         const float*, float*, const float*, const float*,
+                         float*, const float*, uint32_t, const float*,
+                         hipStream_t);
 void launch_energy_kin(int, float*, const int32_t*, const float*,
                        const float*, uint32_t, hipStream_t);
 void launch_energy_power(int, float*, float*, float*, const int32_t*,
@@ -366,6 +384,47 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_timestep(S.dims, I(0), F(1), F(2), S.N, F(3), F(4),
                                 F(5), F(6), F(7), s);
+            } } },
+        // ---- semi-implicit midpoint scheme ----------------------------------
+        { "basic/time_scheme/midpoint.cl:predictor",
+          { { "r", "u", "dudt", "rho", "drhodt", "r_in", "u_in", "dudt_in",
+              "rho_in", "drhodt_in" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_predictor(S.dims, F(0), F(1), F(2), F(3), F(4), F(5),
+                                    F(6), F(7), F(8), F(9), S.N, s);
+            } } },
+        { "basic/time_scheme/midpoint.cl:midpoint",
+          { { "imove", "u_in", "u", "dudt", "rho_in", "rho", "drhodt", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_advance(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                  F(6), S.N, F(7), s);
+            } } },
+        { "basic/time_scheme/midpoint.cl:midpoint_r",
+          { { "imove", "r_in", "r", "u", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_advance_r(S.dims, I(0), F(1), F(2), F(3), S.N, F(4),
+                                    s);
+            } } },
+        { "basic/time_scheme/midpoint.cl:relax",
+          { { "imove", "dudt_in", "dudt", "drhodt_in", "drhodt",
+              "relax_midpoint" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_relax(S.dims, I(0), F(1), F(2), F(3), F(4), S.N, F(5),
+                                s);
+            } } },
+        { "basic/time_scheme/midpoint.cl:residuals",
+          { { "imove", "m", "u", "dudt_in", "dudt", "rho", "p", "drhodt_in",
+              "drhodt", "residual_midpoint" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_residuals(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                    F(6), F(7), F(8), F(9), S.N, s);
+            } } },
+        { "basic/time_scheme/midpoint.cl:corrector",
+          { { "imove", "r_in", "r", "u_in", "u", "dudt", "rho_in", "rho",
+              "drhodt", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_mp_corrector(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                    F(6), F(7), F(8), S.N, F(9), s);
             } } },
         // ---- energy (cfd/energy_kin.xml, cfd/energy.xml) -------------------
         { "cfd/Energy/EnergyKin.cl:entry",

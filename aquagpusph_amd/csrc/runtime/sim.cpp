@@ -43,6 +43,10 @@ Sim::~Sim()
 {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (auto& kv : segments) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
     tools.clear();
     for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
                      (void*)clist.heads, (void*)clist.n_heads, clist.ws,
@@ -255,6 +259,7 @@ void Sim::load_sets()
 void Sim::setup()
 {
     for (auto& t : tools) t->setup(*this);
+    match_scopes();
     // cell-parallel walks unless AQ_WALK=thread asks for the per-particle
     // ones (which also serve more than 2^28 particles)
     const char* walk = getenv("AQ_WALK");
@@ -523,16 +528,124 @@ void Sim::run_fused(hipStream_t s)
     if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
 }
 
-void Sim::run_tools(hipStream_t s)
+// Scopes of while/if/end (Conditional.cpp setup: an opener's scope closes
+// at the first end that brings the count back to zero; an end's opener is
+// found the same way backwards).
+void Sim::match_scopes()
 {
-    std::set<Tool*> ran;
-    for (auto& t : tools) {
-        if (t->once() && t->done) continue;
-        last_tool = t->name();
-        t->execute(*this, s);
-        ran.insert(t.get());
+    const size_t n = tools.size();
+    jump.assign(n, 0);
+    if_taken.assign(n, 0);
+    has_control = false;
+    std::vector<char> matched(n, 0);
+    for (size_t i = 0; i < n; i++) {
+        if (tools[i]->scope() <= 0) continue;
+        has_control = true;
+        int scope = 1;
+        size_t j = i;
+        while (j + 1 < n) {
+            j++;
+            scope += tools[j]->scope();
+            if (!scope) break;
+        }
+        if (scope)
+            throw Error(AQ_EPARSE, "unclosed scope of tool \"" +
+                                       tools[i]->name() + "\": add an end tool");
+        jump[i] = j;
+        jump[j] = i;
+        matched[j] = 1;
     }
-    for (auto* t : ran) t->done = true;
+    for (size_t i = 0; i < n; i++)
+        if (tools[i]->scope() < 0 && !matched[i])
+            throw Error(AQ_EPARSE, "tool \"" + tools[i]->name() +
+                                       "\" closes no while/if scope");
+}
+
+// The tool after a control tool.  while: the body while the condition
+// holds, else past its end.  if: the body once when the condition holds
+// (its end hands control back, and the if then jumps out).  end: back to
+// its opener.
+size_t Sim::control(size_t pc, hipStream_t s)
+{
+    Tool* t = tools[pc].get();
+    if (t->scope() < 0) return jump[pc];
+    auto* c = static_cast<ControlTool*>(t);
+    if (t->type() == "while") return c->eval(*this, s) ? pc + 1 : jump[pc] + 1;
+    if (if_taken[pc]) {
+        if_taken[pc] = 0;
+        return jump[pc] + 1;
+    }
+    if (c->eval(*this, s)) {
+        if_taken[pc] = 1;
+        return pc + 1;
+    }
+    return jump[pc] + 1;
+}
+
+// Tool::execute: a `once` tool runs the first time it is reached only.
+void Sim::run_tools(hipStream_t s,
+                    const std::function<void(size_t, bool)>& hook)
+{
+    size_t pc = 0;
+    while (pc < tools.size()) {
+        Tool* t = tools[pc].get();
+        last_tool = t->name();
+        if (t->scope()) {
+            pc = control(pc, s);
+            continue;
+        }
+        if (!(t->once() && t->done)) {
+            if (hook) hook(pc, false);
+            t->execute(*this, s);
+            if (hook) hook(pc, true);
+            t->done = true;
+        }
+        pc++;
+    }
+}
+
+// Graph replay of a pipeline with control tools: each run of tools between
+// control tools is captured once into its own graph; the host walks the
+// control flow between them (a condition is read back when evaluated, as
+// the reference does).
+void Sim::step_segments(int nsteps)
+{
+    for (int k = 0; k < nsteps; k++) {
+        size_t pc = 0;
+        while (pc < tools.size()) {
+            if (tools[pc]->scope()) {
+                last_tool = tools[pc]->name();
+                pc = control(pc, stream);
+                continue;
+            }
+            auto it = segments.find(pc);
+            if (it == segments.end()) {
+                Segment g;
+                g.end = pc;
+                while (g.end < tools.size() && !tools[g.end]->scope()) g.end++;
+                AQ_HIP(hipStreamBeginCapture(stream,
+                                             hipStreamCaptureModeThreadLocal));
+                try {
+                    for (size_t i = pc; i < g.end; i++) {
+                        Tool* t = tools[i].get();
+                        if (t->once() && t->done) continue;
+                        last_tool = t->name();
+                        t->execute(*this, stream);
+                    }
+                } catch (...) {
+                    hipGraph_t tmp;
+                    (void)hipStreamEndCapture(stream, &tmp);
+                    throw;
+                }
+                AQ_HIP(hipStreamEndCapture(stream, &g.graph));
+                AQ_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr,
+                                           0));
+                it = segments.emplace(pc, g).first;
+            }
+            AQ_HIP(hipGraphLaunch(it->second.exec, stream));
+            pc = it->second.end;
+        }
+    }
 }
 
 void Sim::sync() { AQ_HIP(hipStreamSynchronize(stream)); }
@@ -570,6 +683,22 @@ void Sim::step_graph(int nsteps)
     if (!is_setup) throw Error(AQ_EINVAL, "setup() has not been called");
     if (nsteps <= 0) return;
     if (vars.host_dirty()) vars.push(stream);
+    if (has_control) {
+        if (!graph_ready) {
+            for (auto& t : tools)
+                if (!t->graph_safe() && !t->scope())
+                    throw Error(AQ_EUNSUP, "tool \"" + t->name() +
+                                               "\" cannot be captured in a "
+                                               "graph");
+            // the first step runs eagerly so that `once` tools are done
+            run_tools(stream);
+            nsteps--;
+            graph_ready = true;
+        }
+        step_segments(nsteps);
+        check_status();
+        return;
+    }
     if (!graph_ready) {
         for (auto& t : tools)
             if (!t->graph_safe())

@@ -15,6 +15,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -117,6 +118,9 @@ class Tool {
     virtual bool graph_safe() const { return true; }
     // registry key of kernel tools ("" for other tools)
     virtual std::string key() const { return ""; }
+    // +1 for the scope openers (while, if), -1 for end (Tool.hpp
+    // scope_modifier); such tools are run by the pipeline, not execute()
+    virtual int scope() const { return 0; }
     const std::string& name() const { return name_; }
     const std::string& type() const { return type_; }
     bool once() const { return once_; }
@@ -125,6 +129,16 @@ class Tool {
   protected:
     std::string name_, type_;
     bool once_;
+};
+
+// While / If condition tools (aquagpusph/CalcServer/Conditional.cpp)
+class ControlTool : public Tool {
+  public:
+    using Tool::Tool;
+    void execute(Sim&, hipStream_t) override {}
+    bool graph_safe() const override { return false; }
+    // evaluate the condition (reads it back: synchronises the stream)
+    virtual bool eval(Sim& S, hipStream_t s) = 0;
 };
 
 // Tool description collected by the XML parser (ProblemSetup::sphTool)
@@ -275,6 +289,18 @@ class Sim {
     bool graph_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // control flow: jump[i] = matching end of an opener, opener of an end
+    bool has_control = false;
+    std::vector<size_t> jump;
+    std::vector<char> if_taken;
+    // with control flow, one graph per run of tools between control tools,
+    // keyed by the run's first tool index
+    struct Segment {
+        size_t end = 0;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<size_t, Segment> segments;
     bool is_setup = false;
     FusionPlan fusion;
     CellList clist;
@@ -298,9 +324,15 @@ class Sim {
     void* ptr(const std::string& name);
     void* ptr(Var* v);
     int assert_id(const std::string& name);
+    // one step of the pipeline in order, following while/if/end; `hook` is
+    // called around each executed tool (index, false before / true after)
+    void run_tools(hipStream_t s,
+                   const std::function<void(size_t, bool)>& hook = {});
 
   private:
-    void run_tools(hipStream_t s);
+    size_t control(size_t pc, hipStream_t s);
+    void match_scopes();
+    void step_segments(int nsteps);
     void plan_fusion();
     void make_defs();
     void load_sets();

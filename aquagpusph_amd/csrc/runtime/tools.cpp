@@ -356,7 +356,9 @@ class ReductionTool : public Tool {
     ReductionTool(const ToolDesc& d)
         : Tool(d.get("name"), "reduction", d.get("once") == "true"), d_(d)
     {
-        const std::string op = squeeze(d.text);
+        // the operation is a C statement; its closing ';' may be left out
+        std::string op = squeeze(d.text);
+        if (!op.empty() && op.back() != ';') op += ';';
         if (op == "c=min(a,b);" || op == "c=(a<b)?a:b;" ||
             op == "c=(a<=b)?a:b;" || op == "c=(b<a)?b:a;")
             op_ = AQ_RED_MIN;
@@ -527,6 +529,74 @@ class AssertTool : public VmTool {
     ToolDesc d_;
 };
 
+// ---- while / if / end: Conditional.cpp ------------------------------------------
+// The condition runs on the device VM into a hidden int scalar, which is
+// read back into pinned memory (a stream synchronisation, the counterpart of
+// clWaitForEvents in Conditional::next_tool) for the host to pick the next
+// tool.
+class CondTool : public ControlTool {
+  public:
+    CondTool(const ToolDesc& d)
+        : ControlTool(d.get("name"), d.get("type"), d.get("once") == "true"),
+          d_(d)
+    {
+    }
+    ~CondTool() override
+    {
+        if (host_) (void)hipHostFree(host_);
+    }
+    int scope() const override { return 1; }
+    void setup(Sim& S) override
+    {
+        if (prog_.p) return;  // the same tool placed twice
+        static int count = 0;
+        Var* res = S.vars.reg("__condition " + std::to_string(count++) + " " +
+                                  name_,
+                              "int", "", "0");
+        slot_ = res->slot;
+        VmProgram p;
+        p.compile(parse_expr(d_.get("condition")),
+                  [&](const std::string& n, SlotRef& r) {
+                      return S.vars.slot(n, r);
+                  });
+        p.store(SlotRef{ slot_, 'i' });
+        p.end();
+        if (vm_depth(p.ops) > 30)
+            throw Error(AQ_EUNSUP, "condition of tool \"" + name_ +
+                                       "\" is too deep");
+        prog_.alloc(p.ops.size() * sizeof(VmOp));
+        AQ_HIP(hipMemcpy(prog_.p, p.ops.data(), p.ops.size() * sizeof(VmOp),
+                         hipMemcpyHostToDevice));
+        AQ_HIP(hipHostMalloc((void**)&host_, sizeof(int32_t),
+                             hipHostMallocDefault));
+    }
+    bool eval(Sim& S, hipStream_t s) override
+    {
+        vm_launch((const VmOp*)prog_.p, S.vars.dev_arena(), S.d_status,
+                  S.d_fail_id, s);
+        AQ_HIP(hipMemcpyAsync(host_, S.vars.dev_arena() + slot_,
+                              sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        AQ_HIP(hipStreamSynchronize(s));
+        return *host_ != 0;
+    }
+
+  private:
+    ToolDesc d_;
+    DevBuf prog_;
+    uint32_t slot_ = 0;
+    int32_t* host_ = nullptr;
+};
+
+class EndTool : public ControlTool {
+  public:
+    EndTool(const ToolDesc& d)
+        : ControlTool(d.get("name"), "end", d.get("once") == "true")
+    {
+    }
+    int scope() const override { return -1; }
+    bool eval(Sim&, hipStream_t) override { return true; }
+};
+
 // ---- copy: Copy.cpp -----------------------------------------------------------
 class CopyTool : public Tool {
   public:
@@ -642,6 +712,8 @@ std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d)
     if (t == "set") return std::make_shared<SetTool>(d);
     if (t == "set_scalar") return std::make_shared<SetScalarTool>(d);
     if (t == "assert") return std::make_shared<AssertTool>(d);
+    if (t == "while" || t == "if") return std::make_shared<CondTool>(d);
+    if (t == "end") return std::make_shared<EndTool>(d);
     if (t == "copy") return std::make_shared<CopyTool>(d);
     if (t == "installable") return std::make_shared<InstallableTool>(d);
     if (t == "dummy" || t.compare(0, 7, "report_") == 0)

@@ -97,6 +97,154 @@ __global__ void predictor_k(const int* __restrict__ imove,
     rho_in[i] = rho[i] + DT * drhodt[i];
 }
 
+// ---- semi-implicit midpoint scheme (basic/time_scheme/midpoint.cl);
+// vectors on every stored component, as the reference's float4 arithmetic
+// midpoint.cl:54-75 (predictor): the state at the start of the step
+template <int D>
+__global__ void mp_predictor_k(const float* __restrict__ r,
+                               const float* __restrict__ u,
+                               const float* __restrict__ dudt,
+                               const float* __restrict__ rho,
+                               const float* __restrict__ drhodt,
+                               float* __restrict__ r_in,
+                               float* __restrict__ u_in,
+                               float* __restrict__ dudt_in,
+                               float* __restrict__ rho_in,
+                               float* __restrict__ drhodt_in, uint32_t N)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        dudt_in[a] = dudt[a];
+        u_in[a] = u[a];
+        r_in[a] = r[a];
+    }
+    drhodt_in[i] = drhodt[i];
+    rho_in[i] = rho[i];
+}
+
+// midpoint.cl:95-112 (midpoint): half a step with the current rates
+template <int D>
+__global__ void mp_advance_k(const int* __restrict__ imove,
+                             const float* __restrict__ u_in,
+                             float* __restrict__ u,
+                             const float* __restrict__ dudt,
+                             const float* __restrict__ rho_in,
+                             float* __restrict__ rho,
+                             const float* __restrict__ drhodt, uint32_t N,
+                             const float* __restrict__ dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= 0) return;
+    const float hdt = 0.5f * (*dtp);
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        u[a] = u_in[a] + hdt * dudt[a];
+    }
+    rho[i] = rho_in[i] + hdt * drhodt[i];
+}
+
+// midpoint.cl:129-142 (midpoint_r)
+template <int D>
+__global__ void mp_advance_r_k(const int* __restrict__ imove,
+                               const float* __restrict__ r_in,
+                               float* __restrict__ r,
+                               const float* __restrict__ u, uint32_t N,
+                               const float* __restrict__ dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= 0) return;
+    const float hdt = 0.5f * (*dtp);
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        r[a] = r_in[a] + hdt * u[a];
+    }
+}
+
+// midpoint.cl:164-179 (relax)
+template <int D>
+__global__ void mp_relax_k(const int* __restrict__ imove,
+                           const float* __restrict__ dudt_in,
+                           float* __restrict__ dudt,
+                           const float* __restrict__ drhodt_in,
+                           float* __restrict__ drhodt, uint32_t N,
+                           const float* __restrict__ fp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= 0) return;
+    const float f = *fp;
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        dudt[a] = f * dudt_in[a] + (1.f - f) * dudt[a];
+    }
+    drhodt[i] = f * drhodt_in[i] + (1.f - f) * drhodt[i];
+}
+
+// midpoint.cl:203-226 (residuals)
+template <int D>
+__global__ void mp_residuals_k(const int* __restrict__ imove,
+                               const float* __restrict__ m,
+                               const float* __restrict__ u,
+                               const float* __restrict__ dudt_in,
+                               const float* __restrict__ dudt,
+                               const float* __restrict__ rho,
+                               const float* __restrict__ p,
+                               const float* __restrict__ drhodt_in,
+                               const float* __restrict__ drhodt,
+                               float* __restrict__ res, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= 0) {
+        res[i] = 0.f;
+        return;
+    }
+    const float rho2 = rho[i] * rho[i];
+    const f3 dd = sub<D>(ldv<D>(dudt, i), ldv<D>(dudt_in, i));
+    res[i] = m[i] * (fabsf(dot<D>(ldv<D>(u, i), dd)) +
+                     fabsf(p[i] / rho2 * (drhodt[i] - drhodt_in[i])));
+}
+
+// midpoint.cl:252-270 (corrector): the whole step with the last rates
+template <int D>
+__global__ void mp_corrector_k(const int* __restrict__ imove,
+                               const float* __restrict__ r_in,
+                               float* __restrict__ r,
+                               const float* __restrict__ u_in,
+                               float* __restrict__ u,
+                               const float* __restrict__ dudt,
+                               const float* __restrict__ rho_in,
+                               float* __restrict__ rho,
+                               const float* __restrict__ drhodt, uint32_t N,
+                               const float* __restrict__ dtp)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] <= 0) return;
+    const float dt = *dtp;
+    const float hdt2 = 0.5f * dt * dt;
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        r[a] = r_in[a] + dt * u_in[a] + hdt2 * dudt[a];
+        u[a] = u_in[a] + dt * dudt[a];
+    }
+    rho[i] = rho_in[i] + dt * drhodt[i];
+}
+
 // basic/time_scheme/improved_euler.cl:125-147
 template <int D>
 __global__ void corrector_k(const int* __restrict__ imove,
@@ -1165,6 +1313,54 @@ void launch_rates(int dims, const uint32_t* iset, const int32_t* imove,
 {
     AQ_DISPATCH(dims, rates_k, AQ_GRID(N), iset, imove, grad_p, lap_u, div_u,
                 dudt, drhodt, visc_dyn, N, g);
+}
+void launch_mp_predictor(int dims, const float* r, const float* u,
+                         const float* dudt, const float* rho,
+                         const float* drhodt, float* r_in, float* u_in,
+                         float* dudt_in, float* rho_in, float* drhodt_in,
+                         uint32_t N, hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_predictor_k, AQ_GRID(N), r, u, dudt, rho, drhodt,
+                r_in, u_in, dudt_in, rho_in, drhodt_in, N);
+}
+void launch_mp_advance(int dims, const int32_t* imove, const float* u_in,
+                       float* u, const float* dudt, const float* rho_in,
+                       float* rho, const float* drhodt, uint32_t N,
+                       const float* dt, hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_advance_k, AQ_GRID(N), imove, u_in, u, dudt, rho_in,
+                rho, drhodt, N, dt);
+}
+void launch_mp_advance_r(int dims, const int32_t* imove, const float* r_in,
+                         float* r, const float* u, uint32_t N,
+                         const float* dt, hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_advance_r_k, AQ_GRID(N), imove, r_in, r, u, N, dt);
+}
+void launch_mp_relax(int dims, const int32_t* imove, const float* dudt_in,
+                     float* dudt, const float* drhodt_in, float* drhodt,
+                     uint32_t N, const float* f, hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_relax_k, AQ_GRID(N), imove, dudt_in, dudt, drhodt_in,
+                drhodt, N, f);
+}
+void launch_mp_residuals(int dims, const int32_t* imove, const float* m,
+                         const float* u, const float* dudt_in,
+                         const float* dudt, const float* rho, const float* p,
+                         const float* drhodt_in, const float* drhodt,
+                         float* res, uint32_t N, hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_residuals_k, AQ_GRID(N), imove, m, u, dudt_in, dudt,
+                rho, p, drhodt_in, drhodt, res, N);
+}
+void launch_mp_corrector(int dims, const int32_t* imove, const float* r_in,
+                         float* r, const float* u_in, float* u,
+                         const float* dudt, const float* rho_in, float* rho,
+                         const float* drhodt, uint32_t N, const float* dt,
+                         hipStream_t s)
+{
+    AQ_DISPATCH(dims, mp_corrector_k, AQ_GRID(N), imove, r_in, r, u_in, u,
+                dudt, rho_in, rho, drhodt, N, dt);
 }
 void launch_energy_kin(int dims, float* ek, const int32_t* imove,
                        const float* u, const float* m, uint32_t N,

@@ -102,6 +102,47 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         binormal<D, R>(normal, tangent, binormal_, N);                         \
     }                                                                          \
+    ORC_EXPORT void orc_midpoint_predictor_##SR##_##D##d(                      \
+        const R* r, const R* u, const R* dudt, const R* rho, const R* drhodt,  \
+        R* r_in, R* u_in, R* dudt_in, R* rho_in, R* drhodt_in, usize N)        \
+    {                                                                          \
+        midpoint_predictor<D, R>(r, u, dudt, rho, drhodt, r_in, u_in, dudt_in, \
+                                 rho_in, drhodt_in, N);                        \
+    }                                                                          \
+    ORC_EXPORT void orc_midpoint_advance_##SR##_##D##d(                        \
+        const int* imove, const R* u_in, R* u, const R* dudt, const R* rho_in, \
+        R* rho, const R* drhodt, usize N, R dt)                                \
+    {                                                                          \
+        midpoint_advance<D, R>(imove, u_in, u, dudt, rho_in, rho, drhodt, N,   \
+                               dt);                                            \
+    }                                                                          \
+    ORC_EXPORT void orc_midpoint_r_##SR##_##D##d(                              \
+        const int* imove, const R* r_in, R* r, const R* u, usize N, R dt)      \
+    {                                                                          \
+        midpoint_r<D, R>(imove, r_in, r, u, N, dt);                            \
+    }                                                                          \
+    ORC_EXPORT void orc_midpoint_relax_##SR##_##D##d(                          \
+        const int* imove, const R* dudt_in, R* dudt, const R* drhodt_in,       \
+        R* drhodt, usize N, R f)                                               \
+    {                                                                          \
+        midpoint_relax<D, R>(imove, dudt_in, dudt, drhodt_in, drhodt, N, f);   \
+    }                                                                          \
+    ORC_EXPORT void orc_midpoint_residuals_##SR##_##D##d(                      \
+        const int* imove, const R* m, const R* u, const R* dudt_in,            \
+        const R* dudt, const R* rho, const R* p, const R* drhodt_in,           \
+        const R* drhodt, R* res, usize N)                                      \
+    {                                                                          \
+        midpoint_residuals<D, R>(imove, m, u, dudt_in, dudt, rho, p,           \
+                                 drhodt_in, drhodt, res, N);                   \
+    }                                                                          \
+    ORC_EXPORT void orc_midpoint_corrector_##SR##_##D##d(                      \
+        const int* imove, const R* r_in, R* r, const R* u_in, R* u,            \
+        const R* dudt, const R* rho_in, R* rho, const R* drhodt, usize N,      \
+        R dt)                                                                  \
+    {                                                                          \
+        midpoint_corrector<D, R>(imove, r_in, r, u_in, u, dudt, rho_in, rho,   \
+                                 drhodt, N, dt);                               \
+    }                                                                          \
     ORC_EXPORT void orc_energy_kin_##SR##_##D##d(                              \
         R* ek, const int* imove, const R* u, const R* m, usize N)              \
     {                                                                          \

@@ -1532,6 +1532,116 @@ void neighs(const int* imove, usize* n_neighs, usize limit, usize N, LL ll)
 }
 
 // ---------------------------------------------------------------------------
+// Semi-implicit midpoint scheme (basic/time_scheme/midpoint.cl).  Vectors
+// are updated on every stored component, as the float4 arithmetic of the
+// reference.
+// ---------------------------------------------------------------------------
+// midpoint.cl:54-75 (predictor): the state at the start of the step
+template <int D, typename R>
+void midpoint_predictor(const R* r, const R* u, const R* dudt, const R* rho,
+                        const R* drhodt, R* r_in, R* u_in, R* dudt_in,
+                        R* rho_in, R* drhodt_in, usize N)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t a = 0; a < (size_t)N * VS; a++) {
+        dudt_in[a] = dudt[a];
+        u_in[a] = u[a];
+        r_in[a] = r[a];
+    }
+    for (size_t i = 0; i < N; i++) {
+        drhodt_in[i] = drhodt[i];
+        rho_in[i] = rho[i];
+    }
+}
+
+// midpoint.cl:95-112 (midpoint): half a step with the current rates
+template <int D, typename R>
+void midpoint_advance(const int* imove, const R* u_in, R* u, const R* dudt,
+                      const R* rho_in, R* rho, const R* drhodt, usize N,
+                      R dt)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) continue;
+        const R hdt = R(0.5f) * dt;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            u[a] = u_in[a] + hdt * dudt[a];
+        }
+        rho[i] = rho_in[i] + hdt * drhodt[i];
+    }
+}
+
+// midpoint.cl:129-142 (midpoint_r)
+template <int D, typename R>
+void midpoint_r(const int* imove, const R* r_in, R* r, const R* u, usize N,
+                R dt)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) continue;
+        const R hdt = R(0.5f) * dt;
+        for (int k = 0; k < VS; k++)
+            r[i * VS + k] = r_in[i * VS + k] + hdt * u[i * VS + k];
+    }
+}
+
+// midpoint.cl:164-179 (relax)
+template <int D, typename R>
+void midpoint_relax(const int* imove, const R* dudt_in, R* dudt,
+                    const R* drhodt_in, R* drhodt, usize N, R f)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) continue;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            dudt[a] = f * dudt_in[a] + (R(1) - f) * dudt[a];
+        }
+        drhodt[i] = f * drhodt_in[i] + (R(1) - f) * drhodt[i];
+    }
+}
+
+// midpoint.cl:203-226 (residuals): m (|u.(dudt - dudt_in)| +
+// |p/rho^2 (drhodt - drhodt_in)|)
+template <int D, typename R>
+void midpoint_residuals(const int* imove, const R* m, const R* u,
+                        const R* dudt_in, const R* dudt, const R* rho,
+                        const R* p, const R* drhodt_in, const R* drhodt,
+                        R* res, usize N)
+{
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) {
+            res[i] = R(0);
+            continue;
+        }
+        const R rho2 = rho[i] * rho[i];
+        const V<D, R> dd = sub<D, R>(ld<D, R>(dudt, i), ld<D, R>(dudt_in, i));
+        res[i] = m[i] * (std::fabs(dot<D, R>(ld<D, R>(u, i), dd)) +
+                         std::fabs(p[i] / rho2 * (drhodt[i] - drhodt_in[i])));
+    }
+}
+
+// midpoint.cl:252-270 (corrector): the whole step with the last rates
+template <int D, typename R>
+void midpoint_corrector(const int* imove, const R* r_in, R* r, const R* u_in,
+                        R* u, const R* dudt, const R* rho_in, R* rho,
+                        const R* drhodt, usize N, R dt)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) continue;
+        const R hdt2 = R(0.5f) * dt * dt;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            r[a] = r_in[a] + dt * u_in[a] + hdt2 * dudt[a];
+            u[a] = u_in[a] + dt * dudt[a];
+        }
+        rho[i] = rho_in[i] + dt * drhodt[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Energy of the fluid (cfd/energy_kin.xml, cfd/energy.xml).  Non-fluid
 // particles get zeros.
 // ---------------------------------------------------------------------------

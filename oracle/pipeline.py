@@ -62,7 +62,8 @@ class OracleSim:
     def __init__(self, case, dtype=np.float32, domain=True, bi=None,
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
-                 min_bound_dist=0.0, delta_corr=None, energy=False):
+                 min_bound_dist=0.0, delta_corr=None, energy=False,
+                 midpoint=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -76,7 +77,22 @@ class OracleSim:
                          gp_model=gp_model, freeslip_iset=freeslip_iset,
                          bie=bie, dr_factor=dr_factor,
                          min_bound_dist=min_bound_dist,
-                         delta_corr=delta_corr, energy=energy)
+                         delta_corr=delta_corr, energy=energy,
+                         midpoint=midpoint)
+        if midpoint is not None:
+            # basic/time_scheme/midpoint.xml (+ midpoint/{r,autostop,
+            # autorelax}.xml): iter_max, relax, r, Residual_max, mindec, inc
+            mp = dict(iter_max=50, relax=0.0, r=False, residual_max=None,
+                      mindec=None, inc=0.25)
+            mp.update(midpoint)
+            self.mp = mp
+            self.relax = np.float32(mp["relax"])
+            self.relax0 = self.relax
+            self.residual = np.float32(0.0)
+            self.residual_prev = np.float32(1.0)
+            self.iter_midpoint = 0
+            self.mp_iters = []  # inner iterations of each step
+            self.residuals = []  # Residual_midpoint of each inner iteration
         N = self.N = int(case["N"])
         R = self.dtype
         f = {}
@@ -232,29 +248,12 @@ class OracleSim:
         sc(f["drhodt"], f["drhodt_in"])
 
     # ---- one time step ----------------------------------------------
-    def step(self):
+    def _forces(self, ll, reinit_lap_p=True):
+        """Interactions stage: the "Sort" to "Rates" tools (reinits, Shepard,
+        interactions, sensors, lap p, wall terms, rates, delta-SPH)."""
         f, N, o = self.f, self.N, self.opts
-        k, R, ll = self._k, self._r, None
+        k, R = self._k, self._r
         n = ctypes.c_uint(N)
-        k("predictor")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
-                       _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
-                       _p(f["u_in"]), _p(f["dudt_in"]), _p(f["rho_in"]),
-                       _p(f["drhodt_in"]), n, R(self.dt))
-        if o["domain"]:
-            k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
-                        _p(f["dudt_in"]), _p(f["m"]), n,
-                        _p(self.domain_min), _p(self.domain_max))
-        self.link_list("r_in")
-        self.reorder()
-        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
-                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
-        k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
-        if o["gp"]:
-            # "Backup associations" + "Sort associations" (GP/Sort.cl)
-            f["associations_in"][...] = f["associations"]
-            lib().orc_gp_sort(_p(f["associations_in"]), _p(f["associations"]),
-                              _p(f["id_sorted"]), n)
-        ll = self._ll()
         H, CONW, CONF = R(self.H), R(self.CONW), R(self.CONF)
         if o["delta_corr"]:
             f["lap_p_corr"][...] = 0
@@ -266,7 +265,8 @@ class OracleSim:
             k("mls_inv")(_p(f["imove"]), _p(f["mls"]), n, ctypes.c_uint(1))
             f["mls_fluid"][...] = f["mls"]
         for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
-            f[key][...] = 0
+            if key != "lap_p" or reinit_lap_p:
+                f[key][...] = 0
         if o["bi"]:
             k("bi_shepard_compute")(
                 _p(f["imove"]), _p(f["r"]), _p(f["normal"]), _p(f["tangent"]),
@@ -390,6 +390,134 @@ class OracleSim:
                             _p(f["m"]), n)
             E = self.energy
             E["energy_Kin"] = self._sum(f["energy_kin"])
+
+    def _step_midpoint(self):
+        """One step of the preset stack basic + domain + cfd + midpoint
+        (+ r / autostop / autorelax) + deltaSPH + BI + variableTimeStep, in
+        the tool order the insert rules give it: the delta-SPH lap p reinit
+        sits before the inner loop, the other reinits inside it."""
+        f, N, o, mp = self.f, self.N, self.opts, self.mp
+        k, R = self._k, self._r
+        n = ctypes.c_uint(N)
+        f32 = np.float32
+        k("midpoint_predictor")(_p(f["r"]), _p(f["u"]), _p(f["dudt"]),
+                                _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
+                                _p(f["u_in"]), _p(f["dudt_in"]),
+                                _p(f["rho_in"]), _p(f["drhodt_in"]), n)
+        if o["domain"]:
+            k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
+                        _p(f["dudt_in"]), _p(f["m"]), n,
+                        _p(self.domain_min), _p(self.domain_max))
+        self.link_list("r_in")
+        self.reorder()
+        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
+                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
+        ll = self._ll()
+        if o["delta"]:
+            f["lap_p"][...] = 0
+        self.iter_midpoint = 0
+        f["drhodt"][...] = f["drhodt_in"]
+        f["dudt"][...] = f["dudt_in"]
+        f["r_in"][...] = f["r"]
+        f["u_in"][...] = f["u"]
+        f["rho_in"][...] = f["rho"]
+        iters = 0
+        while self.iter_midpoint < mp["iter_max"]:
+            f["dudt_in"][...] = f["dudt"]
+            f["drhodt_in"][...] = f["drhodt"]
+            k("midpoint_advance")(_p(f["imove"]), _p(f["u_in"]), _p(f["u"]),
+                                  _p(f["dudt"]), _p(f["rho_in"]),
+                                  _p(f["rho"]), _p(f["drhodt"]), n,
+                                  R(self.dt))
+            if mp["r"]:
+                k("midpoint_r")(_p(f["imove"]), _p(f["r_in"]), _p(f["r"]),
+                                _p(f["u"]), n, R(self.dt))
+            k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
+                     _p(f["p"]), _p(self.sets["refd"]), n, R(self.cs),
+                     R(self.p0))
+            self._forces(ll, reinit_lap_p=False)
+            if mp["residual_max"] is not None:
+                res = np.zeros(N, self.dtype)
+                k("midpoint_residuals")(_p(f["imove"]), _p(f["m"]),
+                                        _p(f["u"]), _p(f["dudt_in"]),
+                                        _p(f["dudt"]), _p(f["rho"]),
+                                        _p(f["p"]), _p(f["drhodt_in"]),
+                                        _p(f["drhodt"]), _p(res), n)
+                self.residual = f32(self._sum(res))
+                self.residuals.append(float(self.residual))
+                # set_scalar tools: double arithmetic of float inputs
+                if mp["mindec"] is not None:
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        ratio = np.float64(self.residual) / \
+                            np.float64(self.residual_prev)
+                    if ratio > float(f32(mp["mindec"])):
+                        self.relax = f32(float(f32(mp["inc"])) +
+                                         (1.0 - float(f32(mp["inc"]))) *
+                                         float(self.relax))
+                    if self.iter_midpoint == 0:
+                        self.relax = self.relax0
+                if float(self.residual) < float(f32(mp["residual_max"])):
+                    self.iter_midpoint = mp["iter_max"]
+                if mp["mindec"] is not None:
+                    if self.iter_midpoint == mp["iter_max"]:
+                        self.relax = f32(0.0)
+                    self.residual_prev = self.residual
+            k("midpoint_relax")(_p(f["imove"]), _p(f["dudt_in"]),
+                                _p(f["dudt"]), _p(f["drhodt_in"]),
+                                _p(f["drhodt"]), n, R(self.relax))
+            self.iter_midpoint += 1
+            iters += 1
+        self.mp_iters.append(iters)
+        k("midpoint_corrector")(_p(f["imove"]), _p(f["r_in"]), _p(f["r"]),
+                                _p(f["u_in"]), _p(f["u"]), _p(f["dudt"]),
+                                _p(f["rho_in"]), _p(f["rho"]),
+                                _p(f["drhodt"]), n, R(self.dt))
+        self._time_step()
+
+    def _time_step(self):
+        """fixed time step, TimeStep.cl + min reduction + assert, t, iter
+        (basic.xml:164-169, cfd/variableTimeStep.xml)."""
+        f, o = self.f, self.opts
+        k, R = self._k, self._r
+        n = ctypes.c_uint(self.N)
+        self.dt = self.dtype.type(float(self.courant) * float(self.h) /
+                                  float(self.cs))
+        if o["variable_dt"]:
+            k("timestep")(_p(f["imove"]), _p(f["u"]), _p(f["dt_var"]), n,
+                          R(self.dt), R(self.dt_min), R(self.courant),
+                          R(self.dt_Ma), R(self.h))
+            self.dt = self.dtype.type(np.min(f["dt_var"]))
+            assert self.dt > 0.0, "dt > 0.0 assertion failed"
+        self.t = self.dtype.type(float(self.t) + float(self.dt))
+        self.iter += 1
+
+    def step(self):
+        if self.opts["midpoint"] is not None:
+            return self._step_midpoint()
+        f, N, o = self.f, self.N, self.opts
+        k, R, ll = self._k, self._r, None
+        n = ctypes.c_uint(N)
+        k("predictor")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
+                       _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
+                       _p(f["u_in"]), _p(f["dudt_in"]), _p(f["rho_in"]),
+                       _p(f["drhodt_in"]), n, R(self.dt))
+        if o["domain"]:
+            k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
+                        _p(f["dudt_in"]), _p(f["m"]), n,
+                        _p(self.domain_min), _p(self.domain_max))
+        self.link_list("r_in")
+        self.reorder()
+        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
+                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
+        if o["gp"]:
+            # "Backup associations" + "Sort associations" (GP/Sort.cl)
+            f["associations_in"][...] = f["associations"]
+            lib().orc_gp_sort(_p(f["associations_in"]), _p(f["associations"]),
+                              _p(f["id_sorted"]), n)
+        ll = self._ll()
+        self._forces(ll)
         if o["bie"]:
             k("bie_elastic_bounce")(_p(f["imove"]), _p(f["r_in"]),
                                     _p(f["normal"]), _p(f["m"]),
@@ -429,16 +557,7 @@ class OracleSim:
                          _p(f["m"]), _p(f["rho"]), n, *ll, R(self.DIMS),
                          R(o["dr_factor"]))
         # "fixed time step" set_scalar (basic.xml:164), double -> float
-        self.dt = self.dtype.type(float(self.courant) * float(self.h) /
-                                  float(self.cs))
-        if o["variable_dt"]:
-            k("timestep")(_p(f["imove"]), _p(f["u"]), _p(f["dt_var"]), n,
-                          R(self.dt), R(self.dt_min), R(self.courant),
-                          R(self.dt_Ma), R(self.h))
-            self.dt = self.dtype.type(np.min(f["dt_var"]))
-            assert self.dt > 0.0, "dt > 0.0 assertion failed"
-        self.t = self.dtype.type(float(self.t) + float(self.dt))
-        self.iter += 1
+        self._time_step()
 
     def run(self, nsteps):
         for _ in range(nsteps):

@@ -139,6 +139,61 @@ def test_energy_presets_match_oracle(cuda, tmp_path, dims, nl, steps):
     sim.close()
 
 
+MIDPOINT_PRESETS = (
+    "resources/Presets/basic.xml", "resources/Presets/basic/domain.xml",
+    "resources/Presets/cfd.xml",
+    "resources/Presets/basic/time_scheme/midpoint.xml",
+    "resources/Presets/basic/time_scheme/midpoint/autostop.xml",
+    "resources/Presets/basic/time_scheme/midpoint/autorelax.xml",
+    "resources/Presets/cfd/deltaSPH.xml", "resources/Presets/cfd/BI.xml",
+    "resources/Presets/cfd/variableTimeStep.xml")
+MIDPOINT_R = "resources/Presets/basic/time_scheme/midpoint/r.xml"
+
+
+@pytest.mark.parametrize("dims,with_r", [(2, False), (2, True), (3, False)])
+def test_midpoint_scheme_matches_oracle(cuda, tmp_path, dims, with_r):
+    """basic/time_scheme/midpoint.xml with autostop and autorelax: a
+    while/end inner loop with device-evaluated conditions read back by the
+    host.  Fields, the inner iteration count, the residual and the adapted
+    relaxation factor against the oracle after 4 steps (1e-4 of each
+    field's largest magnitude; the residual sum to 1e-3); graph replay
+    (one graph per run of tools between control tools) equals eager
+    execution bit for bit."""
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    presets = MIDPOINT_PRESETS
+    if with_r:
+        i = presets.index("resources/Presets/basic/time_scheme/midpoint.xml")
+        presets = presets[:i + 1] + (MIDPOINT_R,) + presets[i + 1:]
+    extra = ('    <Variables>\n'
+             '        <Variable name="iter_midpoint_max" type="unsigned int" '
+             'value="4" />\n'
+             '        <Variable name="relax_midpoint" type="float" '
+             'value="0.1" />\n'
+             '    </Variables>')
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets,
+                               extra_xml=extra)
+    sim = Simulation(xml, dims)
+    gsim = Simulation(xml, dims)
+    o = op.OracleSim(case, midpoint=dict(iter_max=4, relax=0.1, r=with_r,
+                                         residual_max=1e-8, mindec=0.2,
+                                         inc=0.25))
+    steps = 4
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    gsim.step(steps, graph=True)
+    _compare(sim, o, 1e-4)
+    assert int(sim.get("iter_midpoint")) == o.iter_midpoint
+    assert float(sim.get("relax_midpoint")) == pytest.approx(
+        float(o.relax), rel=1e-5, abs=1e-7)
+    assert float(sim.get("Residual_midpoint")) == pytest.approx(
+        float(o.residual), rel=1e-3)
+    for f in ("r", "u", "rho", "p", "dudt", "drhodt", "lap_p"):
+        np.testing.assert_array_equal(sim.get(f), gsim.get(f), err_msg=f)
+    sim.close()
+    gsim.close()
+
+
 GP_FIELDS = FIELDS + ("associations", "gp_icell", "gp_rho", "gp_p", "gp_u")
 
 
