@@ -82,11 +82,15 @@ def write_particles(case, path):
                header="aquagpusph_amd particles: " + ", ".join(fields))
 
 
-def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml=""):
+def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml="",
+               save=None, timing=None):
     """Write Main.xml (+ Fluid.dat when load is True) and return its path.
 
     With load=False the ParticlesSet has no Load element and the arrays are
-    uploaded afterwards through Simulation.set (large cases)."""
+    uploaded afterwards through Simulation.set (large cases).  `save` =
+    (format, file pattern, fields) adds a <Save> to the set; `timing` is a
+    list of (name, type, value) <Timing> options, e.g. ("End", "Steps", 10),
+    ("Output", "IPF", 5)."""
     os.makedirs(folder, exist_ok=True)
     dims = case["dims"]
     p = case["params"]
@@ -118,7 +122,17 @@ def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml=""):
         write_particles(case, os.path.join(folder, "Fluid.dat"))
         lines.append('        <Load format="FastASCII" file="Fluid.dat" '
                      'fields="%s" />' % ", ".join(fields_of(case)))
+    if save is not None:
+        fmt, pattern, fl = save
+        lines.append('        <Save format="%s" file="%s" fields="%s" />'
+                     % (fmt, pattern, ", ".join(fl)))
     lines.append("    </ParticlesSet>")
+    if timing:
+        lines.append("    <Timing>")
+        for name, typ, val in timing:
+            lines.append('        <Option name="%s" type="%s" value="%s" />'
+                         % (name, typ, val))
+        lines.append("    </Timing>")
     if extra_xml:
         lines.append(extra_xml)
     lines.append("</sphInput>")

@@ -162,6 +162,24 @@ AQ_API int aq_sim_definition(aq_sim* s, const char* name, char* buf,
     });
 }
 
+// The AQUAgpusph main loop (main.cpp:160-180): steps until the Timing end
+// criteria, writing the particle sets and the state file at each output
+// point (at most max_steps steps when max_steps > 0).
+AQ_API int aq_sim_run(aq_sim* s, long max_steps, int graph, long* steps)
+{
+    return guard([&] {
+        const long n = SIM(s).run(max_steps, graph != 0);
+        if (steps) *steps = n;
+    });
+}
+
+// Write every set's Save files (original particle order) and the state
+// file now (FileManager::save, FileManager.cpp:146-155).
+AQ_API int aq_sim_save(aq_sim* s)
+{
+    return guard([&] { SIM(s).save_now(); });
+}
+
 // Per-tool GPU time: runs `nsteps` eager steps with HIP events around every
 // tool and writes the mean milliseconds of tool i to ms[i] (the
 // report_performance counterpart, aquagpusph/CalcServer/Reports/

@@ -99,46 +99,5 @@ void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
     }
 }
 
-void save_ascii(Sim& S, const std::string& path, size_t offset, size_t n,
-                const std::vector<std::string>& fields)
-{
-    std::vector<Var*> vs;
-    std::vector<std::vector<uint32_t>> data;
-    for (auto& name : fields) {
-        Var& v = S.vars.at(name);
-        if (!v.is_array() || v.length < offset + n)
-            throw Error(AQ_EINVAL, "cannot save '" + name + "'");
-        vs.push_back(&v);
-        data.emplace_back(n * v.type.stored);
-        AQ_HIP(hipMemcpy(data.back().data(),
-                         (char*)v.dev + offset * v.type.elem_bytes(),
-                         n * v.type.elem_bytes(), hipMemcpyDeviceToHost));
-    }
-    std::ofstream f(path);
-    if (!f) throw Error(AQ_ENOTFOUND, "cannot write '" + path + "'");
-    f << "# aquagpusph_amd particles:";
-    for (auto& n_ : fields) f << " " << n_;
-    f << "\n";
-    char b[64];
-    for (size_t i = 0; i < n; i++) {
-        for (size_t k = 0; k < vs.size(); k++) {
-            const Type& ty = vs[k]->type;
-            for (int c = 0; c < ty.ncomp; c++) {
-                const uint32_t w = data[k][i * ty.stored + c];
-                if (ty.base == 'f') {
-                    float x;
-                    memcpy(&x, &w, 4);
-                    snprintf(b, sizeof(b), "%.9g", (double)x);
-                } else if (ty.base == 'i')
-                    snprintf(b, sizeof(b), "%d", (int32_t)w);
-                else
-                    snprintf(b, sizeof(b), "%u", w);
-                f << b << (c + 1 < ty.ncomp ? " " : "");
-            }
-            f << (k + 1 < vs.size() ? ", " : "\n");
-        }
-    }
-}
-
 }  // namespace rt
 }  // namespace aq

@@ -301,6 +301,21 @@ class Sim {
         hipGraphExec_t exec = nullptr;
     };
     std::map<size_t, Segment> segments;
+    // output (aquagpusph/TimeManager.cpp, FileManager.cpp): next file index
+    // and last file of every set, the state file, the last output point
+    struct Output {
+        bool started = false;
+        std::vector<unsigned> next_index;
+        std::vector<std::string> last_file;
+        std::string state_file;
+        double time = 0.0;
+        unsigned long step = 0;
+    } out;
+    // run until the Timing end criteria, saving at the output points;
+    // returns the steps run (at most max_steps when > 0)
+    long run(long max_steps, bool graph);
+    // write every set's Save files and the state file now
+    void save_now();
     bool is_setup = false;
     FusionPlan fusion;
     CellList clist;
@@ -343,8 +358,6 @@ std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d);
 // particle file I/O (FastASCII / ASCII rows, comma or blank separated)
 void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
                     const std::vector<std::string>& fields);
-void save_ascii(Sim& S, const std::string& path, size_t offset, size_t n,
-                const std::vector<std::string>& fields);
 
 // scalar bytecode on the device
 void vm_launch(const VmOp* ops, uint8_t* arena, uint32_t* status,

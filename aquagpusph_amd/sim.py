@@ -61,6 +61,9 @@ class Simulation:
                                            ctypes.POINTER(ctypes.c_double)]
         L.aq_sim_walk_ms.argtypes = [ctypes.c_void_p,
                                      ctypes.POINTER(ctypes.c_double)]
+        L.aq_sim_run.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_long)]
+        L.aq_sim_save.argtypes = [ctypes.c_void_p]
         h = ctypes.c_void_p()
         _lib.check(L.aq_sim_create(os.fsencode(os.path.abspath(xml_path)),
                                    dims, ctypes.byref(h)))
@@ -82,6 +85,20 @@ class Simulation:
     def step(self, n=1, graph=False):
         f = self._L.aq_sim_step_graph if graph else self._L.aq_sim_step
         _lib.check(f(self._h, int(n)))
+
+    def run(self, max_steps=0, graph=True):
+        """The AQUAgpusph main loop: step until the case's Timing end
+        criteria (at most max_steps steps when > 0), writing the particle
+        <Save> files and AQUAgpusph.save.N.xml in the working directory at
+        every output point.  Returns the steps run."""
+        n = ctypes.c_long()
+        _lib.check(self._L.aq_sim_run(self._h, int(max_steps), int(graph),
+                                      ctypes.byref(n)))
+        return n.value
+
+    def save(self):
+        """Write the <Save> files and the state file now."""
+        _lib.check(self._L.aq_sim_save(self._h))
 
     def sync(self):
         _lib.check(self._L.aq_sim_sync(self._h))

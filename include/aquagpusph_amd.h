@@ -127,6 +127,12 @@ AQ_API unsigned aq_radix_key_bits(uint64_t max_val);
 AQ_API int aq_permute(const void* src, void* dst, size_t esize,
                       const uint32_t* perm, uint32_t N, void* stream);
 
+/* aquagpusph/CalcServer/UnSort.cl.in:31-42: out[id[i]] = in[i] for i < N
+ * (back to the original particle order), elem_bytes a multiple of 4
+ * (4, 8, 16 or 64). */
+AQ_API int aq_unsort(const uint32_t* id, const void* in, void* out,
+                     uint32_t N, uint32_t elem_bytes, void* stream);
+
 /* ---- streaming particle kernels --------------------------------------- */
 
 /* improved Euler predictor, basic/time_scheme/improved_euler.cl:75-103 */
@@ -351,6 +357,15 @@ AQ_API int aq_sim_definition(aq_sim* sim, const char* name, char* buf,
 /* Mean GPU milliseconds per tool over nsteps eager steps, into ms[n_tools]
  * (the report_performance counterpart, CalcServer/Reports/Performance.cpp). */
 AQ_API int aq_sim_profile_tools(aq_sim* sim, int nsteps, double* ms);
+/* The AQUAgpusph main loop (aquagpusph/main.cpp:160-180, TimeManager.cpp:
+ * 103-150): steps (graph replays when graph != 0) until the Timing end
+ * criteria, writing each ParticlesSet's <Save> files, in the original
+ * particle order, and the state file AQUAgpusph.save.N.xml (in the working
+ * directory) at every output point; at most max_steps steps when
+ * max_steps > 0.  *steps receives the number of steps run. */
+AQ_API int aq_sim_run(aq_sim* sim, long max_steps, int graph, long* steps);
+/* Write the <Save> files and the state file now (FileManager.cpp:146-155). */
+AQ_API int aq_sim_save(aq_sim* sim);
 /* Mean GPU milliseconds of the fused fluid neighbour walk kernel alone
  * (HIP events around its launch) over the last aq_sim_profile_tools call;
  * -1 when the pipeline has no fused walk.  No counterpart in the reference:
