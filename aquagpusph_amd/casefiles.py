@@ -83,20 +83,37 @@ def write_particles(case, path):
 
 
 def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml="",
-               save=None, timing=None):
+               save=None, timing=None, portals=None):
     """Write Main.xml (+ Fluid.dat when load is True) and return its path.
 
     With load=False the ParticlesSet has no Load element and the arrays are
     uploaded afterwards through Simulation.set (large cases).  `save` =
     (format, file pattern, fields) adds a <Save> to the set; `timing` is a
     list of (name, type, value) <Timing> options, e.g. ("End", "Steps", 10),
-    ("Output", "IPF", 5)."""
+    ("Output", "IPF", 5).  `portals` is a list of (prefix, out_r, in_r, n)
+    plane portals: cfd/portal.xml is included once per prefix, after the
+    presets, and set_scalar tools set portal_out_r, portal_in_r and portal_n
+    before each of that prefix's mirror and teleport tools (the layout of
+    examples/2D/taylor_green/src/templates/BCs.xml)."""
     os.makedirs(folder, exist_ok=True)
     dims = case["dims"]
     p = case["params"]
     lines = ['<?xml version="1.0" ?>', "<sphInput>"]
     for pr in presets:
         lines.append('    <Include file="%s" />' % pr)
+    for px, _, _, _ in portals or ():
+        lines.append('    <Include file="resources/Presets/cfd/portal.xml" '
+                     'prefix="%s" />' % px)
+    if portals:
+        lines.append("    <Tools>")
+        for px, out_r, in_r, n in portals:
+            for var, v in (("portal_out_r", out_r), ("portal_in_r", in_r),
+                           ("portal_n", n)):
+                lines.append('        <Tool action="insert" before="%scfd '
+                             'portal* mirror*" type="set_scalar" name="%s%s" '
+                             'in="%s" value="%s"/>'
+                             % (px, px, var, var, _vec(v, dims)))
+        lines.append("    </Tools>")
     lines.append("    <Variables>")
     var = [("g", "vec", _vec(p["g"], dims)),
            ("h", "float", _f(p["h"])),

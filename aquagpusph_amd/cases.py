@@ -186,6 +186,116 @@ def still_tank(dims, dr, L=0.3, H=0.15, W=0.2, hfac=2.0, refd=998.0,
     return case
 
 
+def taylor_green(n=32, L=2.0 * math.pi, U=1.0, hfac=4.0, cs=50.0,
+                 courant=0.1, refd=1.0, Re=2000.0, alpha=0.0, delta=0.0,
+                 drift=(0.0, 0.0)):
+    """Periodic 2D Taylor-Green vortices (the reference's
+    examples/2D/taylor_green/src/Create.py:44-117): n x n fluid particles on
+    the cell-centred lattice of [-L/2, L/2]^2, u = U sin(kx) cos(ky),
+    v = -U cos(kx) sin(ky), p = rho0 U^2 (cos 2kx + cos 2ky) / 4 with
+    k = 2 pi / L, rho = rho0 + p / cs^2, background pressure p0 = 3 rho0 U^2,
+    mu = max(alpha/8 rho0 h cs, rho0 U L / Re).  No walls: the box is closed
+    by the eight plane portals of taylor_green_portals.  `drift` adds a
+    uniform velocity (the vortices then travel through the portals)."""
+    dr = L / n
+    h = hfac * dr
+    k = 2.0 * math.pi / L
+    xs = -0.5 * L + (np.arange(n) + 0.5) * dr
+    pos = _lattice(xs, xs)
+    N = pos.shape[0]
+    x, y = pos[:, 0], pos[:, 1]
+    r = np.zeros((N, 2))
+    r[:, :] = pos
+    u = np.zeros((N, 2))
+    u[:, 0] = U * np.sin(k * x) * np.cos(k * y)
+    u[:, 1] = -U * np.cos(k * x) * np.sin(k * y)
+    u += np.asarray(drift, float)[None, :]
+    press = refd * U ** 2 * (np.cos(2.0 * k * x) + np.cos(2.0 * k * y)) / 4.0
+    rho = refd + press / cs ** 2
+    m = np.full(N, refd * dr ** 2)
+    visc = max(alpha / 8.0 * refd * h * cs, refd * U * L / Re)
+    params = dict(h=h, cs=cs, courant=courant, g=[0.0, 0.0], refd=[refd],
+                  visc_dyn=[visc], delta=[delta], p0=3.0 * refd * U ** 2,
+                  dt_min=0.0, dt_Ma=0.1, domain_min=[-L, -L],
+                  domain_max=[L, L], dr=dr, hfac=hfac, L=L,
+                  E_kin=float(np.sum(0.5 * m * (u[:, 0] ** 2 + u[:, 1] ** 2))))
+    return dict(dims=2, N=N, n_fluid=N, r=r, normal=np.zeros((N, 2)),
+                tangent=np.zeros((N, 2)), u=u, dudt=np.zeros((N, 2)),
+                rho=rho, drhodt=np.zeros(N), m=m,
+                imove=np.ones(N, np.int32), iset=np.zeros(N, np.uint32),
+                params=params, bc="portal")
+
+
+def taylor_green_portals(L):
+    """The eight portals that make [-L/2, L/2]^2 periodic, in the order of
+    examples/2D/taylor_green/src/templates/BCs.xml:21-54: (prefix, out_r,
+    in_r, n).  The diagonal pairs carry the corner neighbourhoods."""
+    a = 0.5 * L
+    s = 1.0 / math.sqrt(2.0)
+    return [("right_", (a, 0.0), (-a, 0.0), (1.0, 0.0)),
+            ("left_", (-a, 0.0), (a, 0.0), (-1.0, 0.0)),
+            ("top_", (0.0, a), (0.0, -a), (0.0, 1.0)),
+            ("bottom_", (0.0, -a), (0.0, a), (0.0, -1.0)),
+            ("topright_", (a, a), (-a, -a), (s, s)),
+            ("topleft_", (-a, a), (a, -a), (-s, s)),
+            ("bottomright_", (a, -a), (-a, a), (s, -s)),
+            ("bottomleft_", (-a, -a), (a, a), (-s, -s))]
+
+
+def periodic_slab(dims=3, n=(16, 8, 8), dr=0.05, U=1.0, hfac=2.0, cs=20.0,
+                  courant=0.1, refd=1000.0, mu=1e-3, delta=0.1, wall=False):
+    """A fluid block [-Lx/2, Lx/2) x [0, Ly) (x [0, Lz)) periodic along x
+    through one pair of plane portals, free surfaces elsewhere, with the
+    sheared flow u = U sin(2 pi z / Lz) along x (z the last axis; n gives
+    (nx, nz) in 2D).  wall=True adds a row of boundary-integral elements
+    (imove -3, area in m, normal -z) under the block and gravity along -z:
+    a periodic channel."""
+    nx = n[0]
+    Lx = nx * dr
+    axes = [-0.5 * Lx + (np.arange(nx) + 0.5) * dr]
+    axes += [(np.arange(n[k]) + 0.5) * dr for k in range(1, dims)]
+    Lz = n[dims - 1] * dr
+    fl = _lattice(*axes)
+    nf = fl.shape[0]
+    walls = np.zeros((0, dims))
+    if wall:
+        walls = _lattice(*(axes[:-1] + [[0.0]]))
+    N = nf + walls.shape[0]
+    vs = 2 if dims == 2 else 4
+    h = hfac * dr
+    r = np.zeros((N, vs))
+    r[:nf, :dims] = fl
+    r[nf:, :dims] = walls
+    normal = np.zeros((N, vs))
+    normal[nf:, dims - 1] = -1.0
+    u = np.zeros((N, vs))
+    u[:nf, 0] = U * np.sin(2.0 * math.pi * fl[:, dims - 1] / Lz)
+    imove = np.ones(N, np.int32)
+    imove[nf:] = -3
+    m = np.full(N, refd * dr ** dims)
+    m[nf:] = dr ** (dims - 1)
+    g = [0.0] * dims
+    if wall:
+        g[dims - 1] = -G
+    hi = [Lx] + [2.0 * n[k] * dr for k in range(1, dims)]
+    params = dict(h=h, cs=cs, courant=courant, g=g, refd=[refd],
+                  visc_dyn=[mu], delta=[delta], p0=0.0,
+                  dt_min=0.0, dt_Ma=0.1, domain_min=[-Lx] * dims,
+                  domain_max=hi, dr=dr, hfac=hfac, L=Lx)
+    return dict(dims=dims, N=N, n_fluid=nf, r=r, normal=normal,
+                tangent=np.zeros((N, vs)), u=u, dudt=np.zeros((N, vs)),
+                rho=np.full(N, refd), drhodt=np.zeros(N), m=m, imove=imove,
+                iset=np.zeros(N, np.uint32), params=params, bc="portal")
+
+
+def slab_portals(L, dims=3):
+    """Portals of periodic_slab: x = +L/2 -> -L/2 and back."""
+    a = 0.5 * L
+    z = (0.0,) * (dims - 1)
+    return [("right_", (a,) + z, (-a,) + z, (1.0,) + z),
+            ("left_", (-a,) + z, (a,) + z, (-1.0,) + z)]
+
+
 def dam_break_dr_for(n_fluid, dims=3, B=0.6, H=0.3, W=0.6):
     """Lattice spacing that gives about n_fluid particles in the column."""
     vol = B * H * (W if dims == 3 else 1.0)

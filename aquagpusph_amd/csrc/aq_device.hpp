@@ -23,6 +23,16 @@ struct f3 {
     float x, y, z;
 };
 
+// float -> uint with saturation (NaN and negatives -> 0, >= 2^32 ->
+// UINT_MAX); the OpenCL (uint) cast is implementation-defined outside
+// [0, 2^32), which the cell hash of a point below r_min reaches
+__device__ __forceinline__ uint32_t cvt_u32(float x)
+{
+    if (!(x > 0.f)) return 0u;
+    if (x >= 4294967296.f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
 template <int D> __device__ __forceinline__ f3 ldv(const float* a, uint32_t i)
 {
     f3 v;
@@ -248,6 +258,35 @@ __device__ __forceinline__ void walk(uint32_t c_i, uint32_t N, uint32_t nx,
             }
         }
     }
+}
+
+// The walk around a cell given explicitly (a moved point, e.g. a mirrored
+// particle): neighbour cells outside the grid (c_j >= n_cells.w, a point
+// beyond the particle bounding box) are skipped, so every ihoc read stays
+// in range whatever the point.
+template <int D, class Body>
+__device__ __forceinline__ void walk_in(uint32_t c_i, uint32_t N,
+                                        const uint32_t* __restrict__ nc,
+                                        const uint32_t* __restrict__ icell,
+                                        const uint32_t* __restrict__ ihoc,
+                                        Body body)
+{
+    const uint32_t nx = nc[0], ny = nc[1], nw = nc[3];
+#pragma unroll 1
+    for (int ci = -1; ci <= 1; ci++)
+#pragma unroll 1
+        for (int cj = -1; cj <= 1; cj++)
+#pragma unroll 1
+            for (int ck = (D == 3 ? -1 : 0); ck <= (D == 3 ? 1 : 0); ck++) {
+                const uint32_t c_j = c_i + (uint32_t)ci + (uint32_t)cj * nx +
+                                     (uint32_t)ck * nx * ny;
+                if (c_j >= nw) continue;
+                uint32_t j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    body(j);
+                    j++;
+                }
+            }
 }
 
 }  // namespace aq

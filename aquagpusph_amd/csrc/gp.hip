@@ -28,13 +28,6 @@ __device__ __forceinline__ bool skip(const Cells& c)
     return c.status && (*c.status & 1u);
 }
 
-__device__ __forceinline__ uint32_t cvt_u32(float x)
-{
-    if (!(x > 0.f)) return 0u;
-    if (x >= 4294967296.f) return 0xFFFFFFFFu;
-    return (uint32_t)x;
-}
-
 // GP/Sort.cl:35-53: associations into the new sorted order
 __global__ void gp_sort_k(const uint32_t* __restrict__ assoc_in,
                           uint32_t* __restrict__ assoc,

@@ -21,15 +21,6 @@ unsigned radix_key_bits(uint64_t max_val);
 constexpr int MM_BS = 256;
 constexpr int MM_MAXB = 1024;
 
-// float -> uint with saturation (NaN and negatives -> 0, >= 2^32 ->
-// UINT_MAX); the OpenCL cast is implementation-defined outside [0, 2^32)
-__device__ __forceinline__ uint32_t cvt_u32(float x)
-{
-    if (!(x > 0.f)) return 0u;
-    if (x >= 4294967296.f) return 0xFFFFFFFFu;
-    return (uint32_t)x;
-}
-
 template <int D>
 __global__ void __launch_bounds__(MM_BS)
     minmax_partial(const float* __restrict__ r, uint32_t N,

@@ -34,6 +34,23 @@ AQ_API int aq_sim_create(const char* xml_path, int dims, aq_sim** out)
     });
 }
 
+AQ_API int aq_problem_tools(const char* xml_path, int dims, char* buf,
+                            size_t size, size_t* needed)
+{
+    return guard([&] {
+        AQ_REQUIRE(xml_path && needed, "aq_problem_tools: null argument");
+        AQ_REQUIRE(dims == 2 || dims == 3, "dims must be 2 or 3");
+        Problem P = load_problem(xml_path, dims);
+        std::string out;
+        for (auto& t : P.tools)
+            out += t->get("name") + "\t" + t->get("type") + "\n";
+        *needed = out.size() + 1;
+        if (!buf) return;
+        AQ_REQUIRE(size >= out.size() + 1, "aq_problem_tools: buffer too small");
+        memcpy(buf, out.c_str(), out.size() + 1);
+    }, false);
+}
+
 AQ_API void aq_sim_destroy(aq_sim* s)
 {
     if (!s) return;

@@ -37,6 +37,43 @@ void launch_timestep(int, const int32_t*, const float*, float*, uint32_t,
                      const float*, hipStream_t);
 void launch_sensors_renorm(int, const int32_t*, const float*, float*, float*,
                            float*, uint32_t, hipStream_t);
+void launch_portal_mirror(int, float*, int32_t*, uint32_t*, uint32_t,
+                          const float*, const float*, const float*,
+                          const float*, const uint32_t*, aq_defs, hipStream_t);
+void launch_portal_unmirror(int, float*, const int32_t*, uint32_t,
+                            const float*, const float*, hipStream_t);
+void launch_portal_teleport(int, float*, uint32_t, const float*, const float*,
+                            const float*, hipStream_t);
+void launch_portal_shepard(int, const int32_t*, const int32_t*, const float*,
+                           const float*, const float*, float*, uint32_t,
+                           const uint32_t*, const uint32_t*, const uint32_t*,
+                           const uint32_t*, aq_defs, float, hipStream_t);
+void launch_portal_interactions(int, const int32_t*, const int32_t*,
+                                const float*, const float*, const float*,
+                                const float*, const float*, float*, float*,
+                                float*, uint32_t, const uint32_t*,
+                                const uint32_t*, const uint32_t*,
+                                const uint32_t*, aq_defs, float, hipStream_t);
+void launch_portal_bi(int, const int32_t*, const int32_t*, const float*,
+                      const float*, const float*, const float*, const float*,
+                      const float*, float*, float*, uint32_t, const uint32_t*,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      aq_defs, float, hipStream_t);
+void launch_portal_lapp(int, const int32_t*, const int32_t*, const float*,
+                        const float*, const float*, const float*, float*,
+                        float*, uint32_t, const uint32_t*, const uint32_t*,
+                        const uint32_t*, const uint32_t*, aq_defs, float,
+                        hipStream_t);
+void launch_portal_lapp_corr(int, const int32_t*, const int32_t*,
+                             const float*, const float*, const float*,
+                             const float*, float*, uint32_t, const uint32_t*,
+                             const uint32_t*, const uint32_t*,
+                             const uint32_t*, aq_defs, float, hipStream_t);
+void launch_portal_mls(int, const int32_t*, const int32_t*, const float*,
+                       const float*, const float*, float*, const uint32_t*,
+                       uint32_t, const uint32_t*, const uint32_t*,
+                       const uint32_t*, const uint32_t*, aq_defs, float,
+                       hipStream_t);
 void launch_mp_predictor(int, const float*, const float*, const float*,
                          const float*, const float*, float*, float*, float*,
                          float*, float*, uint32_t, hipStream_t);
@@ -384,6 +421,86 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_timestep(S.dims, I(0), F(1), F(2), S.N, F(3), F(4),
                                 F(5), F(6), F(7), s);
+            } } },
+        // ---- plane portals (cfd/portal.xml) ---------------------------------
+        { "cfd/Boundary/Portal/Mirror.cl:mirror",
+          { { "r", "imirrored", "icell", "portal_in_r", "portal_out_r",
+              "portal_n", "r_min", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_mirror(S.dims, F(0), I(1), U(2), S.N, F(3), F(4),
+                                     F(5), F(6), U(7), S.kdefs, s);
+            } } },
+        { "cfd/Boundary/Portal/Mirror.cl:unmirror",
+          { { "r", "imirrored", "portal_in_r", "portal_out_r" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_unmirror(S.dims, F(0), I(1), S.N, F(2), F(3), s);
+            } } },
+        { "cfd/Boundary/Portal/Mirror.cl:teleport",
+          { { "r", "portal_in_r", "portal_out_r", "portal_n" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_teleport(S.dims, F(0), S.N, F(1), F(2), F(3), s);
+            } } },
+        { "cfd/Boundary/Portal/Shepard.cl:entry",
+          { { "imove", "imirrored", "r", "rho", "m", "shepard", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_shepard(S.dims, I(0), I(1), F(2), F(3), F(4),
+                                      F(5), S.N, U(6), U(7), U(8), S.d_status,
+                                      S.kdefs, support_threshold(S.kdefs.H),
+                                      s);
+            } } },
+        { "cfd/Boundary/Portal/Interactions.cl:entry",
+          { { "imove", "imirrored", "r", "u", "rho", "m", "p", "grad_p",
+              "lap_u", "div_u", "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_interactions(
+                    S.dims, I(0), I(1), F(2), F(3), F(4), F(5), F(6), F(7),
+                    F(8), F(9), S.N, U(10), U(11), U(12), S.d_status, S.kdefs,
+                    support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/Boundary/Portal/BoundaryIntegrals.cl:entry",
+          { { "imove", "imirrored", "r", "normal", "u", "rho", "m", "p",
+              "grad_p", "div_u", "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_bi(S.dims, I(0), I(1), F(2), F(3), F(4), F(5),
+                                 F(6), F(7), F(8), F(9), S.N, U(10), U(11),
+                                 U(12), S.d_status, S.kdefs,
+                                 support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/Boundary/Portal/deltaSPH.cl:lapp",
+          { { "imove", "imirrored", "r", "rho", "m", "p", "lap_p", "icell",
+              "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_lapp(S.dims, I(0), I(1), F(2), F(3), F(4), F(5),
+                                   F(6), nullptr, S.N, U(7), U(8), U(9),
+                                   S.d_status, S.kdefs,
+                                   support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/Boundary/Portal/deltaSPH.cl:full",
+          { { "imove", "imirrored", "r", "rho", "m", "p", "lap_p_corr",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_lapp(S.dims, I(0), I(1), F(2), F(3), F(4), F(5),
+                                   nullptr, F(6), S.N, U(7), U(8), U(9),
+                                   S.d_status, S.kdefs,
+                                   support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/Boundary/Portal/deltaSPH.cl:lapp_corr",
+          { { "imove", "imirrored", "r", "rho", "m", "lap_p_corr", "lap_p",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_lapp_corr(S.dims, I(0), I(1), F(2), F(3), F(4),
+                                        F(5), F(6), S.N, U(7), U(8), U(9),
+                                        S.d_status, S.kdefs,
+                                        support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/Boundary/Portal/MLS.cl:entry",
+          { { "imove", "imirrored", "r", "rho", "m", "mls", "mls_imove",
+              "icell", "ihoc", "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_portal_mls(S.dims, I(0), I(1), F(2), F(3), F(4), F(5),
+                                  U(6), S.N, U(7), U(8), U(9), S.d_status,
+                                  S.kdefs, support_threshold(S.kdefs.H), s);
             } } },
         // ---- semi-implicit midpoint scheme ----------------------------------
         { "basic/time_scheme/midpoint.cl:predictor",

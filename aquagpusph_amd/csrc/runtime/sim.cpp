@@ -87,12 +87,18 @@ float Sim::def_float(const std::string& n) const
     return strtof(v.c_str(), nullptr);
 }
 
-void Sim::load(const std::string& xml_path)
+Problem load_problem(const std::string& xml_path, int dims)
 {
     Problem P;
     P.dims = dims;
     P.base_path = package_dir();
     parse_problem(xml_path, P, "");
+    return P;
+}
+
+void Sim::load(const std::string& xml_path)
+{
+    Problem P = load_problem(xml_path, dims);
     build(P);
 }
 

@@ -194,6 +194,9 @@ struct Problem {
 
 void parse_problem(const std::string& path, Problem& P,
                    const std::string& prefix);
+// The problem of a case file, with the package folder as root path (no GPU
+// work).
+Problem load_problem(const std::string& xml_path, int dims);
 
 // ---- kernel registry ----------------------------------------------------------
 struct KernelBinding;

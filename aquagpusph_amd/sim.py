@@ -26,6 +26,23 @@ def _base(type_name):
     return "f"
 
 
+def problem_tools(xml_path, dims=3):
+    """The (name, type) tool list a case file resolves to, in execution
+    order, parsed on the host (aq_problem_tools: no GPU needed)."""
+    L = _lib.lib()
+    L.aq_problem_tools.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                   ctypes.c_char_p, ctypes.c_size_t,
+                                   ctypes.POINTER(ctypes.c_size_t)]
+    path = os.fsencode(os.path.abspath(xml_path))
+    need = ctypes.c_size_t(0)
+    _lib.check(L.aq_problem_tools(path, dims, None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    _lib.check(L.aq_problem_tools(path, dims, buf, need.value,
+                                  ctypes.byref(need)))
+    return [tuple(line.split("\t")) for line in
+            buf.value.decode().splitlines()]
+
+
 class Simulation:
     def __init__(self, xml_path, dims=3):
         L = _lib.lib()

@@ -102,6 +102,74 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         binormal<D, R>(normal, tangent, binormal_, N);                         \
     }                                                                          \
+    ORC_EXPORT void orc_portal_mirror_##SR##_##D##d(                           \
+        R* r, int* imirrored, usize* icell, usize N, const R* in_r,            \
+        const R* out_r, const R* n, const R* r_min, const usize* nc, R H)      \
+    {                                                                          \
+        portal_mirror<D, R>(r, imirrored, icell, N, in_r, out_r, n, r_min, nc, \
+                            H);                                                \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_unmirror_##SR##_##D##d(                         \
+        R* r, const int* imirrored, usize N, const R* in_r, const R* out_r)    \
+    {                                                                          \
+        portal_unmirror<D, R>(r, imirrored, N, in_r, out_r);                   \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_teleport_##SR##_##D##d(                         \
+        R* r, usize N, const R* in_r, const R* out_r, const R* n)              \
+    {                                                                          \
+        portal_teleport<D, R>(r, N, in_r, out_r, n);                           \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_shepard_##SR##_##D##d(                          \
+        const int* imove, const int* imirrored, const R* r, const R* rho,      \
+        const R* m, R* shep, usize N, const usize* ic, const usize* hoc,       \
+        const usize* nc, R H, R CONW)                                          \
+    {                                                                          \
+        portal_shepard<D, R>(imove, imirrored, r, rho, m, shep, N,             \
+                             LL{ ic, hoc, nc }, H, CONW);                      \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_interactions_##SR##_##D##d(                     \
+        const int* imove, const int* imirrored, const R* r, const R* u,        \
+        const R* rho, const R* m, const R* p, R* grad_p, R* lap_u, R* div_u,   \
+        usize N, const usize* ic, const usize* hoc, const usize* nc, R H,      \
+        R CONF, int lap)                                                       \
+    {                                                                          \
+        portal_interactions<D, R>(imove, imirrored, r, u, rho, m, p, grad_p,   \
+                                  lap_u, div_u, N, LL{ ic, hoc, nc }, H, CONF, \
+                                  lap);                                        \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_bi_##SR##_##D##d(                               \
+        const int* imove, const int* imirrored, const R* r, const R* normal,   \
+        const R* u, const R* rho, const R* m, const R* p, R* grad_p,           \
+        R* div_u, usize N, const usize* ic, const usize* hoc, const usize* nc, \
+        R H, R CONW)                                                           \
+    {                                                                          \
+        portal_bi<D, R>(imove, imirrored, r, normal, u, rho, m, p, grad_p,     \
+                        div_u, N, LL{ ic, hoc, nc }, H, CONW);                 \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_lapp_##SR##_##D##d(                             \
+        const int* imove, const int* imirrored, const R* r, const R* rho,      \
+        const R* m, const R* p, R* lap_p, R* lap_p_corr, usize N,              \
+        const usize* ic, const usize* hoc, const usize* nc, R H, R CONF)       \
+    {                                                                          \
+        portal_lapp<D, R>(imove, imirrored, r, rho, m, p, lap_p, lap_p_corr,   \
+                          N, LL{ ic, hoc, nc }, H, CONF);                      \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_lapp_corr_##SR##_##D##d(                        \
+        const int* imove, const int* imirrored, const R* r, const R* rho,      \
+        const R* m, const R* lap_p_corr, R* lap_p, usize N, const usize* ic,   \
+        const usize* hoc, const usize* nc, R H, R CONF)                        \
+    {                                                                          \
+        portal_lapp_corr<D, R>(imove, imirrored, r, rho, m, lap_p_corr,        \
+                               lap_p, N, LL{ ic, hoc, nc }, H, CONF);          \
+    }                                                                          \
+    ORC_EXPORT void orc_portal_mls_##SR##_##D##d(                              \
+        const int* imove, const int* imirrored, const R* r, const R* rho,      \
+        const R* m, R* mls, usize N, unsigned mls_imove, const usize* ic,      \
+        const usize* hoc, const usize* nc, R H, R CONF)                        \
+    {                                                                          \
+        portal_mls<D, R>(imove, imirrored, r, rho, m, mls, N, mls_imove,       \
+                         LL{ ic, hoc, nc }, H, CONF);                          \
+    }                                                                          \
     ORC_EXPORT void orc_midpoint_predictor_##SR##_##D##d(                      \
         const R* r, const R* u, const R* dudt, const R* rho, const R* drhodt,  \
         R* r_in, R* u_in, R* dudt_in, R* rho_in, R* drhodt_in, usize N)        \

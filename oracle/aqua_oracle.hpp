@@ -1532,6 +1532,232 @@ void neighs(const int* imove, usize* n_neighs, usize limit, usize N, LL ll)
 }
 
 // ---------------------------------------------------------------------------
+// Plane portals (cfd/portal.xml, cfd/Boundary/Portal/*.cl).  Particles within
+// SUPPORT h of the out plane are moved by in_r - out_r (and re-hashed into
+// the cell of their new place); those "mirrored" particles then add the
+// terms of the unmirrored particles around the in plane, walking the cells
+// of their new place; a cell run ends at a mirrored particle whose icell
+// changed (as in the reference) and cells outside the grid are skipped.
+// ---------------------------------------------------------------------------
+// Portal/Mirror.cl:35-52 (cell)
+template <int D, typename R>
+usize portal_cell(const R* r, const R* r_min, const usize* nc, R H)
+{
+    const R idist = R(1) / (R(2) * H);
+    const usize cx = cvt_u32((r[0] - r_min[0]) * idist) + 3u;
+    const usize cy = cvt_u32((r[1] - r_min[1]) * idist) + 3u;
+    if (D == 3) {
+        const usize cz = cvt_u32((r[2] - r_min[2]) * idist) + 3u;
+        return cx - 1u + (cy - 1u) * nc[0] + (cz - 1u) * nc[0] * nc[1];
+    }
+    return cx - 1u + (cy - 1u) * nc[0];
+}
+
+// Portal/Mirror.cl:71-95 (mirror), on every stored component
+template <int D, typename R>
+void portal_mirror(R* r, int* imirrored, usize* icell, usize N,
+                   const R* in_r, const R* out_r, const R* n, const R* r_min,
+                   const usize* nc, R H)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        R rij[4] = { 0, 0, 0, 0 };
+        R dist = R(0);
+        for (int k = 0; k < VS; k++) rij[k] = r[(size_t)i * VS + k] - out_r[k];
+        dist = rij[0] * n[0];
+        for (int k = 1; k < VS; k++) dist = dist + rij[k] * n[k];
+        if (std::fabs(dist) > R(2) * H) {
+            imirrored[i] = 0;
+            continue;
+        }
+        imirrored[i] = 1;
+        for (int k = 0; k < VS; k++) r[(size_t)i * VS + k] = in_r[k] + rij[k];
+        icell[i] = portal_cell<D, R>(r + (size_t)i * VS, r_min, nc, H);
+    }
+}
+
+// Portal/Mirror.cl:108-124 (unmirror)
+template <int D, typename R>
+void portal_unmirror(R* r, const int* imirrored, usize N, const R* in_r,
+                     const R* out_r)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (!imirrored[i]) continue;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = (size_t)i * VS + k;
+            r[a] = out_r[k] + (r[a] - in_r[k]);
+        }
+    }
+}
+
+// Portal/Mirror.cl:136-153 (teleport): past the out plane -> in plane
+template <int D, typename R>
+void portal_teleport(R* r, usize N, const R* in_r, const R* out_r,
+                     const R* n)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        R rij[4] = { 0, 0, 0, 0 };
+        for (int k = 0; k < VS; k++) rij[k] = r[(size_t)i * VS + k] - out_r[k];
+        R dist = rij[0] * n[0];
+        for (int k = 1; k < VS; k++) dist = dist + rij[k] * n[k];
+        if (dist < R(0)) continue;
+        for (int k = 0; k < VS; k++) r[(size_t)i * VS + k] = in_r[k] + rij[k];
+    }
+}
+
+// Portal/Shepard.cl:58-100
+template <int D, typename R>
+void portal_shepard(const int* imove, const int* imirrored, const R* r,
+                    const R* rho, const R* m, R* shep, usize N, LL ll, R H,
+                    R CONW)
+{
+    for (usize i = 0; i < N; i++) {
+        if ((imove[i] < -3) || (imove[i] > 1) || !imirrored[i]) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R acc = shep[i];
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((imove[j] != 1) || imirrored[j]) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            acc += kernelW<D, R>(q) * CONW * m[j] / rho[j];
+        });
+        shep[i] = acc;
+    }
+}
+
+// Portal/Interactions.cl:67-140 (fluid and ghost candidates)
+template <int D, typename R>
+void portal_interactions(const int* imove, const int* imirrored, const R* r,
+                         const R* u, const R* rho, const R* m, const R* p,
+                         R* grad_p, R* lap_u, R* div_u, usize N, LL ll, R H,
+                         R CONF, int lap)
+{
+    const R CLEARY = (D == 2) ? R(8) : R(10);
+    for (usize i = 0; i < N; i++) {
+        if (!imirrored[i] || (imove[i] != 1)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
+        const R p_i = p[i], rho_i = rho[i];
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imirrored[j] || ((imove[j] != 1) && (imove[j] != -1)))
+                return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j], p_j = p[j];
+            const V<D, R> du = sub<D, R>(ld<D, R>(u, j), u_i);
+            const R udr = dot<D, R>(du, r_ij);
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j];
+            addsv<D, R>(grad_p, i, (p_i + p_j) / (rho_i * rho_j) * f_ij, r_ij);
+            if (lap == 1) {
+                const R r2 = (q * q + R(0.01f)) * H * H;
+                addsv<D, R>(lap_u, i,
+                            f_ij * CLEARY * udr / (r2 * rho_i * rho_j), r_ij);
+            } else {
+                addsv<D, R>(lap_u, i, f_ij * R(2) / (rho_i * rho_j), du);
+            }
+            div_u[i] += udr * f_ij * rho_i / rho_j;
+        });
+    }
+}
+
+// Portal/BoundaryIntegrals.cl:55-128 (the normal is the `normal` argument)
+template <int D, typename R>
+void portal_bi(const int* imove, const int* imirrored, const R* r,
+               const R* normal, const R* u, const R* rho, const R* m,
+               const R* p, R* grad_p, R* div_u, usize N, LL ll, R H, R CONW)
+{
+    for (usize i = 0; i < N; i++) {
+        if (!imirrored[i] || (imove[i] != 1)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
+        const R p_i = p[i], rho_i = rho[i];
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imirrored[j] || (imove[j] != -3)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const V<D, R> du = sub<D, R>(ld<D, R>(u, j), u_i);
+            const R w_ij = kernelW<D, R>(q) * CONW * m[j];
+            addsv<D, R>(grad_p, i, (p_i + p[j]) / rho_i * w_ij, n_j);
+            div_u[i] += rho_i * dot<D, R>(du, n_j) * w_ij;
+        });
+    }
+}
+
+// Portal/deltaSPH.cl:121-191 (lapp) and 50-119 (full)
+template <int D, typename R>
+void portal_lapp(const int* imove, const int* imirrored, const R* r,
+                 const R* rho, const R* m, const R* p, R* lap_p, R* lap_p_corr,
+                 usize N, LL ll, R H, R CONF)
+{
+    for (usize i = 0; i < N; i++) {
+        if (!imirrored[i] || (imove[i] != 1)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const R p_i = p[i];
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imirrored[j] || (imove[j] != 1)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            if (lap_p) lap_p[i] += (p[j] - p_i) * f_ij;
+            if (lap_p_corr) addsv<D, R>(lap_p_corr, i, (p[j] - p_i) * f_ij, r_ij);
+        });
+    }
+}
+
+// Portal/deltaSPH.cl:193-246 (lapp_corr)
+template <int D, typename R>
+void portal_lapp_corr(const int* imove, const int* imirrored, const R* r,
+                      const R* rho, const R* m, const R* lap_p_corr, R* lap_p,
+                      usize N, LL ll, R H, R CONF)
+{
+    for (usize i = 0; i < N; i++) {
+        if (!imirrored[i] || (imove[i] != 1)) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        const V<D, R> g_i = ld<D, R>(lap_p_corr, i);
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (imirrored[j] || (imove[j] != 1)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            V<D, R> g;
+            const V<D, R> g_j = ld<D, R>(lap_p_corr, j);
+            for (int k = 0; k < 3; k++) g.c[k] = g_j.c[k] + g_i.c[k];
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            lap_p[i] -= R(0.5f) * dot<D, R>(g, r_ij) * f_ij;
+        });
+    }
+}
+
+// Portal/MLS.cl:60-110 (entry): mls += outer(r_ij, f_ij r_ij)
+template <int D, typename R>
+void portal_mls(const int* imove, const int* imirrored, const R* r,
+                const R* rho, const R* m, R* mls, usize N, unsigned mls_imove,
+                LL ll, R H, R CONF)
+{
+    const int n = (D == 2) ? 2 : 4, S = (D == 2) ? 4 : 16;
+    for (usize i = 0; i < N; i++) {
+        if ((imove[i] != 1) || !imirrored[i]) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        R* M = mls + (size_t)i * S;
+        walk_cell<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((imove[j] != (int)mls_imove) || imirrored[j]) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+            for (int a = 0; a < D; a++)
+                for (int b = 0; b < D; b++)
+                    M[a * n + b] += r_ij.c[a] * (f_ij * r_ij.c[b]);
+        });
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Semi-implicit midpoint scheme (basic/time_scheme/midpoint.cl).  Vectors
 // are updated on every stored component, as the float4 arithmetic of the
 // reference.

@@ -63,7 +63,7 @@ class OracleSim:
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
-                 midpoint=None):
+                 midpoint=None, portals=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -79,6 +79,8 @@ class OracleSim:
                          min_bound_dist=min_bound_dist,
                          delta_corr=delta_corr, energy=energy,
                          midpoint=midpoint)
+        if portals and midpoint is not None:
+            raise ValueError("portals are modelled with improved Euler only")
         if midpoint is not None:
             # basic/time_scheme/midpoint.xml (+ midpoint/{r,autostop,
             # autorelax}.xml): iter_max, relax, r, Residual_max, mindec, inc
@@ -147,6 +149,22 @@ class OracleSim:
             f["gp_p"] = np.zeros(N, R)
             for k in ("gp_u", "gp_r_in", "gp_u_in"):
                 f[k] = np.zeros((N, self.vs), R)
+        if portals:
+            # cfd/portal.xml once per (prefix, out_r, in_r, n), in include
+            # order; the set_scalar tools before each mirror/teleport tool
+            # give every stage its own portal's planes
+            f["imirrored"] = np.zeros(N, np.int32)
+            f["icell_backup"] = np.zeros(N, np.uint32)
+            self.portals = []
+            for _, out_r, in_r, n in portals:
+                pv = {}
+                for key, v in (("out_r", out_r), ("in_r", in_r), ("n", n)):
+                    a = np.zeros(self.vs, R)
+                    a[:len(v)] = np.asarray(v, np.float32)[:self.vs]
+                    pv[key] = a
+                self.portals.append(pv)
+        else:
+            self.portals = []
         self.f = f
         self.ihoc = np.zeros(0, np.uint32)
         self.n_cells = np.zeros(4, np.uint32)
@@ -206,6 +224,69 @@ class OracleSim:
         f = self.f
         return (_p(f["icell"]), _p(self.ihoc), _p(self.n_cells))
 
+    def _portal_stage(self, stage, ll):
+        """The tools one cfd/portal.xml include puts at one place, for each
+        portal in turn: icell backup, mirror, the stage's kernels, unmirror,
+        icell restore (cfd/portal.xml stages 1-3).  stage: "mls" (before
+        imove1_MLS), "forces" (before Interactions) or "lapp_corr" (before
+        LapP Correction)."""
+        f, o = self.f, self.opts
+        k, R = self._k, self._r
+        n = ctypes.c_uint(self.N)
+        H, CONW, CONF = R(self.H), R(self.CONW), R(self.CONF)
+        for pv in self.portals:
+            f["icell_backup"][...] = f["icell"]
+            k("portal_mirror")(_p(f["r"]), _p(f["imirrored"]), _p(f["icell"]),
+                               n, _p(pv["in_r"]), _p(pv["out_r"]),
+                               _p(pv["n"]), _p(self.r_min), _p(self.n_cells),
+                               H)
+            if stage == "mls":
+                k("portal_mls")(_p(f["imove"]), _p(f["imirrored"]),
+                                _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                                _p(f["mls"]), n, ctypes.c_uint(1), *ll, H,
+                                CONF)
+            elif stage == "forces":
+                k("portal_shepard")(_p(f["imove"]), _p(f["imirrored"]),
+                                    _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                                    _p(f["shepard"]), n, *ll, H, CONW)
+                k("portal_interactions")(
+                    _p(f["imove"]), _p(f["imirrored"]), _p(f["r"]),
+                    _p(f["u"]), _p(f["rho"]), _p(f["m"]), _p(f["p"]),
+                    _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]), n, *ll,
+                    H, CONF, ctypes.c_int(o["lap_form"]))
+                if o["bi"]:
+                    k("portal_bi")(_p(f["imove"]), _p(f["imirrored"]),
+                                   _p(f["r"]), _p(f["normal"]), _p(f["u"]),
+                                   _p(f["rho"]), _p(f["m"]), _p(f["p"]),
+                                   _p(f["grad_p"]), _p(f["div_u"]), n, *ll,
+                                   H, CONW)
+                if o["delta"]:
+                    k("portal_lapp")(_p(f["imove"]), _p(f["imirrored"]),
+                                     _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                                     _p(f["p"]), _p(f["lap_p"]), None, n,
+                                     *ll, H, CONF)
+                if o["delta_corr"] == "full":
+                    k("portal_lapp")(_p(f["imove"]), _p(f["imirrored"]),
+                                     _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                                     _p(f["p"]), None, _p(f["lap_p_corr"]),
+                                     n, *ll, H, CONF)
+            else:
+                k("portal_lapp_corr")(_p(f["imove"]), _p(f["imirrored"]),
+                                      _p(f["r"]), _p(f["rho"]), _p(f["m"]),
+                                      _p(f["lap_p_corr"]), _p(f["lap_p"]), n,
+                                      *ll, H, CONF)
+            k("portal_unmirror")(_p(f["r"]), _p(f["imirrored"]), n,
+                                 _p(pv["in_r"]), _p(pv["out_r"]))
+            f["icell"][...] = f["icell_backup"]
+
+    def _portal_teleport(self):
+        """"cfd portal mirror_teleport" of each portal (before Corrector,
+        i.e. after the corrector kernel)."""
+        for pv in self.portals:
+            self._k("portal_teleport")(_p(self.f["r"]), ctypes.c_uint(self.N),
+                                       _p(pv["in_r"]), _p(pv["out_r"]),
+                                       _p(pv["n"]))
+
     # ---- link-list (LinkList.cpp:326-494) ----------------------------
     def link_list(self, r_name="r_in"):
         f, L, N = self.f, lib(), self.N
@@ -262,6 +343,8 @@ class OracleSim:
             k("mls_entry")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
                            _p(f["m"]), _p(f["mls"]), n, ctypes.c_uint(1), *ll,
                            H, CONF)
+            if self.portals:
+                self._portal_stage("mls", ll)
             k("mls_inv")(_p(f["imove"]), _p(f["mls"]), n, ctypes.c_uint(1))
             f["mls_fluid"][...] = f["mls"]
         for key in ("lap_p", "lap_u", "div_u", "grad_p", "shepard"):
@@ -368,6 +451,8 @@ class OracleSim:
             k("bi_shepard_apply")(_p(f["imove"]), _p(f["shepard"]),
                                   _p(f["grad_p"]), _p(f["lap_u"]),
                                   _p(f["div_u"]), n)
+        if self.portals:
+            self._portal_stage("forces", ll)
         if o["delta_corr"] == "full":
             k("lapp_full_mls")(_p(f["imove"]), _p(f["mls"]),
                                _p(f["lap_p_corr"]), n)
@@ -375,6 +460,8 @@ class OracleSim:
             k("lapp_corr")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
                            _p(f["m"]), _p(f["lap_p_corr"]), _p(f["lap_p"]), n,
                            *ll, H, CONF)
+            if self.portals:
+                self._portal_stage("lapp_corr", ll)
         k("rates")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
                    _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]),
                    _p(f["dudt"]), _p(f["drhodt"]), _p(self.sets["visc_dyn"]),
@@ -556,6 +643,8 @@ class OracleSim:
             k("bie_pst")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
                          _p(f["m"]), _p(f["rho"]), n, *ll, R(self.DIMS),
                          R(o["dr_factor"]))
+        if self.portals:
+            self._portal_teleport()
         # "fixed time step" set_scalar (basic.xml:164), double -> float
         self._time_step()
 

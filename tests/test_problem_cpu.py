@@ -1,0 +1,93 @@
+"""The tool list that case files resolve to, parsed on the host
+(aq_problem_tools), against the include/insert rules of the reference's
+State.cpp: prefixed includes, insert before a glob (every match), try_insert
+that finds nothing, ifdef.  The order checked here is the one the oracle
+pipeline replays."""
+import pytest
+
+from aquagpusph_amd import casefiles, cases
+from aquagpusph_amd.sim import problem_tools
+
+BASE = ("resources/Presets/basic.xml",
+        "resources/Presets/basic/time_scheme/improved_euler.xml",
+        "resources/Presets/basic/domain.xml",
+        "resources/Presets/cfd.xml",
+        "resources/Presets/cfd/deltaSPH.xml")
+
+SETS = ["portal_out_r", "portal_in_r", "portal_n"]
+
+
+def _names(xml, dims):
+    return [n for n, _ in problem_tools(xml, dims)]
+
+
+def _block(names, first, count):
+    i = names.index(first)
+    return names[i:i + count]
+
+
+def test_taylor_green_portal_tool_order(tmp_path):
+    case = cases.taylor_green(n=16)
+    ports = cases.taylor_green_portals(case["params"]["L"])
+    xml = casefiles.write_case(case, str(tmp_path), presets=BASE,
+                               portals=ports)
+    names = _names(xml, 2)
+    # the forces stage of every portal, in include order, right before the
+    # Interactions dummy and after the ordinary delta-SPH Laplacian
+    stage = ["cfd portal icell backup"] + [None] * 3 + [
+        "cfd portal mirror", "cfd portal Shepard", "cfd portal interactions",
+        "cfd portal delta-SPH", "cfd portal unmirror",
+        "cfd portal icell restore"]
+    at = names.index("cfd lap p") + 1
+    for px, _, _, _ in ports:
+        want = [px + s if s else None for s in stage]
+        want[1:4] = [px + s for s in SETS]
+        assert names[at:at + len(want)] == want, (px, names[at:at + 12])
+        at += len(want)
+    assert names[at] == "Interactions"
+    # no MLS / LapP Correction in this stack: the try_inserts add nothing,
+    # no BI: no portal BI, no delta-SPH-full: no portal full
+    assert not any("portal_mls" in n or "portal_deltaSPH" in n or
+                   "portal BI" in n or "delta-SPH full" in n for n in names)
+    # teleports after the corrector kernel, each after its planes
+    at = names.index("corrector") + 1
+    for px, _, _, _ in ports:
+        want = [px + s for s in SETS] + [px + "cfd portal mirror_teleport"]
+        assert names[at:at + 4] == want
+        at += 4
+    assert names[at] == "Corrector"
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_slab_portal_stages_with_mls_and_bi(tmp_path, dims):
+    n = (16, 8) if dims == 2 else (12, 6, 6)
+    case = cases.periodic_slab(dims, n, wall=True)
+    ports = cases.slab_portals(case["params"]["L"], dims)
+    pres = BASE + ("resources/Presets/cfd/deltaSPH-full.xml",
+                   "resources/Presets/cfd/BI.xml")
+    xml = casefiles.write_case(case, str(tmp_path), presets=pres,
+                               portals=ports)
+    names = _names(xml, dims)
+    # MLS stage between the MLS sums and their inversion
+    a, b = names.index("imove1_MLS interactions"), names.index("imove1_MLS")
+    mls = [px + s for px, _, _, _ in ports for s in
+           ["cfd portal_mls icell backup"] + SETS +
+           ["cfd portal_mls mirror", "cfd portal_mls MLS",
+            "cfd portal_mls unmirror", "cfd portal_mls icell restore"]]
+    assert names[a + 1:b] == mls
+    # forces stage after the BI tools, with portal BI and the full term
+    i = names.index("cfd BI Shepard") + 1
+    assert names[i:i + 12] == ["right_cfd portal icell backup"] + [
+        "right_" + s for s in SETS] + [
+        "right_cfd portal " + s for s in
+        ("mirror", "Shepard", "interactions", "BI", "delta-SPH",
+         "delta-SPH full", "unmirror", "icell restore")]
+    # lap_p correction stage after the ordinary correction
+    i = names.index("cfd lap p apply correction") + 1
+    j = names.index("LapP Correction")
+    assert names[i:j] == [px + s for px, _, _, _ in ports for s in
+                          ["cfd portal_deltaSPH icell backup"] + SETS +
+                          ["cfd portal_deltaSPH mirror",
+                           "cfd portal_deltaSPH lapp_corr",
+                           "cfd portal_deltaSPH unmirror",
+                           "cfd portal_deltaSPH icell restore"]]

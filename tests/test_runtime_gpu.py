@@ -26,9 +26,9 @@ FIELDS = ("icell", "id_sorted", "id_unsorted", "id", "imove", "r", "u", "rho",
           "lap_p", "dt_var")
 
 
-def _compare(sim, o, tol):
+def _compare(sim, o, tol, fields=FIELDS):
     errs = {}
-    for k in FIELDS:
+    for k in fields:
         a = sim.get(k)
         b = o.f[k]
         if a.dtype.kind in "iu":
@@ -276,6 +276,82 @@ def test_delta_sph_corrections_match_oracle(cuda, tmp_path, dims, corr,
         else:
             errs[k] = relerr(a, b)
     print(dims, corr, steps, errs)
+    assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
+    sim.close()
+
+
+PORTAL_BASE = ("resources/Presets/basic.xml",
+               "resources/Presets/basic/time_scheme/improved_euler.xml",
+               "resources/Presets/basic/domain.xml",
+               "resources/Presets/cfd.xml",
+               "resources/Presets/cfd/deltaSPH.xml")
+MORRIS = ('    <Definitions><Define name="__LAP_FORMULATION__" '
+          'value="__LAP_MORRIS__" evaluate="false"/></Definitions>')
+
+
+@pytest.mark.parametrize("steps", [2, 20])
+def test_taylor_green_portals_match_oracle(cuda, tmp_path, steps):
+    """The periodic Taylor-Green vortices of the reference's
+    examples/2D/taylor_green: eight cfd/portal.xml includes (sides and
+    corners) with their set_scalar plane tools, Morris Laplacian,
+    delta-SPH.  A uniform drift carries particles through the portals
+    within the 20 steps (teleports)."""
+    case = cases.taylor_green(n=32, hfac=4.0, drift=(5.0, 4.0))
+    ports = cases.taylor_green_portals(case["params"]["L"])
+    xml = casefiles.write_case(case, str(tmp_path), presets=PORTAL_BASE,
+                               extra_xml=MORRIS, portals=ports)
+    sim = Simulation(xml, 2)
+    o = op.OracleSim(case, bi=False, variable_dt=False, lap_form=2,
+                     portals=ports)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    # (fixed time step: no dt_var)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4,
+                    [k for k in FIELDS if k != "dt_var"])
+    print(steps, errs)
+    np.testing.assert_array_equal(sim.get("imirrored"), o.f["imirrored"])
+    ids = o.f["id"].astype(np.int64)
+    jumped = np.abs(o.f["r"][:, 0] - case["r"][ids, 0]) > \
+        0.5 * case["params"]["L"]
+    assert jumped.any() == (steps > 2)
+    sim.close()
+
+
+@pytest.mark.parametrize("dims,wall,corr,steps", [
+    (2, True, None, 2), (3, True, None, 2), (2, False, "full", 2),
+    (3, False, "full", 2), (2, True, "full", 10), (3, True, "simple", 4)])
+def test_periodic_slab_portals_match_oracle(cuda, tmp_path, dims, wall, corr,
+                                            steps):
+    """One portal pair along x over a sheared block: with a boundary-integral
+    wall (cfd portal BI), and with cfd/deltaSPH-full.xml (the portal MLS,
+    full and lapp_corr stages) or deltaSPH-simple."""
+    n = (16, 8) if dims == 2 else (12, 6, 6)
+    case = cases.periodic_slab(dims, n, wall=wall)
+    ports = cases.slab_portals(case["params"]["L"], dims)
+    pres = list(PORTAL_BASE)
+    if corr:
+        pres.append("resources/Presets/cfd/deltaSPH-%s.xml" % corr)
+    if wall:
+        pres.append("resources/Presets/cfd/BI.xml")
+    pres.append("resources/Presets/cfd/variableTimeStep.xml")
+    xml = casefiles.write_case(case, str(tmp_path), presets=tuple(pres),
+                               portals=ports)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, bi=wall, delta_corr=corr, portals=ports)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    keys = FIELDS + (("lap_p_corr",) if corr else ()) + \
+        (("mls_fluid",) if corr == "full" else ())
+    errs = {}
+    for k in keys:
+        a, b = sim.get(k), o.f[k]
+        if a.dtype.kind in "iu":
+            np.testing.assert_array_equal(a.view(b.dtype), b, err_msg=k)
+        else:
+            errs[k] = relerr(a, b)
+    print(dims, wall, corr, steps, errs)
     assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
     sim.close()
 
