@@ -45,9 +45,11 @@ __device__ void block_reduce_store(T (&v)[NC], T* out)
     }
 }
 
-template <int OP, typename T, int NC>
+// TI: element type read; T: type the reduction runs and writes in (a float
+// array summed in double when T = double)
+template <int OP, typename TI, typename T, int NC>
 __global__ void __launch_bounds__(RD_BS)
-    reduce_partial(const T* __restrict__ in, uint32_t n, T* __restrict__ part,
+    reduce_partial(const TI* __restrict__ in, uint32_t n, T* __restrict__ part,
                    const T* __restrict__ ident)
 {
     T v[NC];
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(RD_BS)
          i += gridDim.x * RD_BS) {
 #pragma unroll
         for (int k = 0; k < NC; k++)
-            v[k] = rop<OP, T>(v[k], in[(size_t)i * NC + k]);
+            v[k] = rop<OP, T>(v[k], (T)in[(size_t)i * NC + k]);
     }
     block_reduce_store<OP, T, NC>(v, part + (size_t)blockIdx.x * NC);
 }
@@ -85,32 +87,32 @@ size_t reduce_workspace(uint32_t n, int ncomp)
     return (size_t)(RD_MAXB + 1) * ncomp * 8 + 256;
 }
 
-template <int OP, typename T, int NC>
+template <int OP, typename TI, typename T, int NC>
 static void run_reduce(const void* in, uint32_t n, void* out,
                        const void* ident_dev, void* ws, hipStream_t s)
 {
     const unsigned nb = std::max(
         1u, std::min<unsigned>(RD_MAXB, blocks_for(n, RD_BS * 8)));
     T* part = (T*)ws;
-    hipLaunchKernelGGL((reduce_partial<OP, T, NC>), dim3(nb), dim3(RD_BS), 0,
-                       s, (const T*)in, n, part, (const T*)ident_dev);
+    hipLaunchKernelGGL((reduce_partial<OP, TI, T, NC>), dim3(nb), dim3(RD_BS),
+                       0, s, (const TI*)in, n, part, (const T*)ident_dev);
     hipLaunchKernelGGL((reduce_final<OP, T, NC>), dim3(1), dim3(RD_BS), 0, s,
                        part, nb, (T*)out, (const T*)ident_dev);
 }
 
-template <int OP, typename T>
+template <int OP, typename TI, typename T = TI>
 static void run_nc(int nc, const void* in, uint32_t n, void* out,
                    const void* ident_dev, void* ws, hipStream_t s)
 {
     switch (nc) {
     case 1:
-        run_reduce<OP, T, 1>(in, n, out, ident_dev, ws, s);
+        run_reduce<OP, TI, T, 1>(in, n, out, ident_dev, ws, s);
         break;
     case 2:
-        run_reduce<OP, T, 2>(in, n, out, ident_dev, ws, s);
+        run_reduce<OP, TI, T, 2>(in, n, out, ident_dev, ws, s);
         break;
     case 4:
-        run_reduce<OP, T, 4>(in, n, out, ident_dev, ws, s);
+        run_reduce<OP, TI, T, 4>(in, n, out, ident_dev, ws, s);
         break;
     default:
         throw Error(AQ_EINVAL, "reduction: ncomp must be 1, 2 or 4");
@@ -130,6 +132,12 @@ static void run_type(int type, int nc, const void* in, uint32_t n, void* out,
         break;
     case AQ_T_UINT:
         run_nc<OP, uint32_t>(nc, in, n, out, ident_dev, ws, s);
+        break;
+    case AQ_T_DOUBLE:
+        run_nc<OP, double>(nc, in, n, out, ident_dev, ws, s);
+        break;
+    case AQ_T_FLOAT_F64:
+        run_nc<OP, float, double>(nc, in, n, out, ident_dev, ws, s);
         break;
     default:
         throw Error(AQ_EINVAL, "reduction: unknown element type");
@@ -177,7 +185,9 @@ AQ_API int aq_reduce(int op, int type, int ncomp, const void* in, uint32_t n,
                    "aq_reduce: ncomp must be 1, 2 or 4");
         // identity goes at the end of the workspace, after the partials
         char* ident_dev = (char*)workspace + (size_t)RD_MAXB * ncomp * 8;
-        AQ_HIP(hipMemcpyAsync(ident_dev, identity, (size_t)ncomp * 4,
+        const size_t ib =
+            (type == AQ_T_DOUBLE || type == AQ_T_FLOAT_F64) ? 8 : 4;
+        AQ_HIP(hipMemcpyAsync(ident_dev, identity, (size_t)ncomp * ib,
                               hipMemcpyHostToDevice, S(stream)));
         reduce(op, type, ncomp, in, n, out, ident_dev, workspace, S(stream));
     });

@@ -390,6 +390,7 @@ void VmProgram::compile(const ExprP& e, const SlotFn& slots)
         SlotRef r;
         if (slots(e->name, r)) {
             const int32_t code = r.base == 'f'   ? VM_LOADF
+                                 : r.base == 'd' ? VM_LOADD
                                  : r.base == 'i' ? VM_LOADI
                                                  : VM_LOADU;
             ops.push_back({ code, (int32_t)r.offset, 0.0 });
@@ -473,8 +474,10 @@ void VmProgram::compile(const ExprP& e, const SlotFn& slots)
 
 void VmProgram::store(const SlotRef& s)
 {
-    const int32_t code =
-        s.base == 'f' ? VM_STOREF : s.base == 'i' ? VM_STOREI : VM_STOREU;
+    const int32_t code = s.base == 'f'   ? VM_STOREF
+                         : s.base == 'd' ? VM_STORED
+                         : s.base == 'i' ? VM_STOREI
+                                         : VM_STOREU;
     ops.push_back({ code, (int32_t)s.offset, 0.0 });
 }
 

@@ -65,6 +65,8 @@ enum VmCode : int32_t {
     VM_F1,
     VM_F2,
     VM_ASSERT,
+    VM_LOADD,
+    VM_STORED,
 };
 
 enum VmFn1 : int32_t {

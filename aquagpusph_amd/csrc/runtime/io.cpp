@@ -25,6 +25,9 @@ void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
             throw Error(AQ_EINVAL, "cannot load the scalar '" + name + "'");
         if (v.length < offset + n)
             throw Error(AQ_EINVAL, "variable '" + name + "' is too short");
+        if (v.type.base == 'd')
+            throw Error(AQ_EUNSUP, "cannot load the double field '" + name +
+                                       "' (FastASCII loads 32-bit fields)");
         vs.push_back(&v);
         ncols += v.type.ncomp;
     }

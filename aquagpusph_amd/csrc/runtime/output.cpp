@@ -91,6 +91,9 @@ static void write_ascii(Sim& S, const std::string& path, size_t off, size_t n,
         if (!v.is_array() || v.length < S.N)
             throw Error(AQ_EINVAL, "cannot save '" + name + "': not an "
                                                             "array of N");
+        if (v.type.base == 'd')
+            throw Error(AQ_EUNSUP, "cannot save the double field '" + name +
+                                       "' (ASCII saves 32-bit fields)");
         vs.push_back(&v);
         big = std::max(big, (size_t)S.N * v.type.elem_bytes());
     }
@@ -156,7 +159,11 @@ static std::string value_text(const Type& ty, const uint8_t* p)
     std::string s;
     char b[64];
     for (int c = 0; c < ty.ncomp; c++) {
-        if (ty.base == 'f') {
+        if (ty.base == 'd') {
+            double x;
+            memcpy(&x, p + 8 * c, 8);
+            snprintf(b, sizeof(b), "%.17g", x);
+        } else if (ty.base == 'f') {
             float x;
             memcpy(&x, p + 4 * c, 4);
             snprintf(b, sizeof(b), "%.9g", (double)x);

@@ -74,6 +74,15 @@ void launch_portal_mls(int, const int32_t*, const int32_t*, const float*,
                        uint32_t, const uint32_t*, const uint32_t*,
                        const uint32_t*, const uint32_t*, aq_defs, float,
                        hipStream_t);
+void launch_predictor64(int, const int32_t*, const float*, const double*,
+                        const float*, const float*, const float*,
+                        const float*, float*, double*, float*, float*, float*,
+                        float*, uint32_t, const float*, hipStream_t);
+void launch_corrector64(int, const int32_t*, float*, double*, float*,
+                        const float*, float*, const float*, const float*,
+                        const float*, uint32_t, const float*, hipStream_t);
+void launch_fluid_mass(const int32_t*, const float*, float*, uint32_t,
+                       hipStream_t);
 void launch_mp_predictor(int, const float*, const float*, const float*,
                          const float*, const float*, float*, float*, float*,
                          float*, float*, uint32_t, hipStream_t);
@@ -254,6 +263,7 @@ namespace rt {
 #define F(k) ((float*)S.ptr(a[k]))
 #define I(k) ((int32_t*)S.ptr(a[k]))
 #define U(k) ((uint32_t*)S.ptr(a[k]))
+#define DB(k) ((double*)S.ptr(a[k]))
 
 static void gather(Sim& S, Var* src, Var* dst, const uint32_t* perm,
                    hipStream_t s)
@@ -501,6 +511,33 @@ static const std::map<std::string, KernelSpec>& table()
                 launch_portal_mls(S.dims, I(0), I(1), F(2), F(3), F(4), F(5),
                                   U(6), S.N, U(7), U(8), U(9), S.d_status,
                                   S.kdefs, support_threshold(S.kdefs.H), s);
+            } } },
+        // ---- double-precision positions (Presets/precision.xml; no
+        // counterpart script in the reference) --------------------------------
+        { "basic/time_scheme/improved_euler_fp64.cl:predictor",
+          { { "imove", "r", "dr64", "u", "dudt", "rho", "drhodt", "r_in",
+              "dr64_in", "u_in", "dudt_in", "rho_in", "drhodt_in", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_predictor64(S.dims, I(0), F(1), DB(2), F(3), F(4), F(5),
+                                   F(6), F(7), DB(8), F(9), F(10), F(11),
+                                   F(12), S.N, F(13), s);
+            } } },
+        { "basic/time_scheme/improved_euler_fp64.cl:corrector",
+          { { "imove", "r", "dr64", "u", "dudt", "rho", "drhodt", "dudt_in",
+              "drhodt_in", "dt" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_corrector64(S.dims, I(0), F(1), DB(2), F(3), F(4), F(5),
+                                   F(6), F(7), F(8), S.N, F(9), s);
+            } } },
+        { "basic/Sort_fp64.cl:entry",
+          { { "dr64_in", "dr64", "id_unsorted" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                gather(S, a[0], a[1], U(2), s);
+            } } },
+        { "cfd/FluidMass.cl:entry",
+          { { "imove", "m", "m_fluid" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_fluid_mass(I(0), F(1), F(2), S.N, s);
             } } },
         // ---- semi-implicit midpoint scheme ----------------------------------
         { "basic/time_scheme/midpoint.cl:predictor",

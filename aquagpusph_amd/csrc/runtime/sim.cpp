@@ -234,6 +234,10 @@ void Sim::load_sets()
             std::vector<double> vals;
             for (auto& c : split_components(sc.second))
                 vals.push_back(vars.eval(c));
+            if (v.type.base == 'd')
+                throw Error(AQ_EUNSUP, "set scalar '" + sc.first +
+                                           "': double arrays are not "
+                                           "supported here");
             std::vector<uint32_t> w(v.type.stored, 0);
             for (int c = 0; c < v.type.ncomp && c < (int)vals.size(); c++) {
                 if (v.type.base == 'f') {
@@ -284,6 +288,13 @@ void Sim::setup()
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
     }
     plan_fusion();
+    // the compensated sums live in the fused cell walk only
+    if (problem.is_defined("__KAHAN_SUMS__") &&
+        !(fusion.active && clist.enabled))
+        throw Error(AQ_EUNSUP,
+                    "__KAHAN_SUMS__ needs the fused cell-parallel walk (cfd "
+                    "Shepard + cfd interactions + cfd lap p in a row, no "
+                    "AQ_WALK=thread or AQ_NO_FUSION)");
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
     is_setup = true;
@@ -399,9 +410,10 @@ void Sim::plan_fusion()
         fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
         fusion.walk_segw = atoi(ws) == 16 ? 16 : 8;
-    fusion.fast_math = relaxed_math(problem.compile_flags);
+    fusion.math = relaxed_math(problem.compile_flags) ? 1 : 0;
     if (const char* fm = getenv("AQ_MATH"))
-        fusion.fast_math = strcmp(fm, "relaxed") == 0;
+        fusion.math = strcmp(fm, "relaxed") == 0 ? 1 : 0;
+    if (problem.is_defined("__KAHAN_SUMS__")) fusion.math = 2;
     fusion.active = true;
 }
 
@@ -433,7 +445,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
-                             int segw, int flagged, int fast, hipStream_t s);
+                             int segw, int flagged, int math, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -517,10 +529,13 @@ void Sim::run_fused(hipStream_t s)
             (const uint32_t*)P(f.n_cells), clist.cell_end, clist.heads,
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
-            f.walk_segw, clist.kinds ? 1 : 0, f.fast_math ? 1 : 0, s);
+            f.walk_segw, clist.kinds ? 1 : 0, f.math, s);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }
+    if (f.math == 2)
+        throw Error(AQ_EUNSUP, "__KAHAN_SUMS__: the cell list is not usable "
+                               "for this walk");
     launch_fluid_pass(dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
                       (const float*)P(f.u), (const float*)P(f.rho),
                       (const float*)P(f.m), (const float*)P(f.p),

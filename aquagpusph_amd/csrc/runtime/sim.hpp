@@ -31,12 +31,13 @@ namespace rt {
 
 // ---- types ------------------------------------------------------------------
 struct Type {
-    char base = 'f';    // 'f' float, 'i' int32, 'u' uint32
+    char base = 'f';    // 'f' float, 'i' int32, 'u' uint32, 'd' double
     int ncomp = 1;      // logical components
     int stored = 1;     // stored components (vec3 is padded to 4)
     bool array = false;
     std::string canon;  // canonical reference type name
-    size_t elem_bytes() const { return 4 * (size_t)stored; }
+    size_t comp_bytes() const { return base == 'd' ? 8 : 4; }
+    size_t elem_bytes() const { return comp_bytes() * (size_t)stored; }
 };
 // Resolve a reference type string ("vec*", "size_t", "svec4", "float", ...)
 // with the aliasing of aquagpusph/Variable.cpp:1547-1625 (addr_bits = 32).
@@ -232,10 +233,11 @@ struct FusionPlan {
     uint8_t* near_buf = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
-    // fused walk arithmetic with the OpenCL C default accuracy (reciprocal
-    // division, hardware square root, contracted multiply-adds) instead of
-    // IEEE-754 correctly rounded division and square root
-    bool fast_math = false;
+    // fused walk arithmetic: 0 IEEE-754 (correctly rounded division and
+    // square root, no contraction); 1 relaxed (reciprocal division,
+    // hardware square root, contracted multiply-adds); 2 IEEE terms with
+    // compensated momentum and density sums (the __KAHAN_SUMS__ definition)
+    int math = 0;
     // profiling: events around the walk launch while `timing` is set
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

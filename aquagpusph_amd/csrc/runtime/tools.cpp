@@ -106,9 +106,11 @@ static std::vector<double> macro_value(const std::string& v0, const Type& t,
 static std::vector<uint32_t> words(const std::vector<double>& v,
                                    const Type& t)
 {
-    std::vector<uint32_t> w(t.stored, 0u);
+    std::vector<uint32_t> w(t.elem_bytes() / 4, 0u);
     for (int k = 0; k < t.ncomp && k < (int)v.size(); k++) {
-        if (t.base == 'f') {
+        if (t.base == 'd') {
+            std::memcpy(&w[2 * k], &v[k], 8);
+        } else if (t.base == 'f') {
             float f = (float)v[k];
             std::memcpy(&w[k], &f, 4);
         } else if (t.base == 'i') {
@@ -380,7 +382,10 @@ class ReductionTool : public Tool {
             throw Error(AQ_EINVAL, "reduction \"" + name_ +
                                        "\": in must be an array, out a "
                                        "scalar");
-        if (in_->type.base != out_->type.base ||
+        // a float array may be reduced into a double of the same shape
+        // (summed in double)
+        const bool widen = in_->type.base == 'f' && out_->type.base == 'd';
+        if ((in_->type.base != out_->type.base && !widen) ||
             in_->type.stored != out_->type.stored)
             throw Error(AQ_EINVAL, "reduction \"" + name_ +
                                        "\": in and out types differ");
@@ -394,9 +399,12 @@ class ReductionTool : public Tool {
     }
     void execute(Sim& S, hipStream_t s) override
     {
-        const int type = in_->type.base == 'f'   ? AQ_T_FLOAT
-                         : in_->type.base == 'i' ? AQ_T_INT
-                                                 : AQ_T_UINT;
+        const char ib = in_->type.base, ob = out_->type.base;
+        const int type = (ib == 'f' && ob == 'd') ? AQ_T_FLOAT_F64
+                         : ib == 'd'              ? AQ_T_DOUBLE
+                         : ib == 'f'              ? AQ_T_FLOAT
+                         : ib == 'i'              ? AQ_T_INT
+                                                  : AQ_T_UINT;
         reduce(op_, type, nc_, in_->dev, (uint32_t)in_->length, S.ptr(out_),
                ident_.p, ws_.p, s);
     }
@@ -429,8 +437,8 @@ class SetTool : public Tool {
     }
     void execute(Sim&, hipStream_t s) override
     {
-        fill_launch(v_->dev, (const uint32_t*)pat_.p, v_->type.stored,
-                    v_->length, s);
+        fill_launch(v_->dev, (const uint32_t*)pat_.p,
+                    (int)(v_->type.elem_bytes() / 4), v_->length, s);
     }
 
   private:
@@ -445,8 +453,10 @@ static int vm_depth(const std::vector<VmOp>& ops)
     int d = 0, mx = 0;
     for (auto& o : ops) {
         switch (o.code) {
-        case VM_PUSH: case VM_LOADF: case VM_LOADI: case VM_LOADU: d++; break;
-        case VM_STOREF: case VM_STOREI: case VM_STOREU: case VM_ASSERT: d--; break;
+        case VM_PUSH: case VM_LOADF: case VM_LOADI: case VM_LOADU:
+        case VM_LOADD: d++; break;
+        case VM_STOREF: case VM_STOREI: case VM_STOREU: case VM_STORED:
+        case VM_ASSERT: d--; break;
         case VM_NEG: case VM_NOT: case VM_F1: case VM_END: break;
         case VM_SEL: d -= 2; break;
         default: d--; break;
@@ -497,7 +507,8 @@ class SetScalarTool : public VmTool {
         VmProgram p;
         for (int k = 0; k < v.type.ncomp && k < (int)comps.size(); k++) {
             p.compile(parse_expr(comps[k]), slots);
-            p.store(SlotRef{ v.slot + 4u * k, v.type.base });
+            p.store(SlotRef{ v.slot + (uint32_t)v.type.comp_bytes() * k,
+                             v.type.base });
         }
         p.end();
         upload(p);

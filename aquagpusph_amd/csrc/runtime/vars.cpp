@@ -37,8 +37,9 @@ Type parse_type(const std::string& t0, int dims)
     else if (starts(t, "uint") && t != "uint") t = "uivec" + t.substr(4);
     else if (starts(t, "float") && t != "float") t = "vec" + t.substr(5);
     else if (starts(t, "fvec")) t = "vec" + t.substr(4);
+    else if (starts(t, "double") && t != "double") t = "dvec" + t.substr(6);
     const std::string dflt = dims == 3 ? "4" : "2";
-    if (t == "vec" || t == "ivec" || t == "uivec") t += dflt;
+    if (t == "vec" || t == "ivec" || t == "uivec" || t == "dvec") t += dflt;
 
     if (t == "matrix") {
         // float4 (2x2) in 2D, float16 (4x4) in 3D, types/{2D,3D}.h
@@ -51,6 +52,8 @@ Type parse_type(const std::string& t0, int dims)
         return T;
     } else if (t == "float") {
         T.base = 'f';
+    } else if (t == "double") {
+        T.base = 'd';
     } else if (t == "int") {
         T.base = 'i';
     } else if (t == "unsigned int") {
@@ -66,6 +69,9 @@ Type parse_type(const std::string& t0, int dims)
         } else if (starts(t, "vec")) {
             T.base = 'f';
             pre = "vec";
+        } else if (starts(t, "dvec")) {
+            T.base = 'd';
+            pre = "dvec";
         } else {
             throw Error(AQ_EUNSUP, "unsupported variable type '" + t0 + "'");
         }
@@ -76,6 +82,9 @@ Type parse_type(const std::string& t0, int dims)
     }
     T.stored = T.ncomp == 3 ? 4 : T.ncomp;
     T.canon = t + (T.array ? "*" : "");
+    if (!T.array && T.elem_bytes() > 16)
+        throw Error(AQ_EUNSUP, "scalar variables of type '" + t0 +
+                                   "' do not fit a 16-byte slot");
     return T;
 }
 
@@ -108,16 +117,21 @@ Var& Variables::at(const std::string& name)
 
 static const char* SUFFIX[4] = { "_x", "_y", "_z", "_w" };
 
+static double load_comp(const uint8_t* p, char base)
+{
+    if (base == 'f') return *(const float*)p;
+    if (base == 'd') return *(const double*)p;
+    if (base == 'i') return *(const int32_t*)p;
+    return *(const uint32_t*)p;
+}
+
 bool Variables::symbol(const std::string& name, double& out) const
 {
     auto it = by_name_.find(name);
     if (it != by_name_.end()) {
         const Var& v = *it->second;
         if (v.is_array()) return false;
-        const uint8_t* p = host_arena_.data() + v.slot;
-        if (v.type.base == 'f') out = *(const float*)p;
-        else if (v.type.base == 'i') out = *(const int32_t*)p;
-        else out = *(const uint32_t*)p;
+        out = load_comp(host_arena_.data() + v.slot, v.type.base);
         return true;
     }
     if (name.size() > 2 && name[name.size() - 2] == '_') {
@@ -127,10 +141,9 @@ bool Variables::symbol(const std::string& name, double& out) const
             if (jt == by_name_.end() || jt->second->is_array()) return false;
             const Var& v = *jt->second;
             if (k >= v.type.ncomp) return false;
-            const uint8_t* p = host_arena_.data() + v.slot + 4 * k;
-            if (v.type.base == 'f') out = *(const float*)p;
-            else if (v.type.base == 'i') out = *(const int32_t*)p;
-            else out = *(const uint32_t*)p;
+            out = load_comp(host_arena_.data() + v.slot +
+                                v.type.comp_bytes() * k,
+                            v.type.base);
             return true;
         }
     }
@@ -154,7 +167,7 @@ bool Variables::slot(const std::string& name, SlotRef& r) const
             if (jt == by_name_.end() || jt->second->is_array()) return false;
             const Var& v = *jt->second;
             if (k >= v.type.ncomp) return false;
-            r.offset = v.slot + 4 * k;
+            r.offset = v.slot + (uint32_t)v.type.comp_bytes() * k;
             r.base = v.type.base;
             return true;
         }
@@ -182,6 +195,7 @@ void Variables::set_host(Var& v, const std::vector<double>& vals)
     for (int k = 0; k < v.type.ncomp && k < (int)vals.size(); k++) {
         const double x = vals[k];
         if (v.type.base == 'f') ((float*)p)[k] = (float)x;
+        else if (v.type.base == 'd') ((double*)p)[k] = x;
         else if (v.type.base == 'i') ((int32_t*)p)[k] = (int32_t)x;
         else ((uint32_t*)p)[k] = (uint32_t)(int64_t)x;
     }
@@ -192,11 +206,8 @@ std::vector<double> Variables::get_host(const Var& v) const
 {
     std::vector<double> out(v.type.ncomp);
     const uint8_t* p = host_arena_.data() + v.slot;
-    for (int k = 0; k < v.type.ncomp; k++) {
-        if (v.type.base == 'f') out[k] = ((const float*)p)[k];
-        else if (v.type.base == 'i') out[k] = ((const int32_t*)p)[k];
-        else out[k] = ((const uint32_t*)p)[k];
-    }
+    for (int k = 0; k < v.type.ncomp; k++)
+        out[k] = load_comp(p + v.type.comp_bytes() * k, v.type.base);
     return out;
 }
 

@@ -75,6 +75,12 @@ __global__ void vm_kernel(const VmOp* __restrict__ ops, uint8_t* arena,
         case VM_LOADU:
             st[sp++] = (double)*(const uint32_t*)(arena + op.arg);
             break;
+        case VM_LOADD:
+            st[sp++] = *(const double*)(arena + op.arg);
+            break;
+        case VM_STORED:
+            *(double*)(arena + op.arg) = st[--sp];
+            break;
         case VM_STOREF:
             *(float*)(arena + op.arg) = (float)st[--sp];
             break;

@@ -261,6 +261,10 @@ void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                            0, s, perm, inv, n);
 }
 
+struct alignas(16) Words8 {
+    uint4 a, b;
+};
+
 void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
              uint32_t n, hipStream_t s)
 {
@@ -285,6 +289,10 @@ void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
     case 16:
         hipLaunchKernelGGL(permute_k<uint4>, g, b, 0, s, (const uint4*)src,
                            (uint4*)dst, perm, n);
+        break;
+    case 32:
+        hipLaunchKernelGGL(permute_k<Words8>, g, b, 0, s, (const Words8*)src,
+                           (Words8*)dst, perm, n);
         break;
     default:
         throw Error(AQ_EINVAL, "aq_permute: element size must be 1,2,4,8,16");

@@ -282,8 +282,31 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
 
 // ---------------------------------------------------------------------------
 // Fused fluid walk (same terms as fluid_pass_k in sph.hip).
+//   MATH 0: IEEE-754 terms and sums, bit-identical to the oracle;
+//   MATH 1: OpenCL relaxed-math terms (reciprocals, hardware sqrt, fma);
+//   MATH 2: the IEEE terms with compensated (Neumaier) sums for the
+//           momentum (grad_p, lap_u) and density (div_u, lap_p) rates.
 // ---------------------------------------------------------------------------
-template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, bool FAST>
+// s + x with the rounding error of every addition collected in c (Kahan-
+// Babuska / Neumaier: the larger operand's low bits are the ones lost)
+__device__ __forceinline__ void nsum(float& s, float& c, float x)
+{
+    const float t = s + x;
+    const bool big = fabsf(s) >= fabsf(x);
+    const float lo = big ? ((s - t) + x) : ((x - t) + s);
+    c += lo;
+    s = t;
+}
+
+template <int D>
+__device__ __forceinline__ void naxpy(f3& acc, f3& c, float s, const f3& v)
+{
+    nsum(acc.x, c.x, s * v.x);
+    nsum(acc.y, c.y, s * v.y);
+    if (D == 3) nsum(acc.z, c.z, s * v.z);
+}
+
+template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, int MATH>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     fluid_pass_cells_k(const int* __restrict__ imove,
                        const float* __restrict__ r,
@@ -333,6 +356,9 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             float p_i = 0.f, rho_i = 1.f;
             f3 gp = { 0.f, 0.f, 0.f }, lu = { 0.f, 0.f, 0.f };
             float dv = 0.f, lp = 0.f, sh = 0.f;
+            // compensations (MATH 2)
+            f3 cgp = { 0.f, 0.f, 0.f }, clu = { 0.f, 0.f, 0.f };
+            float cdv = 0.f, clp = 0.f;
             if (act) {
                 r_i = ldv<D>(r, i);
                 u_i = ldv<D>(u, i);
@@ -344,6 +370,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 if (LAPP) lp = lap_p[i];
                 sh = (SHEP == 2) ? 1.f : (SHEP == 1 ? shepard[i] : 0.f);
             }
+            constexpr bool FAST = MATH == 1;
             const float irho_i = FAST ? __builtin_amdgcn_rcpf(rho_i) : 0.f;
             // ---- fluid neighbours ------------------------------------------
             bool nearw = false;
@@ -398,17 +425,28 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     const float udr = dot<D>(u_ij, r_ij);
                     const float fk = kernelF<D>(q) * CONF;
                     const float f_ij = fk * m_j;
-                    axpy<D>(gp, (p_i + p_j) / (rho_i * rho_j) * f_ij, r_ij);
+                    const float sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
+                    float sl;
                     if constexpr (LAPF == AQ_LAP_MONAGHAN) {
                         const float r2 = (q * q + 0.01f) * H * H;
-                        axpy<D>(lu,
-                                f_ij * CLEARY * udr / (r2 * rho_i * rho_j),
-                                r_ij);
+                        sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
                     } else {
-                        axpy<D>(lu, f_ij * 2.f / (rho_i * rho_j), u_ij);
+                        sl = f_ij * 2.f / (rho_i * rho_j);
                     }
-                    dv += udr * f_ij * rho_i / rho_j;
-                    if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
+                    const float td = udr * f_ij * rho_i / rho_j;
+                    if constexpr (MATH == 2) {
+                        naxpy<D>(gp, cgp, sg, r_ij);
+                        naxpy<D>(lu, clu, sl,
+                                 LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
+                        nsum(dv, cdv, td);
+                        if (LAPP) nsum(lp, clp, (p_j - p_i) * (f_ij / rho_j));
+                    } else {
+                        axpy<D>(gp, sg, r_ij);
+                        axpy<D>(lu, sl,
+                                LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
+                        dv += td;
+                        if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
+                    }
                 });
             // ---- wall elements (boundary-integral Shepard) ------------------
             if (SHEP == 2 && __ballot(nearw) != 0) {
@@ -447,6 +485,12 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     });
             }
             if (act) {
+                if constexpr (MATH == 2) {
+                    gp = { gp.x + cgp.x, gp.y + cgp.y, gp.z + cgp.z };
+                    lu = { lu.x + clu.x, lu.y + clu.y, lu.z + clu.z };
+                    dv += cdv;
+                    lp += clp;
+                }
                 stxyz<D>(grad_p, i, gp);
                 stxyz<D>(lap_u, i, lu);
                 div_u[i] = dv;
@@ -760,7 +804,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
-                             int flagged, int fast, hipStream_t s)
+                             int flagged, int math, hipStream_t s)
 {
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
@@ -772,8 +816,9 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                 walk_ranges() };
 #define AQ_FC2(DD, SH, LP, SW, LF)                                             \
     do {                                                                       \
-        auto* ker = fast ? fluid_pass_cells_k<DD, SH, LP, SW, LF, true>        \
-                         : fluid_pass_cells_k<DD, SH, LP, SW, LF, false>;      \
+        auto* ker = math == 1   ? fluid_pass_cells_k<DD, SH, LP, SW, LF, 1>    \
+                    : math == 2 ? fluid_pass_cells_k<DD, SH, LP, 8, LF, 2>     \
+                                : fluid_pass_cells_k<DD, SH, LP, SW, LF, 0>;   \
         const unsigned nb = nblocks ? nblocks : resident_blocks((const void*)ker, \
                                                     64 * WC_WAVES);            \
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 
-_NP = {"f": np.float32, "i": np.int32, "u": np.uint32}
+_NP = {"f": np.float32, "i": np.int32, "u": np.uint32, "d": np.float64}
 
 
 def _base(type_name):
@@ -23,6 +23,8 @@ def _base(type_name):
         return "u"
     if t.startswith("ivec") or t == "int":
         return "i"
+    if t.startswith("dvec") or t == "double":
+        return "d"
     return "f"
 
 
@@ -140,7 +142,7 @@ class Simulation:
     def get(self, name):
         t, n, eb = self.info(name)
         dt = _NP[_base(t)]
-        comps = eb // 4
+        comps = eb // np.dtype(dt).itemsize
         a = np.empty(n * comps, dt)
         _lib.check(self._L.aq_sim_get(self._h, name.encode(),
                                       a.ctypes.data_as(ctypes.c_void_p),

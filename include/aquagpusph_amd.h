@@ -309,10 +309,13 @@ AQ_API int aq_neighs(int dims, const int32_t* imove, uint32_t* n_neighs,
 #define AQ_T_FLOAT 0 /* float */
 #define AQ_T_INT 1   /* int32 */
 #define AQ_T_UINT 2  /* uint32 */
+#define AQ_T_DOUBLE 3    /* double */
+#define AQ_T_FLOAT_F64 4 /* float elements reduced in double, double out */
 /* Reduce n elements of `ncomp` components each (1 for scalars, 2/4 for
  * vec) component-wise with op into out[ncomp] (device).  identity is the
  * reference's "null" value per component (host array of ncomp values, as
- * float/int32/uint32 bits per `type`).  workspace: aq_reduce_workspace(). */
+ * float/int32/uint32/double bits per the output type of `type`).
+ * workspace: aq_reduce_workspace(). */
 AQ_API int aq_reduce(int op, int type, int ncomp, const void* in, uint32_t n,
                      void* out, const void* identity, void* workspace,
                      void* stream);

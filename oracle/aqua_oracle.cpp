@@ -301,6 +301,22 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         lapp<D, R>(imove, r, rho, m, p, lap_p, N, LL{ ic, hoc, nc }, H, CONF); \
     }                                                                          \
+    ORC_EXPORT void orc_interactions_kahan_##SR##_##D##d(                      \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        const R* p, R* grad_p, R* lap_u, R* div_u, usize N, const usize* ic,   \
+        const usize* hoc, const usize* nc, R H, R CONF, int lap_form)          \
+    {                                                                          \
+        interactions<D, R>(imove, r, u, rho, m, p, grad_p, lap_u, div_u, N,    \
+                           LL{ ic, hoc, nc }, H, CONF, lap_form, true);        \
+    }                                                                          \
+    ORC_EXPORT void orc_lapp_kahan_##SR##_##D##d(                              \
+        const int* imove, const R* r, const R* rho, const R* m, const R* p,    \
+        R* lap_p, usize N, const usize* ic, const usize* hoc, const usize* nc, \
+        R H, R CONF)                                                           \
+    {                                                                          \
+        lapp<D, R>(imove, r, rho, m, p, lap_p, N, LL{ ic, hoc, nc }, H, CONF,  \
+                   true);                                                      \
+    }                                                                          \
     ORC_EXPORT void orc_sensors_##SR##_##D##d(                                 \
         const int* imove, const R* r, const R* m, R* u, R* rho, R* p, usize N, \
         const usize* ic, const usize* hoc, const usize* nc, R H, R CONW)       \
@@ -482,6 +498,37 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
 
 ORC_DEFINE(float, f32, 2)
 ORC_DEFINE(float, f32, 3)
+
+// precision mode: float state with double positions (r + dr64)
+#define ORC_DEFINE64(D)                                                        \
+    ORC_EXPORT void orc_predictor64_##D##d(                                    \
+        const int* imove, const float* r, const double* dr64, const float* u,  \
+        const float* dudt, const float* rho, const float* drhodt,             \
+        float* r_in, double* dr64_in, float* u_in, float* dudt_in,            \
+        float* rho_in, float* drhodt_in, usize N, float dt)                    \
+    {                                                                          \
+        predictor64<D>(imove, r, dr64, u, dudt, rho, drhodt, r_in, dr64_in,    \
+                       u_in, dudt_in, rho_in, drhodt_in, N, dt);               \
+    }                                                                          \
+    ORC_EXPORT void orc_corrector64_##D##d(                                    \
+        const int* imove, float* r, double* dr64, float* u, const float* dudt, \
+        float* rho, const float* drhodt, const float* dudt_in,                 \
+        const float* drhodt_in, usize N, float dt)                             \
+    {                                                                          \
+        corrector64<D>(imove, r, dr64, u, dudt, rho, drhodt, dudt_in,          \
+                       drhodt_in, N, dt);                                      \
+    }
+ORC_DEFINE64(2)
+ORC_DEFINE64(3)
+
+// the compensated sum of x[0..n) in order: out[0] = s, out[1] = c
+ORC_EXPORT void orc_nsum_f32(const float* x, usize n, float* out)
+{
+    float s = 0.f, c = 0.f;
+    for (usize i = 0; i < n; i++) nsum(s, c, x[i]);
+    out[0] = s;
+    out[1] = c;
+}
 ORC_DEFINE(double, f64, 2)
 ORC_DEFINE(double, f64, 3)
 

@@ -381,6 +381,59 @@ void corrector(const int* imove, R* r, R* u, const R* dudt, R* rho,
     }
 }
 
+// The precision mode's improved Euler (Presets/precision.xml): the same
+// update with the position carried in double as r + dr64 (r float, dr64 the
+// double remainder); velocity and density as above.
+template <int D>
+void predictor64(const int* imove, const float* r, const double* dr64,
+                 const float* u, const float* dudt, const float* rho,
+                 const float* drhodt, float* r_in, double* dr64_in,
+                 float* u_in, float* dudt_in, float* rho_in,
+                 float* drhodt_in, usize N, float dt)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        float DT = dt;
+        if (imove[i] <= 0) DT = 0.f;
+        const float hdt2 = 0.5f * DT * DT;
+        const double dDT = DT, dh = hdt2;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            dudt_in[a] = dudt[a];
+            u_in[a] = u[a] + DT * dudt[a];
+            const double x = ((double)r[a] + dr64[a]) + dDT * (double)u[a] +
+                             dh * (double)dudt[a];
+            r_in[a] = (float)x;
+            dr64_in[a] = x - (double)r_in[a];
+        }
+        drhodt_in[i] = drhodt[i];
+        rho_in[i] = rho[i] + DT * drhodt[i];
+    }
+}
+
+template <int D>
+void corrector64(const int* imove, float* r, double* dr64, float* u,
+                 const float* dudt, float* rho, const float* drhodt,
+                 const float* dudt_in, const float* drhodt_in, usize N,
+                 float dt)
+{
+    const int VS = Dim<D>::VS;
+    for (size_t i = 0; i < N; i++) {
+        if (imove[i] <= 0) continue;
+        const float DT = 0.5f * dt;
+        const double DT2 = (double)DT * (double)DT;
+        for (int k = 0; k < VS; k++) {
+            const size_t a = i * VS + k;
+            const float da = dudt[a] - dudt_in[a];
+            u[a] += DT * da;
+            const double x = ((double)r[a] + dr64[a]) + DT2 * (double)da;
+            r[a] = (float)x;
+            dr64[a] = x - (double)r[a];
+        }
+        rho[i] += DT * (drhodt[i] - drhodt_in[i]);
+    }
+}
+
 // basic/Domain.cl:48-90
 template <int D, typename R>
 void domain(int* imove, R* r_in, R* u_in, R* dudt_in, R* m, usize N,
@@ -609,12 +662,23 @@ void shepard(const int* imove, const R* r, const R* rho, const R* m,
     }
 }
 
+// s + x with every addition's rounding error collected in c (Kahan-Babuska /
+// Neumaier compensated summation: the larger operand's low bits are the
+// ones lost).  The precision mode's walk sums with it; the result is s + c.
+template <typename R> inline void nsum(R& s, R& c, R x)
+{
+    const R t = s + x;
+    const R lo = (std::fabs(s) >= std::fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    c += lo;
+    s = t;
+}
+
 // cfd/Interactions.cl:60-145 (Monaghan-Cleary Laplacian when lap_form==1,
-// Morris when lap_form==2)
+// Morris when lap_form==2); kahan: the same terms summed with nsum
 template <int D, typename R>
 void interactions(const int* imove, const R* r, const R* u, const R* rho,
                   const R* m, const R* p, R* grad_p, R* lap_u, R* div_u,
-                  usize N, LL ll, R H, R CONF, int lap_form)
+                  usize N, LL ll, R H, R CONF, int lap_form, bool kahan = false)
 {
     const int VS = Dim<D>::VS;
     const R CLEARY = (D == 2) ? R(8.f) : R(10.f);
@@ -628,6 +692,7 @@ void interactions(const int* imove, const R* r, const R* u, const R* rho,
         const R rho_i = rho[i];
         V<D, R> gp = ld<D, R>(grad_p, i), lu = ld<D, R>(lap_u, i);
         R du = div_u[i];
+        R cg[3] = { 0, 0, 0 }, cl[3] = { 0, 0, 0 }, cd = 0;
         walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
             if (i == j) return;
             if (imove[j] != 1) return;
@@ -640,29 +705,44 @@ void interactions(const int* imove, const R* r, const R* u, const R* rho,
             const R udr = dot<D, R>(u_ij, r_ij);
             const R f_ij = kernelF<D, R>(q) * CONF * m[j];
             const R sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
-            for (int k = 0; k < D; k++) gp.c[k] += sg * r_ij.c[k];
+            R sl;
+            const V<D, R>* lv;
             if (lap_form == 1) {
                 const R r2 = (q * q + R(0.01f)) * H * H;
-                const R sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
-                for (int k = 0; k < D; k++) lu.c[k] += sl * r_ij.c[k];
+                sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
+                lv = &r_ij;
             } else {
-                const R sl = f_ij * R(2.f) / (rho_i * rho_j);
-                for (int k = 0; k < D; k++) lu.c[k] += sl * u_ij.c[k];
+                sl = f_ij * R(2.f) / (rho_i * rho_j);
+                lv = &u_ij;
             }
-            du += udr * f_ij * rho_i / rho_j;
+            const R td = udr * f_ij * rho_i / rho_j;
+            if (kahan) {
+                for (int k = 0; k < D; k++) {
+                    nsum(gp.c[k], cg[k], sg * r_ij.c[k]);
+                    nsum(lu.c[k], cl[k], sl * lv->c[k]);
+                }
+                nsum(du, cd, td);
+            } else {
+                for (int k = 0; k < D; k++) {
+                    gp.c[k] += sg * r_ij.c[k];
+                    lu.c[k] += sl * lv->c[k];
+                }
+                du += td;
+            }
         });
         for (int k = 0; k < D; k++) {
-            grad_p[(size_t)i * VS + k] = gp.c[k];
-            lap_u[(size_t)i * VS + k] = lu.c[k];
+            grad_p[(size_t)i * VS + k] = kahan ? gp.c[k] + cg[k] : gp.c[k];
+            lap_u[(size_t)i * VS + k] = kahan ? lu.c[k] + cl[k] : lu.c[k];
         }
-        div_u[i] = du;
+        div_u[i] = kahan ? du + cd : du;
     }
 }
 
-// basic/deltaSPH.cl:191-242 (lapp) with cfd/deltaSPH.cl:29 (fluid only)
+// basic/deltaSPH.cl:191-242 (lapp) with cfd/deltaSPH.cl:29 (fluid only);
+// kahan: summed with nsum
 template <int D, typename R>
 void lapp(const int* imove, const R* r, const R* rho, const R* m, const R* p,
-          R* lap_p, usize N, LL ll, R H, R CONF)
+          R* lap_p, usize N, LL ll, R H, R CONF, bool kahan = false)
 {
 #pragma omp parallel for schedule(dynamic, 256)
     for (long long ii = 0; ii < (long long)N; ii++) {
@@ -670,16 +750,17 @@ void lapp(const int* imove, const R* r, const R* rho, const R* m, const R* p,
         if (imove[i] != 1) continue;
         const V<D, R> r_i = ld<D, R>(r, i);
         const R p_i = p[i];
-        R acc = lap_p[i];
+        R acc = lap_p[i], c = 0;
         walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
             if ((i == j) || (imove[j] != 1)) return;
             const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
-            acc += (p[j] - p_i) * f_ij;
+            if (kahan) nsum(acc, c, (p[j] - p_i) * f_ij);
+            else acc += (p[j] - p_i) * f_ij;
         });
-        lap_p[i] = acc;
+        lap_p[i] = kahan ? acc + c : acc;
     }
 }
 

@@ -63,7 +63,7 @@ class OracleSim:
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
-                 midpoint=None, portals=None):
+                 midpoint=None, portals=None, precision=False):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -81,6 +81,12 @@ class OracleSim:
                          midpoint=midpoint)
         if portals and midpoint is not None:
             raise ValueError("portals are modelled with improved Euler only")
+        # Presets/precision.xml: compensated walk sums, double positions
+        # (r + dr64), double mass and energy sums; a float32 model
+        self.precision = bool(precision)
+        if precision and (midpoint is not None or dtype != np.float32):
+            raise ValueError("the precision mode is a float32 improved-Euler "
+                             "model")
         if midpoint is not None:
             # basic/time_scheme/midpoint.xml (+ midpoint/{r,autostop,
             # autorelax}.xml): iter_max, relax, r, Residual_max, mindec, inc
@@ -149,6 +155,10 @@ class OracleSim:
             f["gp_p"] = np.zeros(N, R)
             for k in ("gp_u", "gp_r_in", "gp_u_in"):
                 f[k] = np.zeros((N, self.vs), R)
+        if precision:
+            f["dr64"] = np.zeros((N, self.vs), np.float64)
+            f["dr64_in"] = np.zeros((N, self.vs), np.float64)
+            self.mass_fp64 = 0.0
         if portals:
             # cfd/portal.xml once per (prefix, out_r, in_r, n), in include
             # order; the set_scalar tools before each mirror/teleport tool
@@ -211,6 +221,13 @@ class OracleSim:
         float64; the GPU's tree order differs, so sums agree to a tolerance
         only)."""
         return self.dtype.type(np.sum(a, dtype=np.float64))
+
+    def _esum(self, a):
+        """An energy sum: float (as _sum), or double in the precision
+        mode (its energy variables are double)."""
+        if self.precision:
+            return np.float64(np.sum(a, dtype=np.float64))
+        return self._sum(a)
 
     def _R(self, x):
         return self.dtype.type(x)
@@ -327,6 +344,8 @@ class OracleSim:
         sc(f["u_in"], f["u"])
         sc(f["dudt"], f["dudt_in"])
         sc(f["drhodt"], f["drhodt_in"])
+        if self.precision:
+            sc(f["dr64_in"], f["dr64"])
 
     # ---- one time step ----------------------------------------------
     def _forces(self, ll, reinit_lap_p=True):
@@ -376,10 +395,12 @@ class OracleSim:
                            _p(f["gp_p"]), _p(f["gp_u"]), n)
             f["r"][...] = f["gp_r_in"]
             f["gp_u_in"][...] = f["u"]
-        k("interactions")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
-                          _p(f["rho"]), _p(f["m"]), _p(f["p"]),
-                          _p(f["grad_p"]), _p(f["lap_u"]), _p(f["div_u"]), n,
-                          *ll, H, CONF, ctypes.c_int(o["lap_form"]))
+        kh = "_kahan" if self.precision else ""
+        k("interactions" + kh)(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                               _p(f["rho"]), _p(f["m"]), _p(f["p"]),
+                               _p(f["grad_p"]), _p(f["lap_u"]),
+                               _p(f["div_u"]), n, *ll, H, CONF,
+                               ctypes.c_int(o["lap_form"]))
         k("sensors")(_p(f["imove"]), _p(f["r"]), _p(f["m"]), _p(f["u"]),
                      _p(f["rho"]), _p(f["p"]), n, *ll, H, CONW)
         k("sensors_renorm")(_p(f["imove"]), _p(f["shepard"]), _p(f["u"]),
@@ -394,8 +415,9 @@ class OracleSim:
                              _p(f["lap_p_corr"]), _p(self.sets["refd"]), n,
                              _p(self.g))
         if o["delta"]:
-            k("lapp")(_p(f["imove"]), _p(f["r"]), _p(f["rho"]), _p(f["m"]),
-                      _p(f["p"]), _p(f["lap_p"]), n, *ll, H, CONF)
+            k("lapp" + kh)(_p(f["imove"]), _p(f["r"]), _p(f["rho"]),
+                           _p(f["m"]), _p(f["p"]), _p(f["lap_p"]), n, *ll, H,
+                           CONF)
         if o["bie"]:
             # BIe.xml: boundary terms of the conservative integrals
             f["grad_w_bi"][...] = 0
@@ -476,7 +498,7 @@ class OracleSim:
             k("energy_kin")(_p(f["energy_kin"]), _p(f["imove"]), _p(f["u"]),
                             _p(f["m"]), n)
             E = self.energy
-            E["energy_Kin"] = self._sum(f["energy_kin"])
+            E["energy_Kin"] = self._esum(f["energy_kin"])
 
     def _step_midpoint(self):
         """One step of the preset stack basic + domain + cfd + midpoint
@@ -585,10 +607,18 @@ class OracleSim:
         f, N, o = self.f, self.N, self.opts
         k, R, ll = self._k, self._r, None
         n = ctypes.c_uint(N)
-        k("predictor")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
-                       _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
-                       _p(f["u_in"]), _p(f["dudt_in"]), _p(f["rho_in"]),
-                       _p(f["drhodt_in"]), n, R(self.dt))
+        if self.precision:
+            getattr(lib(), "orc_predictor64_%dd" % self.dims)(
+                _p(f["imove"]), _p(f["r"]), _p(f["dr64"]), _p(f["u"]),
+                _p(f["dudt"]), _p(f["rho"]), _p(f["drhodt"]), _p(f["r_in"]),
+                _p(f["dr64_in"]), _p(f["u_in"]), _p(f["dudt_in"]),
+                _p(f["rho_in"]), _p(f["drhodt_in"]), n,
+                ctypes.c_float(float(self.dt)))
+        else:
+            k("predictor")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                           _p(f["dudt"]), _p(f["rho"]), _p(f["drhodt"]),
+                           _p(f["r_in"]), _p(f["u_in"]), _p(f["dudt_in"]),
+                           _p(f["rho_in"]), _p(f["drhodt_in"]), n, R(self.dt))
         if o["domain"]:
             k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
                         _p(f["dudt_in"]), _p(f["m"]), n,
@@ -621,13 +651,25 @@ class OracleSim:
                               _p(self.g))
             E = self.energy
             for c in ("k", "p", "c"):
-                E["energy_dE%sdt" % c] = self._sum(f["energy_de%sdt" % c])
-                E["energy_E%s" % c] = self.dtype.type(
-                    float(E["energy_E%s" % c]) +
-                    float(self.dt) * float(E["energy_dE%sdt" % c]))
-        k("corrector")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["dudt"]),
-                       _p(f["rho"]), _p(f["drhodt"]), _p(f["dudt_in"]),
-                       _p(f["drhodt_in"]), n, R(self.dt))
+                E["energy_dE%sdt" % c] = self._esum(f["energy_de%sdt" % c])
+                e = float(E["energy_E%s" % c]) + \
+                    float(self.dt) * float(E["energy_dE%sdt" % c])
+                E["energy_E%s" % c] = np.float64(e) if self.precision else \
+                    self.dtype.type(e)
+        if self.precision:
+            getattr(lib(), "orc_corrector64_%dd" % self.dims)(
+                _p(f["imove"]), _p(f["r"]), _p(f["dr64"]), _p(f["u"]),
+                _p(f["dudt"]), _p(f["rho"]), _p(f["drhodt"]),
+                _p(f["dudt_in"]), _p(f["drhodt_in"]), n,
+                ctypes.c_float(float(self.dt)))
+            # "fluid mass" + "total fluid mass" (double sum)
+            self.mass_fp64 = float(np.sum(
+                f["m"][f["imove"] == 1].astype(np.float64)))
+        else:
+            k("corrector")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                           _p(f["dudt"]), _p(f["rho"]), _p(f["drhodt"]),
+                           _p(f["dudt_in"]), _p(f["drhodt_in"]), n,
+                           R(self.dt))
         if o["energy"]:
             # "cfd Energy" and its sums, after the corrector
             k("energy_parts")(_p(f["energy_ek"]), _p(f["energy_ep"]),
@@ -638,7 +680,7 @@ class OracleSim:
                               R(self.cs))
             E = self.energy
             for c in ("k", "p", "c"):
-                E["energy_E%s_ref" % c] = self._sum(f["energy_e%s" % c])
+                E["energy_E%s_ref" % c] = self._esum(f["energy_e%s" % c])
         if o["bie"]:
             k("bie_pst")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
                          _p(f["m"]), _p(f["rho"]), n, *ll, R(self.DIMS),

@@ -280,6 +280,68 @@ def test_delta_sph_corrections_match_oracle(cuda, tmp_path, dims, corr,
     sim.close()
 
 
+PRECISION = "resources/Presets/precision.xml"
+
+
+@pytest.mark.parametrize("dims,energy,steps", [(2, False, 2), (3, False, 2),
+                                               (2, True, 20), (3, True, 10)])
+def test_precision_preset_matches_oracle(cuda, tmp_path, dims, energy, steps):
+    """Presets/precision.xml on the dam break: compensated walk sums,
+    double positions r + dr64, double mass and energy sums, against the
+    oracle's precision mode (the tolerances of the plain pipeline; the
+    double positions' displacement likewise; the double mass exactly)."""
+    case = cases.dam_break(dims, 0.3 / (25 if dims == 2 else 8))
+    pres = (ENERGY_PRESETS if energy else casefiles.PBI_PRESETS) + \
+        (PRECISION,)
+    xml = casefiles.write_case(case, str(tmp_path), presets=pres)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, energy=energy, precision=True)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    tol = 1e-5 if steps <= 2 else 1e-4
+    errs = _compare(sim, o, tol)
+    dr = sim.get("dr64")
+    assert dr.dtype == np.float64
+    assert np.abs(o.f["dr64"]).max() > 0
+    # the double positions r + dr64 (dr64 alone is a rounding residual,
+    # which flips with the last bit of r): their displacement from the
+    # start, to tol
+    ids = o.f["id"].astype(np.int64)
+    r0 = case["r"].astype(np.float32).astype(np.float64)[ids]
+    x_gpu = sim.get("r").astype(np.float64) + dr - r0
+    x_orc = o.f["r"].astype(np.float64) + o.f["dr64"] - r0
+    errs["r64"] = relerr(x_gpu, x_orc)
+    assert errs["r64"] <= tol, errs
+    m = float(sim.get("mass_fp64"))
+    assert m == pytest.approx(o.mass_fp64, rel=1e-12)
+    if energy:
+        for k, v in o.energy.items():
+            assert isinstance(sim.get(k), np.float64), k
+        # (the compressibility part is held as in the plain energy test)
+        for k in ("energy_Kin", "energy_Ek_ref", "energy_Ep_ref",
+                  "energy_Ek", "energy_Ep"):
+            a, b = float(sim.get(k)), float(o.energy[k])
+            assert a == pytest.approx(b, rel=tol, abs=1e-9), k
+    print(dims, energy, steps, errs)
+    sim.close()
+
+
+def test_precision_walk_differs_from_plain(cuda, tmp_path):
+    """The compensated walk is in use: one step of the same case with and
+    without the preset gives different (and oracle-matching) rates."""
+    case = cases.dam_break(2, 0.3 / 25)
+    a = Simulation(casefiles.write_case(case, str(tmp_path / "a")), 2)
+    b = Simulation(casefiles.write_case(
+        case, str(tmp_path / "b"),
+        presets=casefiles.PBI_PRESETS + (PRECISION,)), 2)
+    a.step(1)
+    b.step(1)
+    assert not np.array_equal(a.get("grad_p"), b.get("grad_p"))
+    a.close()
+    b.close()
+
+
 PORTAL_BASE = ("resources/Presets/basic.xml",
                "resources/Presets/basic/time_scheme/improved_euler.xml",
                "resources/Presets/basic/domain.xml",
