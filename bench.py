@@ -34,12 +34,14 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 
 
-def build_sim(dr, dims=3):
+def build_sim(dr, dims=3, precision=False):
     from aquagpusph_amd import casefiles, cases
     from aquagpusph_amd.sim import Simulation
     case = cases.dam_break(dims, dr)
     d = tempfile.mkdtemp(prefix="aq_bench_")
-    xml = casefiles.write_case(case, d, load=False)
+    pres = casefiles.PBI_PRESETS + (
+        ("resources/Presets/precision.xml",) if precision else ())
+    xml = casefiles.write_case(case, d, load=False, presets=pres)
     sim = Simulation(xml, dims)
     casefiles.upload(sim, case)
     return sim, case
@@ -184,6 +186,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-sizes", action="store_true",
                     help="skip the ~1M and ~16M fluid-particle points")
+    ap.add_argument("--precision", action="store_true",
+                    help="add Presets/precision.xml (compensated walk sums, "
+                         "double positions) to the workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (for the --pmc "
                          "counter scale) and exit")
@@ -197,7 +202,7 @@ def main():
         copy_calibration()
         return
 
-    sim, case = build_sim(a.dr)
+    sim, case = build_sim(a.dr, precision=a.precision)
     N = sim.N
     nf = int(case["n_fluid"])
     # warmup (the first call runs one eager step, then captures the graph)
@@ -269,7 +274,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if not a.precision else "f32 (f64 positions)",
         "data": "synthetic",
         "config": {
             "workload": "3D dam break, tank 1.6x0.6x0.6 m, column "
@@ -277,7 +282,8 @@ def main():
                         "improved Euler, variable dt" % a.dr,
             "n_particles": N, "n_fluid": nf, "dr": a.dr,
             "presets": "basic+improved_euler+domain+cfd+deltaSPH+BI+"
-                       "variableTimeStep",
+                       "variableTimeStep" + ("+precision" if a.precision
+                                             else ""),
             "parallelism": "single GPU, HIP graph per step"},
         "fluid_particle_steps_per_s": nf * a.steps / el,
         "roofline": roof,
