@@ -131,19 +131,43 @@ def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml="",
         lines.append('        <Variable name="%s" type="%s" value="%s" />'
                      % (name, t, v))
     lines.append("    </Variables>")
-    lines.append('    <ParticlesSet n="%d">' % int(case["N"]))
-    for k in ("refd", "visc_dyn", "delta"):
-        lines.append('        <Scalar name="%s" value="%s" />'
-                     % (k, _f(p[k][0])))
-    if load:
-        write_particles(case, os.path.join(folder, "Fluid.dat"))
-        lines.append('        <Load format="FastASCII" file="Fluid.dat" '
-                     'fields="%s" />' % ", ".join(fields_of(case)))
-    if save is not None:
-        fmt, pattern, fl = save
-        lines.append('        <Save format="%s" file="%s" fields="%s" />'
-                     % (fmt, pattern, ", ".join(fl)))
-    lines.append("    </ParticlesSet>")
+    if "sets" in case:
+        # one ParticlesSet per case of a batch, particles contiguous
+        o = 0
+        for k, (n, sc) in enumerate(case["sets"]):
+            lines.append('    <ParticlesSet n="%d">' % int(n))
+            for name, v in sc.items():
+                txt = (_vec(v, dims) if isinstance(v, (tuple, list))
+                       else _f(v))
+                lines.append('        <Scalar name="%s" value="%s" />'
+                             % (name, txt))
+            if load:
+                sub_case = {key: (val[o:o + n] if key in
+                                  ("r", "normal", "tangent", "u", "dudt",
+                                   "rho", "drhodt", "m", "imove") else val)
+                            for key, val in case.items()}
+                sub_case["N"] = n
+                fname = "Fluid_%d.dat" % k
+                write_particles(sub_case, os.path.join(folder, fname))
+                lines.append('        <Load format="FastASCII" file="%s" '
+                             'fields="%s" />'
+                             % (fname, ", ".join(fields_of(case))))
+            lines.append("    </ParticlesSet>")
+            o += n
+    else:
+        lines.append('    <ParticlesSet n="%d">' % int(case["N"]))
+        for k in ("refd", "visc_dyn", "delta"):
+            lines.append('        <Scalar name="%s" value="%s" />'
+                         % (k, _f(p[k][0])))
+        if load:
+            write_particles(case, os.path.join(folder, "Fluid.dat"))
+            lines.append('        <Load format="FastASCII" file="Fluid.dat" '
+                         'fields="%s" />' % ", ".join(fields_of(case)))
+        if save is not None:
+            fmt, pattern, fl = save
+            lines.append('        <Save format="%s" file="%s" fields="%s" />'
+                         % (fmt, pattern, ", ".join(fl)))
+        lines.append("    </ParticlesSet>")
     if timing:
         lines.append("    <Timing>")
         for name, typ, val in timing:

@@ -296,6 +296,129 @@ def slab_portals(L, dims=3):
             ("left_", (-a,) + z, (a,) + z, (-1.0,) + z)]
 
 
+def sloshing_tank(width, fill, res=200, refd=998.0, alpha=0.02, mu=8.9e-4,
+                  delta=0.1, courant=0.1, amp=0.03, freq_ratio=1.0):
+    """A closed square sloshing tank of side `width` filled to fill * width,
+    in the tank's own units (lengths / width, time unchanged: the Froude
+    scaling keeps every tank at dr = 1/res, h = 2 dr): fluid lattice, one
+    layer of boundary-integral elements on the four walls, hydrostatic
+    pressure.  The tank moves horizontally as amp * width * sin(w t) with w
+    = freq_ratio times the first sloshing mode sqrt(g k tanh(k d)), k = pi /
+    width.  In tank units: g' = g / width, c0' = 10 sqrt(g' fill),
+    excitation amplitude amp, mu' = mu / width^2.  Returns a case with the
+    per-set values in params["set"]."""
+    dr = 1.0 / res
+    case = dam_break(2, dr, L=1.0, D=1.0, B=1.0, H=fill, hfac=2.0,
+                     refd=refd, alpha=alpha, mu=mu, delta=delta,
+                     courant=courant,
+                     walls=("bottom", "top", "left", "right"))
+    gs = G / width
+    cs = 10.0 * math.sqrt(gs * fill)
+    h = 2.0 * dr
+    nf = case["n_fluid"]
+    y = case["r"][:nf, 1]
+    press = refd * gs * (fill - y)
+    case["rho"][:nf] = refd + press / cs ** 2
+    visc = max(alpha / 8.0 * refd * h * cs, mu / width ** 2)
+    k = math.pi  # the tank is 1 wide in its units
+    w1 = math.sqrt(gs * k * math.tanh(k * fill))
+    p = case["params"]
+    p.update(cs=cs, g=[0.0, -gs], visc_dyn=[visc], dt_min=0.0,
+             width=width, fill=fill)
+    p["set"] = dict(refd=refd, visc_dyn=visc, delta=delta, cs_set=cs,
+                    g_set=(0.0, -gs), exc_amp=amp, exc_omega=freq_ratio * w1,
+                    dt_set=0.0, t_set=0.0)
+    return case
+
+
+def sloshing_batch(n_tanks, seed=0, res=200, gap=None, **kw):
+    """n_tanks independent sloshing tanks in one case, one particle set per
+    tank (the batched-ensemble workload): widths uniform in 0.5-1.5 m, fill
+    ratios 0.2-0.6, excitation amplitudes 1-5 % of the width at 0.8-1.2
+    times the first mode (seeded).  Each tank in its own units (side 1,
+    dr = 1/res); the tanks sit on a square grid, `gap` (default 4 h) apart
+    so that no pair interaction crosses from one to another.  The
+    concatenated arrays keep each tank's particles contiguous, in tank
+    order; case["sets"] lists (n, per-set scalars)."""
+    rng = np.random.default_rng(seed)
+    widths = rng.uniform(0.5, 1.5, n_tanks)
+    fills = rng.uniform(0.2, 0.6, n_tanks)
+    amps = rng.uniform(0.01, 0.05, n_tanks)
+    ratios = rng.uniform(0.8, 1.2, n_tanks)
+    h = 2.0 / res
+    gap = 4.0 * h if gap is None else gap
+    cols = int(math.ceil(math.sqrt(n_tanks)))
+    tanks = []
+    for k in range(n_tanks):
+        t = sloshing_tank(widths[k], fills[k], res=res, amp=amps[k],
+                          freq_ratio=ratios[k], **kw)
+        # dam_break puts the tank at [-1, 0] x [0, 1]
+        t["r"][:, 0] += 1.0 + (k % cols) * (1.0 + gap)
+        t["r"][:, 1] += (k // cols) * (1.0 + gap)
+        tanks.append(t)
+    out = {"dims": 2, "N": sum(t["N"] for t in tanks),
+           "n_fluid": sum(t["n_fluid"] for t in tanks)}
+    for key in ("r", "normal", "tangent", "u", "dudt"):
+        out[key] = np.concatenate([t[key] for t in tanks])
+    for key in ("rho", "drhodt", "m", "imove"):
+        out[key] = np.concatenate([t[key] for t in tanks])
+    out["iset"] = np.concatenate([np.full(t["N"], k, np.uint32)
+                                  for k, t in enumerate(tanks)])
+    out["sets"] = [(t["N"], t["params"]["set"]) for t in tanks]
+    lo = out["r"].min(axis=0) - 1.0
+    hi = out["r"].max(axis=0) + 1.0
+    p0 = tanks[0]["params"]
+    out["params"] = dict(h=h, cs=max(t["params"]["cs"] for t in tanks),
+                         courant=p0["courant"], g=[0.0, -G],
+                         refd=[s["refd"] for _, s in out["sets"]],
+                         visc_dyn=[s["visc_dyn"] for _, s in out["sets"]],
+                         delta=[s["delta"] for _, s in out["sets"]], p0=0.0,
+                         dt_min=0.0, dt_Ma=0.1, domain_min=list(lo),
+                         domain_max=list(hi), dr=1.0 / res, hfac=2.0)
+    out["tanks"] = tanks
+    out["bc"] = "BI"
+    return out
+
+
+def batch_member(batch, k):
+    """Tank k of a sloshing_batch as a stand-alone case at the same place:
+    its scalars become the global ones (cs, g, refd, visc_dyn, delta) and
+    params["excitation"] = (exc_amp, exc_omega, g) for a time-dependent
+    gravity."""
+    o = sum(n for n, _ in batch["sets"][:k])
+    n, sc = batch["sets"][k]
+    out = {"dims": 2, "N": n}
+    for key in ("r", "normal", "tangent", "u", "dudt", "rho", "drhodt", "m",
+                "imove"):
+        out[key] = batch[key][o:o + n].copy()
+    out["n_fluid"] = int((out["imove"] == 1).sum())
+    out["iset"] = np.zeros(n, np.uint32)
+    p = dict(batch["params"])
+    p.update(cs=sc["cs_set"], g=list(sc["g_set"]), refd=[sc["refd"]],
+             visc_dyn=[sc["visc_dyn"]], delta=[sc["delta"]],
+             excitation=(sc["exc_amp"], sc["exc_omega"], list(sc["g_set"])))
+    out["params"] = p
+    out["bc"] = "BI"
+    return out
+
+
+def excitation_xml(case):
+    """The set_scalar tool that makes a stand-alone case's gravity follow
+    its excitation: g = g0 + exc_amp exc_omega^2 sin(exc_omega t) along x,
+    updated at the start of the interactions stage (as the batched kernels
+    do per set)."""
+    a, w, g0 = case["params"]["excitation"]
+    f = lambda x: "%.9g" % float(np.float32(x))
+    return ("    <Variables>\n"
+            '        <Variable name="exc_amp" type="float" value="%s" />\n'
+            '        <Variable name="exc_omega" type="float" value="%s" />\n'
+            "    </Variables>\n    <Tools>\n"
+            '        <Tool action="insert" after="Sort" type="set_scalar" '
+            'name="excitation" in="g" value="%s + exc_amp*exc_omega*exc_omega'
+            '*sin(exc_omega*t), %s"/>\n    </Tools>'
+            % (f(a), f(w), f(g0[0]), f(g0[1])))
+
+
 def dam_break_dr_for(n_fluid, dims=3, B=0.6, H=0.3, W=0.6):
     """Lattice spacing that gives about n_fluid particles in the column."""
     vol = B * H * (W if dims == 3 else 1.0)

@@ -255,6 +255,17 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
     });
 }
 
+AQ_API int aq_sim_autotune(aq_sim* s, int nsteps, char* report, size_t len)
+{
+    return guard([&] {
+        const std::string r = SIM(s).autotune(nsteps);
+        if (report && len) {
+            strncpy(report, r.c_str(), len - 1);
+            report[len - 1] = '\0';
+        }
+    });
+}
+
 AQ_API int aq_sim_walk_ms(aq_sim* s, double* ms)
 {
     return guard(

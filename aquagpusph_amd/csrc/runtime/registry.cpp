@@ -83,6 +83,38 @@ void launch_corrector64(int, const int32_t*, float*, double*, float*,
                         const float*, uint32_t, const float*, hipStream_t);
 void launch_fluid_mass(const int32_t*, const float*, float*, uint32_t,
                        hipStream_t);
+void launch_bt_eos(const uint32_t*, const int32_t*, const float*, float*,
+                   const float*, const float*, uint32_t, const float*,
+                   hipStream_t);
+void launch_bt_rates(int, const uint32_t*, const int32_t*, const float*,
+                     const float*, const float*, float*, float*, const float*,
+                     const float*, const float*, const float*, const float*,
+                     uint32_t, hipStream_t);
+void launch_bt_timestep(int, const uint32_t*, const int32_t*, const float*,
+                        float*, uint32_t, const float*, const float*,
+                        const float*, const float*, const float*,
+                        hipStream_t);
+void launch_bt_dt_min(const uint32_t*, const float*, uint32_t, uint32_t,
+                      float*, hipStream_t);
+void launch_bt_predictor(int, const uint32_t*, const int32_t*, const float*,
+                         const float*, const float*, const float*,
+                         const float*, float*, float*, float*, float*, float*,
+                         uint32_t, const float*, hipStream_t);
+void launch_bt_corrector(int, const uint32_t*, const int32_t*, float*, float*,
+                         const float*, float*, const float*, const float*,
+                         const float*, uint32_t, const float*, hipStream_t);
+void launch_bt_delta(const uint32_t*, const int32_t*, const float*,
+                     const float*, float*, const float*, const float*,
+                     uint32_t, const float*, hipStream_t);
+void launch_bt_bi_gradp(int, const uint32_t*, const int32_t*, const float*,
+                        const float*, const float*, float*, const float*,
+                        const float*, const float*, const float*,
+                        const float*, const float*, uint32_t, hipStream_t);
+void launch_bt_bi_interp_shepard(const uint32_t*, const int32_t*,
+                                 const float*, float*, float*, const float*,
+                                 const float*, uint32_t, const float*,
+                                 hipStream_t);
+void launch_bt_time(float*, const float*, uint32_t, hipStream_t);
 void launch_mp_predictor(int, const float*, const float*, const float*,
                          const float*, const float*, float*, float*, float*,
                          float*, float*, uint32_t, hipStream_t);
@@ -538,6 +570,74 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "m", "m_fluid" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_fluid_mass(I(0), F(1), F(2), S.N, s);
+            } } },
+        // ---- batched ensembles (Presets/batch.xml; per-set forms of the
+        // scalar-parameter kernels, no counterpart script in the reference) --
+        { "basic/batch/EOS.cl:entry",
+          { { "iset", "imove", "rho", "p", "refd", "cs_set", "p0" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_eos(U(0), I(1), F(2), F(3), F(4), F(5), S.N, F(6), s);
+            } } },
+        { "cfd/batch/Rates.cl:entry",
+          { { "iset", "imove", "grad_p", "lap_u", "div_u", "dudt", "drhodt",
+              "visc_dyn", "g_set", "exc_amp", "exc_omega", "t_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_rates(S.dims, U(0), I(1), F(2), F(3), F(4), F(5),
+                                F(6), F(7), F(8), F(9), F(10), F(11), S.N, s);
+            } } },
+        { "cfd/batch/TimeStep.cl:entry",
+          { { "iset", "imove", "u", "dt_var", "cs_set", "dt_min", "courant",
+              "dt_Ma", "h" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_timestep(S.dims, U(0), I(1), F(2), F(3), S.N, F(4),
+                                   F(5), F(6), F(7), F(8), s);
+            } } },
+        { "cfd/batch/TimeStep.cl:min",
+          { { "iset", "dt_var", "dt_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_dt_min(U(0), F(1), S.N, (uint32_t)a[2]->length,
+                                 F(2), s);
+            } } },
+        { "basic/batch/improved_euler.cl:predictor",
+          { { "iset", "imove", "r", "u", "dudt", "rho", "drhodt", "r_in",
+              "u_in", "dudt_in", "rho_in", "drhodt_in", "dt_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_predictor(S.dims, U(0), I(1), F(2), F(3), F(4), F(5),
+                                    F(6), F(7), F(8), F(9), F(10), F(11),
+                                    S.N, F(12), s);
+            } } },
+        { "basic/batch/improved_euler.cl:corrector",
+          { { "iset", "imove", "r", "u", "dudt", "rho", "drhodt", "dudt_in",
+              "drhodt_in", "dt_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_corrector(S.dims, U(0), I(1), F(2), F(3), F(4), F(5),
+                                    F(6), F(7), F(8), S.N, F(9), s);
+            } } },
+        { "cfd/batch/deltaSPH.cl:entry",
+          { { "iset", "imove", "rho", "lap_p", "drhodt", "refd", "delta",
+              "dt_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_delta(U(0), I(1), F(2), F(3), F(4), F(5), F(6), S.N,
+                                F(7), s);
+            } } },
+        { "cfd/batch/BI/GradP.cl:entry",
+          { { "iset", "imove", "shepard", "lap_u", "dudt", "grad_p",
+              "visc_dyn", "refd", "g_set", "exc_amp", "exc_omega", "t_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_bi_gradp(S.dims, U(0), I(1), F(2), F(3), F(4), F(5),
+                                   F(6), F(7), F(8), F(9), F(10), F(11), S.N,
+                                   s);
+            } } },
+        { "cfd/batch/BI/InterpolationShepard.cl:entry",
+          { { "iset", "imove", "shepard", "rho", "p", "refd", "cs_set", "p0" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_bi_interp_shepard(U(0), I(1), F(2), F(3), F(4), F(5),
+                                            F(6), S.N, F(7), s);
+            } } },
+        { "basic/batch/Time.cl:entry",
+          { { "t_set", "dt_set" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bt_time(F(0), F(1), (uint32_t)a[0]->length, s);
             } } },
         // ---- semi-implicit midpoint scheme ----------------------------------
         { "basic/time_scheme/midpoint.cl:predictor",

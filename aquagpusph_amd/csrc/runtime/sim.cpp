@@ -445,7 +445,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
-                             int segw, int flagged, int math, hipStream_t s);
+                             int segw, int flagged, int math,
+                             unsigned ranges, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -529,7 +530,7 @@ void Sim::run_fused(hipStream_t s)
             (const uint32_t*)P(f.n_cells), clist.cell_end, clist.heads,
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
-            f.walk_segw, clist.kinds ? 1 : 0, f.math, s);
+            f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges, s);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

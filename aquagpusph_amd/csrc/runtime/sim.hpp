@@ -81,6 +81,7 @@ class Variables {
     std::vector<double> get_host(const Var& v) const;
     void* scalar_dev(const Var& v) const { return dev_arena_ + v.slot; }
     void* scalar_host(const Var& v) { return host_arena_.data() + v.slot; }
+    std::vector<uint8_t>& host_arena() { return host_arena_; }
 
     // whole-arena transfers (synchronous w.r.t. the stream)
     void push(hipStream_t s);
@@ -233,6 +234,8 @@ struct FusionPlan {
     uint8_t* near_buf = nullptr;
     unsigned walk_blocks = 0;  // 0: as many workgroups as fit at once
     int walk_segw = 8;  // accept-bit words per lane and segment (8 or 16)
+    // cell hand-out ranges (1 or 8, one per XCD); 0: AQ_WALK_RANGES or 8
+    unsigned walk_ranges = 0;
     // fused walk arithmetic: 0 IEEE-754 (correctly rounded division and
     // square root, no contraction); 1 relaxed (reciprocal division,
     // hardware square root, contracted multiply-adds); 2 IEEE terms with
@@ -275,6 +278,10 @@ class Sim {
     void setup();                            // tool setup + initial upload
     void step(int nsteps);                   // eager execution
     void step_graph(int nsteps);             // HIP-graph replay
+    // time the fused-walk launch configurations on this case (state saved
+    // and restored) and keep the fastest; returns a report
+    std::string autotune(int nsteps);
+    void drop_graphs();
     void check_status();                     // raise on device failures
     void sync();
 

@@ -804,7 +804,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* cell_end, const uint32_t* heads,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
-                             int flagged, int math, hipStream_t s)
+                             int flagged, int math, unsigned ranges,
+                             hipStream_t s)
 {
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
@@ -813,7 +814,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
     WalkArgs w{ icell,  ihoc,   n_cells, cell_end, heads,
                 n_heads, next, status, N,       (uint32_t)flagged,
-                walk_ranges() };
+                ranges ? std::min<unsigned>(ranges, NRANGE) : walk_ranges() };
 #define AQ_FC2(DD, SH, LP, SW, LF)                                             \
     do {                                                                       \
         auto* ker = math == 1   ? fluid_pass_cells_k<DD, SH, LP, SW, LF, 1>    \

@@ -158,6 +158,17 @@ class Simulation:
                                       a.ctypes.data_as(ctypes.c_void_p),
                                       a.nbytes))
 
+    def autotune(self, steps=3):
+        """Time the fused-walk launch configurations on this case (the
+        state is restored afterwards), keep the fastest; returns the
+        report lines."""
+        L = self._L
+        L.aq_sim_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_char_p, ctypes.c_size_t]
+        buf = ctypes.create_string_buffer(4096)
+        _lib.check(L.aq_sim_autotune(self._h, steps, buf, 4096))
+        return buf.value.decode().splitlines()
+
     def device_ptr(self, name):
         return self._L.aq_sim_device_ptr(self._h, name.encode())
 

@@ -176,6 +176,77 @@ def copy_calibration(nbytes=1 << 30, reps=5):
                       "reps": reps}))
 
 
+def batch_bench(n_tanks=256, res=200, steps=20, warmup=3, sample=8,
+                tune=True):
+    """The batched-ensemble workload: n_tanks independent excited 2D
+    sloshing tanks (widths 0.5-1.5 m, fill 0.2-0.6, dr = width/200, BI
+    walls, delta-SPH, per-tank dt) in one case (Presets/batch.xml: one
+    link-list, one walk launch for every tank), after the walk autotuner;
+    beside it, `sample` of the tanks run one at a time, scaled to all."""
+    from aquagpusph_amd import casefiles, cases
+    from aquagpusph_amd.sim import Simulation
+    b = cases.sloshing_batch(n_tanks, res=res)
+    d = tempfile.mkdtemp(prefix="aq_batch_")
+    xml = casefiles.write_case(
+        b, d, load=False,
+        presets=casefiles.PBI_PRESETS + ("resources/Presets/batch.xml",))
+    sim = Simulation(xml, 2)
+    casefiles.upload(sim, b)
+    tuning = sim.autotune(2) if tune else []
+    sim.step(max(1, warmup), graph=True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps, graph=True)
+    sim.sync()
+    el = time.perf_counter() - t0
+    N = sim.N
+    dts = np.asarray(sim.get("dt_set"), np.float64)
+    sim.close()
+    # the same tanks one case at a time (a sample, scaled to all)
+    alone = []
+    for k in range(min(sample, n_tanks)):
+        case = cases.batch_member(b, k)
+        dk = tempfile.mkdtemp(prefix="aq_tank_")
+        one = Simulation(casefiles.write_case(
+            case, dk, load=False, extra_xml=cases.excitation_xml(case)), 2)
+        casefiles.upload(one, case)
+        one.step(max(1, warmup), graph=True)
+        one.sync()
+        t1 = time.perf_counter()
+        one.step(steps, graph=True)
+        one.sync()
+        alone.append((case["N"], 1e3 * (time.perf_counter() - t1) / steps))
+        one.close()
+    ms_alone = np.mean([m for _, m in alone]) * n_tanks
+    return {
+        "metric": "particle-steps/s",
+        "value": N * steps / el,
+        "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "%d 2D sloshing tanks, widths 0.5-1.5 m, "
+                               "fill 0.2-0.6, dr=width/%d, BI walls, "
+                               "delta-SPH, harmonic excitation, per-tank dt"
+                               % (n_tanks, res),
+                   "n_particles": N,
+                   "tank_particles_min_max": [min(n for n, _ in b["sets"]),
+                                              max(n for n, _ in b["sets"])],
+                   "presets": "basic+improved_euler+domain+cfd+deltaSPH+"
+                              "BI+variableTimeStep+batch",
+                   "parallelism": "single GPU, one launch per tool for "
+                                  "every tank, HIP graph per step"},
+        "dt_set_min_max": [float(dts.min()), float(dts.max())],
+        "autotune": tuning,
+        "one_at_a_time": {"sampled_tanks": len(alone),
+                          "ms_per_step_each": [round(m, 4)
+                                               for _, m in alone],
+                          "ms_per_step_all_tanks_est": ms_alone,
+                          "batched_speedup": ms_alone / (1e3 * el / steps)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +260,9 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="add Presets/precision.xml (compensated walk sums, "
                          "double positions) to the workload")
+    ap.add_argument("--batch", type=int, default=0, metavar="TANKS",
+                    help="run the batched sloshing-tank ensemble of TANKS "
+                         "tanks instead (prints its own line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB copy kernel (for the --pmc "
                          "counter scale) and exit")
@@ -200,6 +274,10 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.batch:
+        print(json.dumps(batch_bench(a.batch, steps=a.steps,
+                                     warmup=a.warmup)))
         return
 
     sim, case = build_sim(a.dr, precision=a.precision)

@@ -368,6 +368,14 @@ AQ_API int aq_sim_definition(aq_sim* sim, const char* name, char* buf,
 /* Mean GPU milliseconds per tool over nsteps eager steps, into ms[n_tools]
  * (the report_performance counterpart, CalcServer/Reports/Performance.cpp). */
 AQ_API int aq_sim_profile_tools(aq_sim* sim, int nsteps, double* ms);
+/* Autotuner (no counterpart in the reference): times the launch
+ * configurations of the fused cell walk (segment width 8/16, one or eight
+ * cell hand-out ranges, resident or one-per-CU grid) over nsteps eager steps
+ * each on this case, restores the state after each, keeps the fastest and
+ * drops captured graphs.  A text report (one line per configuration and the
+ * choice) goes to report[len]. */
+AQ_API int aq_sim_autotune(aq_sim* sim, int nsteps, char* report,
+                           size_t len);
 /* The AQUAgpusph main loop (aquagpusph/main.cpp:160-180, TimeManager.cpp:
  * 103-150): steps (graph replays when graph != 0) until the Timing end
  * criteria, writing each ParticlesSet's <Save> files, in the original

@@ -20,6 +20,7 @@ evaluated in double and narrowed to the variable type; preset definitions
 their "%#G" text (aquagpusph/CalcServer/CalcServer.cpp:241-260).
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -63,7 +64,8 @@ class OracleSim:
                  delta=True, variable_dt=True, lap_form=1, gp_model=2,
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
-                 midpoint=None, portals=None, precision=False):
+                 midpoint=None, portals=None, precision=False,
+                 excitation=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -84,6 +86,10 @@ class OracleSim:
         # Presets/precision.xml: compensated walk sums, double positions
         # (r + dr64), double mass and energy sums; a float32 model
         self.precision = bool(precision)
+        # (amp, omega, g0): the gravity follows a horizontal harmonic motion
+        # of the frame, g = g0 + amp omega^2 sin(omega t) along x, set (in
+        # double, narrowed) at the start of the interactions stage
+        self.excitation = excitation
         if precision and (midpoint is not None or dtype != np.float32):
             raise ValueError("the precision mode is a float32 improved-Euler "
                              "model")
@@ -634,6 +640,14 @@ class OracleSim:
             lib().orc_gp_sort(_p(f["associations_in"]), _p(f["associations"]),
                               _p(f["id_sorted"]), n)
         ll = self._ll()
+        if self.excitation is not None:
+            a, w, g0 = (np.float32(self.excitation[0]),
+                        np.float32(self.excitation[1]),
+                        np.asarray(self.excitation[2], np.float32))
+            a, w, t = float(a), float(w), float(np.float32(self.t))
+            self.g[0] = self.dtype.type(np.float32(
+                float(g0[0]) + a * w * w * math.sin(w * t)))
+            self.g[1] = self.dtype.type(g0[1])
         self._forces(ll)
         if o["bie"]:
             k("bie_elastic_bounce")(_p(f["imove"]), _p(f["r_in"]),

@@ -91,3 +91,23 @@ def test_slab_portal_stages_with_mls_and_bi(tmp_path, dims):
                            "cfd portal_deltaSPH lapp_corr",
                            "cfd portal_deltaSPH unmirror",
                            "cfd portal_deltaSPH icell restore"]]
+
+
+def test_batch_preset_tool_order(tmp_path):
+    """Presets/batch.xml swaps the scalar-parameter kernels for their
+    per-set forms in place and adds the per-set time step and time after
+    the global ones; one particle set per tank with its scalars."""
+    b = cases.sloshing_batch(3, res=20)
+    pres = casefiles.PBI_PRESETS + ("resources/Presets/batch.xml",)
+    xml = casefiles.write_case(b, str(tmp_path), presets=pres)
+    tools = problem_tools(xml, 2)
+    names = [n for n, _ in tools]
+    base = _names(casefiles.write_case(b, str(tmp_path / "plain")), 2)
+    added = ["batch reset time steps", "batch time steps", "batch times"]
+    assert [n for n in names if n not in added] == base
+    i = names.index("cfd minimum time step")
+    assert names[i + 1:i + 3] == added[:2]
+    assert names[names.index("t = t + dt") + 1] == "batch times"
+    text = open(xml).read()
+    assert text.count("<ParticlesSet") == 3
+    assert text.count('<Scalar name="exc_omega"') == 3

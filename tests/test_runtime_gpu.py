@@ -342,6 +342,95 @@ def test_precision_walk_differs_from_plain(cuda, tmp_path):
     b.close()
 
 
+BATCH = "resources/Presets/batch.xml"
+
+
+@pytest.mark.parametrize("steps", [2, 20])
+def test_sloshing_tank_matches_oracle(cuda, tmp_path, steps):
+    """One excited sloshing tank (tank units, BI walls) with its gravity
+    set each step by a set_scalar tool, against the oracle's excitation."""
+    b = cases.sloshing_batch(2, res=40, seed=3)
+    case = cases.batch_member(b, 1)
+    xml = casefiles.write_case(case, str(tmp_path),
+                               extra_xml=cases.excitation_xml(case))
+    sim = Simulation(xml, 2)
+    o = op.OracleSim(case, excitation=case["params"]["excitation"])
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4)
+    np.testing.assert_allclose(sim.get("g"), o.g, rtol=1e-6)
+    print(steps, errs)
+    sim.close()
+
+
+@pytest.mark.parametrize("steps", [2, 10])
+def test_batch_matches_member_runs(cuda, tmp_path, steps):
+    """Four tanks in one batched run (Presets/batch.xml: one link-list, one
+    walk, per-set cs, g, excitation, dt, t) against each tank run alone at
+    the same place: every field of every tank, its dt and its time."""
+    b = cases.sloshing_batch(4, res=40, seed=5)
+    xml = casefiles.write_case(b, str(tmp_path / "batch"),
+                               presets=casefiles.PBI_PRESETS + (BATCH,))
+    sim = Simulation(xml, 2)
+    sim.step(steps)
+    ids = sim.get("id").astype(np.int64)
+    dt_set, t_set = sim.get("dt_set"), sim.get("t_set")
+    o = 0
+    errs = {}
+    for k, (n, sc) in enumerate(b["sets"]):
+        case = cases.batch_member(b, k)
+        one = Simulation(casefiles.write_case(
+            case, str(tmp_path / ("tank%d" % k)),
+            extra_xml=cases.excitation_xml(case)), 2)
+        one.step(steps)
+        oid = one.get("id").astype(np.int64)
+        mine = (ids >= o) & (ids < o + n)
+        order = np.argsort(ids[mine])
+        order1 = np.argsort(oid)
+        for f in ("r", "u", "rho", "p", "dudt", "drhodt", "imove"):
+            a = sim.get(f)[mine][order]
+            c = one.get(f)[order1]
+            if a.dtype.kind in "iu":
+                np.testing.assert_array_equal(a, c, err_msg=f)
+            else:
+                errs[(k, f)] = relerr(a, c)
+        assert float(dt_set[k]) == pytest.approx(float(one.get("dt")),
+                                                 rel=1e-5)
+        assert float(t_set[k]) == pytest.approx(float(one.get("t")),
+                                                rel=1e-5)
+        one.close()
+        o += n
+    print(steps, max(errs.values()))
+    assert max(errs.values()) <= (1e-5 if steps <= 2 else 1e-4), errs
+    # the tanks advance with different steps
+    assert len(set(np.asarray(dt_set).tolist())) > 1
+    sim.close()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_autotune_keeps_state_and_results(cuda, tmp_path, dims):
+    """The walk autotuner times its configurations on the live case but
+    restores the state after each; every configuration walks in the same
+    order, so the tuned run equals the untuned one bit for bit."""
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    xml = casefiles.write_case(case, str(tmp_path))
+    a = Simulation(xml, dims)
+    b = Simulation(xml, dims)
+    a.step(2, graph=True)
+    b.step(2, graph=True)
+    rep = a.autotune(2)
+    assert len(rep) == 9 and rep[-1].startswith("chosen:"), rep
+    for f in ("r", "u", "rho", "p", "id", "t"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    a.step(3, graph=True)
+    b.step(3, graph=True)
+    for f in ("r", "u", "rho", "p", "id", "shepard", "t", "dt"):
+        np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    a.close()
+    b.close()
+
+
 PORTAL_BASE = ("resources/Presets/basic.xml",
                "resources/Presets/basic/time_scheme/improved_euler.xml",
                "resources/Presets/basic/domain.xml",
