@@ -368,6 +368,28 @@ AQ_API int aq_sim_definition(aq_sim* sim, const char* name, char* buf,
 /* Mean GPU milliseconds per tool over nsteps eager steps, into ms[n_tools]
  * (the report_performance counterpart, CalcServer/Reports/Performance.cpp). */
 AQ_API int aq_sim_profile_tools(aq_sim* sim, int nsteps, double* ms);
+/* ---- LES eddy-viscosity network (no counterpart in the reference) ----
+ * A 7 -> 64 -> 64 -> 1 perceptron with GELU hidden units; BF16 matrix
+ * instructions, FP32 accumulation.  params: aq_mlp_nparam() floats, FP32
+ * master copy laid out W1[64][16] (inputs padded from 7; columns 7..15
+ * zero), b1[64], W2[64][64] (out, in), b2[64], w3[64], b3.  x: m rows of
+ * 8 floats (7 inputs + 1 ignored). */
+AQ_API int aq_mlp_nparam(void);
+AQ_API size_t aq_mlp_workspace(uint32_t m);
+/* y[i] = network(x[i]) for i < m (device pointers). */
+AQ_API int aq_mlp_forward(const float* x, uint32_t m, const float* params,
+                          float* y, void* stream);
+/* grad[nparam + 1]: the gradient of the mean squared error over the m
+ * samples and, last, the loss itself.  Deterministic (fixed summation
+ * order).  workspace: aq_mlp_workspace(m) bytes. */
+AQ_API int aq_mlp_grad(const float* x, const float* target, uint32_t m,
+                       const float* params, float* grad, void* workspace,
+                       void* stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1) on
+ * the master copy from grad (its first nparam entries). */
+AQ_API int aq_mlp_adam(float* params, const float* grad, float* adam_m,
+                       float* adam_v, int step, float lr, void* stream);
+
 /* Autotuner (no counterpart in the reference): times the launch
  * configurations of the fused cell walk (segment width 8/16, one or eight
  * cell hand-out ranges, resident or one-per-CU grid) over nsteps eager steps
