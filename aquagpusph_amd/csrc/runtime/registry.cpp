@@ -115,6 +115,20 @@ void launch_bt_bi_interp_shepard(const uint32_t*, const int32_t*,
                                  const float*, uint32_t, const float*,
                                  hipStream_t);
 void launch_bt_time(float*, const float*, uint32_t, hipStream_t);
+void launch_les_gradu(int, const int32_t*, const float*, const float*,
+                      const float*, const float*, float*, uint32_t,
+                      const uint32_t*, const uint32_t*, const uint32_t*,
+                      aq_defs, float, hipStream_t);
+void launch_les_features(int, const int32_t*, const float*, float*, uint32_t,
+                         float, hipStream_t);
+void launch_les_nut(int, const int32_t*, const float*, const float*, float*,
+                    uint32_t, float, int, float, hipStream_t);
+void launch_les_interactions(int, const int32_t*, const float*, const float*,
+                             const float*, const float*, const float*, float*,
+                             uint32_t, const uint32_t*, const uint32_t*,
+                             const uint32_t*, aq_defs, float, hipStream_t);
+void launch_mlp_forward(const float*, uint32_t, const float*, float*,
+                        hipStream_t);
 void launch_mp_predictor(int, const float*, const float*, const float*,
                          const float*, const float*, float*, float*, float*,
                          float*, float*, uint32_t, hipStream_t);
@@ -638,6 +652,54 @@ static const std::map<std::string, KernelSpec>& table()
           { { "t_set", "dt_set" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_bt_time(F(0), F(1), (uint32_t)a[0]->length, s);
+            } } },
+        // ---- LES closure (Presets/cfd/LES*.xml; no counterpart script in
+        // the reference) -------------------------------------------------
+        { "cfd/LES/GradU.cl:entry",
+          { { "imove", "r", "u", "rho", "m", "grad_u", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_les_gradu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
+                                 S.N, U(6), U(7), U(8), S.kdefs,
+                                 support_threshold(S.kdefs.H), s);
+            } } },
+        { "cfd/LES/Features.cl:entry",
+          { { "imove", "grad_u", "les_in", "hfac" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_les_features(S.dims, I(0), F(1), F(2), S.N,
+                                    (float)S.vars.get_host(*a[3])[0], s);
+            } } },
+        { "cfd/LES/MLP.cl:entry",
+          { { "les_in", "les_params", "les_c" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                AQ_REQUIRE(a[0]->length >= 8 * (size_t)S.N &&
+                               a[1]->length >= 5313 &&
+                               a[2]->length >= S.N,
+                           "LES network: les_in (8 N), les_params (5313) or "
+                           "les_c (N) too short");
+                launch_mlp_forward(F(0), S.N, F(1), F(2), s);
+            } } },
+        { "cfd/LES/Nut.cl:entry",
+          { { "imove", "grad_u", "les_c", "nu_t" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_les_nut(S.dims, I(0), F(1), F(2), F(3), S.N,
+                               S.kdefs.H, 0, 0.f, s);
+            } } },
+        { "cfd/LES/Nut.cl:smagorinsky",
+          { { "imove", "grad_u", "nu_t", "les_cs" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_les_nut(S.dims, I(0), F(1), nullptr, F(2), S.N,
+                               S.kdefs.H, 1,
+                               (float)S.vars.get_host(*a[3])[0], s);
+            } } },
+        { "cfd/LES/Interactions.cl:entry",
+          { { "imove", "r", "u", "rho", "m", "nu_t", "dudt", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_les_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                        F(5), F(6), S.N, U(7), U(8), U(9),
+                                        S.kdefs, support_threshold(S.kdefs.H),
+                                        s);
             } } },
         // ---- semi-implicit midpoint scheme ----------------------------------
         { "basic/time_scheme/midpoint.cl:predictor",

@@ -301,6 +301,22 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         lapp<D, R>(imove, r, rho, m, p, lap_p, N, LL{ ic, hoc, nc }, H, CONF); \
     }                                                                          \
+    ORC_EXPORT void orc_les_gradu_##SR##_##D##d(                               \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        R* grad_u, usize N, const usize* ic, const usize* hoc,                 \
+        const usize* nc, R H, R CONF)                                          \
+    {                                                                          \
+        les_gradu<D, R>(imove, r, u, rho, m, grad_u, N, LL{ ic, hoc, nc }, H,  \
+                        CONF);                                                 \
+    }                                                                          \
+    ORC_EXPORT void orc_les_interactions_##SR##_##D##d(                        \
+        const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
+        const R* nu_t, R* dudt, usize N, const usize* ic, const usize* hoc,    \
+        const usize* nc, R H, R CONF)                                          \
+    {                                                                          \
+        les_interactions<D, R>(imove, r, u, rho, m, nu_t, dudt, N,             \
+                               LL{ ic, hoc, nc }, H, CONF);                    \
+    }                                                                          \
     ORC_EXPORT void orc_interactions_kahan_##SR##_##D##d(                      \
         const int* imove, const R* r, const R* u, const R* rho, const R* m,    \
         const R* p, R* grad_p, R* lap_u, R* div_u, usize N, const usize* ic,   \

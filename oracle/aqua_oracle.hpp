@@ -1839,6 +1839,63 @@ void portal_mls(const int* imove, const int* imirrored, const R* r,
 }
 
 // ---------------------------------------------------------------------------
+// LES closure (csrc/les.hip; no counterpart in the reference): the SPH
+// velocity gradient G_i = sum_j m_j/rho_j (u_j - u_i) (x) F_ij r_ij (matrix
+// layout of MLS: row stride 2 in 2D, 4 in 3D) and the eddy-viscosity term
+// dudt_i += sum_j F_ij m_j (rho_i nu_i + rho_j nu_j)/(rho_i rho_j) u_ij.
+// ---------------------------------------------------------------------------
+template <int D, typename R>
+void les_gradu(const int* imove, const R* r, const R* u, const R* rho,
+               const R* m, R* grad_u, usize N, LL ll, R H, R CONF)
+{
+    const int n = (D == 2) ? 2 : 4, S = (D == 2) ? 4 : 16;
+    for (usize i = 0; i < N; i++) {
+        R G[16] = { 0 };
+        if (imove[i] == 1) {
+            const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
+            walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+                if (j == i || imove[j] != 1) return;
+                const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+                const R q = length<D, R>(r_ij) / H;
+                if (q >= R(2)) return;
+                const R w = kernelF<D, R>(q) * CONF * m[j] / rho[j];
+                const V<D, R> du = sub<D, R>(ld<D, R>(u, j), u_i);
+                for (int a = 0; a < D; a++)
+                    for (int b = 0; b < D; b++)
+                        G[a * n + b] += du.c[a] * (w * r_ij.c[b]);
+            });
+        }
+        for (int k = 0; k < S; k++) grad_u[(size_t)i * S + k] = G[k];
+    }
+}
+
+template <int D, typename R>
+void les_interactions(const int* imove, const R* r, const R* u, const R* rho,
+                      const R* m, const R* nu_t, R* dudt, usize N, LL ll,
+                      R H, R CONF)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
+        const R rho_i = rho[i], mu_i = rho_i * nu_t[i];
+        R acc[3] = { 0, 0, 0 };
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if (j == i || imove[j] != 1) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const R q = length<D, R>(r_ij) / H;
+            if (q >= R(2)) return;
+            const R rho_j = rho[j];
+            const R f = kernelF<D, R>(q) * CONF * m[j];
+            const R sc = f * (mu_i + rho_j * nu_t[j]) / (rho_i * rho_j);
+            const V<D, R> du = sub<D, R>(ld<D, R>(u, j), u_i);
+            for (int k = 0; k < D; k++) acc[k] += sc * du.c[k];
+        });
+        for (int k = 0; k < D; k++) dudt[(size_t)i * VS + k] += acc[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Semi-implicit midpoint scheme (basic/time_scheme/midpoint.cl).  Vectors
 // are updated on every stored component, as the float4 arithmetic of the
 // reference.

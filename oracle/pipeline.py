@@ -65,7 +65,7 @@ class OracleSim:
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
                  midpoint=None, portals=None, precision=False,
-                 excitation=None):
+                 excitation=None, les=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -90,6 +90,9 @@ class OracleSim:
         # of the frame, g = g0 + amp omega^2 sin(omega t) along x, set (in
         # double, narrowed) at the start of the interactions stage
         self.excitation = excitation
+        # cfd/LES.xml (les = {"params": P}) or cfd/LES-smagorinsky.xml
+        # (les = {"Cs": C}): the eddy viscosity after the delta-SPH rate
+        self.les = les
         if precision and (midpoint is not None or dtype != np.float32):
             raise ValueError("the precision mode is a float32 improved-Euler "
                              "model")
@@ -161,6 +164,10 @@ class OracleSim:
             f["gp_p"] = np.zeros(N, R)
             for k in ("gp_u", "gp_r_in", "gp_u_in"):
                 f[k] = np.zeros((N, self.vs), R)
+        if les is not None:
+            f["grad_u"] = np.zeros((N, 4 if self.dims == 2 else 16), R)
+            f["nu_t"] = np.zeros(N, R)
+            f["les_c"] = np.zeros(N, R)
         if precision:
             f["dr64"] = np.zeros((N, self.vs), np.float64)
             f["dr64_in"] = np.zeros((N, self.vs), np.float64)
@@ -301,6 +308,33 @@ class OracleSim:
             k("portal_unmirror")(_p(f["r"]), _p(f["imirrored"]), n,
                                  _p(pv["in_r"]), _p(pv["out_r"]))
             f["icell"][...] = f["icell_backup"]
+
+    def _les(self, ll):
+        """cfd/LES*.xml, before Rates: velocity gradient, (features and
+        network,) eddy viscosity, eddy-viscosity term."""
+        from oracle import les as olq
+        from oracle import mlp as omlp
+        f, N, k, R = self.f, self.N, self._k, self._r
+        n = ctypes.c_uint(N)
+        H, CONF = R(self.H), R(self.CONF)
+        k("les_gradu")(_p(f["imove"]), _p(f["r"]), _p(f["u"]), _p(f["rho"]),
+                       _p(f["m"]), _p(f["grad_u"]), n, *ll, H, CONF)
+        if "params" in self.les:
+            x = olq.features(f["grad_u"], self.dims,
+                             float(np.float32(self.les.get("hdr", 2.0))))
+            x = x.astype(np.float32)
+            c, _ = omlp.forward(self.les["params"], x, bf=True)
+            f["les_c"][...] = c
+            nu = olq.eddy_viscosity(f["imove"], f["grad_u"], self.dims,
+                                    float(self.H), c=f["les_c"])
+        else:
+            nu = olq.eddy_viscosity(f["imove"], f["grad_u"], self.dims,
+                                    float(self.H),
+                                    Cs=float(np.float32(self.les["Cs"])))
+        f["nu_t"][...] = nu
+        k("les_interactions")(_p(f["imove"]), _p(f["r"]), _p(f["u"]),
+                              _p(f["rho"]), _p(f["m"]), _p(f["nu_t"]),
+                              _p(f["dudt"]), n, *ll, H, CONF)
 
     def _portal_teleport(self):
         """"cfd portal mirror_teleport" of each portal (before Corrector,
@@ -499,6 +533,8 @@ class OracleSim:
                             _p(f["lap_p"]), _p(f["drhodt"]),
                             _p(self.sets["refd"]), _p(self.sets["delta"]), n,
                             R(self.dt))
+        if self.les is not None:
+            self._les(ll)
         if o["energy"]:
             # "cfd Energy kin" + its sum, after "Rates"
             k("energy_kin")(_p(f["energy_kin"]), _p(f["imove"]), _p(f["u"]),

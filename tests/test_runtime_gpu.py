@@ -431,6 +431,51 @@ def test_autotune_keeps_state_and_results(cuda, tmp_path, dims):
     b.close()
 
 
+@pytest.mark.parametrize("dims,mode,steps", [(2, "smag", 2), (3, "smag", 2),
+                                           (2, "mlp", 2), (3, "mlp", 2),
+                                           (2, "mlp", 10)])
+def test_les_presets_match_oracle(cuda, tmp_path, dims, mode, steps):
+    """cfd/LES-smagorinsky.xml and cfd/LES.xml (the BF16 network with given
+    parameters) on the dam break with a smooth sheared initial velocity:
+    velocity gradient and eddy viscosity against the oracle to 1e-5 (the
+    network's to 2e-3: its FP32 accumulation order differs from the
+    oracle's), every field to 1e-4 (with the fluid moving from the start,
+    the BI transcendentals' 1e-7 differences reach p and lap p through
+    the cancellation in rho - rho0)."""
+    from oracle import mlp as omlp
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    fl = case["imove"] == 1
+    x, y = case["r"][fl, 0], case["r"][fl, dims - 1]
+    case["u"][fl, 0] = 0.3 * np.sin(2 * np.pi * y / 0.3)
+    case["u"][fl, dims - 1] = 0.3 * np.sin(2 * np.pi * x / 0.6)
+    pres = casefiles.PBI_PRESETS + (
+        "resources/Presets/cfd/LES%s.xml"
+        % ("" if mode == "mlp" else "-smagorinsky"),)
+    xml = casefiles.write_case(case, str(tmp_path), presets=pres)
+    sim = Simulation(xml, dims)
+    if mode == "mlp":
+        P = omlp.init(5)
+        P[omlp.OFF_W3:omlp.OFF_B3] *= 0.1
+        P[omlp.OFF_B3] = 0.05
+        sim.set("les_params", P)
+        les = {"params": P, "hdr": case["params"]["hfac"]}
+    else:
+        les = {"Cs": 0.15}
+    o = op.OracleSim(case, les=les)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = _compare(sim, o, 1e-4)
+    for k in ("grad_u", "nu_t"):
+        errs[k] = relerr(sim.get(k), o.f[k])
+    print(dims, mode, steps, errs)
+    tol = 1e-5 if steps <= 2 else 1e-4
+    assert errs["nu_t"] <= (2e-3 if mode == "mlp" else tol), errs
+    assert errs["grad_u"] <= tol, errs
+    assert np.abs(o.f["nu_t"]).max() > 0
+    sim.close()
+
+
 PORTAL_BASE = ("resources/Presets/basic.xml",
                "resources/Presets/basic/time_scheme/improved_euler.xml",
                "resources/Presets/basic/domain.xml",
