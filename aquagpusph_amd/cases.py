@@ -186,6 +186,53 @@ def still_tank(dims, dr, L=0.3, H=0.15, W=0.2, hfac=2.0, refd=998.0,
     return case
 
 
+def decaying_box(n, seed=0, k0=4.0, urms=1.0, nu=1e-4, cs=20.0, refd=1.0,
+                 courant=0.1, delta=0.1, p0_factor=3.0):
+    """Decaying 2D turbulence in a closed unit box: n x n fluid particles,
+    boundary-integral walls on the four sides, no gravity, and a seeded
+    solenoidal velocity field with energy spectrum E(k) ~ k^4
+    exp(-2 (k/k0)^2), k in units of 2 pi (the spectrum of the BASELINE's
+    decaying-turbulence cube, here in 2D), scaled to the RMS speed urms.
+    The field is the curl of a stream function sum a_mn sin(m pi x)
+    sin(n pi y), so it is divergence-free and does not cross the walls.
+    Kinematic viscosity nu (no artificial viscosity), background pressure
+    p0 = p0_factor rho0 urms^2, delta-SPH delta."""
+    dr = 1.0 / n
+    case = dam_break(2, dr, L=1.0, D=1.0, B=1.0, H=1.0, refd=refd,
+                     alpha=0.0, mu=refd * nu, delta=delta, courant=courant,
+                     walls=("bottom", "top", "left", "right"))
+    case["r"][:, 0] += 1.0  # dam_break puts the tank at [-1, 0]
+    nf = case["n_fluid"]
+    x, y = case["r"][:nf, 0], case["r"][:nf, 1]
+    rng = np.random.default_rng(seed)
+    K = int(np.ceil(4.0 * k0)) + 1
+    kk = 2.0 * math.pi * k0
+    u = np.zeros(nf)
+    v = np.zeros(nf)
+    for mm in range(1, 2 * K + 1):
+        sx, cx = np.sin(mm * math.pi * x), np.cos(mm * math.pi * x)
+        for nn in range(1, 2 * K + 1):
+            k = math.pi * math.hypot(mm, nn)
+            E = k ** 4 * math.exp(-2.0 * (k / kk) ** 2)
+            if E < 1e-12 * kk ** 4:
+                continue
+            a = rng.normal() * math.sqrt(E) / k
+            sy, cy = np.sin(nn * math.pi * y), np.cos(nn * math.pi * y)
+            u += a * nn * math.pi * sx * cy    # d psi / dy
+            v -= a * mm * math.pi * cx * sy    # -d psi / dx
+    s = urms / math.sqrt(np.mean(u * u + v * v))
+    case["u"][:nf, 0] = u * s
+    case["u"][:nf, 1] = v * s
+    case["rho"][:] = refd
+    p = case["params"]
+    h = 2.0 * dr
+    p.update(cs=cs, g=[0.0, 0.0], p0=p0_factor * refd * urms ** 2,
+             visc_dyn=[refd * nu], dt_min=0.0, dt=0.0,
+             domain_min=[-0.5, -0.5], domain_max=[1.5, 1.5], nu=nu)
+    case["params"] = p
+    return case
+
+
 def taylor_green(n=32, L=2.0 * math.pi, U=1.0, hfac=4.0, cs=50.0,
                  courant=0.1, refd=1.0, Re=2000.0, alpha=0.0, delta=0.0,
                  drift=(0.0, 0.0)):
