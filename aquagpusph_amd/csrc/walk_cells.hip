@@ -152,6 +152,28 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
     return uni(L.cum[NC]);
 }
 
+// Bounding box of the active lanes' positions (uniform; +-inf if none)
+template <int D>
+__device__ __forceinline__ void wave_box(const f3& r_i, bool act, f3& bmin,
+                                         f3& bmax)
+{
+    const float inf = __builtin_huge_valf();
+    float lo[3] = { act ? r_i.x : inf, act ? r_i.y : inf,
+                    act ? r_i.z : inf };
+    float hi[3] = { act ? r_i.x : -inf, act ? r_i.y : -inf,
+                    act ? r_i.z : -inf };
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], o));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
+        }
+    }
+    bmin = f3{ lo[0], lo[1], D == 3 ? lo[2] : 0.f };
+    bmax = f3{ hi[0], hi[1], D == 3 ? hi[2] : 0.f };
+}
+
 // Walk the accepted candidates of one particle per lane.  `kind` selects
 // the candidates (0: fluid particles, imove == 1; 1: wall elements,
 // imove == -3); with kind 0 and `flag_el`, wall elements in range are also
@@ -162,10 +184,20 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
 // dummy index on the idle lanes, measured 25% slower on the 3D dam break:
 // the idle lanes' loads and the longer dependent chains cost more than the
 // exec-mask changes they save.)
+//
+// Box prefilter: a candidate farther than the support from the bounding box
+// of the active lanes' positions is accepted by no lane, so it is not
+// staged; the survivors of each 64-candidate chunk are compacted in their
+// order (each lane's accepted candidates keep their order, so the sums are
+// unchanged).  The box distance is computed like the lanes' own test, one
+// rounded subtraction, square and sum per axis in the same order, and
+// rounding is monotone, so it never exceeds a lane's s: the prefilter
+// drops only candidates every lane would reject.
 template <int D, int SEGW, class Pair>
 __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
                                           uint32_t total, const f3& r_i,
-                                          bool act, float smax,
+                                          bool act, const f3& bmin,
+                                          const f3& bmax, float smax,
                                           const float* __restrict__ r,
                                           const int* __restrict__ imove,
                                           int kind, bool flag_el,
@@ -220,49 +252,73 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             const uint32_t t = seg + cb + lane;
             float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
             bool take = false, is_el = false;
+            uint32_t j = 0;
             if (cb + lane < nseg) {
                 while (t >= L.cum[kl + 1]) kl++;
-                const uint32_t j = L.start[kl] + (t - L.cum[kl]);
+                j = L.start[kl] + (t - L.cum[kl]);
                 const f3 rj = ldv_o<D>(r, j);
                 const int im_j = ldi_o(imove, j);
                 is_el = (im_j == -3);
                 take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
-                st = make_float4(rj.x, rj.y, rj.z, 0.f);
-                L.cand[slot][cb + lane] = j;
+                if (take) {
+                    const float dx =
+                        fmaxf(fmaxf(bmin.x - rj.x, rj.x - bmax.x), 0.f);
+                    const float dy =
+                        fmaxf(fmaxf(bmin.y - rj.y, rj.y - bmax.y), 0.f);
+                    float d2 = dx * dx;
+                    d2 = d2 + dy * dy;
+                    if constexpr (D == 3) {
+                        const float dz =
+                            fmaxf(fmaxf(bmin.z - rj.z, rj.z - bmax.z), 0.f);
+                        d2 = d2 + dz * dz;
+                    }
+                    take = !(d2 > smax);
+                }
+                st = make_float4(rj.x, rj.y, rj.z, is_el ? 1.f : 0.f);
             }
             const uint64_t tm = __ballot(take);
-            const uint64_t em = __ballot(take && is_el);
+            const uint32_t cnt = (uint32_t)__popcll(tm);
             uint32_t lo = 0, hi = 0;
-            if (tm != 0) {
-                L.stage[lane] = st;
+            if (cnt != 0) {
+                // compact the survivors into stage[0 .. cnt), in order
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(tm >> 32),
+                    __builtin_amdgcn_mbcnt_lo((uint32_t)tm, 0u));
+                if (take) {
+                    L.stage[rank] = st;
+                    L.cand[slot][cb + rank] = j;
+                }
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_s_waitcnt(0);
+                const uint32_t c1 = cnt < 32u ? cnt : 32u;
 #pragma unroll 8
-                for (int k = 0; k < 32; k++) {
+                for (uint32_t k = 0; k < c1; k++) {
                     const float4 c = L.stage[k];
                     const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
                     const float s = dot<D>(r_ij, r_ij);
                     lo = (lo << 1) | (!(s > smax) ? 1u : 0u);
                 }
-                if (nseg - cb > 32) {
+                if (c1 < 32u) lo <<= 32u - c1;
+                if (cnt > 32u) {
 #pragma unroll 8
-                    for (int k = 32; k < 64; k++) {
+                    for (uint32_t k = 32; k < cnt; k++) {
                         const float4 c = L.stage[k];
                         const f3 r_ij =
                             sub<D>(f3{ c.x, c.y, c.z }, r_i);
                         const float s = dot<D>(r_ij, r_ij);
                         hi = (hi << 1) | (!(s > smax) ? 1u : 0u);
                     }
+                    if (cnt < 64u) hi <<= 64u - cnt;
                 }
-                lo &= __builtin_bitreverse32((uint32_t)tm);
-                hi &= __builtin_bitreverse32((uint32_t)(tm >> 32));
                 if (!act) lo = hi = 0;
-                if (!kind && em != 0) {
+                if (!kind && __ballot(take && is_el) != 0) {
                     // wall elements in range: flagged, then dropped
-                    const uint32_t el =
-                        __builtin_bitreverse32((uint32_t)em);
-                    const uint32_t eh =
-                        __builtin_bitreverse32((uint32_t)(em >> 32));
+                    uint32_t el = 0, eh = 0;
+                    for (uint32_t k = 0; k < cnt; k++) {
+                        if (L.stage[k].w == 0.f) continue;
+                        if (k < 32u) el |= 0x80000000u >> k;
+                        else eh |= 0x80000000u >> (k - 32u);
+                    }
                     eln = eln || ((lo & el) | (hi & eh)) != 0;
                     lo &= ~el;
                     hi &= ~eh;
@@ -374,8 +430,11 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             const float irho_i = FAST ? __builtin_amdgcn_rcpf(rho_i) : 0.f;
             // ---- fluid neighbours ------------------------------------------
             bool nearw = false;
+            f3 bmin, bmax;
+            wave_box<D>(r_i, act, bmin, bmax);
             ring_walk<D, SEGW>(
-                L, lane, total, r_i, act, smax, r, imove, 0, flag_el, nearw,
+                L, lane, total, r_i, act, bmin, bmax, smax, r, imove, 0,
+                flag_el, nearw,
                 [&](uint32_t j) {
                     if constexpr (FAST) {
                         // OpenCL C default accuracy: reciprocal-based
@@ -455,8 +514,8 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 const uint32_t total_el =
                     neighbour_table<D>(w, c_i, L, lane, AQ_CELL_ELEM);
                 ring_walk<D, SEGW>(
-                    L, lane, total_el, r_i, act && nearw, smax, r, imove, 1,
-                    false, unused, [&](uint32_t j) {
+                    L, lane, total_el, r_i, act && nearw, bmin, bmax, smax,
+                    r, imove, 1, false, unused, [&](uint32_t j) {
                         const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
                         const float s = dot<D>(r_ij, r_ij);
                         const float q = sqrtf(s) / H;
@@ -552,10 +611,11 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 pol.init(i, r_i, st);
             }
             bool unused = false;
-            ring_walk<D, SEGW>(L, lane, total, r_i, act, smax, r, imove,
-                               Pol::KIND, false, unused, [&](uint32_t j) {
-                                   pol.pair(i, j, r_i, st);
-                               });
+            f3 bmin, bmax;
+            wave_box<D>(r_i, act, bmin, bmax);
+            ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax, smax, r,
+                               imove, Pol::KIND, false, unused,
+                               [&](uint32_t j) { pol.pair(i, j, r_i, st); });
             if (act) pol.finish(i, st);
             __builtin_amdgcn_wave_barrier();
         }
