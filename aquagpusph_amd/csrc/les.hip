@@ -10,7 +10,9 @@
 //            symmetric and antisymmetric parts of G and n = S:S + W:W,
 //            S:S/n, W:W/n, tr(S^3)/n^1.5, tr(W^2 S)/n^1.5, tr(W^2 S^2)/n^2,
 //            tr(W^2 S W S^2)/n^3, and h/dr
-//   nu_t     max(c, 0) H^2 |S| or (C_s H)^2 |S|, |S| = sqrt(2 S:S)
+//   nu_t     max(c, c_min) H^2 |S| or (C_s H)^2 |S|, |S| = sqrt(2 S:S)
+//            (c_min = les_cmin, 0 by default; a negative floor lets the
+//            network return energy to the resolved scales, backscatter)
 //   dudt_i  += sum_j F_ij m_j (rho_i nu_i + rho_j nu_j) / (rho_i rho_j)
 //              (u_j - u_i)
 // Pairs are fluid-fluid within the support, walked in the reference order.
@@ -136,12 +138,12 @@ __global__ void __launch_bounds__(BS)
     (void)imove;
 }
 
-// mode 0: nu_t = max(c, 0) H^2 |S|; mode 1: (C_s H)^2 |S|
+// mode 0: nu_t = max(c, cmin) H^2 |S|; mode 1: (C_s H)^2 |S|
 template <int D>
 __global__ void __launch_bounds__(BS)
     les_nut_k(const int* __restrict__ imove, const float* __restrict__ grad_u,
               const float* __restrict__ c, float* __restrict__ nu_t,
-              uint32_t N, float H, int mode, float Cs)
+              uint32_t N, float H, int mode, float Cs, float cmin)
 {
     constexpr int n = MS_<D>::n, SS = MS_<D>::s;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -158,7 +160,7 @@ __global__ void __launch_bounds__(BS)
             s2 += sab * sab;
         }
     const float smag = sqrtf(2.f * s2);
-    const float k = mode == 0 ? fmaxf(c[i], 0.f) : Cs * Cs;
+    const float k = mode == 0 ? fmaxf(c[i], cmin) : Cs * Cs;
     nu_t[i] = k * H * H * smag;
 }
 
@@ -226,10 +228,11 @@ void launch_les_features(int dims, const int32_t* imove, const float* grad_u,
 }
 void launch_les_nut(int dims, const int32_t* imove, const float* grad_u,
                     const float* c, float* nu_t, uint32_t N, float H,
-                    int mode, float Cs, hipStream_t s)
+                    int mode, float Cs, float cmin, hipStream_t s)
 {
     if (!N) return;
-    AQ_LD(dims, les_nut_k, AQ_LG(N), imove, grad_u, c, nu_t, N, H, mode, Cs);
+    AQ_LD(dims, les_nut_k, AQ_LG(N), imove, grad_u, c, nu_t, N, H, mode, Cs,
+          cmin);
 }
 void launch_les_interactions(int dims, const int32_t* imove, const float* r,
                              const float* u, const float* rho, const float* m,

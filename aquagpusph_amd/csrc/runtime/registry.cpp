@@ -122,7 +122,7 @@ void launch_les_gradu(int, const int32_t*, const float*, const float*,
 void launch_les_features(int, const int32_t*, const float*, float*, uint32_t,
                          float, hipStream_t);
 void launch_les_nut(int, const int32_t*, const float*, const float*, float*,
-                    uint32_t, float, int, float, hipStream_t);
+                    uint32_t, float, int, float, float, hipStream_t);
 void launch_les_interactions(int, const int32_t*, const float*, const float*,
                              const float*, const float*, const float*, float*,
                              uint32_t, const uint32_t*, const uint32_t*,
@@ -680,17 +680,18 @@ static const std::map<std::string, KernelSpec>& table()
                 launch_mlp_forward(F(0), S.N, F(1), F(2), s);
             } } },
         { "cfd/LES/Nut.cl:entry",
-          { { "imove", "grad_u", "les_c", "nu_t" },
+          { { "imove", "grad_u", "les_c", "nu_t", "les_cmin" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_les_nut(S.dims, I(0), F(1), F(2), F(3), S.N,
-                               S.kdefs.H, 0, 0.f, s);
+                               S.kdefs.H, 0, 0.f,
+                               (float)S.vars.get_host(*a[4])[0], s);
             } } },
         { "cfd/LES/Nut.cl:smagorinsky",
           { { "imove", "grad_u", "nu_t", "les_cs" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_les_nut(S.dims, I(0), F(1), nullptr, F(2), S.N,
                                S.kdefs.H, 1,
-                               (float)S.vars.get_host(*a[3])[0], s);
+                               (float)S.vars.get_host(*a[3])[0], 0.f, s);
             } } },
         { "cfd/LES/Interactions.cl:entry",
           { { "imove", "r", "u", "rho", "m", "nu_t", "dudt", "icell", "ihoc",

@@ -45,8 +45,8 @@ def strain_rate(grad_u, dims):
     return np.sqrt(2.0 * np.einsum("nab,nab->n", S, S))
 
 
-def eddy_viscosity(imove, grad_u, dims, H, c=None, Cs=None):
-    """max(c, 0) H^2 |S| (network) or (Cs H)^2 |S| (Smagorinsky); 0 off
+def eddy_viscosity(imove, grad_u, dims, H, c=None, Cs=None, cmin=0.0):
+    """max(c, cmin) H^2 |S| (network) or (Cs H)^2 |S| (Smagorinsky); 0 off
     the fluid.  A float32 grad_u is evaluated in float32 with the kernel's
     operation order (S:S summed over a then b, ((k H) H) |S|)."""
     if np.asarray(grad_u).dtype == np.float32:
@@ -59,10 +59,10 @@ def eddy_viscosity(imove, grad_u, dims, H, c=None, Cs=None):
                 sab = f(0.5) * (G[:, a, b] + G[:, b, a])
                 s2 = s2 + sab * sab
         smag = np.sqrt(f(2) * s2)
-        k = (np.maximum(np.asarray(c, f), f(0)) if Cs is None
+        k = (np.maximum(np.asarray(c, f), f(cmin)) if Cs is None
              else np.full(G.shape[0], f(Cs) * f(Cs), f))
         nu = ((k * f(H)) * f(H)) * smag
         return np.where(np.asarray(imove) == 1, nu, f(0))
-    k = np.maximum(np.asarray(c, np.float64), 0.0) if Cs is None else Cs * Cs
+    k = np.maximum(np.asarray(c, np.float64), cmin) if Cs is None else Cs * Cs
     nu = k * H * H * strain_rate(grad_u, dims)
     return np.where(np.asarray(imove) == 1, nu, 0.0)

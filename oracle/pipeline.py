@@ -326,7 +326,9 @@ class OracleSim:
             c, _ = omlp.forward(self.les["params"], x, bf=True)
             f["les_c"][...] = c
             nu = olq.eddy_viscosity(f["imove"], f["grad_u"], self.dims,
-                                    float(self.H), c=f["les_c"])
+                                    float(self.H), c=f["les_c"],
+                                    cmin=float(np.float32(
+                                        self.les.get("cmin", 0.0))))
         else:
             nu = olq.eddy_viscosity(f["imove"], f["grad_u"], self.dims,
                                     float(self.H),

@@ -8,12 +8,14 @@ solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, u_rms = 1).
      smoothing length); the filtered field's SPH gradient on the coarse
      lattice gives the network inputs (the same operator the closure uses),
      the filtered subgrid stress tau = <uu> - <u><u> gives the target
-     coefficient c* = -(tau_d : S) / (2 S:S) / (H^2 |S|), clipped to
-     [0, 2] (interior points only);
+     coefficient c* = -(tau_d : S) / (2 S:S) / (H^2 |S|) (interior points
+     only), clipped to [0, 2] for the eddy-viscosity network and to
+     [c_min, 2] (c_min < 0) for the one allowed to return energy
+     (backscatter, les_cmin = c_min);
   2. training on the GPU: the BF16 network kernels (aq_mlp_grad, aq_mlp_adam)
      on minibatches of 10k samples;
-  3. coarse runs (n x n): plain SPH, the learned closure (cfd/LES.xml) and
-     Smagorinsky (cfd/LES-smagorinsky.xml, C_s = 0.15);
+  3. coarse runs (n x n): plain SPH, the learned closures (cfd/LES.xml)
+     and Smagorinsky (cfd/LES-smagorinsky.xml, C_s = 0.15);
   4. the kinetic energy of every run at the same times, its error against
      the fine run (mean over samples of |E - E_fine| / E_fine(0)), and the
      GPU time of every run.
@@ -51,7 +53,7 @@ MORRIS = ('    <Definitions><Define name="__LAP_FORMULATION__" '
           'value="__LAP_MORRIS__" evaluate="false"/></Definitions>')
 
 
-def _sim(case, extra=(), params=None):
+def _sim(case, extra=(), params=None, cmin=None):
     from aquagpusph_amd.sim import Simulation
     d = tempfile.mkdtemp(prefix="aq_les_")
     xml = casefiles.write_case(case, d, presets=BASE + tuple(extra),
@@ -60,6 +62,8 @@ def _sim(case, extra=(), params=None):
     casefiles.upload(sim, case)
     if params is not None:
         sim.set("les_params", params)
+    if cmin is not None:
+        sim.set("les_cmin", np.array([cmin], np.float32))
     return sim
 
 
@@ -70,10 +74,11 @@ def _ke(sim):
     return float(0.5 * np.sum(m[fl] * np.sum(u[fl] ** 2, axis=1)))
 
 
-def run(case, samples, steps_per_sample, extra=(), params=None, snap=None):
+def run(case, samples, steps_per_sample, extra=(), params=None, snap=None,
+        cmin=None):
     """KE at every sample, GPU seconds of the stepping, and (snap: the
     sample indices) particle snapshots."""
-    sim = _sim(case, extra, params)
+    sim = _sim(case, extra, params, cmin)
     ke = [_ke(sim)]
     gpu_s = 0.0
     snaps = []
@@ -149,7 +154,7 @@ def training_samples(fine_snaps, coarse, hc):
         inner = np.all((xc > 2.5 * hc) & (xc < 1.0 - 2.5 * hc), axis=1) & \
             (smag > 1e-3 * smag.max())
         X.append(olq.features(G, 2, 2.0)[inner])
-        Y.append(np.clip(c[inner], 0.0, 2.0))
+        Y.append(np.clip(c[inner], -2.0, 2.0))
     return np.concatenate(X).astype(np.float32), \
         np.concatenate(Y).astype(np.float32)
 
@@ -194,7 +199,8 @@ def _p_t(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def study(n=128, t_end=1.0, samples=20, snapshots=8, iters=400, log=None):
+def study(n=128, t_end=1.0, samples=20, snapshots=8, iters=400, log=None,
+          cmin=-0.1):
     coarse = cases.decaying_box(n)
     fine = cases.decaying_box(2 * n)
     hc = coarse["params"]["h"]
@@ -207,15 +213,18 @@ def study(n=128, t_end=1.0, samples=20, snapshots=8, iters=400, log=None):
     say("training samples")
     X, Y = training_samples(snaps, coarse, hc)
     say("training on %d samples" % len(Y))
-    P, losses, t_train = train(X, Y, iters=iters)
+    P, losses, t_train = train(X, np.clip(Y, 0.0, 2.0), iters=iters)
+    Pb, losses_b, _ = train(X, np.clip(Y, cmin, 2.0), iters=iters)
+    les = ("resources/Presets/cfd/LES.xml",)
     out = {}
-    for name, extra, params in (
-            ("plain", (), None),
-            ("learned", ("resources/Presets/cfd/LES.xml",), P),
+    for name, extra, params, cm in (
+            ("plain", (), None, None),
+            ("learned", les, P, None),
+            ("learned_backscatter", les, Pb, cmin),
             ("smagorinsky", ("resources/Presets/cfd/LES-smagorinsky.xml",),
-             None)):
+             None, None)):
         say("coarse run: %s" % name)
-        ke, t, _, tc = run(coarse, samples, spc, extra, params)
+        ke, t, _, tc = run(coarse, samples, spc, extra, params, cmin=cm)
         out[name] = dict(ke=ke.tolist(), gpu_s=t, t_end=tc,
                          ke_err=float(np.mean(np.abs(ke - ke_f)) / ke_f[0]))
     out["fine"] = dict(ke=ke_f.tolist(), gpu_s=t_f, t_end=tf)
@@ -226,8 +235,12 @@ def study(n=128, t_end=1.0, samples=20, snapshots=8, iters=400, log=None):
                     "nu = 1e-4, coarse %dx%d, fine %dx%d, t = %.3g s"
                     % (n, n, 2 * n, 2 * n, t_end),
         "samples": samples, "steps_per_sample_coarse": spc,
-        "training": {"samples": int(len(Y)), "target_mean": float(Y.mean()),
-                     "iters": iters, "loss": losses, "gpu_s": t_train},
+        "training": {"samples": int(len(Y)),
+                     "target_mean": float(np.clip(Y, 0.0, 2.0).mean()),
+                     "target_mean_signed": float(np.clip(Y, cmin, 2.0).mean()),
+                     "target_negative_fraction": float((Y < 0).mean()),
+                     "cmin": cmin, "iters": iters, "loss": losses,
+                     "loss_backscatter": losses_b, "gpu_s": t_train},
         "runs": out,
         "time_ratio_fine_over_learned": t_f / out["learned"]["gpu_s"],
     }
