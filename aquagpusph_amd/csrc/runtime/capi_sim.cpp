@@ -1,7 +1,5 @@
 // C ABI of the runtime (include/aquagpusph_amd.h, "simulation" section).
 #include <cstring>
-#include <dlfcn.h>
-#include <filesystem>
 
 #include "sim.hpp"
 
@@ -32,23 +30,6 @@ AQ_API int aq_sim_create(const char* xml_path, int dims, aq_sim** out)
         h->sim->setup();
         *out = h.release();
     });
-}
-
-AQ_API int aq_problem_tools(const char* xml_path, int dims, char* buf,
-                            size_t size, size_t* needed)
-{
-    return guard([&] {
-        AQ_REQUIRE(xml_path && needed, "aq_problem_tools: null argument");
-        AQ_REQUIRE(dims == 2 || dims == 3, "dims must be 2 or 3");
-        Problem P = load_problem(xml_path, dims);
-        std::string out;
-        for (auto& t : P.tools)
-            out += t->get("name") + "\t" + t->get("type") + "\n";
-        *needed = out.size() + 1;
-        if (!buf) return;
-        AQ_REQUIRE(size >= out.size() + 1, "aq_problem_tools: buffer too small");
-        memcpy(buf, out.c_str(), out.size() + 1);
-    }, false);
 }
 
 AQ_API void aq_sim_destroy(aq_sim* s)
@@ -285,16 +266,7 @@ AQ_API int aq_xml_resolve(const char* xml_path, int dims, char* out,
     return guard([&] {
         AQ_REQUIRE(xml_path, "aq_xml_resolve: null path");
         AQ_REQUIRE(dims == 2 || dims == 3, "dims must be 2 or 3");
-        Problem P;
-        P.dims = dims;
-        {
-            Dl_info info;
-            if (dladdr((void*)&aq_xml_resolve, &info) && info.dli_fname)
-                P.base_path = std::filesystem::path(info.dli_fname)
-                                  .parent_path()
-                                  .string();
-        }
-        parse_problem(xml_path, P, "");
+        Problem P = load_problem(xml_path, dims);
         std::string txt;
         for (auto& t : P.tools) {
             std::string key;

@@ -30,19 +30,19 @@ def _base(type_name):
 
 def problem_tools(xml_path, dims=3):
     """The (name, type) tool list a case file resolves to, in execution
-    order, parsed on the host (aq_problem_tools: no GPU needed)."""
+    order, parsed on the host (aq_xml_resolve: no GPU needed)."""
     L = _lib.lib()
-    L.aq_problem_tools.argtypes = [ctypes.c_char_p, ctypes.c_int,
-                                   ctypes.c_char_p, ctypes.c_size_t,
-                                   ctypes.POINTER(ctypes.c_size_t)]
+    L.aq_xml_resolve.argtypes = [ctypes.c_char_p, ctypes.c_int,
+                                 ctypes.c_char_p, ctypes.c_size_t,
+                                 ctypes.POINTER(ctypes.c_size_t)]
     path = os.fsencode(os.path.abspath(xml_path))
     need = ctypes.c_size_t(0)
-    _lib.check(L.aq_problem_tools(path, dims, None, 0, ctypes.byref(need)))
+    _lib.check(L.aq_xml_resolve(path, dims, None, 0, ctypes.byref(need)))
     buf = ctypes.create_string_buffer(need.value)
-    _lib.check(L.aq_problem_tools(path, dims, buf, need.value,
-                                  ctypes.byref(need)))
-    return [tuple(line.split("\t")) for line in
-            buf.value.decode().splitlines()]
+    _lib.check(L.aq_xml_resolve(path, dims, buf, need.value,
+                                ctypes.byref(need)))
+    return [tuple(line.split("\t")[:2]) for line in
+            buf.value.decode().splitlines() if not line.startswith("#")]
 
 
 class Simulation:

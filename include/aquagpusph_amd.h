@@ -351,14 +351,6 @@ AQ_API int aq_sim_set(aq_sim* sim, const char* name, const void* host,
                       size_t bytes);
 /* Device address of an array variable or of a scalar's slot. */
 AQ_API void* aq_sim_device_ptr(aq_sim* sim, const char* name);
-/* Host only, no GPU: parse the case and write the resolved tool list, one
- * "name<TAB>type" line per tool in execution order, into buf (the list
- * State.cpp's include/insert rules give, which the reference's CalcServer
- * setup walks).  *needed gets the bytes required including the final NUL;
- * buf may be null to query it.  No counterpart in the reference API: it is
- * for checks and tooling on machines without a GPU. */
-AQ_API int aq_problem_tools(const char* xml_path, int dims, char* buf,
-                            size_t size, size_t* needed);
 AQ_API int aq_sim_n_tools(aq_sim* sim);
 AQ_API const char* aq_sim_tool_name(aq_sim* sim, int i);
 AQ_API const char* aq_sim_tool_type(aq_sim* sim, int i);

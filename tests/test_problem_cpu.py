@@ -1,5 +1,5 @@
 """The tool list that case files resolve to, parsed on the host
-(aq_problem_tools), against the include/insert rules of the reference's
+(aq_xml_resolve), against the include/insert rules of the reference's
 State.cpp: prefixed includes, insert before a glob (every match), try_insert
 that finds nothing, ifdef.  The order checked here is the one the oracle
 pipeline replays."""
