@@ -133,6 +133,50 @@ __device__ __forceinline__ void faxpy(f3& acc, float s, const f3& v)
     if constexpr (D == 3) acc.z = fmaf(s, v.z, acc.z);
 }
 
+// ---- correctly rounded division and square root without range scaling ----
+// hipcc expands an IEEE float a / b (b > 0) to
+//   y  = fma(fma(-b, rcp(b), 1), rcp(b), rcp(b))       reciprocal, one Newton step
+//   q0 = a y;  q1 = fma(a - b q0, y, q0);  q2 = fma(a - b q1, y, q1)
+// wrapped in v_div_scale (operands rescaled by 2^+-64 near the exponent
+// limits), v_div_fmas (the last fma, rescaled) and v_div_fixup (zeros,
+// infinities, NaN, overflow, and the quotient's sign).  For b in
+// [2^-100, 2^100] and a == 0 or a / b in [2^-120, 2^90] with
+// |a| >= 2^-100, v_div_scale leaves both operands as they are, v_div_fmas
+// is a plain fma and v_div_fixup returns the quotient unchanged, so div_nr
+// below, the same instructions without the three range steps, is a / b bit
+// for bit.  The remainders are negated (b q - a, exact under the fma like
+// a - b q), which gives a zero numerator's sign to the quotient as
+// v_div_fixup does.  Callers check the operand ranges before taking this
+// path (the fused fluid walk bounds them once per launch from the fields'
+// ranges, plus two per-pair tests).
+struct Rcp {
+    float b, y;
+};
+__device__ __forceinline__ Rcp rcp_nr(float b)
+{
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    return { b, fmaf(fmaf(-b, y0, 1.f), y0, y0) };
+}
+__device__ __forceinline__ float div_nr(float a, const Rcp& d)
+{
+    const float q0 = a * d.y;
+    const float q1 = fmaf(-fmaf(d.b, q0, -a), d.y, q0);
+    return fmaf(-fmaf(d.b, q1, -a), d.y, q1);
+}
+// sqrtf(s) for s == 0 or s >= 2^-96: hipcc's sequence (hardware estimate,
+// then one ulp down or up by the signs of the fma residuals) without the
+// 2^32 prescaling of s < 2^-96 and the zero / infinity pass-through; zero
+// goes through the residual tests unchanged.
+__device__ __forceinline__ float sqrt_nr(float s)
+{
+    const float y = __builtin_amdgcn_sqrtf(s);
+    const float yd = __uint_as_float(__float_as_uint(y) - 1u);
+    const float yu = __uint_as_float(__float_as_uint(y) + 1u);
+    const float rd = fmaf(-yd, y, s), ru = fmaf(-yu, y, s);
+    const float z = (rd <= 0.f) ? yd : y;
+    return (ru > 0.f) ? yu : z;
+}
+
 template <int D> __device__ __forceinline__ float dot(const f3& a, const f3& b)
 {
     float s = a.x * b.x;

@@ -414,6 +414,8 @@ void Sim::plan_fusion()
     if (const char* fm = getenv("AQ_MATH"))
         fusion.math = strcmp(fm, "relaxed") == 0 ? 1 : 0;
     if (problem.is_defined("__KAHAN_SUMS__")) fusion.math = 2;
+    if (const char* dv = getenv("AQ_DIV"))
+        fusion.fast_div = strcmp(dv, "ieee") != 0;
     fusion.active = true;
 }
 
@@ -446,7 +448,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
                              int segw, int flagged, int math,
-                             unsigned ranges, hipStream_t s);
+                             unsigned ranges, int fast_div, hipStream_t s);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -530,7 +532,8 @@ void Sim::run_fused(hipStream_t s)
             (const uint32_t*)P(f.n_cells), clist.cell_end, clist.heads,
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
-            f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges, s);
+            f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges,
+            f.fast_div ? 1 : 0, s);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

@@ -241,6 +241,10 @@ struct FusionPlan {
     // hardware square root, contracted multiply-adds); 2 IEEE terms with
     // compensated momentum and density sums (the __KAHAN_SUMS__ definition)
     int math = 0;
+    // IEEE terms (math 0 and 2) through the division and square root
+    // sequences without range scaling where the fields' ranges allow them
+    // (bit-identical); AQ_DIV=ieee keeps the compiler's sequences
+    bool fast_div = true;
     // profiling: events around the walk launch while `timing` is set
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -26,6 +26,7 @@
 // accepted the current candidate.
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "aq_device.hpp"
 #include "aq_internal.hpp"
@@ -362,6 +363,156 @@ __device__ __forceinline__ void naxpy(f3& acc, f3& c, float s, const f3& v)
     if (D == 3) nsum(acc.z, c.z, s * v.z);
 }
 
+// ---------------------------------------------------------------------------
+// Range record of the fluid fields, for the division and square root path
+// without range scaling (div_nr, sqrt_nr): 7 words RREC words after the
+// non-empty cell count, bit patterns of non-negative floats compared as
+// integers (a negative rho or m, an infinity or a NaN lands above every
+// finite positive pattern): min |p| over p != 0, max |p|, min and max rho,
+// min and max m, max |u| component over the fluid particles.
+// ---------------------------------------------------------------------------
+constexpr uint32_t RREC = 384;
+constexpr uint32_t NREC = 7;
+__constant__ const uint32_t rec_init[NREC] = { 0x7F800000u, 0u, 0xFFFFFFFFu,
+                                               0u, 0xFFFFFFFFu, 0u, 0u };
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        x = min(x, (uint32_t)__shfl_xor((int)x, o));
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+        x = max(x, (uint32_t)__shfl_xor((int)x, o));
+    return x;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+    fluid_ranges_k(const int* __restrict__ imove, const float* __restrict__ p,
+                   const float* __restrict__ rho, const float* __restrict__ m,
+                   const float* __restrict__ u, uint32_t N,
+                   uint32_t* __restrict__ rec)
+{
+    uint32_t v[NREC];
+#pragma unroll
+    for (uint32_t k = 0; k < NREC; k++) v[k] = rec_init[k];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += gridDim.x * blockDim.x) {
+        if (imove[i] != 1) continue;
+        const uint32_t pa = __float_as_uint(p[i]) & 0x7FFFFFFFu;
+        if (pa) v[0] = min(v[0], pa);
+        v[1] = max(v[1], pa);
+        const uint32_t rb = __float_as_uint(rho[i]), mb = __float_as_uint(m[i]);
+        v[2] = min(v[2], rb);
+        v[3] = max(v[3], rb);
+        v[4] = min(v[4], mb);
+        v[5] = max(v[5], mb);
+        const f3 ui = ldv<D>(u, i);
+        uint32_t ua = max(__float_as_uint(ui.x) & 0x7FFFFFFFu,
+                          __float_as_uint(ui.y) & 0x7FFFFFFFu);
+        if constexpr (D == 3) ua = max(ua, __float_as_uint(ui.z) & 0x7FFFFFFFu);
+        v[6] = max(v[6], ua);
+    }
+    __shared__ uint32_t part[4][NREC];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t k = 0; k < NREC; k++) {
+        const bool is_min = (k == 0 || k == 2 || k == 4);
+        const uint32_t x = is_min ? wave_min(v[k]) : wave_max(v[k]);
+        if (lane == 0) part[wv][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NREC) {
+        const uint32_t k = threadIdx.x;
+        const bool is_min = (k == 0 || k == 2 || k == 4);
+        uint32_t x = part[0][k];
+        for (int w2 = 1; w2 < 4; w2++)
+            x = is_min ? min(x, part[w2][k]) : max(x, part[w2][k]);
+        if (is_min) atomicMin(rec + k, x);
+        else atomicMax(rec + k, x);
+    }
+}
+
+// The threshold T (a power of two, 0 when the path is not usable) such
+// that every division of the fused walk's pair terms meets div_nr's
+// conditions (b in [2^-100, 2^100]; a == 0 or |a| >= 2^-100 and a / b in
+// [2^-120, 2^90]) for all fluid pairs with s == 0 or s >= 2^-96 and
+// udr == 0 or |udr| >= T (both tested per pair), from the range record.
+// Each operand's bounds follow its expression with a factor 2 of margin
+// for the roundings:
+//   sqrt(s) / H          sqrt(s) in {0} U [2^-48, sqrt(smax)]
+//   f_ij = kF(q) CONF m_j, kF = wcon a^3, a = 2 - q: a == 0 or
+//                        |a| in [2^-23, 2] (a nonzero difference of 2 and
+//                        a float in [0, 4) is a multiple of 2^-23)
+//   (p_i + p_j) / (rho_i rho_j)   a nonzero sum of two floats of magnitude
+//                                 >= Pmin is >= Pmin 2^-24
+//   f_ij CLEARY udr / (r2 rho_i rho_j), r2 = (q^2 + 0.01) H^2   (Monaghan)
+//   2 f_ij / (rho_i rho_j)                                     (Morris)
+//   udr f_ij rho_i / rho_j,  f_ij / rho_j,  |udr| <= 3 (2 Umax) sqrt(smax)
+// A zero numerator (f_ij == 0 at q == 2, udr == 0, p_i + p_j == 0) needs
+// no bound.
+template <int D, int LAPF, bool LAPP>
+__device__ float fast_threshold(const uint32_t* __restrict__ rec, float H,
+                                float CONF, float smax)
+{
+    uint32_t b[NREC];
+#pragma unroll
+    for (uint32_t k = 0; k < NREC; k++) b[k] = uni(rec[k]);
+    if (b[1] >= 0x7F800000u || b[3] >= 0x7F800000u ||
+        b[5] >= 0x7F800000u || b[6] >= 0x7F800000u)
+        return 0.f;  // an infinity, a NaN, a negative rho or m
+    if (b[2] > b[3] || b[4] > b[5]) return 0.f;  // no fluid particle
+    const double Pmin = __uint_as_float(b[0]), Pmax = __uint_as_float(b[1]);
+    const double Rmin = __uint_as_float(b[2]), Rmax = __uint_as_float(b[3]);
+    const double Mmin = __uint_as_float(b[4]), Mmax = __uint_as_float(b[5]);
+    const double Umax = __uint_as_float(b[6]);
+    const double h = H, conf = CONF;
+    if (!(h > 0.0) || !(conf > 0.0) || !(smax > 0.f) || !(Mmin > 0.0) ||
+        !(Rmin > 0.0))
+        return 0.f;
+    auto win = [](double nlo, double nhi, double dlo, double dhi) {
+        return dlo >= 0x1p-100 && dhi <= 0x1p100 && nlo >= 0x1p-100 &&
+               nhi <= 0x1p100 && nlo / dhi >= 0x1p-120 &&
+               nhi / dlo <= 0x1p90;
+    };
+    constexpr double CLEARY = (D == 2) ? 8.0 : 10.0;
+    const double wcon = ((D == 2) ? 1.09375 : 0.8203125) * 0.318309886;
+    const double rs = 2.0 * sqrt((double)smax);
+    const double flo = wcon * 0x1p-69 * conf * Mmin * 0.5;
+    const double fhi = wcon * 8.0 * conf * Mmax * 2.0;
+    const double ud = 12.0 * Umax * rs;
+    const double rrlo = Rmin * Rmin * 0.5, rrhi = Rmax * Rmax * 2.0;
+    // q <= sqrt(smax) / H, about 2: r2 <= 4.2 H^2
+    const double slo = 0.01 * h * h * 0.5 * rrlo, shi = 4.2 * h * h * 2.0 * rrhi;
+    // smallest T for the two terms with udr in the numerator
+    double need = fmax(0x1p-100 / (flo * Rmin * 0.5),
+                       0x1p-120 * Rmax * 2.0 / (flo * Rmin * 0.5));
+    if constexpr (LAPF == AQ_LAP_MONAGHAN)
+        need = fmax(need, fmax(0x1p-100 / (flo * CLEARY * 0.5),
+                               0x1p-120 * shi / (flo * CLEARY * 0.5)));
+    if (!(need <= 0x1p-20)) return 0.f;  // not worth the per-pair fall-backs
+    int e = 0;
+    frexp(need, &e);
+    const double T = ldexp(1.0, e);  // a power of two >= need
+    bool ok = win(0x1p-49, rs, h, h);
+    if (b[0] != 0x7F800000u)  // some p != 0
+        ok = ok && win(Pmin * 0x1p-25, 4.0 * Pmax, rrlo, rrhi);
+    if constexpr (LAPF == AQ_LAP_MONAGHAN)
+        ok = ok && win(flo * CLEARY * T * 0.5, fhi * CLEARY * ud * 2.0, slo,
+                       shi);
+    else
+        ok = ok && win(flo, 4.0 * fhi, rrlo, rrhi);
+    ok = ok && win(T * flo * Rmin * 0.5, ud * fhi * Rmax * 2.0, Rmin * 0.5,
+                   Rmax * 2.0);
+    if (LAPP) ok = ok && win(flo, fhi, Rmin * 0.5, Rmax * 2.0);
+    return ok ? (float)T : 0.f;
+}
+
 template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, int MATH>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     fluid_pass_cells_k(const int* __restrict__ imove,
@@ -377,7 +528,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                        float* __restrict__ lap_u, float* __restrict__ div_u,
                        float* __restrict__ lap_p, uint8_t* __restrict__ near,
                        WalkArgs w, float H, float CONW, float CONF, float DIMS,
-                       float smax, float iH)
+                       float smax, float iH, const uint32_t* __restrict__ rec)
 {
     constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
     __shared__ WaveLds<SEGW> lds[WC_WAVES];
@@ -390,6 +541,13 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     // wall elements in range are needed for the boundary-integral Shepard
     // sum and for the near-wall flags
     const bool flag_el = (SHEP == 2) || (near != nullptr);
+    // divisions and square roots without range scaling (div_nr, sqrt_nr)
+    // when the range record allows them; IEEE sequences otherwise
+    float fdT = 0.f;
+    if constexpr (MATH != 1)
+        if (rec) fdT = fast_threshold<D, LAPF, LAPP>(rec, H, CONF, smax);
+    const bool fastdiv = fdT > 0.f;
+    const Rcp RH = rcp_nr(H);
 
     uint32_t range = blockIdx.x % w.ranges;
     while (true) {
@@ -432,10 +590,80 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             bool nearw = false;
             f3 bmin, bmax;
             wave_box<D>(r_i, act, bmin, bmax);
-            ring_walk<D, SEGW>(
-                L, lane, total, r_i, act, bmin, bmax, smax, r, imove, 0,
-                flag_el, nearw,
-                [&](uint32_t j) {
+            // pair terms; FD selects div_nr for the divisions
+            auto terms = [&](auto fd, const f3& r_ij, float q, float m_j,
+                             float rho_j, float p_j, const f3& u_ij,
+                             float udr) {
+                constexpr bool FD = decltype(fd)::value;
+                if (SHEP == 1) sh += kernelW<D>(q) * CONW * m_j / rho_j;
+                // The reference skips j == i here.  For i itself r_ij,
+                // u_ij and p_j - p_i are exact zeros, so every term below
+                // is +-0 and leaves the accumulators unchanged (they are
+                // never -0: they start at +0 and an exact cancellation
+                // rounds to +0); with finite fields the sums are
+                // bit-identical without the test.
+                const float fk = kernelF<D>(q) * CONF;
+                const float f_ij = fk * m_j;
+                float sg, sl, td, tl = 0.f;
+                if constexpr (FD) {
+                    const Rcp Rr = rcp_nr(rho_i * rho_j);
+                    const Rcp Rj = rcp_nr(rho_j);
+                    sg = div_nr(p_i + p_j, Rr) * f_ij;
+                    if constexpr (LAPF == AQ_LAP_MONAGHAN) {
+                        const float r2 = (q * q + 0.01f) * H * H;
+                        sl = div_nr(f_ij * CLEARY * udr,
+                                    rcp_nr(r2 * rho_i * rho_j));
+                    } else {
+                        sl = div_nr(f_ij * 2.f, Rr);
+                    }
+                    td = div_nr(udr * f_ij * rho_i, Rj);
+                    if (LAPP) tl = (p_j - p_i) * div_nr(f_ij, Rj);
+                } else {
+                    sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
+                    if constexpr (LAPF == AQ_LAP_MONAGHAN) {
+                        const float r2 = (q * q + 0.01f) * H * H;
+                        sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
+                    } else {
+                        sl = f_ij * 2.f / (rho_i * rho_j);
+                    }
+                    td = udr * f_ij * rho_i / rho_j;
+                    if (LAPP) tl = (p_j - p_i) * (f_ij / rho_j);
+                }
+                if constexpr (MATH == 2) {
+                    naxpy<D>(gp, cgp, sg, r_ij);
+                    naxpy<D>(lu, clu, sl,
+                             LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
+                    nsum(dv, cdv, td);
+                    if (LAPP) nsum(lp, clp, tl);
+                } else {
+                    axpy<D>(gp, sg, r_ij);
+                    axpy<D>(lu, sl, LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
+                    dv += td;
+                    if (LAPP) lp += tl;
+                }
+            };
+            // div_nr / sqrt_nr where the per-pair bounds hold (s == 0 or
+            // s >= 2^-96, udr == 0 or |udr| >= T), the IEEE sequences for
+            // the other pairs
+            auto pair_fd = [&](uint32_t j) {
+                const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+                const float s = dot<D>(r_ij, r_ij);
+                const float qf = div_nr(sqrt_nr(s), RH);
+                const float m_j = ldf_o(m, j);
+                const float rho_j = ldf_o(rho, j);
+                const float p_j = ldf_o(p, j);
+                const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
+                const float udr = dot<D>(u_ij, r_ij);
+                const bool bad = ((s < 0x1p-96f) & (s != 0.f)) |
+                                 ((udr != 0.f) & (fabsf(udr) < fdT));
+                if (!bad)
+                    terms(std::true_type{}, r_ij, qf, m_j, rho_j, p_j, u_ij,
+                          udr);
+                else
+                    terms(std::false_type{}, r_ij, sqrtf(s) / H, m_j, rho_j,
+                          p_j, u_ij, udr);
+            };
+            auto pair_ieee = [&](uint32_t j) {
                     if constexpr (FAST) {
                         // OpenCL C default accuracy: reciprocal-based
                         // division, hardware square root, contracted
@@ -472,41 +700,26 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     const float q = sqrtf(s) / H;
                     const float m_j = ldf_o(m, j);
                     const float rho_j = ldf_o(rho, j);
-                    if (SHEP == 1) sh += kernelW<D>(q) * CONW * m_j / rho_j;
-                    // The reference skips j == i here.  For i itself r_ij,
-                    // u_ij and p_j - p_i are exact zeros, so every term
-                    // below is +-0 and leaves the accumulators unchanged
-                    // (they are never -0: they start at +0 and an exact
-                    // cancellation rounds to +0); with finite fields the
-                    // sums are bit-identical without the test.
                     const float p_j = ldf_o(p, j);
                     const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
                     const float udr = dot<D>(u_ij, r_ij);
-                    const float fk = kernelF<D>(q) * CONF;
-                    const float f_ij = fk * m_j;
-                    const float sg = (p_i + p_j) / (rho_i * rho_j) * f_ij;
-                    float sl;
-                    if constexpr (LAPF == AQ_LAP_MONAGHAN) {
-                        const float r2 = (q * q + 0.01f) * H * H;
-                        sl = f_ij * CLEARY * udr / (r2 * rho_i * rho_j);
-                    } else {
-                        sl = f_ij * 2.f / (rho_i * rho_j);
-                    }
-                    const float td = udr * f_ij * rho_i / rho_j;
-                    if constexpr (MATH == 2) {
-                        naxpy<D>(gp, cgp, sg, r_ij);
-                        naxpy<D>(lu, clu, sl,
-                                 LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
-                        nsum(dv, cdv, td);
-                        if (LAPP) nsum(lp, clp, (p_j - p_i) * (f_ij / rho_j));
-                    } else {
-                        axpy<D>(gp, sg, r_ij);
-                        axpy<D>(lu, sl,
-                                LAPF == AQ_LAP_MONAGHAN ? r_ij : u_ij);
-                        dv += td;
-                        if (LAPP) lp += (p_j - p_i) * (f_ij / rho_j);
-                    }
-                });
+                    terms(std::false_type{}, r_ij, q, m_j, rho_j, p_j, u_ij,
+                          udr);
+            };
+            if constexpr (MATH != 1) {
+                if (fastdiv) {
+                    ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
+                                       smax, r, imove, 0, flag_el, nearw,
+                                       pair_fd);
+                } else {
+                    ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
+                                       smax, r, imove, 0, flag_el, nearw,
+                                       pair_ieee);
+                }
+            } else {
+                ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax, smax,
+                                   r, imove, 0, flag_el, nearw, pair_ieee);
+            }
             // ---- wall elements (boundary-integral Shepard) ------------------
             if (SHEP == 2 && __ballot(nearw) != 0) {
                 bool self_added = false;
@@ -814,9 +1027,10 @@ template <int D> struct PolBiInteractions {
 };
 
 // Clears the NRANGE range counters.
-__global__ void zero_u32_k(uint32_t* p)
+__global__ void zero_u32_k(uint32_t* p, uint32_t* rec)
 {
     if (threadIdx.x < NRANGE) p[threadIdx.x * RSTRIDE] = 0u;
+    if (rec && threadIdx.x < NREC) rec[threadIdx.x] = rec_init[threadIdx.x];
 }
 
 // Cell hand-out ranges: NRANGE (one per XCD) unless AQ_WALK_RANGES=1.
@@ -865,13 +1079,26 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
                              int flagged, int math, unsigned ranges,
-                             hipStream_t s)
+                             int fast_div, hipStream_t s)
 {
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     // the hand-out counters live after the non-empty cell count
     uint32_t* next = const_cast<uint32_t*>(n_heads) + RSTRIDE;
-    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
+    // the range record (relaxed arithmetic has no use for it)
+    uint32_t* rec =
+        (fast_div && math != 1) ? const_cast<uint32_t*>(n_heads) + RREC
+                                : nullptr;
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next, rec);
+    if (rec && N) {
+        const unsigned nb = std::min(1024u, (N + 255u) / 256u);
+        if (dims == 2)
+            hipLaunchKernelGGL(fluid_ranges_k<2>, dim3(nb), dim3(256), 0, s,
+                               imove, p, rho, m, u, N, rec);
+        else
+            hipLaunchKernelGGL(fluid_ranges_k<3>, dim3(nb), dim3(256), 0, s,
+                               imove, p, rho, m, u, N, rec);
+    }
     WalkArgs w{ icell,  ihoc,   n_cells, cell_end, heads,
                 n_heads, next, status, N,       (uint32_t)flagged,
                 ranges ? std::min<unsigned>(ranges, NRANGE) : walk_ranges() };
@@ -885,7 +1112,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \
                            u, rho, m, p, normal, tangent, binormal, shepard,   \
                            grad_p, lap_u, div_u, lap_p, near, w, d.H, d.CONW,  \
-                           d.CONF, d.DIMS, smax, 1.f / d.H);                   \
+                           d.CONF, d.DIMS, smax, 1.f / d.H, rec);              \
     } while (0)
 #define AQ_FC(DD, SH, LP, SW)                                                  \
     do {                                                                       \
@@ -920,7 +1147,8 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
 {
     AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     uint32_t* next = const_cast<uint32_t*>(c.n_heads) + RSTRIDE;
-    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next);
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next,
+                       (uint32_t*)nullptr);
     WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
                 c.n_heads, next,     c.status,  c.N,        c.flagged,
                 walk_ranges() };

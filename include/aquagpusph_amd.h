@@ -409,6 +409,14 @@ AQ_API int aq_sim_walk_ms(aq_sim* sim, double* ms);
  * and "#N" lines.  *needed receives the buffer size required. */
 AQ_API int aq_xml_resolve(const char* xml_path, int dims, char* out,
                           size_t len, size_t* needed);
+/* Hardware check of the fused walk's division and square root sequences
+ * without range scaling (no counterpart in the reference, which leaves
+ * division to the OpenCL compiler): n random operand pairs over the whole
+ * window where the walk uses them are divided (and square-rooted) both ways
+ * on the current device; mismatches[0] / [1] receive the number of
+ * quotients / square roots whose bit patterns differ from IEEE a / b and
+ * sqrtf (0 expected).  Synchronous. */
+AQ_API int aq_check_nr_arith(uint64_t n, uint64_t seed, uint64_t* mismatches);
 /* Registry keys ("cfd/Interactions.cl:entry", ...) of the HIP kernels. */
 AQ_API int aq_sim_n_registered_kernels(void);
 AQ_API const char* aq_sim_registered_kernel(int i);

@@ -1,0 +1,73 @@
+"""The fused walk's division and square root sequences without range
+scaling (div_nr, sqrt_nr): equal bit for bit to the compiler's IEEE a / b
+and sqrtf over their whole operand window on the hardware
+(aq_check_nr_arith), and the walk that uses them equal bit for bit to the
+walk on the IEEE sequences (AQ_DIV=ieee) and to the separate per-particle
+kernels, including the first step from rest (zero numerators of both
+signs), lattice neighbours at q == 2 (zero and near-zero kernel
+derivatives) and a random cloud."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from aquagpusph_amd import _lib, casefiles, cases
+from aquagpusph_amd.sim import Simulation
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("shepard", "grad_p", "lap_u", "div_u", "lap_p", "r", "u", "rho",
+          "p")
+
+
+def test_nr_sequences_match_ieee(cuda):
+    L = _lib.lib()
+    L.aq_check_nr_arith.argtypes = [ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_uint64)]
+    for seed in (1, 12345):
+        out = (ctypes.c_uint64 * 2)()
+        _lib.check(L.aq_check_nr_arith(1 << 26, seed, out))
+        assert (out[0], out[1]) == (0, 0), (seed, out[0], out[1])
+
+
+def _pair(xml, dims, monkeypatch, steps, **env):
+    sims = []
+    for div in ("nr", "ieee"):
+        monkeypatch.setenv("AQ_DIV", div)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        sims.append(Simulation(xml, dims))
+    for _ in range(steps):
+        for s in sims:
+            s.step(2, graph=True)
+        for f in FIELDS:
+            np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f),
+                                          err_msg=f)
+    return sims
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_walk_nr_equals_ieee_dam_break(cuda, tmp_path, dims, monkeypatch):
+    case = cases.dam_break(dims, 0.3 / (20 if dims == 2 else 6))
+    xml = casefiles.write_case(case, str(tmp_path))
+    _pair(xml, dims, monkeypatch, 3)
+
+
+def test_walk_nr_equals_separate_kernels_dense(cuda, tmp_path, monkeypatch):
+    from test_runtime_gpu import _dense_case
+    case = _dense_case(3, 20000, 5)
+    presets = tuple(p for p in casefiles.PBI_PRESETS
+                    if not p.endswith("BI.xml"))
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets)
+    monkeypatch.setenv("AQ_NO_FUSION", "1")
+    monkeypatch.setenv("AQ_WALK", "thread")
+    a = Simulation(xml, 3)
+    monkeypatch.setenv("AQ_NO_FUSION", "0")
+    monkeypatch.setenv("AQ_WALK", "cells")
+    monkeypatch.setenv("AQ_DIV", "nr")
+    b = Simulation(xml, 3)
+    for _ in range(2):
+        a.step(2)
+        b.step(2)
+        for f in FIELDS:
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
