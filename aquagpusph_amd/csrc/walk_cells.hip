@@ -365,13 +365,17 @@ __device__ __forceinline__ void naxpy(f3& acc, f3& c, float s, const f3& v)
 
 // ---------------------------------------------------------------------------
 // Range record of the fluid fields, for the division and square root path
-// without range scaling (div_nr, sqrt_nr): 7 words RREC words after the
-// non-empty cell count, bit patterns of non-negative floats compared as
+// without range scaling (div_nr, sqrt_nr): 7 words from RREC words after
+// the non-empty cell count on, RSTRIDE words apart (one cache line each, so
+// that the workgroups' atomics on different words proceed in parallel), bit
+// patterns of non-negative floats compared as
 // integers (a negative rho or m, an infinity or a NaN lands above every
 // finite positive pattern): min |p| over p != 0, max |p|, min and max rho,
 // min and max m, max |u| component over the fluid particles.
 // ---------------------------------------------------------------------------
-constexpr uint32_t RREC = 384;
+constexpr uint32_t RREC = 288;
+// the cell list's count buffer holds 512 words (runtime/tools.cpp)
+static_assert(RREC + 6 * RSTRIDE < 512, "range record beyond the count buffer");
 constexpr uint32_t NREC = 7;
 __constant__ const uint32_t rec_init[NREC] = { 0x7F800000u, 0u, 0xFFFFFFFFu,
                                                0u, 0xFFFFFFFFu, 0u, 0u };
@@ -401,22 +405,42 @@ __global__ void __launch_bounds__(256)
     uint32_t v[NREC];
 #pragma unroll
     for (uint32_t k = 0; k < NREC; k++) v[k] = rec_init[k];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += gridDim.x * blockDim.x) {
-        if (imove[i] != 1) continue;
-        const uint32_t pa = __float_as_uint(p[i]) & 0x7FFFFFFFu;
-        if (pa) v[0] = min(v[0], pa);
-        v[1] = max(v[1], pa);
-        const uint32_t rb = __float_as_uint(rho[i]), mb = __float_as_uint(m[i]);
-        v[2] = min(v[2], rb);
-        v[3] = max(v[3], rb);
-        v[4] = min(v[4], mb);
-        v[5] = max(v[5], mb);
-        const f3 ui = ldv<D>(u, i);
-        uint32_t ua = max(__float_as_uint(ui.x) & 0x7FFFFFFFu,
-                          __float_as_uint(ui.y) & 0x7FFFFFFFu);
-        if constexpr (D == 3) ua = max(ua, __float_as_uint(ui.z) & 0x7FFFFFFFu);
-        v[6] = max(v[6], ua);
+    // four particles per thread and pass, every load issued before the
+    // fluid tests
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < N;
+         i0 += 4 * stride) {
+        int im[4];
+        uint32_t pa[4], rb[4], mb[4];
+        f3 uu[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const uint32_t i = i0 + t * stride;
+            im[t] = 0;
+            if (i < N) {
+                im[t] = imove[i];
+                pa[t] = __float_as_uint(p[i]) & 0x7FFFFFFFu;
+                rb[t] = __float_as_uint(rho[i]);
+                mb[t] = __float_as_uint(m[i]);
+                uu[t] = ldv<D>(u, i);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (im[t] != 1) continue;
+            const uint32_t pa_ = pa[t], rb_ = rb[t], mb_ = mb[t];
+            const f3 ui = uu[t];
+            if (pa_) v[0] = min(v[0], pa_);
+            v[1] = max(v[1], pa_);
+            v[2] = min(v[2], rb_);
+            v[3] = max(v[3], rb_);
+            v[4] = min(v[4], mb_);
+            v[5] = max(v[5], mb_);
+            uint32_t ua = max(__float_as_uint(ui.x) & 0x7FFFFFFFu,
+                              __float_as_uint(ui.y) & 0x7FFFFFFFu);
+            if constexpr (D == 3) ua = max(ua, __float_as_uint(ui.z) & 0x7FFFFFFFu);
+            v[6] = max(v[6], ua);
+        }
     }
     __shared__ uint32_t part[4][NREC];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -433,8 +457,8 @@ __global__ void __launch_bounds__(256)
         uint32_t x = part[0][k];
         for (int w2 = 1; w2 < 4; w2++)
             x = is_min ? min(x, part[w2][k]) : max(x, part[w2][k]);
-        if (is_min) atomicMin(rec + k, x);
-        else atomicMax(rec + k, x);
+        if (is_min) atomicMin(rec + k * RSTRIDE, x);
+        else atomicMax(rec + k * RSTRIDE, x);
     }
 }
 
@@ -462,7 +486,7 @@ __device__ float fast_threshold(const uint32_t* __restrict__ rec, float H,
 {
     uint32_t b[NREC];
 #pragma unroll
-    for (uint32_t k = 0; k < NREC; k++) b[k] = uni(rec[k]);
+    for (uint32_t k = 0; k < NREC; k++) b[k] = uni(rec[k * RSTRIDE]);
     if (b[1] >= 0x7F800000u || b[3] >= 0x7F800000u ||
         b[5] >= 0x7F800000u || b[6] >= 0x7F800000u)
         return 0.f;  // an infinity, a NaN, a negative rho or m
@@ -1030,7 +1054,8 @@ template <int D> struct PolBiInteractions {
 __global__ void zero_u32_k(uint32_t* p, uint32_t* rec)
 {
     if (threadIdx.x < NRANGE) p[threadIdx.x * RSTRIDE] = 0u;
-    if (rec && threadIdx.x < NREC) rec[threadIdx.x] = rec_init[threadIdx.x];
+    if (rec && threadIdx.x < NREC)
+        rec[threadIdx.x * RSTRIDE] = rec_init[threadIdx.x];
 }
 
 // Cell hand-out ranges: NRANGE (one per XCD) unless AQ_WALK_RANGES=1.
@@ -1091,7 +1116,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                                 : nullptr;
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next, rec);
     if (rec && N) {
-        const unsigned nb = std::min(1024u, (N + 255u) / 256u);
+        const unsigned nb = std::min(512u, (N + 1023u) / 1024u);
         if (dims == 2)
             hipLaunchKernelGGL(fluid_ranges_k<2>, dim3(nb), dim3(256), 0, s,
                                imove, p, rho, m, u, N, rec);
