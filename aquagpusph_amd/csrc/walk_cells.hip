@@ -805,15 +805,26 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
 //   pol.owner(i, im_i)  the particles that run the kernel (the reference
 //                       kernel's early exits)
 //   Pol::State          per-particle accumulators
-//   pol.init(i, r_i, st), pol.pair(i, j, r_i, st), pol.finish(i, st)
+//   pol.init(i, r_i, st), pol.pair<NRQ>(i, j, r_i, st), pol.finish(i, st)
+//                       (NRQ: q by div_nr, see qof)
 // pair() is called for the accepted candidates of each particle in the
 // reference walk order, so every sum is bit-identical to the per-particle
 // kernel of sph.hip.
 // ---------------------------------------------------------------------------
+// q = sqrt(s) / H.  NRQ: the quotient by div_nr, which is the IEEE quotient
+// for H in [2^-40, 2^40] and every accepted pair: sqrtf(s) is 0 or at
+// least 2^-75, and at most about 2 H.
+template <bool NRQ> __device__ __forceinline__ float qof(float s, float H)
+{
+    if constexpr (NRQ) return div_nr(sqrtf(s), rcp_nr(H));
+    else return sqrtf(s) / H;
+}
+
 template <int D, int SEGW, class Pol>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     cell_kernel_k(Pol pol, const int* __restrict__ imove,
-                  const float* __restrict__ r, WalkArgs w, float smax)
+                  const float* __restrict__ r, WalkArgs w, float smax,
+                  int nrq)
 {
     __shared__ WaveLds<SEGW> lds[WC_WAVES];
     const int lane = threadIdx.x & 63;
@@ -850,9 +861,18 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             bool unused = false;
             f3 bmin, bmax;
             wave_box<D>(r_i, act, bmin, bmax);
-            ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax, smax, r,
-                               imove, Pol::KIND, false, unused,
-                               [&](uint32_t j) { pol.pair(i, j, r_i, st); });
+            if (nrq)
+                ring_walk<D, SEGW>(
+                    L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                    Pol::KIND, false, unused, [&](uint32_t j) {
+                        pol.template pair<true>(i, j, r_i, st);
+                    });
+            else
+                ring_walk<D, SEGW>(
+                    L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                    Pol::KIND, false, unused, [&](uint32_t j) {
+                        pol.template pair<false>(i, j, r_i, st);
+                    });
             if (act) pol.finish(i, st);
             __builtin_amdgcn_wave_barrier();
         }
@@ -882,6 +902,7 @@ template <int D> struct PolBiShepard {
         st.acc = 1.f;
         st.self_added = false;
     }
+    template <bool NRQ>
     __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
                          State& st) const
     {
@@ -895,7 +916,7 @@ template <int D> struct PolBiShepard {
         const float eps = 1E-8f * H;
         const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
         const float s = dot<D>(r_ij, r_ij);
-        const float q = sqrtf(s) / H;
+        const float q = qof<NRQ>(s, H);
         const f3 n_j = ldv_o<D>(normal, j);
         const f3 t_j = ldv_o<D>(tangent, j);
         const f3 b_j = ldv_o<D>(binormal, j);
@@ -942,6 +963,7 @@ template <int D> struct PolBiLapU {
         st.u_i = ldv<D>(u, i);
         st.lu = ldv<D>(lap_u, i);
     }
+    template <bool NRQ>
     __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
                          State& st) const
     {
@@ -949,7 +971,7 @@ template <int D> struct PolBiLapU {
         if (i == j) return;
         const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
         const float s = dot<D>(r_ij, r_ij);
-        const float q = sqrtf(s) / H;
+        const float q = qof<NRQ>(s, H);
         const float rho_j = ldf_o(rho, j);
         const float f_ij = kernelF<D>(q) * CONF * ldf_o(m, j);
         const f3 u_ij = sub<D>(ldv_o<D>(u, j), st.u_i);
@@ -988,12 +1010,13 @@ template <int D> struct PolBiInterp {
         st.gp_i.z = rho_i * st.gp_i.z;
         st.acc = 0.f;
     }
+    template <bool NRQ>
     __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
                          State& st) const
     {
         const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
         const float s = dot<D>(r_ij, r_ij);
-        const float q = sqrtf(s) / H;
+        const float q = qof<NRQ>(s, H);
         const float w_ij =
             kernelW<D>(q) * CONW * ldf_o(m, j) / ldf_o(rho, j);
         st.acc += (ldf_o(p, j) - dot<D>(st.gp_i, r_ij)) * w_ij;
@@ -1029,12 +1052,13 @@ template <int D> struct PolBiInteractions {
         st.gp = ldv<D>(grad_p, i);
         st.dv = div_u[i];
     }
+    template <bool NRQ>
     __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
                          State& st) const
     {
         const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
         const float s = dot<D>(r_ij, r_ij);
-        const float q = sqrtf(s) / H;
+        const float q = qof<NRQ>(s, H);
         const f3 n_j = ldv_o<D>(normal, j);
         const float area_j = ldf_o(m, j);
         const float p_j = ldf_o(p, j);
@@ -1179,8 +1203,9 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
                 walk_ranges() };
     auto* ker = cell_kernel_k<D, 8, Pol>;
     const unsigned nb = resident_blocks((const void*)ker, 64 * WC_WAVES);
+    const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
-                       r, w, smax);
+                       r, w, smax, nrq);
 }
 
 void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
