@@ -43,6 +43,7 @@ def _pair(xml, dims, monkeypatch, steps, **env):
         for f in FIELDS:
             np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f),
                                           err_msg=f)
+    assert float(np.abs(sims[0].get("u")).max()) > 1.0
     return sims
 
 
@@ -71,3 +72,24 @@ def test_walk_nr_equals_separate_kernels_dense(cuda, tmp_path, monkeypatch):
         b.step(2)
         for f in FIELDS:
             np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+
+
+@pytest.mark.parametrize("dims,res", [(2, 60), (3, 14)])
+def test_walk_nr_equals_ieee_long_dam_break(cuda, tmp_path, dims, res,
+                                            monkeypatch):
+    """6000 steps of a collapsing column, from rest to a flow faster than
+    1 m/s (checked): the walk on div_nr / sqrt_nr and the walk on the IEEE
+    sequences stay bit-identical at every 1000th step."""
+    case = cases.dam_break(dims, 0.3 / res)
+    xml = casefiles.write_case(case, str(tmp_path))
+    sims = []
+    for div in ("nr", "ieee"):
+        monkeypatch.setenv("AQ_DIV", div)
+        sims.append(Simulation(xml, dims))
+    for _ in range(6):
+        for s in sims:
+            s.step(1000, graph=True)
+        for f in FIELDS:
+            np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f),
+                                          err_msg=f)
+    assert float(np.abs(sims[0].get("u")).max()) > 1.0
