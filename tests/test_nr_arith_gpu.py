@@ -43,7 +43,6 @@ def _pair(xml, dims, monkeypatch, steps, **env):
         for f in FIELDS:
             np.testing.assert_array_equal(sims[0].get(f), sims[1].get(f),
                                           err_msg=f)
-    assert float(np.abs(sims[0].get("u")).max()) > 1.0
     return sims
 
 
