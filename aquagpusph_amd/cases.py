@@ -191,8 +191,8 @@ def decaying_box(n, seed=0, k0=4.0, urms=1.0, nu=1e-4, cs=20.0, refd=1.0,
     """Decaying 2D turbulence in a closed unit box: n x n fluid particles,
     boundary-integral walls on the four sides, no gravity, and a seeded
     solenoidal velocity field with energy spectrum E(k) ~ k^4
-    exp(-2 (k/k0)^2), k in units of 2 pi (the spectrum of the BASELINE's
-    decaying-turbulence cube, here in 2D), scaled to the RMS speed urms.
+    exp(-2 (k/k0)^2), k in units of 2 pi (the spectrum of the
+    usual decaying-turbulence cube benchmark, here in 2D), scaled to the RMS speed urms.
     The field is the curl of a stream function sum a_mn sin(m pi x)
     sin(n pi y), so it is divergence-free and does not cross the walls.
     Kinematic viscosity nu (no artificial viscosity), background pressure

@@ -269,7 +269,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not "
-              "supported (see DESIGN.md for how the path would shard)"
+              "supported (one rank per GPU, each owning a slab of cells "
+              "with halo exchange, is the intended sharding)"
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     if a.copy_calibration:
