@@ -6,8 +6,6 @@
 
 #include <cstdint>
 
-extern "C" __device__ float __ocml_acospi_f32(float);
-
 namespace aq {
 
 // ---- vec storage: float2 in 2D, float4 in 3D (types/2D.h:23, 3D.h:23) ----
@@ -241,11 +239,74 @@ __device__ __forceinline__ float cl_max(float x, float y)
     return (x < y) ? y : x;
 }
 
+// ---- atan, acospi and pow of the boundary-integral kernel terms ----------
+// The reference evaluates these with its OpenCL runtime's atan / acospi /
+// pow (Wendland2D.hcl / Wendland3D.hcl:137-185), whose only contract is the
+// OpenCL C ULP bounds (atan 5, acospi 5, pow 16).  Here they are one fixed
+// float sequence, the Cephes method (S. L. Moshier, Cephes Mathematical
+// Library: atanf.c, asinf.c, acosf.c): range reduction by tan(pi/8) and
+// tan(3pi/8) for atan, by 1/2 for asin, and the Cephes minimax polynomials
+// evaluated with fused multiply-adds.  The CPU oracle runs the same
+// sequence, so the two agree bit for bit; both are checked against the
+// bounds over the float range.
+__device__ __forceinline__ float atan_cl(float x)
+{
+    const float ax = fabsf(x);
+    float a = ax, base = 0.f;
+    if (ax > 2.414213562373095f) {  // tan(3 pi / 8)
+        base = 1.5707963267948966f;
+        a = -1.f / ax;
+    } else if (ax > 0.4142135623730950f) {  // tan(pi / 8)
+        base = 0.7853981633974483f;
+        a = (ax - 1.f) / (ax + 1.f);
+    }
+    const float z = a * a;
+    float p = __builtin_fmaf(8.05374449538e-2f, z, -1.38776856032e-1f);
+    p = __builtin_fmaf(p, z, 1.99777106478e-1f);
+    p = __builtin_fmaf(p, z, -3.33329491539e-1f);
+    const float r = base + __builtin_fmaf(p * z, a, a);
+    return copysignf(r, x);
+}
+
+// asin on [0, 1/2] (Cephes asinf polynomial)
+__device__ __forceinline__ float asin_half_cl(float a)
+{
+    const float z = a * a;
+    float p = __builtin_fmaf(4.2163199048e-2f, z, 2.4181311049e-2f);
+    p = __builtin_fmaf(p, z, 4.5470025998e-2f);
+    p = __builtin_fmaf(p, z, 7.4953002686e-2f);
+    p = __builtin_fmaf(p, z, 1.6666752422e-1f);
+    return __builtin_fmaf(p * z, a, a);
+}
+
+// acos(x)/pi for x in [0, 1] (the only range the kernel terms reach):
+// acos x = 2 asin(sqrt((1 - x)/2)) above 1/2, pi/2 - asin x below
+__device__ __forceinline__ float acospi_cl(float x)
+{
+    float ac;
+    if (x > 0.5f) {
+        const float s = sqrtf(0.5f * (1.f - x));
+        ac = 2.f * asin_half_cl(s);
+    } else {
+        ac = 1.5707963267948966f - asin_half_cl(x);
+    }
+    return ac * 0.318309886183791f;
+}
+
+// pow(area, 1 / (DIMS - 1)): the exponent is 1 in 2D and 1/2 in 3D, where
+// the correctly rounded identity and square root are the exact results
+__device__ __forceinline__ float pow_cl(float x, float y)
+{
+    if (y == 1.f) return x;
+    if (y == 0.5f) return sqrtf(x);
+    return powf(x, y);
+}
+
 __device__ __forceinline__ float omega3(float a, float b)
 {
     const float a2 = a * a;
     const float b2 = b * b;
-    return __ocml_acospi_f32(cl_min(
+    return acospi_cl(cl_min(
         sqrtf((1.f + a2 + b2) / ((1.f + a2) * (1.f + b2))), 1.f));
 }
 
@@ -255,7 +316,7 @@ __device__ __forceinline__ float kernelS_D(float d, float t, float b, float s)
     if constexpr (D == 2) {
         const float wcon = 0.5f * AQ_IPI;
         const float dr = 0.5f * s;
-        return -wcon * (atanf((t + dr) / d) - atanf((t - dr) / d));
+        return -wcon * (atan_cl((t + dr) / d) - atan_cl((t - dr) / d));
     } else {
         const float wcon = 0.25f;
         const float dr = 0.5f * sqrtf(s);

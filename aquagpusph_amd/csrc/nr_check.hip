@@ -57,6 +57,15 @@ __global__ void __launch_bounds__(256)
     if (ns) atomicAdd(bad + 1, ns);
 }
 
+__global__ void __launch_bounds__(256)
+    clmath_k(int op, const float* __restrict__ x, float* __restrict__ y,
+             uint32_t n)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    y[k] = op == 0 ? atan_cl(x[k]) : acospi_cl(x[k]);
+}
+
 }  // namespace
 
 }  // namespace aq
@@ -84,6 +93,20 @@ AQ_API int aq_check_nr_arith(uint64_t n, uint64_t seed, uint64_t* mismatches)
         AQ_HIP(hipFree(d));
         mismatches[0] = h[0];
         mismatches[1] = h[1];
+    });
+}
+
+AQ_API int aq_clmath_eval(int op, const float* x, float* y, uint32_t n,
+                          void* stream)
+{
+    using namespace aq;
+    return guard([&] {
+        AQ_REQUIRE(op == 0 || op == 1, "aq_clmath_eval: op is 0 or 1");
+        AQ_REQUIRE(n == 0 || (x && y), "aq_clmath_eval: null pointer");
+        if (n == 0) return;
+        hipLaunchKernelGGL(clmath_k, dim3((n + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, op, x, y, n);
+        AQ_HIP(hipGetLastError());
     });
 }
 

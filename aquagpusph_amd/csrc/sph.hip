@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(BS)
                     const float r_t = fabsf(dot<D>(r_ij, t_j));
                     const float r_b = fabsf(dot<D>(r_ij, b_j));
                     const float dr =
-                        0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                        0.55f * pow_cl(area_j, 1.f / (DIMS - 1.f));
                     if ((r_t <= dr) && (r_b <= dr)) {
                         self_added = true;
                         acc -= 0.5f;
@@ -1148,7 +1148,7 @@ __global__ void __launch_bounds__(BS)
                                 const float r_t = fabsf(dot<D>(r_ij, t_j));
                                 const float r_b = fabsf(dot<D>(r_ij, b_j));
                                 const float dr =
-                                    0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                                    0.55f * pow_cl(area_j, 1.f / (DIMS - 1.f));
                                 if ((r_t <= dr) && (r_b <= dr)) {
                                     self_added = true;
                                     sh -= 0.5f;

@@ -766,7 +766,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                                 const float r_t = fabsf(dot<D>(r_ij, t_j));
                                 const float r_b = fabsf(dot<D>(r_ij, b_j));
                                 const float dr =
-                                    0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                                    0.55f * pow_cl(area_j, 1.f / (DIMS - 1.f));
                                 if ((r_t <= dr) && (r_b <= dr)) {
                                     self_added = true;
                                     sh -= 0.5f;
@@ -926,7 +926,7 @@ template <int D> struct PolBiShepard {
             if (!st.self_added) {
                 const float r_t = fabsf(dot<D>(r_ij, t_j));
                 const float r_b = fabsf(dot<D>(r_ij, b_j));
-                const float dr = 0.55f * powf(area_j, 1.f / (DIMS - 1.f));
+                const float dr = 0.55f * pow_cl(area_j, 1.f / (DIMS - 1.f));
                 if ((r_t <= dr) && (r_b <= dr)) {
                     st.self_added = true;
                     st.acc -= 0.5f;

@@ -417,6 +417,13 @@ AQ_API int aq_xml_resolve(const char* xml_path, int dims, char* out,
  * quotients / square roots whose bit patterns differ from IEEE a / b and
  * sqrtf (0 expected).  Synchronous. */
 AQ_API int aq_check_nr_arith(uint64_t n, uint64_t seed, uint64_t* mismatches);
+/* The float atan (op 0) and acospi (op 1) of the boundary-integral kernel
+ * terms, y[k] = f(x[k]) for device arrays of n floats, on `stream` (the
+ * reference takes atan / acospi from its OpenCL runtime:
+ * KernelFunctions/Wendland2D.hcl, Wendland3D.hcl:137-185).  For checking
+ * the device sequence against the CPU one. */
+AQ_API int aq_clmath_eval(int op, const float* x, float* y, uint32_t n,
+                          void* stream);
 /* Registry keys ("cfd/Interactions.cl:entry", ...) of the HIP kernels. */
 AQ_API int aq_sim_n_registered_kernels(void);
 AQ_API const char* aq_sim_registered_kernel(int i);

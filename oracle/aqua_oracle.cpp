@@ -567,3 +567,53 @@ ORC_EXPORT double orc_kernel_sd(int dims, double d, double t, double b,
     return dims == 2 ? kernelS_D<2, double>(d, t, b, s)
                      : kernelS_D<3, double>(d, t, b, s);
 }
+
+// ---- the float atan / acospi sequence of the boundary-integral terms -----
+// op 0: atan, op 1: acospi.  y[k] = the float sequence at x[k].
+ORC_EXPORT void orc_clmath_f32(int op, const float* x, float* y, usize n)
+{
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)n; k++)
+        y[k] = op == 0 ? atan_cl(x[k]) : acospi_cl(x[k]);
+}
+
+// Largest error, in units of the float spacing at the exact value (libm in
+// double), of the float sequence over the float bit patterns lo, lo+stride,
+// ... <= hi (op 0: atan, op 1: acospi).  out[0] = max ulp, out[1] = the
+// argument where it occurs.
+ORC_EXPORT void orc_clmath_ulp(int op, unsigned lo, unsigned hi,
+                               unsigned stride, double* out)
+{
+    const long long n = ((long long)hi - (long long)lo) / stride + 1;
+    double best = 0.0, arg = 0.0;
+#pragma omp parallel
+    {
+        double b = 0.0, a = 0.0;
+#pragma omp for schedule(static)
+        for (long long k = 0; k < n; k++) {
+            const unsigned bits = lo + (unsigned)(k * stride);
+            float x;
+            std::memcpy(&x, &bits, 4);
+            if (!std::isfinite(x)) continue;
+            const double ex = op == 0 ? std::atan((double)x)
+                                      : std::acos((double)x) / M_PI;
+            const double got = op == 0 ? atan_cl(x) : acospi_cl(x);
+            const double ax = std::fabs(ex);
+            int e = 0;
+            std::frexp(ax, &e);  // ax in [2^(e-1), 2^e)
+            const double ulp = std::ldexp(1.0, std::max(e - 24, -149));
+            const double err = std::fabs(got - ex) / ulp;
+            if (err > b) {
+                b = err;
+                a = x;
+            }
+        }
+#pragma omp critical
+        if (b > best) {
+            best = b;
+            arg = a;
+        }
+    }
+    out[0] = best;
+    out[1] = arg;
+}

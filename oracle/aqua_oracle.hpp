@@ -129,11 +129,73 @@ template <typename R> inline R cl_sign(R x)
 template <typename R> inline R cl_min(R x, R y) { return (y < x) ? y : x; }
 template <typename R> inline R cl_max(R x, R y) { return (x < y) ? y : x; }
 
-// acospi(x) = acos(x)/pi.  glibc has no acospi; the double evaluation
-// rounded once to R is within the OpenCL 5-ulp bound for float.
+// atan, acospi and pow of the boundary-integral terms
+// (KernelFunctions/Wendland2D.hcl, Wendland3D.hcl:137-185).  The reference
+// takes them from its OpenCL runtime, bound only by the OpenCL C ULP limits
+// (atan 5, acospi 5, pow 16).  The float instantiation runs one fixed
+// sequence, the Cephes method (S. L. Moshier, Cephes Mathematical Library:
+// atanf.c, asinf.c, acosf.c): range reduction by tan(pi/8) and tan(3pi/8)
+// for atan, by 1/2 for asin, and the Cephes minimax polynomials evaluated
+// with fused multiply-adds (std::fma rounds once, like the device fma).
+// The double instantiation uses libm.
+inline float atan_cl(float x)
+{
+    const float ax = std::fabs(x);
+    float a = ax, base = 0.f;
+    if (ax > 2.414213562373095f) {
+        base = 1.5707963267948966f;
+        a = -1.f / ax;
+    } else if (ax > 0.4142135623730950f) {
+        base = 0.7853981633974483f;
+        a = (ax - 1.f) / (ax + 1.f);
+    }
+    const float z = a * a;
+    float p = std::fma(8.05374449538e-2f, z, -1.38776856032e-1f);
+    p = std::fma(p, z, 1.99777106478e-1f);
+    p = std::fma(p, z, -3.33329491539e-1f);
+    const float r = base + std::fma(p * z, a, a);
+    return std::copysign(r, x);
+}
+
+inline float asin_half_cl(float a)
+{
+    const float z = a * a;
+    float p = std::fma(4.2163199048e-2f, z, 2.4181311049e-2f);
+    p = std::fma(p, z, 4.5470025998e-2f);
+    p = std::fma(p, z, 7.4953002686e-2f);
+    p = std::fma(p, z, 1.6666752422e-1f);
+    return std::fma(p * z, a, a);
+}
+
+// acos(x)/pi on [0, 1]
+inline float acospi_cl(float x)
+{
+    float ac;
+    if (x > 0.5f) {
+        const float s = std::sqrt(0.5f * (1.f - x));
+        ac = 2.f * asin_half_cl(s);
+    } else {
+        ac = 1.5707963267948966f - asin_half_cl(x);
+    }
+    return ac * 0.318309886183791f;
+}
+
+template <typename R> inline R cl_atan(R x) { return std::atan(x); }
+template <> inline float cl_atan<float>(float x) { return atan_cl(x); }
+
 template <typename R> inline R cl_acospi(R x)
 {
-    return R(std::acos((double)x) / 3.14159265358979323846);
+    return std::acos(x) / R(3.14159265358979323846);
+}
+template <> inline float cl_acospi<float>(float x) { return acospi_cl(x); }
+
+// pow(area, 1 / (DIMS - 1)): exponent 1 (2D) or 1/2 (3D), whose exact
+// results are the identity and the correctly rounded square root
+template <typename R> inline R cl_pow(R x, R y)
+{
+    if (y == R(1)) return x;
+    if (y == R(0.5)) return std::sqrt(x);
+    return std::pow(x, y);
 }
 
 template <typename R> inline R Omega3(R a, R b)
@@ -149,7 +211,7 @@ template <int D, typename R> inline R kernelS_D(R d, R t, R b, R s)
     if (D == 2) {
         const R wcon = R(0.5f * kIPI);
         const R dr = R(0.5f) * s;
-        return -wcon * (std::atan((t + dr) / d) - std::atan((t - dr) / d));
+        return -wcon * (cl_atan<R>((t + dr) / d) - cl_atan<R>((t - dr) / d));
     }
     const R wcon = R(0.25f);
     const R dr = R(0.5f) * std::sqrt(s);
@@ -827,7 +889,7 @@ void bi_shepard_compute(const int* imove, const R* r, const R* normal,
                     const R r_t = std::fabs(dot<D, R>(r_ij, t_j));
                     const R r_b = std::fabs(dot<D, R>(r_ij, b_j));
                     const R dr = R(0.55f) *
-                                 std::pow(area_j, R(1.f) / (DIMS - R(1.f)));
+                                 cl_pow<R>(area_j, R(1.f) / (DIMS - R(1.f)));
                     if ((r_t <= dr) && (r_b <= dr)) {
                         self_added = true;
                         acc -= R(0.5f);

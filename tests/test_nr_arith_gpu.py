@@ -30,6 +30,37 @@ def test_nr_sequences_match_ieee(cuda):
         assert (out[0], out[1]) == (0, 0), (seed, out[0], out[1])
 
 
+def test_clmath_sequences_match_oracle(cuda):
+    """The device atan / acospi of the boundary-integral terms equal the
+    CPU oracle's float sequence bit for bit (the same Cephes-method
+    operations; the oracle's ulp bounds are checked on the CPU): atan over a
+    stride of every finite float of both signs, acospi over [0, 1]."""
+    import torch
+    from oracle import pipeline as op
+    L = _lib.lib()
+    one = int(np.float32(1.0).view(np.uint32))
+    cases_ = [(0, np.arange(0, 0x7F800000, 127, dtype=np.uint32)),
+              (0, np.arange(0x80000000, 0xFF800000, 131, dtype=np.uint32)),
+              (1, np.arange(0, one + 1, 61, dtype=np.uint32))]
+    for opc, bits in cases_:
+        x = bits.view(np.float32)
+        want = np.zeros_like(x)
+        op.lib().orc_clmath_f32(ctypes.c_int(opc),
+                                ctypes.c_void_p(x.ctypes.data),
+                                ctypes.c_void_p(want.ctypes.data),
+                                ctypes.c_uint(x.size))
+        xd = torch.from_numpy(x.copy()).cuda()
+        yd = torch.empty_like(xd)
+        _lib.call("aq_clmath_eval", ctypes.c_int(opc),
+                  ctypes.c_void_p(xd.data_ptr()),
+                  ctypes.c_void_p(yd.data_ptr()), ctypes.c_uint32(x.size),
+                  ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        got = yd.cpu().numpy()
+        bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+        assert bad.size == 0, (opc, bad.size, x[bad[:4]], got[bad[:4]],
+                               want[bad[:4]])
+
+
 def _pair(xml, dims, monkeypatch, steps, **env):
     sims = []
     for div in ("nr", "ieee"):
