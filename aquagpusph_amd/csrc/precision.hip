@@ -30,13 +30,13 @@ __global__ void __launch_bounds__(BS)
                   double* __restrict__ dr64_in, float* __restrict__ u_in,
                   float* __restrict__ dudt_in, float* __restrict__ rho_in,
                   float* __restrict__ drhodt_in, uint32_t N,
-                  const float* __restrict__ dtp)
+                  const float* __restrict__ dtp, const uint32_t* status)
 {
     constexpr int VS = (D == 2) ? 2 : 4;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float DT = *dtp;
-    if (imove[i] <= 0) DT = 0.f;
+    if ((imove[i] <= 0) || (status && *status)) DT = 0.f;
     const float hdt2 = 0.5f * DT * DT;
     const double dDT = DT, dh = hdt2;
 #pragma unroll
@@ -65,11 +65,12 @@ __global__ void __launch_bounds__(BS)
                   const float* __restrict__ drhodt,
                   const float* __restrict__ dudt_in,
                   const float* __restrict__ drhodt_in, uint32_t N,
-                  const float* __restrict__ dtp)
+                  const float* __restrict__ dtp, const uint32_t* status)
 {
     constexpr int VS = (D == 2) ? 2 : 4;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (status && *status) return;  // a failure was flagged: state kept
     if (imove[i] <= 0) return;
     const float DT = 0.5f * (*dtp);
     const double DT2 = (double)DT * (double)DT;
@@ -103,34 +104,36 @@ void launch_predictor64(int dims, const int32_t* imove, const float* r,
                         const float* rho, const float* drhodt, float* r_in,
                         double* dr64_in, float* u_in, float* dudt_in,
                         float* rho_in, float* drhodt_in, uint32_t N,
-                        const float* dt, hipStream_t s)
+                        const float* dt, const uint32_t* status, hipStream_t s)
 {
     if (!N) return;
     const dim3 g(blocks_for(N, BS)), b(BS);
     if (dims == 2)
         hipLaunchKernelGGL(predictor64_k<2>, g, b, 0, s, imove, r, dr64, u,
                            dudt, rho, drhodt, r_in, dr64_in, u_in, dudt_in,
-                           rho_in, drhodt_in, N, dt);
+                           rho_in, drhodt_in, N, dt, status);
     else
         hipLaunchKernelGGL(predictor64_k<3>, g, b, 0, s, imove, r, dr64, u,
                            dudt, rho, drhodt, r_in, dr64_in, u_in, dudt_in,
-                           rho_in, drhodt_in, N, dt);
+                           rho_in, drhodt_in, N, dt, status);
 }
 
 void launch_corrector64(int dims, const int32_t* imove, float* r,
                         double* dr64, float* u, const float* dudt,
                         float* rho, const float* drhodt, const float* dudt_in,
                         const float* drhodt_in, uint32_t N, const float* dt,
-                        hipStream_t s)
+                        const uint32_t* status, hipStream_t s)
 {
     if (!N) return;
     const dim3 g(blocks_for(N, BS)), b(BS);
     if (dims == 2)
         hipLaunchKernelGGL(corrector64_k<2>, g, b, 0, s, imove, r, dr64, u,
-                           dudt, rho, drhodt, dudt_in, drhodt_in, N, dt);
+                           dudt, rho, drhodt, dudt_in, drhodt_in, N, dt,
+                           status);
     else
         hipLaunchKernelGGL(corrector64_k<3>, g, b, 0, s, imove, r, dr64, u,
-                           dudt, rho, drhodt, dudt_in, drhodt_in, N, dt);
+                           dudt, rho, drhodt, dudt_in, drhodt_in, N, dt,
+                           status);
 }
 
 void launch_fluid_mass(const int32_t* imove, const float* m, float* m_fluid,

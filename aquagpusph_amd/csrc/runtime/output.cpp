@@ -136,7 +136,7 @@ static void write_ascii(Sim& S, const std::string& path, size_t off, size_t n,
                 if (ty.base == 'f') {
                     float x;
                     memcpy(&x, &w, 4);
-                    snprintf(b, sizeof(b), "%.7g", (double)x);
+                    snprintf(b, sizeof(b), "%.9g", (double)x);  // round trip
                 } else if (ty.base == 'i')
                     snprintf(b, sizeof(b), "%d", (int32_t)w);
                 else

@@ -16,10 +16,10 @@ void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
 void launch_predictor(int, const int32_t*, const float*, const float*,
                       const float*, const float*, const float*, float*, float*,
                       float*, float*, float*, uint32_t, const float*,
-                      hipStream_t);
+                      const uint32_t*, hipStream_t);
 void launch_corrector(int, const int32_t*, float*, float*, const float*,
                       float*, const float*, const float*, const float*,
-                      uint32_t, const float*, hipStream_t);
+                      uint32_t, const float*, const uint32_t*, hipStream_t);
 void launch_domain(int, int32_t*, float*, float*, float*, float*, uint32_t,
                    const float*, const float*, hipStream_t);
 void launch_eos(const uint32_t*, const int32_t*, const float*, float*,
@@ -77,10 +77,12 @@ void launch_portal_mls(int, const int32_t*, const int32_t*, const float*,
 void launch_predictor64(int, const int32_t*, const float*, const double*,
                         const float*, const float*, const float*,
                         const float*, float*, double*, float*, float*, float*,
-                        float*, uint32_t, const float*, hipStream_t);
+                        float*, uint32_t, const float*, const uint32_t*,
+                        hipStream_t);
 void launch_corrector64(int, const int32_t*, float*, double*, float*,
                         const float*, float*, const float*, const float*,
-                        const float*, uint32_t, const float*, hipStream_t);
+                        const float*, uint32_t, const float*, const uint32_t*,
+                        hipStream_t);
 void launch_fluid_mass(const int32_t*, const float*, float*, uint32_t,
                        hipStream_t);
 void launch_bt_eos(const uint32_t*, const int32_t*, const float*, float*,
@@ -347,14 +349,15 @@ static const std::map<std::string, KernelSpec>& table()
               "dudt_in", "rho_in", "drhodt_in", "dt" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_predictor(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
-                                 F(6), F(7), F(8), F(9), F(10), S.N, F(11), s);
+                                 F(6), F(7), F(8), F(9), F(10), S.N, F(11),
+                                 S.d_status, s);
             } } },
         { "basic/time_scheme/improved_euler.cl:corrector",
           { { "imove", "iset", "r", "u", "dudt", "rho", "drhodt", "dudt_in",
               "drhodt_in", "dt" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_corrector(S.dims, I(0), F(2), F(3), F(4), F(5), F(6),
-                                 F(7), F(8), S.N, F(9), s);
+                                 F(7), F(8), S.N, F(9), S.d_status, s);
             } } },
         { "basic/time_scheme/euler.cl:predictor",
           { { "r", "u", "dudt", "rho", "drhodt", "r_in", "u_in", "dudt_in",
@@ -566,14 +569,15 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_predictor64(S.dims, I(0), F(1), DB(2), F(3), F(4), F(5),
                                    F(6), F(7), DB(8), F(9), F(10), F(11),
-                                   F(12), S.N, F(13), s);
+                                   F(12), S.N, F(13), S.d_status, s);
             } } },
         { "basic/time_scheme/improved_euler_fp64.cl:corrector",
           { { "imove", "r", "dr64", "u", "dudt", "rho", "drhodt", "dudt_in",
               "drhodt_in", "dt" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_corrector64(S.dims, I(0), F(1), DB(2), F(3), F(4), F(5),
-                                   F(6), F(7), F(8), S.N, F(9), s);
+                                   F(6), F(7), F(8), S.N, F(9), S.d_status,
+                                   s);
             } } },
         { "basic/Sort_fp64.cl:entry",
           { { "dr64_in", "dr64", "id_unsorted" },

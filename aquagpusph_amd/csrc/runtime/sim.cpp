@@ -33,9 +33,11 @@ Sim::Sim(int d) : dims(d), vars(d)
 {
     if (d != 2 && d != 3) throw Error(AQ_EINVAL, "dims must be 2 or 3");
     AQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    AQ_HIP(hipMalloc(&d_status, sizeof(uint32_t)));
+    // [0] failure bits, [1] iter at the end of the first failed step
+    AQ_HIP(hipMalloc(&d_status, 2 * sizeof(uint32_t)));
     AQ_HIP(hipMalloc(&d_fail_id, sizeof(int32_t)));
     AQ_HIP(hipMemset(d_status, 0, sizeof(uint32_t)));
+    AQ_HIP(hipMemset(d_status + 1, 0xff, sizeof(uint32_t)));
     AQ_HIP(hipMemset(d_fail_id, 0xff, sizeof(int32_t)));
 }
 
@@ -93,6 +95,9 @@ Problem load_problem(const std::string& xml_path, int dims)
     P.dims = dims;
     P.base_path = package_dir();
     parse_problem(xml_path, P, "");
+    // the <Reports> run after every tool (CalcServer.cpp:472-520)
+    for (auto& r : P.reports) P.tools.push_back(r);
+    P.reports.clear();
     return P;
 }
 
@@ -627,6 +632,17 @@ void Sim::run_tools(hipStream_t s,
         }
         pc++;
     }
+    stamp_status(s);
+}
+
+void Sim::stamp_status(hipStream_t s)
+{
+    Var* it = vars.get("iter");
+    const uint32_t* iter =
+        (it && !it->is_array() && it->type.base == 'u')
+            ? (const uint32_t*)vars.scalar_dev(*it)
+            : nullptr;
+    status_stamp(d_status, iter, s);
 }
 
 // Graph replay of a pipeline with control tools: each run of tools between
@@ -670,6 +686,7 @@ void Sim::step_segments(int nsteps)
             AQ_HIP(hipGraphLaunch(it->second.exec, stream));
             pc = it->second.end;
         }
+        stamp_status(stream);
     }
 }
 
@@ -677,22 +694,28 @@ void Sim::sync() { AQ_HIP(hipStreamSynchronize(stream)); }
 
 void Sim::check_status()
 {
-    uint32_t st = 0;
+    uint32_t st[2] = { 0, 0xFFFFFFFFu };
     int32_t id = -1;
-    AQ_HIP(hipMemcpyAsync(&st, d_status, 4, hipMemcpyDeviceToHost, stream));
+    AQ_HIP(hipMemcpyAsync(st, d_status, 8, hipMemcpyDeviceToHost, stream));
     AQ_HIP(hipMemcpyAsync(&id, d_fail_id, 4, hipMemcpyDeviceToHost, stream));
     AQ_HIP(hipStreamSynchronize(stream));
-    if (st & 1u)
+    // the step that first flagged it, by the iteration counter at its end;
+    // the corrector has left the particles as they were since
+    const std::string when =
+        st[1] == 0xFFFFFFFFu
+            ? std::string()
+            : " in the step ending at iter = " + std::to_string(st[1]) +
+                  " (the particles are kept as they were before it)";
+    if (st[0] & 1u)
         throw Error(AQ_ENOMEM,
                     "link-list: the cell grid outgrew the ihoc capacity "
-                    "(particles left the domain box); results of this step "
-                    "are invalid");
-    if (st & 2u)
+                    "(particles left the domain box)" + when);
+    if (st[0] & 2u)
         throw Error(AQ_EASSERT,
                     "assertion failed in tool " +
                         (id >= 0 && id < (int)assert_names.size()
                              ? assert_names[id]
-                             : std::string("?")));
+                             : std::string("?")) + when);
 }
 
 void Sim::step(int nsteps)

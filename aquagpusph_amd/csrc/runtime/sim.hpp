@@ -186,6 +186,8 @@ struct Problem {
         variables;  // name, type, length, value
     std::vector<Definition> definitions;
     std::vector<std::shared_ptr<ToolDesc>> tools;
+    // <Reports> (State.cpp:1395-1500), appended to the tools once parsed
+    std::vector<std::shared_ptr<ToolDesc>> reports;
     std::vector<SetDesc> sets;
     TimingDesc timing;
     std::string base_path;
@@ -297,7 +299,9 @@ class Sim {
     aq_defs kdefs{};
     uint32_t N = 0;
     hipStream_t stream = nullptr;
-    uint32_t* d_status = nullptr;   // bit0 ihoc overflow, bit1 assert failed
+    // [0]: bit0 ihoc overflow, bit1 assert failed; [1]: iter at the end of
+    // the first step that flagged one (0xFFFFFFFF: none)
+    uint32_t* d_status = nullptr;
     int32_t* d_fail_id = nullptr;
     std::vector<std::string> assert_names;
     Problem problem;
@@ -359,6 +363,8 @@ class Sim {
     // called around each executed tool (index, false before / true after)
     void run_tools(hipStream_t s,
                    const std::function<void(size_t, bool)>& hook = {});
+    // record the iteration of the first failed step (end of each step)
+    void stamp_status(hipStream_t s);
 
   private:
     size_t control(size_t pc, hipStream_t s);
@@ -378,6 +384,7 @@ void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
 // scalar bytecode on the device
 void vm_launch(const VmOp* ops, uint8_t* arena, uint32_t* status,
                int32_t* fail_id, hipStream_t s);
+void status_stamp(uint32_t* status, const uint32_t* iter, hipStream_t s);
 
 }  // namespace rt
 }  // namespace aq

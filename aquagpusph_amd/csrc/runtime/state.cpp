@@ -313,6 +313,32 @@ static void parse_tools(const XmlNode& root, Problem& P,
         }
 }
 
+// <Reports><Report type="screen|file|performance" .../>: the report tools
+// of the same names, run after every tool (State.cpp:1395-1500)
+static void parse_reports(const XmlNode& root, Problem& P,
+                          const std::string& prefix)
+{
+    for (auto* rs : root.all("Reports"))
+        for (auto* e : rs->all("Report")) {
+            if (!e->has("type")) throw Error(AQ_EPARSE, "Report without type");
+            auto t = std::make_shared<ToolDesc>();
+            for (auto& kv : e->attrs) t->attrs[kv.first] = kv.second;
+            t->attrs["name"] = prefix + e->attr("name");
+            t->attrs["type"] = "report_" + e->attr("type");
+            t->attrs["once"] = "false";
+            t->xml_dir = P.xml_stack.empty() ? "" : P.xml_stack.back();
+            const std::string ty = e->attr("type");
+            if ((ty == "screen" || ty == "file" || ty == "particles") &&
+                !e->has("fields"))
+                throw Error(AQ_EPARSE, "report \"" + t->get("name") +
+                                           "\" without fields");
+            if ((ty == "file" || ty == "particles") && !e->has("path"))
+                throw Error(AQ_EPARSE, "report \"" + t->get("name") +
+                                           "\" without an output path");
+            P.reports.push_back(t);
+        }
+}
+
 static void parse_timing(const XmlNode& root, Problem& P)
 {
     for (auto* ts : root.all("Timing"))
@@ -386,6 +412,7 @@ void parse_problem(const std::string& path, Problem& P,
     parse_variables(*root, P);
     parse_definitions(*root, P);
     parse_tools(*root, P, prefix);
+    parse_reports(*root, P, prefix);
     parse_timing(*root, P);
     parse_sets(*root, P);
     includes(true);

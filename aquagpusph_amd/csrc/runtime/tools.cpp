@@ -1,7 +1,11 @@
 // Built-in tool types (the reference's CalcServer/*.cpp tools).
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <dlfcn.h>
+#include <sstream>
 #include <filesystem>
 #include <limits>
 
@@ -123,11 +127,254 @@ static std::vector<uint32_t> words(const std::vector<double>& v,
     return w;
 }
 
-// ---- dummy / reports ------------------------------------------------------
+// ---- dummy ---------------------------------------------------------------------
 class DummyTool : public Tool {
   public:
     using Tool::Tool;
     void execute(Sim&, hipStream_t) override {}
+};
+
+// ---- reports: Reports/Screen.cpp, TabFile.cpp, Performance.cpp -----------------
+// The reported variables are copied into pinned memory on the stream and a
+// host function, enqueued behind the copy, formats them; both are stream
+// operations, so a report inside a captured step becomes a copy node and a
+// host node of the graph and never stalls the stream.
+struct ReportField {
+    Var* v;
+    size_t off, bytes;
+};
+
+// Variable::asString: scalars as numbers, vectors as "(x,y,...)", arrays as
+// their elements in brackets.
+static std::string report_value(const Type& ty, const uint8_t* p, size_t n)
+{
+    auto one = [&](const uint8_t* q) {
+        std::string s;
+        char b[64];
+        for (int c = 0; c < ty.ncomp; c++) {
+            if (ty.base == 'd') {
+                double x;
+                memcpy(&x, q + 8 * c, 8);
+                snprintf(b, sizeof(b), "%.17g", x);
+            } else if (ty.base == 'f') {
+                float x;
+                memcpy(&x, q + 4 * c, 4);
+                snprintf(b, sizeof(b), "%.9g", (double)x);
+            } else if (ty.base == 'i') {
+                int32_t x;
+                memcpy(&x, q + 4 * c, 4);
+                snprintf(b, sizeof(b), "%d", x);
+            } else {
+                uint32_t x;
+                memcpy(&x, q + 4 * c, 4);
+                snprintf(b, sizeof(b), "%u", x);
+            }
+            s += b;
+            if (c + 1 < ty.ncomp) s += ",";
+        }
+        return ty.ncomp > 1 ? "(" + s + ")" : s;
+    };
+    if (!ty.array) return one(p);
+    std::string s = "[";
+    for (size_t k = 0; k < n; k++) {
+        s += one(p + k * ty.elem_bytes());
+        if (k + 1 < n) s += ",";
+    }
+    return s + "]";
+}
+
+class ReportTool : public Tool {
+  public:
+    // kind 0: screen, 1: file, 2: performance
+    ReportTool(const ToolDesc& d, int kind)
+        : Tool(d.get("name"), d.get("type"), d.get("once") == "true"),
+          d_(d), kind_(kind)
+    {
+    }
+    ~ReportTool() override
+    {
+        if (host_) (void)hipHostFree(host_);
+    }
+    void setup(Sim& S) override
+    {
+        if (host_) return;  // the same tool placed twice
+        S_ = &S;
+        std::string fields = d_.get("fields");
+        if (kind_ == 2) fields = "t,end_t,iter,end_iter,frame,end_frame";
+        if (kind_ != 2 && trim(fields).empty())
+            throw Error(AQ_EPARSE, "report \"" + name_ + "\" without fields");
+        // Report::processFields: ';' breaks lines, ',' or ' ' separates
+        size_t off = 0;
+        std::stringstream lines(fields);
+        std::string line;
+        while (std::getline(lines, line, ';')) {
+            for (auto& c : line)
+                if (c == ' ') c = ',';
+            std::stringstream ls(line);
+            std::string n;
+            size_t in_line = 0;
+            while (std::getline(ls, n, ',')) {
+                n = trim(n);
+                if (n.empty()) continue;
+                Var* v = S.vars.get(n);
+                if (!v)
+                    throw Error(AQ_ENOTFOUND, "report \"" + name_ +
+                                                  "\" asks for the undeclared "
+                                                  "variable \"" + n + "\"");
+                const size_t b = v->is_array() ? v->bytes
+                                               : v->type.elem_bytes();
+                if (b > (1u << 16))
+                    throw Error(AQ_EUNSUP, "report \"" + name_ + "\": \"" +
+                                               n + "\" is too large to "
+                                               "report (64 KiB at most)");
+                fields_.push_back({ v, off, b });
+                off += (b + 15) & ~(size_t)15;
+                in_line++;
+            }
+            per_line_.push_back(in_line);
+        }
+        bytes_ = off ? off : 16;
+        AQ_HIP(hipHostMalloc((void**)&host_, bytes_, hipHostMallocDefault));
+        if (kind_ == 1) {
+            path_ = d_.get("path");
+            if (path_.empty())
+                throw Error(AQ_EPARSE, "report \"" + name_ +
+                                           "\" of type file without a path");
+            file_.reset(fopen(path_.c_str(), "w"));
+            if (!file_)
+                throw Error(AQ_ENOTFOUND, "cannot write '" + path_ + "'");
+            std::string h = "# ";
+            for (auto& f : fields_) h += f.v->name + " ";
+            h += "\n";
+            fputs(h.c_str(), file_.get());
+            fflush(file_.get());
+        } else if (kind_ == 2) {
+            path_ = d_.get("path");
+            for (auto& v : S.vars.all())
+                if (v->is_array()) mem_ += v->bytes;
+        }
+    }
+    void execute(Sim& S, hipStream_t s) override
+    {
+        for (auto& f : fields_)
+            AQ_HIP(hipMemcpyAsync(host_ + f.off, S.ptr(f.v), f.bytes,
+                                  hipMemcpyDeviceToHost, s));
+        AQ_HIP(hipLaunchHostFunc(s, &ReportTool::cb, this));
+    }
+
+  private:
+    static void cb(void* p) { ((ReportTool*)p)->print(); }
+    std::string text(const char* sep_in_line) const
+    {
+        std::string out;
+        size_t k = 0;
+        for (size_t l = 0; l < per_line_.size(); l++) {
+            for (size_t j = 0; j < per_line_[l]; j++, k++) {
+                const auto& f = fields_[k];
+                const size_t n = f.v->is_array() ? f.v->length : 1;
+                out += report_value(f.v->type, host_ + f.off, n);
+                if (j + 1 < per_line_[l]) out += sep_in_line;
+            }
+            if (l + 1 < per_line_.size()) out += kind_ == 1 ? " " : "\n";
+        }
+        return out;
+    }
+    double scalar(size_t k) const
+    {
+        const auto& f = fields_[k];
+        const uint8_t* q = host_ + f.off;
+        if (f.v->type.base == 'f') {
+            float x;
+            memcpy(&x, q, 4);
+            return x;
+        }
+        if (f.v->type.base == 'd') {
+            double x;
+            memcpy(&x, q, 8);
+            return x;
+        }
+        uint32_t x;
+        memcpy(&x, q, 4);
+        return x;
+    }
+    void print()
+    {
+        if (kind_ == 0) {
+            // Screen: the tool name, then each line of name=value pairs
+            std::string out = name_ + ":\n";
+            size_t k = 0;
+            for (size_t l = 0; l < per_line_.size(); l++) {
+                for (size_t j = 0; j < per_line_[l]; j++, k++) {
+                    const auto& f = fields_[k];
+                    const size_t n = f.v->is_array() ? f.v->length : 1;
+                    out += f.v->name + "=" +
+                           report_value(f.v->type, host_ + f.off, n);
+                    out += (j + 1 < per_line_[l]) ? " " : "\n";
+                }
+            }
+            fputs(out.c_str(), stdout);
+            fflush(stdout);
+        } else if (kind_ == 1) {
+            // TabFile: one line per execution, space separated
+            const std::string out = text(" ") + "\n";
+            fputs(out.c_str(), file_.get());
+            fflush(file_.get());
+        } else {
+            // Performance: memory, wall time since the last execution,
+            // progress by time / steps / frames and the time to arrival
+            const auto now = std::chrono::steady_clock::now();
+            const double el = last_set_ ? std::chrono::duration<double>(
+                                              now - last_).count()
+                                        : 0.0;
+            last_ = now;
+            last_set_ = true;
+            double prog = 0.0;
+            for (int c = 0; c < 3; c++) {
+                const double a = scalar(2 * c), b = scalar(2 * c + 1);
+                if (b > 0) prog = std::max(prog, a / b);
+            }
+            total_ += el;
+            const double eta = prog > 0 ? total_ * (1.0 / prog - 1.0) : 0.0;
+            char b[256];
+            snprintf(b, sizeof(b),
+                     "%s:\nMemory=%zuMB\nElapsed=%.6gs\nPercentage=%.2f"
+                     "   ETA=%.6gs\n",
+                     name_.c_str(), mem_ >> 20, el, 100.0 * prog, eta);
+            fputs(b, stdout);
+            fflush(stdout);
+            if (!path_.empty()) {
+                FILE* f = fopen(path_.c_str(), "w");
+                if (f) {
+                    fprintf(f,
+                            "{\n    \"progress\": %.9g,\n    \"elapsed\": "
+                            "%.9g,\n    \"snapshot\": {\"step\": %.0f, "
+                            "\"samples\": []}\n}\n",
+                            prog, el * 1e9, scalar(2));
+                    fclose(f);
+                }
+            }
+        }
+    }
+
+    struct FileClose {
+        void operator()(FILE* f) const
+        {
+            if (f) fclose(f);
+        }
+    };
+    ToolDesc d_;
+    int kind_;
+    Sim* S_ = nullptr;
+    std::vector<ReportField> fields_;
+    std::vector<size_t> per_line_;
+    uint8_t* host_ = nullptr;
+    size_t bytes_ = 0;
+    std::string path_;
+    std::unique_ptr<FILE, FileClose> file_;
+    size_t mem_ = 0;
+    std::chrono::steady_clock::time_point last_;
+    bool last_set_ = false;
+    double total_ = 0.0;
 };
 
 // ---- kernel: registry of HIP implementations --------------------------------
@@ -727,8 +974,10 @@ std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d)
     if (t == "end") return std::make_shared<EndTool>(d);
     if (t == "copy") return std::make_shared<CopyTool>(d);
     if (t == "installable") return std::make_shared<InstallableTool>(d);
-    if (t == "dummy" || t.compare(0, 7, "report_") == 0)
-        return std::make_shared<DummyTool>(d.get("name"), t, once);
+    if (t == "dummy") return std::make_shared<DummyTool>(d.get("name"), t, once);
+    if (t == "report_screen") return std::make_shared<ReportTool>(d, 0);
+    if (t == "report_file") return std::make_shared<ReportTool>(d, 1);
+    if (t == "report_performance") return std::make_shared<ReportTool>(d, 2);
     throw Error(AQ_EUNSUP, "tool \"" + d.get("name") + "\": type '" + t +
                                "' is not supported by this build");
 }

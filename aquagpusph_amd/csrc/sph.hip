@@ -76,13 +76,14 @@ __global__ void predictor_k(const int* __restrict__ imove,
                             float* __restrict__ dudt_in,
                             float* __restrict__ rho_in,
                             float* __restrict__ drhodt_in, uint32_t N,
-                            const float* dtp)
+                            const float* dtp, const uint32_t* status)
 {
     constexpr int VS = (D == 2) ? 2 : 4;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     float DT = *dtp;
-    if (imove[i] <= 0) DT = 0.f;
+    // a failure flagged earlier: predict no motion (the state is kept)
+    if ((imove[i] <= 0) || (status && *status)) DT = 0.f;
     const float hdt2 = 0.5f * DT * DT;
 #pragma unroll
     for (int k = 0; k < VS; k++) {
@@ -254,11 +255,14 @@ __global__ void corrector_k(const int* __restrict__ imove,
                             const float* __restrict__ drhodt,
                             const float* __restrict__ dudt_in,
                             const float* __restrict__ drhodt_in, uint32_t N,
-                            const float* dtp)
+                            const float* dtp, const uint32_t* status)
 {
     constexpr int VS = (D == 2) ? 2 : 4;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    // a failure flagged earlier (ihoc overflow, failed assert): the state
+    // stays where it was
+    if (status && *status) return;
     if (imove[i] > 0) {
         const float DT = 0.5f * (*dtp);
         const float DT2 = DT * DT;
@@ -1257,18 +1261,20 @@ void launch_predictor(int dims, const int32_t* imove, const float* r,
                       const float* u, const float* dudt, const float* rho,
                       const float* drhodt, float* r_in, float* u_in,
                       float* dudt_in, float* rho_in, float* drhodt_in,
-                      uint32_t N, const float* dt, hipStream_t s)
+                      uint32_t N, const float* dt, const uint32_t* status,
+                      hipStream_t s)
 {
     AQ_DISPATCH(dims, predictor_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
-                r_in, u_in, dudt_in, rho_in, drhodt_in, N, dt);
+                r_in, u_in, dudt_in, rho_in, drhodt_in, N, dt, status);
 }
 void launch_corrector(int dims, const int32_t* imove, float* r, float* u,
                       const float* dudt, float* rho, const float* drhodt,
                       const float* dudt_in, const float* drhodt_in,
-                      uint32_t N, const float* dt, hipStream_t s)
+                      uint32_t N, const float* dt, const uint32_t* status,
+                      hipStream_t s)
 {
     AQ_DISPATCH(dims, corrector_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
-                dudt_in, drhodt_in, N, dt);
+                dudt_in, drhodt_in, N, dt, status);
 }
 void launch_euler_predictor(int dims, const float* r, const float* u,
                             const float* dudt, const float* rho,
@@ -1622,7 +1628,8 @@ AQ_API int aq_predictor(int dims, const int32_t* imove, const float* r,
         AQ_PTRS(imove, r, u, dudt, rho, drhodt, r_in, u_in, dudt_in, rho_in,
                 drhodt_in, dt);
         launch_predictor(dims, imove, r, u, dudt, rho, drhodt, r_in, u_in,
-                         dudt_in, rho_in, drhodt_in, N, dt, S(stream));
+                         dudt_in, rho_in, drhodt_in, N, dt, nullptr,
+                         S(stream));
     });
 }
 
@@ -1636,7 +1643,7 @@ AQ_API int aq_corrector(int dims, const int32_t* imove, float* r, float* u,
         if (!N) return;
         AQ_PTRS(imove, r, u, dudt, rho, drhodt, dudt_in, drhodt_in, dt);
         launch_corrector(dims, imove, r, u, dudt, rho, drhodt, dudt_in,
-                         drhodt_in, N, dt, S(stream));
+                         drhodt_in, N, dt, nullptr, S(stream));
     });
 }
 

@@ -57,8 +57,13 @@ def test_run_saves_unsorted_files_and_resumes(cuda, tmp_path, monkeypatch):
     du/dt at the element's slot of the sorted arrays, which the sort does
     not carry along (basic/Sort.cl:119-123 moves only du/dt_in), so with BI
     walls a resumed run differs from the continued one by that slot history
-    (in the reference as well); without them the two agree to the 7 digits
-    the particle file is written with."""
+    (in the reference as well).  Without them the particle files, written
+    with 9 significant digits, read back to the same float bits, and the
+    resumed run equals the continued one bit for bit as long as no two
+    particles that share a cell reached it from different cells before the
+    resume (the stable sort keeps such ties in the order of the previous
+    step, a resume in the order of the particle ids), which holds over the
+    first steps of the dam break."""
     monkeypatch.chdir(tmp_path)
     dims = 2
     case = cases.dam_break(dims, 0.3 / 20)
@@ -76,8 +81,7 @@ def test_run_saves_unsorted_files_and_resumes(cuda, tmp_path, monkeypatch):
     assert names == ["Fluid.00000.dat", "Fluid.00001.dat", "Fluid.00002.dat"]
     assert os.path.exists(tmp_path / "AQUAgpusph.save.0.xml")
     assert int(sim.get("frame")) == 2 and int(sim.get("iter")) == 8
-    # the last file holds the state in the original particle order, to the
-    # 7 significant digits it is written with
+    # the last file holds the state in the original particle order, exactly
     ids = sim.get("id").astype(np.int64)
     widths = []
     for f in fields:
@@ -88,9 +92,8 @@ def test_run_saves_unsorted_files_and_resumes(cuda, tmp_path, monkeypatch):
         v = sim.get(f).astype(np.float64).reshape(len(ids), -1)
         orig = np.zeros_like(v)
         orig[ids] = v
-        np.testing.assert_allclose(col, orig, rtol=1e-6,
-                                   atol=1e-6 * (np.abs(orig).max() + 1e-30),
-                                   err_msg=f)
+        np.testing.assert_array_equal(col.astype(np.float32),
+                                      orig.astype(np.float32), err_msg=f)
     # resume from the state file and compare with the continued run
     res = Simulation(str(tmp_path / "AQUAgpusph.save.0.xml"), dims)
     for k in ("t", "dt", "iter", "frame"):
@@ -106,8 +109,44 @@ def test_run_saves_unsorted_files_and_resumes(cuda, tmp_path, monkeypatch):
         o[i] = v
         return o
 
-    for f in ("r", "u", "rho"):
-        a, b = orig(res, f), orig(sim, f)
-        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), f
+    for f in ("r", "u", "rho", "p"):
+        np.testing.assert_array_equal(orig(res, f), orig(sim, f), err_msg=f)
     sim.close()
     res.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_time_control_reports(cuda, tmp_path, monkeypatch, capfd, graph):
+    """basic/timing.report.xml: the file report writes "iter t dt" after
+    every step (Reports/TabFile.cpp), in eager and in graph-replayed steps
+    (a pinned copy plus a host node), and the screen report prints the
+    same values; unbuilt report types are refused."""
+    monkeypatch.chdir(tmp_path)
+    case = cases.dam_break(2, 0.3 / 10)
+    presets = casefiles.PBI_PRESETS + (
+        "resources/Presets/basic/timing.report.xml",)
+    xml = casefiles.write_case(case, str(tmp_path / "case"), presets=presets)
+    sim = Simulation(xml, 2)
+    rows = []
+    for k in range(3):
+        sim.step(2, graph=graph)
+        rows.append((int(sim.get("iter")), float(np.float32(sim.get("t"))),
+                     float(np.float32(sim.get("dt")))))
+    sim.close()
+    lines = [l.split() for l in open(tmp_path / "Timing.dat")]
+    assert lines[0][0] == "#" and lines[0][1:] == ["iter", "t", "dt"]
+    vals = [(int(a), float(b), float(c)) for a, b, c in lines[1:]]
+    assert len(vals) == 6
+    for k, (it, t, dt) in enumerate(rows):
+        assert vals[2 * k + 1][0] == it
+        assert np.float32(vals[2 * k + 1][1]) == np.float32(t)
+    assert [v[0] for v in vals] == list(range(1, 7))
+    out = capfd.readouterr().out
+    assert out.count("Time control:") == 6
+    assert "iter=6 " in out
+    bad = tmp_path / "bad"
+    xml = casefiles.write_case(case, str(bad), extra_xml=(
+        '<Reports><Report type="dump" name="d" fields="r" path="x.dat"/>'
+        '</Reports>'))
+    with pytest.raises(Exception, match="not supported"):
+        Simulation(xml, 2)

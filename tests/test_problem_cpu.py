@@ -111,3 +111,27 @@ def test_batch_preset_tool_order(tmp_path):
     text = open(xml).read()
     assert text.count("<ParticlesSet") == 3
     assert text.count('<Scalar name="exc_omega"') == 3
+
+
+def test_reports_run_after_every_tool(tmp_path):
+    """<Reports> are appended after all tools (CalcServer.cpp:472-520), as
+    report_<type> tools; timing.report.xml holds a screen and a file
+    report, performance.report.xml a performance report."""
+    case = cases.dam_break(2, 0.3 / 10)
+    presets = casefiles.PBI_PRESETS + (
+        "resources/Presets/basic/timing.report.xml",
+        "resources/Presets/basic/performance.report.xml")
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets)
+    tl = problem_tools(xml, 2)
+    assert tl[-3:] == [("Time control", "report_screen"),
+                       ("Time control file", "report_file"),
+                       ("Performance", "report_performance")]
+    names = [n for n, _ in tl]
+    assert names.index("iter += 1") < names.index("Time control")
+    # a file report without a path is rejected at parse time
+    bad = tmp_path / "bad.xml"
+    bad.write_text('<?xml version="1.0" ?><sphInput><Reports>'
+                   '<Report type="file" name="x" fields="t"/></Reports>'
+                   '</sphInput>')
+    with pytest.raises(Exception, match="path"):
+        problem_tools(str(bad), 2)

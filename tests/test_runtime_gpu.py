@@ -767,3 +767,34 @@ def test_neighbour_count_preset(cuda, tmp_path):
     sim.set("neighs_limit", np.uint32(n.max()))
     with pytest.raises(_lib.AquaError, match="check number of neighbours"):
         sim.step(1)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_failure_names_its_step_and_freezes_the_state(cuda, tmp_path, graph):
+    """A failure flagged in the middle of step(10) (here an assert that
+    fails once iter reaches 5) is reported with the step it happened in,
+    and the predictor / corrector leave the particles as they were at the
+    end of that step for the remaining steps of the call."""
+    case = cases.dam_break(2, 0.3 / 10)
+    xml = casefiles.write_case(case, str(tmp_path), extra_xml="""
+    <Tools>
+        <Tool action="insert" before="End" name="stop at five" type="assert" condition="iter &lt; 5"/>
+    </Tools>""")
+    sim = Simulation(xml, 2)
+    with pytest.raises(_lib.AquaError, match="iter = 5"):
+        sim.step(10, graph=graph)
+    ref = Simulation(casefiles.write_case(case, str(tmp_path / "ref")), 2)
+    ref.step(5)
+
+    def by_id(s, f):
+        i = s.get("id").astype(np.int64)
+        v = s.get(f)
+        o = np.zeros_like(v)
+        o[i] = v
+        return o
+
+    for f in ("r", "u", "rho"):
+        np.testing.assert_array_equal(by_id(sim, f), by_id(ref, f),
+                                      err_msg=f)
+    sim.close()
+    ref.close()
