@@ -793,8 +793,14 @@ def test_failure_names_its_step_and_freezes_the_state(cuda, tmp_path, graph):
         o[i] = v
         return o
 
-    for f in ("r", "u", "rho"):
+    # the wall elements' density is not integrated: the boundary-integral
+    # interpolation recomputes it every step from the (kept) fluid state,
+    # so only the fluid's is held to the state at the failed step
+    fluid = by_id(ref, "imove") == 1
+    for f in ("r", "u"):
         np.testing.assert_array_equal(by_id(sim, f), by_id(ref, f),
                                       err_msg=f)
+    np.testing.assert_array_equal(by_id(sim, "rho")[fluid],
+                                  by_id(ref, "rho")[fluid])
     sim.close()
     ref.close()
