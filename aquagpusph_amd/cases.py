@@ -466,6 +466,77 @@ def excitation_xml(case):
             % (f(a), f(w), f(g0[0]), f(g0[1])))
 
 
+def with_sensors(case, pos, normal, tangent=None):
+    """The case plus a second particle set of pressure sensors: fixed
+    massless particles (imove = 0, m = 0, rho = refd, u = 0) at `pos` with
+    the given normals/tangents, set 1 with the same refd / visc_dyn / delta
+    (the sensor set of examples/3D/spheric_testcase2_dambreak: Create.py
+    writes Sensors.dat with mass 0 and imove 0, templates/Sensors.xml loads
+    it as its own ParticlesSet)."""
+    dims = case["dims"]
+    vs = 2 if dims == 2 else 4
+    pos = np.asarray(pos, float).reshape(-1, dims)
+    ns = pos.shape[0]
+
+    def vec(a):
+        out = np.zeros((ns, vs))
+        if a is not None:
+            out[:, :dims] = np.broadcast_to(np.asarray(a, float)[..., :dims],
+                                            (ns, dims))
+        return out
+
+    p = dict(case["params"])
+    sc = {"refd": p["refd"][0], "visc_dyn": p["visc_dyn"][0],
+          "delta": p["delta"][0]}
+    out = dict(case)
+    N = int(case["N"])
+    out["N"] = N + ns
+    out["r"] = np.concatenate([case["r"], vec(pos)])
+    out["normal"] = np.concatenate([case["normal"], vec(normal)])
+    out["tangent"] = np.concatenate([case["tangent"], vec(tangent)])
+    for k in ("u", "dudt"):
+        out[k] = np.concatenate([case[k], np.zeros((ns, vs))])
+    out["rho"] = np.concatenate([case["rho"], np.full(ns, sc["refd"])])
+    out["drhodt"] = np.concatenate([case["drhodt"], np.zeros(ns)])
+    out["m"] = np.concatenate([case["m"], np.zeros(ns)])
+    out["imove"] = np.concatenate([case["imove"], np.zeros(ns, np.int32)])
+    out["iset"] = np.concatenate([case.get("iset", np.zeros(N, np.uint32)),
+                                  np.ones(ns, np.uint32)]).astype(np.uint32)
+    out["sets"] = [(N, sc), (ns, dict(sc))]
+    for k in ("refd", "visc_dyn", "delta"):
+        p[k] = [p[k][0], p[k][0]]
+    out["params"] = p
+    out["n_sensors"] = ns
+    return out
+
+
+def dam_break_sensors(dims, B=0.6, H=0.3, W=0.6):
+    """Sensor points for the dam break: on the left wall inside the column
+    (normal +x, into the fluid), on the floor under it (normal +z), inside
+    the column, and one on the far wall that the water reaches later."""
+    v = dims - 1
+    pts, nrm = [], []
+
+    def add(x, z, n):
+        q = [0.0] * dims
+        q[0], q[v] = x, z
+        if dims == 3:
+            q[1] = 0.5 * W
+        pts.append(q)
+        nn = [0.0] * dims
+        nn[n[0]] = n[1]
+        nrm.append(nn)
+
+    for k in range(4):
+        add(-B, (0.07 + 0.2 * k) * H, (0, 1.0))
+    for k in range(3):
+        add(-B + (0.15 + 0.3 * k) * B, 0.0, (v, 1.0))
+    for k in range(3):
+        add(-B + (0.25 + 0.25 * k) * B, (0.3 + 0.2 * k) * H, (v, 1.0))
+    add(0.9, 0.05 * H, (0, -1.0))
+    return np.array(pts), np.array(nrm)
+
+
 def dam_break_dr_for(n_fluid, dims=3, B=0.6, H=0.3, W=0.6):
     """Lattice spacing that gives about n_fluid particles in the column."""
     vol = B * H * (W if dims == 3 else 1.0)

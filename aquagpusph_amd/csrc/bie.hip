@@ -197,6 +197,68 @@ __global__ void __launch_bounds__(BS)
     stxyz<D>(dudt, i, a);
 }
 
+// cfd/Boundary/ElasticBounce.cl:entry: the plain elastic bounce of the
+// fluid on ghost-particle (-2) and boundary-integral (-3) wall elements.  A
+// fluid particle within R = DR_FACTOR dr of an element (tangentially), on
+// the fluid side of it, that would come closer than MIN_BOUND_DIST dr in
+// the next dt loses (1 + ELASTIC_FACTOR) times its normal relative velocity
+// and acceleration.  u and dudt are updated in place; the next elements of
+// the walk see the corrected values (the reference's order).
+template <int D>
+__global__ void __launch_bounds__(BS)
+    elastic_bounce_k(const int* __restrict__ imove, const float* __restrict__ r,
+                     const float* __restrict__ normal, float* __restrict__ u,
+                     float* __restrict__ dudt, Cells c, const float* drp,
+                     const float* dtp, float DR_FACTOR, float MIN_BOUND_DIST,
+                     float ELASTIC_FACTOR)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    if (skip(c)) return;
+    if (imove[i] != 1) return;
+    const float dt = *dtp, dr = *drp;
+    const float Rr = DR_FACTOR * dr;
+    const float e1 = 1.f + ELASTIC_FACTOR;
+    const f3 r_i = ldv<D>(r, i);
+    f3 ui = ldv<D>(u, i);
+    f3 ai = ldv<D>(dudt, i);
+    bool hit = false;
+    walk<D>(c.icell[i], c.N, c.n_cells[0], c.n_cells[1], c.icell, c.ihoc,
+            [&](uint32_t j) {
+                const int mj = imove[j];
+                if ((mj != -2) && (mj != -3)) return;
+                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
+                const f3 n_j = ldv<D>(normal, j);
+                const float r0 = dot<D>(r_ij, n_j);
+                if (r0 < 0.f) return;  // behind the element
+                f3 rt;
+                rt.x = r_ij.x - r0 * n_j.x;
+                rt.y = r_ij.y - r0 * n_j.y;
+                rt.z = (D == 3) ? r_ij.z - r0 * n_j.z : 0.f;
+                if (dot<D>(rt, rt) >= Rr * Rr) return;  // passing too far
+                const float u_n = dot<D>(sub<D>(ui, ldv<D>(u, j)), n_j);
+                const float a_n = dot<D>(sub<D>(ai, ldv<D>(dudt, j)), n_j);
+                const float dist = dt * u_n + 0.5f * dt * dt * a_n;
+                if (dist < 0.f) return;  // already moving away
+                if (r0 - dist <= MIN_BOUND_DIST * dr) {
+                    const float sa = e1 * a_n, su = e1 * u_n;
+                    ai.x = ai.x - sa * n_j.x;
+                    ai.y = ai.y - sa * n_j.y;
+                    ui.x = ui.x - su * n_j.x;
+                    ui.y = ui.y - su * n_j.y;
+                    if (D == 3) {
+                        ai.z = ai.z - sa * n_j.z;
+                        ui.z = ui.z - su * n_j.z;
+                    }
+                    hit = true;
+                }
+            });
+    if (hit) {
+        stxyz<D>(dudt, i, ai);
+        stxyz<D>(u, i, ui);
+    }
+}
+
 // BIe/PST.cl:entry (r of the fluid particle is moved in place)
 template <int D>
 __global__ void __launch_bounds__(BS)
@@ -285,6 +347,19 @@ void launch_bie_bounce(int dims, const int32_t* imove, const float* r_in,
     Cells c{ icell, ihoc, n_cells, status, N };
     AQ_D(dims, bie_bounce_k, AQ_G(N), imove, r_in, normal, m, u_in, dudt, c, dt,
          dr_factor, min_bound_dist);
+}
+
+void launch_elastic_bounce(int dims, const int32_t* imove, const float* r,
+                           const float* normal, float* u, float* dudt,
+                           uint32_t N, const float* dr, const float* dt,
+                           const uint32_t* icell, const uint32_t* ihoc,
+                           const uint32_t* n_cells, const uint32_t* status,
+                           float dr_factor, float min_bound_dist,
+                           float elastic_factor, hipStream_t s)
+{
+    Cells c{ icell, ihoc, n_cells, status, N };
+    AQ_D(dims, elastic_bounce_k, AQ_G(N), imove, r, normal, u, dudt, c, dr, dt,
+         dr_factor, min_bound_dist, elastic_factor);
 }
 
 void launch_bie_pst(int dims, const int32_t* imove, float* r,

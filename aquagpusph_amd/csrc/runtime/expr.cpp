@@ -1,5 +1,6 @@
 #include "expr.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
@@ -484,6 +485,32 @@ void VmProgram::store(const SlotRef& s)
 void VmProgram::assert_true(int32_t id)
 {
     ops.push_back({ VM_ASSERT, id, 0.0 });
+}
+
+void VmProgram::end()
+{
+    ops.push_back({ VM_END, 0, 0.0 });
+    int sp = 0, top = 0;
+    for (const VmOp& op : ops) {
+        switch (op.code) {
+        case VM_END: break;
+        case VM_PUSH: case VM_LOADF: case VM_LOADI: case VM_LOADU:
+        case VM_LOADD: sp++; break;
+        case VM_STOREF: case VM_STOREI: case VM_STOREU: case VM_STORED:
+        case VM_ASSERT: sp--; break;
+        case VM_NEG: case VM_NOT: case VM_F1: break;
+        case VM_SEL: sp -= 2; break;
+        default: sp--; break;  // binary operators and two-argument calls
+        }
+        if (sp < 0)
+            throw Error(AQ_EINVAL, "scalar expression: malformed program");
+        top = std::max(top, sp);
+    }
+    if (top > VM_STACK)
+        throw Error(AQ_EUNSUP, "scalar expression too deep for the device "
+                               "interpreter (" + std::to_string(top) +
+                               " values at once, at most " +
+                               std::to_string(VM_STACK) + ")");
 }
 
 }  // namespace rt

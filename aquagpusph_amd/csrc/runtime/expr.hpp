@@ -89,13 +89,18 @@ struct SlotRef {
 };
 typedef std::function<bool(const std::string&, SlotRef&)> SlotFn;
 
+// Depth of the device interpreter's value stack.
+constexpr int VM_STACK = 32;
+
 class VmProgram {
   public:
     // push code that leaves the value of e on the stack
     void compile(const ExprP& e, const SlotFn& slots);
     void store(const SlotRef& s);
     void assert_true(int32_t id);
-    void end() { ops.push_back({ VM_END, 0, 0.0 }); }
+    // close the program; raises AQ_EUNSUP when an expression needs more
+    // than VM_STACK values at once (the device stack is a fixed array)
+    void end();
     std::vector<VmOp> ops;
 };
 

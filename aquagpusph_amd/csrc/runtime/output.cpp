@@ -122,10 +122,20 @@ static void write_ascii(Sim& S, const std::string& path, size_t off, size_t n,
     if (!f) throw Error(AQ_ENOTFOUND, "cannot write '" + path + "'");
     char b[64];
     snprintf(b, sizeof(b), "%g", t);
-    f << "# particles written by aquagpusph_amd\n# t = " << b << " s\n# ";
+    // The header spans 18 lines, 17 comment lines and a blank one, as the
+    // reference's (ASCII::print_header): readers such as the reference's
+    // check scripts skip exactly that many lines before the data.
+    std::string names;
     for (size_t k = 0; k < fields.size(); k++)
-        f << fields[k] << (k + 1 < fields.size() ? ", " : "\n");
-    f << "\n";
+        names += fields[k] + (k + 1 < fields.size() ? ", " : "");
+    const std::string rule(57, '#');
+    f << rule << "\n#\n#    aquagpusph_amd particle file (ASCII)\n#\n"
+      << "#    t = " << b << " s\n#    particles: " << n
+      << "\n#    fields: " << names
+      << "\n#\n#    one particle per line, in the original order;\n"
+      << "#    fields separated by ',', vector components by ' '\n"
+      << "#    (the layout of the FastASCII loader)\n#\n#\n#\n#\n#\n"
+      << rule << "\n\n";
     std::string line;
     for (size_t i = 0; i < n; i++) {
         line.clear();
@@ -334,8 +344,13 @@ void Sim::save_now()
             if (fmt != "ascii" && fmt != "fastascii")
                 throw Error(AQ_EUNSUP, "particles output format '" + s.format +
                                            "' is not supported (ASCII)");
+            // a name without an index takes ".{index}.dat" (ASCII::create)
+            std::string pattern = s.file;
+            if (pattern.find("{index}") == std::string::npos &&
+                pattern.find("%d") == std::string::npos)
+                pattern += ".{index}.dat";
             const std::string path =
-                new_file_path(s.file, out.next_index[k], 5);
+                new_file_path(pattern, out.next_index[k], 5);
             write_ascii(*this, path, off, sd.n, s.fields, t);
             out.next_index[k]++;
             if (j == 0) out.last_file[k] = path;

@@ -272,6 +272,11 @@ void launch_bie_bounce(int, const int32_t*, const float*, const float*,
                        const float*, const uint32_t*, const uint32_t*,
                        const uint32_t*, const uint32_t*, float, float,
                        hipStream_t);
+void launch_elastic_bounce(int, const int32_t*, const float*, const float*,
+                           float*, float*, uint32_t, const float*,
+                           const float*, const uint32_t*, const uint32_t*,
+                           const uint32_t*, const uint32_t*, float, float,
+                           float, hipStream_t);
 void launch_bie_pst(int, const int32_t*, float*, const float*, const float*,
                     const float*, uint32_t, const uint32_t*, const uint32_t*,
                     const uint32_t*, const uint32_t*, aq_defs, float,
@@ -1000,6 +1005,17 @@ static const std::map<std::string, KernelSpec>& table()
                                   S.N, F(6), U(7), U(8), U(9), S.d_status,
                                   def_or(S, "__DR_FACTOR__", 0.5f),
                                   def_or(S, "__MIN_BOUND_DIST__", 0.f), s);
+            } } },
+        { "cfd/Boundary/ElasticBounce.cl:entry",
+          { { "imove", "r", "normal", "u", "dudt", "dr", "dt", "icell", "ihoc",
+              "n_cells" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_elastic_bounce(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                      S.N, F(5), F(6), U(7), U(8), U(9),
+                                      S.d_status,
+                                      def_or(S, "__DR_FACTOR__", 1.5f),
+                                      def_or(S, "__MIN_BOUND_DIST__", 0.3f),
+                                      def_or(S, "__ELASTIC_FACTOR__", 0.f), s);
             } } },
         { "cfd/Boundary/BIe/PST.cl:entry",
           { { "imove", "r", "normal", "m", "rho", "icell", "ihoc",

@@ -614,7 +614,8 @@ size_t Sim::control(size_t pc, hipStream_t s)
 
 // Tool::execute: a `once` tool runs the first time it is reached only.
 void Sim::run_tools(hipStream_t s,
-                    const std::function<void(size_t, bool)>& hook)
+                    const std::function<void(size_t, bool)>& hook,
+                    bool stop_on_assert)
 {
     size_t pc = 0;
     while (pc < tools.size()) {
@@ -629,6 +630,10 @@ void Sim::run_tools(hipStream_t s,
             t->execute(*this, s);
             if (hook) hook(pc, true);
             t->done = true;
+            if (stop_on_assert && t->type() == "assert") {
+                stamp_status(s);
+                check_status();
+            }
         }
         pc++;
     }
@@ -722,7 +727,7 @@ void Sim::step(int nsteps)
 {
     if (!is_setup) throw Error(AQ_EINVAL, "setup() has not been called");
     if (vars.host_dirty()) vars.push(stream);
-    for (int k = 0; k < nsteps; k++) run_tools(stream);
+    for (int k = 0; k < nsteps; k++) run_tools(stream, {}, true);
     check_status();
 }
 

@@ -360,9 +360,14 @@ class Sim {
     void* ptr(Var* v);
     int assert_id(const std::string& name);
     // one step of the pipeline in order, following while/if/end; `hook` is
-    // called around each executed tool (index, false before / true after)
+    // called around each executed tool (index, false before / true after).
+    // With stop_on_assert (eager steps only: it synchronises the stream),
+    // a failed assert raises right away and the tools after it in the step
+    // are not run (the reference skips them: Tool.cpp:112-118,
+    // tests/3D/Assert).
     void run_tools(hipStream_t s,
-                   const std::function<void(size_t, bool)>& hook = {});
+                   const std::function<void(size_t, bool)>& hook = {},
+                   bool stop_on_assert = false);
     // record the iteration of the first failed step (end of each step)
     void stamp_status(hipStream_t s);
 

@@ -298,6 +298,9 @@ static void parse_tools(const XmlNode& root, Problem& P,
             } else if (type == "installable") {
                 tool_attr(*t, *e, "path", nullptr);
                 t->attrs["path"] = find_path(t->get("path"), P);
+                // every other attribute reaches aq_tool_create as key/value
+                for (auto& kv : e->attrs)
+                    if (!t->attrs.count(kv.first)) t->attrs[kv.first] = kv.second;
             } else {
                 for (auto& kv : e->attrs)
                     if (!t->attrs.count(kv.first)) t->attrs[kv.first] = kv.second;

@@ -565,10 +565,14 @@ class LinkListTool : public Tool {
 };
 
 // ---- radix-sort: RadixSort.cpp ----------------------------------------------
+// type="sort" (Sort.cpp, a bitonic sort) runs the same stable radix sort:
+// its contract is sorted keys with consistent permutations (perm[i] is the
+// original index of the i-th sorted key, inv_perm its inverse), which the
+// stable order fulfils; among equal keys the bitonic order is not specified.
 class RadixSortTool : public Tool {
   public:
     RadixSortTool(const ToolDesc& d)
-        : Tool(d.get("name"), "radix-sort", d.get("once") == "true"), d_(d)
+        : Tool(d.get("name"), d.get("type"), d.get("once") == "true"), d_(d)
     {
     }
     void setup(Sim& S) override
@@ -578,11 +582,11 @@ class RadixSortTool : public Tool {
         inv_ = &S.vars.at(d_.get("inv_perm"));
         for (Var* v : { in_, perm_, inv_ })
             if (!v->is_array() || v->type.base != 'u' || v->type.ncomp != 1)
-                throw Error(AQ_EINVAL, "radix-sort \"" + name_ +
+                throw Error(AQ_EINVAL, type_ + " \"" + name_ +
                                            "\": variables must be "
                                            "unsigned int arrays");
         if (perm_->length != in_->length || inv_->length != in_->length)
-            throw Error(AQ_EINVAL, "radix-sort \"" + name_ +
+            throw Error(AQ_EINVAL, type_ + " \"" + name_ +
                                        "\": lengths mismatch");
         ws_.alloc(radix_sort_workspace((uint32_t)in_->length));
     }
@@ -751,12 +755,14 @@ class SetScalarTool : public VmTool {
         auto slots = [&](const std::string& n, SlotRef& r) {
             return S.vars.slot(n, r);
         };
+        // every component is evaluated from the values before the tool
+        // runs, then stored (a swap "v_y,v_x" exchanges them)
         VmProgram p;
-        for (int k = 0; k < v.type.ncomp && k < (int)comps.size(); k++) {
-            p.compile(parse_expr(comps[k]), slots);
+        const int nc = std::min(v.type.ncomp, (int)comps.size());
+        for (int k = 0; k < nc; k++) p.compile(parse_expr(comps[k]), slots);
+        for (int k = nc - 1; k >= 0; k--)
             p.store(SlotRef{ v.slot + (uint32_t)v.type.comp_bytes() * k,
                              v.type.base });
-        }
         p.end();
         upload(p);
     }
@@ -965,7 +971,8 @@ std::shared_ptr<Tool> make_tool(Sim& S, const ToolDesc& d)
     const bool once = d.get("once") == "true";
     if (t == "kernel") return std::make_shared<KernelTool>(d);
     if (t == "link-list") return std::make_shared<LinkListTool>(d);
-    if (t == "radix-sort") return std::make_shared<RadixSortTool>(d);
+    if (t == "radix-sort" || t == "sort")
+        return std::make_shared<RadixSortTool>(d);
     if (t == "reduction") return std::make_shared<ReductionTool>(d);
     if (t == "set") return std::make_shared<SetTool>(d);
     if (t == "set_scalar") return std::make_shared<SetScalarTool>(d);

@@ -6,8 +6,6 @@
 namespace aq {
 namespace rt {
 
-constexpr int VM_STACK = 32;
-
 __device__ double dfn1(int32_t f, double x)
 {
     switch (f) {

@@ -28,9 +28,12 @@ def _base(type_name):
     return "f"
 
 
-def problem_tools(xml_path, dims=3):
+def problem_tools(xml_path, dims=3, keys=False):
     """The (name, type) tool list a case file resolves to, in execution
-    order, parsed on the host (aq_xml_resolve: no GPU needed)."""
+    order, parsed on the host (aq_xml_resolve: no GPU needed).  With
+    keys=True each entry is (name, type, key): the registry key
+    "<script under resources/Scripts>:<entry_point>" of a kernel tool, the
+    `in` variable of the others."""
     L = _lib.lib()
     L.aq_xml_resolve.argtypes = [ctypes.c_char_p, ctypes.c_int,
                                  ctypes.c_char_p, ctypes.c_size_t,
@@ -41,8 +44,16 @@ def problem_tools(xml_path, dims=3):
     buf = ctypes.create_string_buffer(need.value)
     _lib.check(L.aq_xml_resolve(path, dims, buf, need.value,
                                 ctypes.byref(need)))
-    return [tuple(line.split("\t")[:2]) for line in
+    return [tuple(line.split("\t")[:3 if keys else 2]) for line in
             buf.value.decode().splitlines() if not line.startswith("#")]
+
+
+def registered_kernels():
+    """Registry keys of every kernel script this library implements."""
+    L = _lib.lib()
+    L.aq_sim_registered_kernel.restype = ctypes.c_char_p
+    return [L.aq_sim_registered_kernel(i).decode()
+            for i in range(L.aq_sim_n_registered_kernels())]
 
 
 class Simulation:

@@ -463,6 +463,15 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
                                  LL{ ic, hoc, nc }, DR_FACTOR,                 \
                                  MIN_BOUND_DIST);                              \
     }                                                                          \
+    ORC_EXPORT void orc_elastic_bounce_##SR##_##D##d(                          \
+        const int* imove, const R* r, const R* normal, R* u, R* dudt, usize N, \
+        R dr, R dt, const usize* ic, const usize* hoc, const usize* nc,        \
+        R DR_FACTOR, R MIN_BOUND_DIST, R ELASTIC_FACTOR)                       \
+    {                                                                          \
+        elastic_bounce<D, R>(imove, r, normal, u, dudt, N, dr, dt,             \
+                             LL{ ic, hoc, nc }, DR_FACTOR, MIN_BOUND_DIST,     \
+                             ELASTIC_FACTOR);                                  \
+    }                                                                          \
     ORC_EXPORT void orc_bie_pst_##SR##_##D##d(                                 \
         const int* imove, R* r, const R* normal, const R* m, const R* rho,     \
         usize N, const usize* ic, const usize* hoc, const usize* nc, R DIMS,   \

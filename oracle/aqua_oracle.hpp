@@ -1357,6 +1357,60 @@ void bie_rates(const int* imove, const R* rho, const R* p, const R* u,
     }
 }
 
+// cfd/Boundary/ElasticBounce.cl:77-148: the plain elastic bounce on ghost
+// (-2) and boundary-integral (-3) elements.  Particle i (fluid) within
+// R = DR_FACTOR dr of element j tangentially and on its fluid side, whose
+// normal approach dist = dt u_n + dt^2/2 a_n (relative to the element)
+// would bring it within MIN_BOUND_DIST dr, loses (1 + ELASTIC_FACTOR) times
+// its normal relative velocity and acceleration; u and dudt are updated in
+// place and the following elements see the corrected values.
+template <int D, typename R>
+void elastic_bounce(const int* imove, const R* r, const R* normal, R* u,
+                    R* dudt, usize N, R dr, R dt, LL ll, R DR_FACTOR,
+                    R MIN_BOUND_DIST, R ELASTIC_FACTOR)
+{
+    const int VS = Dim<D>::VS;
+    const R Rr = DR_FACTOR * dr;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const usize i = (usize)ii;
+        if (imove[i] != 1) continue;
+        const V<D, R> r_i = ld<D, R>(r, i);
+        V<D, R> ui = ld<D, R>(u, i);
+        V<D, R> ai = ld<D, R>(dudt, i);
+        bool hit = false;
+        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            if ((imove[j] != -2) && (imove[j] != -3)) return;
+            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> n_j = ld<D, R>(normal, j);
+            const R r0 = dot<D, R>(r_ij, n_j);
+            if (r0 < R(0)) return;
+            V<D, R> rt;
+            for (int k = 0; k < 3; k++)
+                rt.c[k] = (k < D) ? r_ij.c[k] - r0 * n_j.c[k] : R(0);
+            if (dot<D, R>(rt, rt) >= Rr * Rr) return;
+            const R u_n = dot<D, R>(sub<D, R>(ui, ld<D, R>(u, j)), n_j);
+            const R a_n = dot<D, R>(sub<D, R>(ai, ld<D, R>(dudt, j)), n_j);
+            const R dist = dt * u_n + R(0.5f) * dt * dt * a_n;
+            if (dist < R(0)) return;
+            if (r0 - dist <= MIN_BOUND_DIST * dr) {
+                const R e1 = R(1.f) + ELASTIC_FACTOR;
+                const R sa = e1 * a_n, su = e1 * u_n;
+                for (int k = 0; k < D; k++) {
+                    ai.c[k] = ai.c[k] - sa * n_j.c[k];
+                    ui.c[k] = ui.c[k] - su * n_j.c[k];
+                }
+                hit = true;
+            }
+        });
+        if (hit)
+            for (int k = 0; k < D; k++) {
+                dudt[(size_t)i * VS + k] = ai.c[k];
+                u[(size_t)i * VS + k] = ui.c[k];
+            }
+    }
+}
+
 // BIe/ElasticBounce.cl:entry: reflect the velocity of fluid particles about
 // to cross an element (within DR_FACTOR dr of its centre, tangentially)
 template <int D, typename R>

@@ -65,7 +65,7 @@ class OracleSim:
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
                  midpoint=None, portals=None, precision=False,
-                 excitation=None, les=None):
+                 excitation=None, les=None, elastic_bounce=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -93,6 +93,13 @@ class OracleSim:
         # cfd/LES.xml (les = {"params": P}) or cfd/LES-smagorinsky.xml
         # (les = {"Cs": C}): the eddy viscosity after the delta-SPH rate
         self.les = les
+        # cfd/elasticBounce.xml: {"dr": lattice spacing, and optionally
+        # "dr_factor" (1.5), "min_bound_dist" (0.3), "elastic_factor" (0)}
+        if elastic_bounce is not None:
+            eb = dict(dr_factor=1.5, min_bound_dist=0.3, elastic_factor=0.0)
+            eb.update(elastic_bounce)
+            elastic_bounce = eb
+        self.elastic_bounce = elastic_bounce
         if precision and (midpoint is not None or dtype != np.float32):
             raise ValueError("the precision mode is a float32 improved-Euler "
                              "model")
@@ -687,6 +694,13 @@ class OracleSim:
                 float(g0[0]) + a * w * w * math.sin(w * t)))
             self.g[1] = self.dtype.type(g0[1])
         self._forces(ll)
+        if self.elastic_bounce is not None:
+            eb = self.elastic_bounce
+            k("elastic_bounce")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
+                                _p(f["u"]), _p(f["dudt"]), n, R(eb["dr"]),
+                                R(self.dt), *ll, R(eb["dr_factor"]),
+                                R(eb["min_bound_dist"]),
+                                R(eb["elastic_factor"]))
         if o["bie"]:
             k("bie_elastic_bounce")(_p(f["imove"]), _p(f["r_in"]),
                                     _p(f["normal"]), _p(f["m"]),

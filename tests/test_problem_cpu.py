@@ -135,3 +135,44 @@ def test_reports_run_after_every_tool(tmp_path):
                    '</sphInput>')
     with pytest.raises(Exception, match="path"):
         problem_tools(str(bad), 2)
+
+
+# the include list of examples/2D/spheric_testcase5_dambreak/src/templates/
+# Main.xml:22-32 (presets only; the case's own Settings/SPH/Time/Fluids
+# files are replaced by this case's variables and particle set)
+TESTCASE5 = ("resources/Presets/basic.xml",
+             "resources/Presets/basic/time_scheme/improved_euler.xml",
+             "resources/Presets/basic/domain.xml",
+             "resources/Presets/cfd.xml",
+             "resources/Presets/cfd/deltaSPH.xml",
+             "resources/Presets/cfd/deltaSPH-full.xml",
+             "resources/Presets/cfd/BI.xml",
+             "resources/Presets/cfd/elasticBounce.xml",
+             "resources/Presets/cfd/variableTimeStep.xml",
+             "resources/Presets/basic/timing.report.xml",
+             "resources/Presets/basic/performance.report.xml")
+
+
+def test_spheric_testcase5_presets_resolve(tmp_path):
+    """The preset stack of the reference's 2D dam-break example resolves,
+    every kernel tool of it has a HIP implementation in the registry, and
+    the elastic bounce runs right after the Rates marker (cfd/
+    elasticBounce.xml inserts after="Rates"), i.e. after the cfd rates and
+    the delta-SPH rate and before the corrector."""
+    from aquagpusph_amd.sim import registered_kernels
+    case = cases.dam_break(2, 0.3 / 10)
+    xml = casefiles.write_case(case, str(tmp_path), presets=TESTCASE5,
+                               extra_xml='    <Variables><Variable name="dr" '
+                               'type="float" value="0.03" /></Variables>')
+    tl = problem_tools(xml, 2, keys=True)
+    reg = set(registered_kernels())
+    missing = [(n, k) for n, t, k in tl if t == "kernel" and k not in reg]
+    assert not missing, missing
+    names = [n for n, _, _ in tl]
+    i = names.index("Rates")
+    assert names[i + 1] == "cfd elastic bounce"
+    assert names.index("cfd rates") < i
+    assert names.index("cfd delta-SPH") < i
+    assert names.index("Corrector") > i + 1
+    keys = dict((n, k) for n, _, k in tl)
+    assert keys["cfd elastic bounce"] == "cfd/Boundary/ElasticBounce.cl:entry"
