@@ -31,7 +31,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_HBM_GBS = 8000.0
+# MI355X FP32 vector peak as published (it counts packed FP32: two FMAs per
+# lane and cycle); the walk is built without packed FP32, so one FMA per
+# lane and cycle, half of it, is what its instructions can reach
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_NONPACKED_TFLOPS = PEAK_FP32_TFLOPS / 2
 
 
 def build_sim(dr, dims=3, precision=False):
@@ -52,9 +56,11 @@ def build_sim(dr, dims=3, precision=False):
 METRIC_SIZES = ((0.00476, "1M"), (0.00189, "16M"))
 
 
-def size_point(dr, steps=10, warmup=2):
+def size_point(dr, steps=10, warmup=2, profile_steps=2):
     """particle-steps/s of the same pipeline at another resolution (HIP
-    graph steps, inputs resident; the same timing as the main line)."""
+    graph steps, inputs resident; the same timing as the main line), with
+    the fused walk's roofline at that size and the share of the step the
+    walk and the other tools take."""
     sim, case = build_sim(dr)
     sim.step(warmup, graph=True)
     sim.sync()
@@ -62,12 +68,46 @@ def size_point(dr, steps=10, warmup=2):
     sim.step(steps, graph=True)
     sim.sync()
     el = time.perf_counter() - t0
+    ms = 1e3 * el / steps
     out = {"dr": dr, "n_particles": sim.N, "n_fluid": int(case["n_fluid"]),
-           "steps": steps, "ms_per_step": 1e3 * el / steps,
+           "steps": steps, "ms_per_step": ms,
            "particle_steps_per_s": sim.N * steps / el,
            "fluid_particle_steps_per_s": int(case["n_fluid"]) * steps / el}
+    prof = sim.profile(profile_steps)
+    walk = sim.walk_ms()
+    gpu_ms = sum(m for _, _, m in prof)
     sim.close()
+    if walk:
+        pc = pair_counts(case)
+        flops = (pc["cand"] * FLOP_CAND + pc["pairs"] * FLOP_PAIR
+                 + pc["elem_pairs"] * FLOP_ELEM)
+        ach = flops / (walk * 1e-3) / 1e12
+        out["roofline"] = {
+            "kernel": "fused fluid walk", "bound": "valu", "achieved": ach,
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP32_TFLOPS,
+            "frac_non_packed": ach / PEAK_FP32_NONPACKED_TFLOPS,
+            "avg_launch_ms": walk,
+            "hbm_achieved_gbs": WALK_BYTES * out["n_particles"]
+            / (walk * 1e-3) / 1e9}
+    # where the rest of the step goes: eager per-tool GPU time (summed
+    # event intervals) against the graph-replayed step
+    out["walk_ms"] = walk
+    out["tools_gpu_ms"] = gpu_ms
+    out["launch_gaps_ms"] = max(0.0, ms - gpu_ms)
     return out
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(case, budget_s=25.0):
@@ -86,7 +126,7 @@ def cpu_baseline(case, budget_s=25.0):
         if el > budget_s or n >= 3:
             break
     return dict(value=case["N"] * n / el, unit="particle-steps/s",
-                cores=threads, kind="port",
+                cores=threads, kind="port", cpu=cpu_model(),
                 sample="%d full steps of the same %d-particle case (after one "
                        "untimed step), FP32 oracle, %d OpenMP threads, "
                        "%.1f s" % (n, case["N"], threads, el))
@@ -316,19 +356,27 @@ def main():
         flops = None
         hbm = None
     traffic = measured_traffic("fused_walk_dambreak3d") if fused else None
+    achieved = flops / t / 1e12 if flops else None
     roof = {
         "kernel": kname,
-        "bound": "mfma",
-        "achieved": flops / t / 1e12 if flops else None,
+        "bound": "valu",
+        "achieved": achieved,
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": flops / t / 1e12 / PEAK_FP32_TFLOPS if flops else None,
+        "frac": achieved / PEAK_FP32_TFLOPS if flops else None,
         "traffic": traffic,
-        "note": "FP32 vector ALU work (no matrix instructions); the MI355X "
-                "FP32 vector and matrix peaks are both 157.3 TFLOP/s, a "
-                "figure that counts packed FP32, which this build does not "
-                "emit (half of it without). The walk is compute-bound: %.0f "
-                "FLOP per %d compulsory bytes, IEEE division and square root"
+        "traffic_source": "profiles/fused_walk_dambreak3d_pmc.md "
+                          "(rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of this "
+                          "kernel on this workload, copy-kernel scaled)",
+        "peak_non_packed": PEAK_FP32_NONPACKED_TFLOPS,
+        "frac_non_packed": (achieved / PEAK_FP32_NONPACKED_TFLOPS
+                            if flops else None),
+        "note": "FP32 vector ALU work, no matrix instructions (bound: the "
+                "VALU). peak is the published MI355X FP32 vector figure, "
+                "which counts packed FP32; this build emits none, so "
+                "peak_non_packed (one FMA per lane and cycle) is the "
+                "attainable roof. The walk is compute-bound: %.0f FLOP per "
+                "%d compulsory bytes, IEEE division and square root"
                 % (flops / N if flops else 0, WALK_BYTES),
         "avg_launch_ms": t * 1e3,
         "tool_ms": dom_ms,
