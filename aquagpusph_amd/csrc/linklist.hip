@@ -8,6 +8,9 @@
 // capacity, so the whole link-list can be replayed from a HIP graph.  An
 // overflowing grid sets bit 0 of a sticky status word and the later kernels
 // skip their work; the caller grows the capacity and re-runs.
+#include <mutex>
+#include <vector>
+
 #include "aq_device.hpp"
 #include "aq_internal.hpp"
 
@@ -357,6 +360,37 @@ void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
     linklist_ihoc(icell, ihoc, N, n_cells, cap, status, s);
 }
 
+// ---- the link-list without allocation -------------------------------------
+// Scratch of aq_linklist_noalloc: the workspace aq_linklist takes as an
+// argument (min/max partials, the radix sort's key and value buffers, digit
+// counts and scan sums) for up to AQ_LINKLIST_NOALLOC_MAX_N particles, in a
+// file-scope device array.  Each GPU holds its own copy, at its own
+// address, until the process ends; host code uses that address, looked up
+// once per device, never the array's name.  Every call on one GPU shares
+// the array and keeps the last call's contents: calls that overlap on one
+// GPU race.  Nothing in it has to start at zero: each kernel of the
+// link-list writes what a later one reads.
+constexpr size_t NOALLOC_BYTES = (size_t)9 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+static void* scratch_ptr()
+{
+    static std::mutex mu;
+    static std::vector<void*> ptrs;
+    int dev = 0;
+    AQ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= ptrs.size()) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        AQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        AQ_REQUIRE(p, "aq_linklist_noalloc: no address for the scratch "
+                      "array on this device");
+        ptrs[dev] = p;
+    }
+    return ptrs[dev];
+}
+
 }  // namespace aq
 
 using namespace aq;
@@ -364,6 +398,34 @@ using namespace aq;
 extern "C" {
 
 AQ_API size_t aq_linklist_workspace(uint32_t N) { return linklist_workspace(N); }
+
+AQ_API int aq_linklist_noalloc(int dims, const float* r, uint32_t N,
+                               float* r_min, float* r_max,
+                               const float* support, const float* h,
+                               uint32_t* n_cells, uint32_t* icell,
+                               uint32_t* id_unsorted, uint32_t* id_sorted,
+                               uint32_t* ihoc, uint32_t ihoc_capacity,
+                               uint32_t* status, void* stream)
+{
+    return guard([&] {
+        check_dims(dims);
+        AQ_REQUIRE(N > 0, "aq_linklist_noalloc: N must be > 0");
+        if (N > AQ_LINKLIST_NOALLOC_MAX_N)
+            throw Error(AQ_EINVAL,
+                        "aq_linklist_noalloc: N = " + std::to_string(N) +
+                            " exceeds the no-allocation limit of " +
+                            std::to_string(AQ_LINKLIST_NOALLOC_MAX_N) +
+                            " particles (use aq_linklist with a workspace)");
+        AQ_REQUIRE(r && r_min && r_max && support && h && n_cells && icell &&
+                       id_unsorted && id_sorted && ihoc && status,
+                   "aq_linklist_noalloc: null pointer");
+        AQ_REQUIRE(linklist_workspace(N) <= NOALLOC_BYTES,
+                   "aq_linklist_noalloc: scratch smaller than the workspace");
+        linklist_all(dims, r, N, r_min, r_max, support, h, n_cells, icell,
+                     id_unsorted, id_sorted, ihoc, ihoc_capacity, status,
+                     scratch_ptr(), S(stream));
+    });
+}
 
 AQ_API int aq_linklist_grid(int dims, const float* r, uint32_t N,
                             float* r_min, float* r_max, const float* support,

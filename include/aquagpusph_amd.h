@@ -105,6 +105,24 @@ AQ_API int aq_linklist(int dims, const float* r, uint32_t N, float* r_min,
 /* Bytes of device workspace aq_linklist / aq_linklist_grid need for N. */
 AQ_API size_t aq_linklist_workspace(uint32_t N);
 
+/* aq_linklist without a workspace argument and without any allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace lives in a fixed device array of the
+ * library (9 MiB per GPU, held until the process ends), so N is limited to
+ * AQ_LINKLIST_NOALLOC_MAX_N particles (larger N: AQ_EINVAL, nothing
+ * launched).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race and must be serialised by
+ * the caller.  Outputs and their meaning are those of aq_linklist. */
+#define AQ_LINKLIST_NOALLOC_MAX_N 1048576u
+AQ_API int aq_linklist_noalloc(int dims, const float* r, uint32_t N,
+                               float* r_min, float* r_max,
+                               const float* support, const float* h,
+                               uint32_t* n_cells, uint32_t* icell,
+                               uint32_t* id_unsorted, uint32_t* id_sorted,
+                               uint32_t* ihoc, uint32_t ihoc_capacity,
+                               uint32_t* status, void* stream);
+
 /* ---- radix sort: aquagpusph/CalcServer/RadixSort.{cpp,cl.in} ---------- */
 
 /* Stable LSD radix sort of n uint32 keys on their low key_bits bits
