@@ -26,6 +26,14 @@ PGP_PRESETS = ("resources/Presets/basic.xml",
                "resources/Presets/cfd/GP.xml",
                "resources/Presets/cfd/variableTimeStep.xml")
 
+# a periodic box of fluid (no walls, no domain)
+PPER_PRESETS = ("resources/Presets/basic.xml",
+                "resources/Presets/basic/time_scheme/improved_euler.xml",
+                "resources/Presets/cfd.xml",
+                "resources/Presets/cfd/deltaSPH.xml",
+                "resources/Presets/cfd/periodic.xml",
+                "resources/Presets/cfd/variableTimeStep.xml")
+
 PBIE_PRESETS = ("resources/Presets/basic.xml",
                 "resources/Presets/basic/time_scheme/improved_euler.xml",
                 "resources/Presets/basic/domain.xml",
@@ -127,6 +135,10 @@ def write_case(case, folder, presets=PBI_PRESETS, load=True, extra_xml="",
     if "domain_min" in p:
         var.append(("domain_min", "vec", _vec(p["domain_min"], dims)))
         var.append(("domain_max", "vec", _vec(p["domain_max"], dims)))
+    if "periodic" in p:
+        # the box of Presets/cfd/periodic.xml
+        var.append(("periodic_min", "vec", _vec(p["periodic"]["min"], dims)))
+        var.append(("periodic_L", "vec", _vec(p["periodic"]["L"], dims)))
     for name, t, v in var:
         lines.append('        <Variable name="%s" type="%s" value="%s" />'
                      % (name, t, v))

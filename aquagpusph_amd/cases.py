@@ -32,7 +32,7 @@ def _lattice(*axes):
 def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
               refd=998.0, alpha=0.02, mu=8.9e-4, delta=0.1, courant=0.1,
               walls=("bottom", "top", "left", "right", "front", "back"),
-              bc="BI"):
+              bc="BI", tait_gamma=None):
     """Dam break with one layer of wall elements.
 
     2D: tank [-B, L-B] x [0, D], column [-B, 0] x [0, H] (g along -y).
@@ -44,6 +44,8 @@ def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
     normal, associated with their element (the layout of the reference's
     examples/2D/spheric_testcase6_movingsquare/src/Create.py:85-146);
     "associations" holds N for every other particle.
+    tait_gamma: the initial density from the hydrostatic pressure through
+    the Tait law of Presets/basic/tait.xml instead of the linear EOS.
     """
     if L is None:
         L = 1.61 if dims == 2 else 1.6
@@ -134,7 +136,11 @@ def dam_break(dims, dr, L=None, D=0.6, B=0.6, H=0.3, W=0.6, hfac=2.0,
     imove[nf + ne:] = -1
     press = np.zeros(N)
     press[:nf] = refd * G * (H - fluid[:, vert])
-    rho = refd + press / cs ** 2
+    if tait_gamma is None:
+        rho = refd + press / cs ** 2
+    else:
+        B = refd * cs ** 2 / tait_gamma
+        rho = refd * (press / B + 1.0) ** (1.0 / tait_gamma)
     rho[nf:] = refd
     m = np.empty(N)
     m[:nf] = refd * dr ** dims
@@ -231,6 +237,68 @@ def decaying_box(n, seed=0, k0=4.0, urms=1.0, nu=1e-4, cs=20.0, refd=1.0,
              domain_min=[-0.5, -0.5], domain_max=[1.5, 1.5], nu=nu)
     case["params"] = p
     return case
+
+
+def solenoidal_field(dims, n, seed=0, k0=4.0, urms=1.0):
+    """A seeded random divergence-free velocity on the n^dims periodic
+    lattice of a unit box, with the energy spectrum of the decaying
+    turbulence benchmark, E(k) ~ k^4 exp(-2 (k/k0)^2) (k in units of
+    2 pi), scaled to the RMS speed urms.  Built in Fourier space: random
+    complex amplitudes of modulus sqrt(E(k) / (shell measure)), projected
+    onto the plane normal to k (the spectral form of div u = 0), and the
+    real part of the inverse FFT.  Returns an (n^dims, dims) array in the
+    lattice order of np.meshgrid(..., indexing="ij")."""
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, d=1.0 / n)  # integer wavenumbers
+    K = np.stack(np.meshgrid(*([k1] * dims), indexing="ij"), axis=0)
+    kk = np.sqrt(np.sum(K ** 2, axis=0))
+    shell = 2.0 * math.pi * kk if dims == 2 else 4.0 * math.pi * kk ** 2
+    E = kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
+    amp = np.zeros_like(kk)
+    # no Nyquist modes (their derivative is not represented on the lattice)
+    nz = (kk > 0) & np.all(np.abs(K) < n / 2, axis=0)
+    amp[nz] = np.sqrt(E[nz] / shell[nz])
+    U = (rng.normal(size=(dims,) + kk.shape) +
+         1j * rng.normal(size=(dims,) + kk.shape)) * amp
+    kd = np.where(nz, kk, 1.0)
+    proj = np.sum(U * K, axis=0) / kd ** 2
+    U = U - K * proj
+    u = np.stack([np.real(np.fft.ifftn(U[a])) for a in range(dims)], -1)
+    u = u.reshape(-1, dims)
+    u -= u.mean(axis=0)
+    return u * (urms / math.sqrt(np.mean(np.sum(u ** 2, axis=1))))
+
+
+def periodic_box(dims, n, seed=0, k0=4.0, urms=1.0, nu=1e-4, cs=10.0,
+                 refd=1.0, courant=0.1, delta=0.1, hfac=2.0, p0_factor=1.0,
+                 L=1.0):
+    """Decaying turbulence in a box of side L periodic along every axis
+    (Presets/cfd/periodic.xml): n^dims fluid particles on the
+    cell-centred lattice, no walls, no gravity, the velocity of
+    solenoidal_field (E(k) ~ k^4 exp(-2 (k/k0)^2), k0 = 4, RMS urms: the
+    reference's configuration of the learned-closure study, triply periodic
+    in 3D).  Kinematic viscosity nu, background pressure p0 = p0_factor
+    rho0 urms^2, delta-SPH delta.  params["periodic"] holds the box."""
+    dr = L / n
+    xs = (np.arange(n) + 0.5) * dr
+    pos = _lattice(*([xs] * dims))
+    N = pos.shape[0]
+    vs = 2 if dims == 2 else 4
+    r = np.zeros((N, vs))
+    r[:, :dims] = pos
+    u = np.zeros((N, vs))
+    u[:, :dims] = solenoidal_field(dims, n, seed, k0, urms)
+    h = hfac * dr
+    params = dict(h=h, cs=cs, courant=courant, g=[0.0] * dims, refd=[refd],
+                  visc_dyn=[refd * nu], delta=[delta],
+                  p0=p0_factor * refd * urms ** 2, dt_min=0.0, dt_Ma=0.1,
+                  dr=dr, hfac=hfac, nu=nu,
+                  periodic={"min": [0.0] * dims, "L": [L] * dims})
+    return dict(dims=dims, N=N, n_fluid=N, r=r, normal=np.zeros((N, vs)),
+                tangent=np.zeros((N, vs)), u=u, dudt=np.zeros((N, vs)),
+                rho=np.full(N, refd), drhodt=np.zeros(N),
+                m=np.full(N, refd * dr ** dims), imove=np.ones(N, np.int32),
+                iset=np.zeros(N, np.uint32), params=params, bc="none")
 
 
 def taylor_green(n=32, L=2.0 * math.pi, U=1.0, hfac=4.0, cs=50.0,

@@ -365,6 +365,90 @@ __device__ __forceinline__ void walk(uint32_t c_i, uint32_t N, uint32_t nx,
     }
 }
 
+// ---- periodic boxes ----------------------------------------------------------
+// A box periodic along the axes whose length per[k] is positive (the vec
+// variable periodic_L; per == nullptr: no periodic axis).  Along such an
+// axis the link-list grid has n_cells[k] cells tiling [periodic_min,
+// periodic_min + L) exactly (no margin cells), and the neighbour cell
+// c + o wraps modulo n_cells[k]; the particles of a wrapped cell are seen
+// at r + shift, shift = -L (wrapped below 0) or +L (wrapped above n - 1),
+// so every pair within the support is found once, at its nearest image
+// (each periodic axis has at least 3 cells of at least the support).
+// Neighbour cell (offset o) of cell c and its shift.  The walk order is the
+// reference's (x offset outermost, then y, then z), over the wrapped cells.
+template <int D>
+__device__ __forceinline__ uint32_t per_neighbour(uint32_t c, int oi, int oj,
+                                                  int ok,
+                                                  const uint32_t* __restrict__ nc,
+                                                  const float* __restrict__ per,
+                                                  f3& sh)
+{
+    const uint32_t nx = nc[0], ny = nc[1];
+    int q[3] = { (int)(c % nx), (int)((c / nx) % ny),
+                 D == 3 ? (int)(c / (nx * ny)) : 0 };
+    const int o[3] = { oi, oj, ok };
+    float s[3] = { 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        const int n = (int)nc[k];
+        int v = q[k] + o[k];
+        const float L = per[k];
+        if (L > 0.f) {
+            if (v < 0) {
+                v += n;
+                s[k] = -L;
+            } else if (v >= n) {
+                v -= n;
+                s[k] = L;
+            }
+        }
+        q[k] = v;
+    }
+    sh = f3{ s[0], s[1], s[2] };
+    return (uint32_t)q[0] + (uint32_t)q[1] * nx +
+           (D == 3 ? (uint32_t)q[2] * nx * ny : 0u);
+}
+
+// r + shift, componentwise, only along the periodic axes (exact: the
+// other components are left as they are, not added a zero)
+template <int D>
+__device__ __forceinline__ f3 per_shift(const f3& r, const f3& sh,
+                                        const float* __restrict__ per)
+{
+    f3 o = r;
+    if (per[0] > 0.f) o.x = r.x + sh.x;
+    if (per[1] > 0.f) o.y = r.y + sh.y;
+    if (D == 3 && per[2] > 0.f) o.z = r.z + sh.z;
+    return o;
+}
+
+// walk<D> over a periodic grid: body(j, shift) for every particle j of the
+// 3^D wrapped cells around c_i, in the reference order.
+template <int D, class Body>
+__device__ __forceinline__ void walk_per(uint32_t c_i, uint32_t N,
+                                         const uint32_t* __restrict__ nc,
+                                         const float* __restrict__ per,
+                                         const uint32_t* __restrict__ icell,
+                                         const uint32_t* __restrict__ ihoc,
+                                         Body body)
+{
+#pragma unroll 1
+    for (int ci = -1; ci <= 1; ci++)
+#pragma unroll 1
+        for (int cj = -1; cj <= 1; cj++)
+#pragma unroll 1
+            for (int ck = (D == 3 ? -1 : 0); ck <= (D == 3 ? 1 : 0); ck++) {
+                f3 sh;
+                const uint32_t c_j = per_neighbour<D>(c_i, ci, cj, ck, nc,
+                                                      per, sh);
+                uint32_t j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    body(j, sh);
+                    j++;
+                }
+            }
+}
+
 // The walk around a cell given explicitly (a moved point, e.g. a mirrored
 // particle): neighbour cells outside the grid (c_j >= n_cells.w, a point
 // beyond the particle bounding box) are skipped, so every ihoc read stays

@@ -78,7 +78,8 @@ template <int D>
 __global__ void __launch_bounds__(MM_BS)
     grid_final(const float* __restrict__ part, uint32_t nb, float* r_min,
                float* r_max, const float* support, const float* h,
-               uint32_t* n_cells, uint32_t capacity, uint32_t* status)
+               uint32_t* n_cells, uint32_t capacity, uint32_t* status,
+               const float* per_min, const float* per_L)
 {
     constexpr int VS = (D == 2) ? 2 : 4;
     __shared__ float smn[VS][MM_BS], smx[VS][MM_BS];
@@ -124,13 +125,26 @@ __global__ void __launch_bounds__(MM_BS)
         }
         const float cell_length = (*support) * (*h);
         uint64_t n[3] = { 1, 1, 1 };
+        bool small = false;
 #pragma unroll
         for (int k = 0; k < D; k++) {
+            if (per_L && per_L[k] > 0.f) {
+                // periodic axis: n cells of L / n >= the support tile the
+                // box exactly (at least 3, so the wrapped neighbour cells
+                // are distinct)
+                n[k] = (uint64_t)cvt_u32(per_L[k] / cell_length);
+                small = small || n[k] < 3;
+                r_min[k] = per_min[k];
+                r_max[k] = per_min[k] + per_L[k];
+                continue;
+            }
             const float v = (hi[k] - lo[k]) / cell_length;
             n[k] = (uint64_t)cvt_u32(v) + 6;
         }
         const uint64_t w = n[0] * n[1] * n[2];
-        if (w > (uint64_t)capacity) {
+        if (small) {
+            *status |= 1u | 4u;  // a periodic box under 3 support lengths
+        } else if (w > (uint64_t)capacity) {
             // keep the previous, consistent grid; flag the overflow
             *status |= 1u;
         } else {
@@ -143,18 +157,38 @@ __global__ void __launch_bounds__(MM_BS)
 }
 
 // iCell, LinkList.cl.in:54-85
+// Along a periodic axis (per_L[k] > 0) the cell coordinate is
+// min(trunc((r - periodic_min) n / L), n - 1), with no margin cells.
 template <int D>
 __global__ void icell_k(uint32_t* __restrict__ icell,
                         const float* __restrict__ r, uint32_t N,
                         const float* r_min, const float* support,
                         const float* h, const uint32_t* n_cells,
-                        const uint32_t* status)
+                        const uint32_t* status, const float* per_min,
+                        const float* per_L)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     if (status && (*status & 1u)) return;
     const float idist = 1.f / ((*support) * (*h));
     const uint32_t nx = n_cells[0], ny = n_cells[1];
+    if (per_L) {
+        constexpr int VS = (D == 2) ? 2 : 4;
+        uint32_t q[3] = { 0u, 0u, 0u };
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const float x = r[(size_t)i * VS + k];
+            if (per_L[k] > 0.f) {
+                const uint32_t n = n_cells[k];
+                const float inv = (float)n / per_L[k];
+                q[k] = min(cvt_u32((x - per_min[k]) * inv), n - 1u);
+            } else {
+                q[k] = cvt_u32((x - r_min[k]) * idist) + 2u;
+            }
+        }
+        icell[i] = q[0] + q[1] * nx + (D == 3 ? q[2] * nx * ny : 0u);
+        return;
+    }
     if constexpr (D == 2) {
         const float2 p = reinterpret_cast<const float2*>(r)[i];
         const uint32_t cx =
@@ -298,7 +332,8 @@ size_t linklist_workspace(uint32_t N)
 void linklist_grid(int dims, const float* r, uint32_t N, float* r_min,
                    float* r_max, const float* support, const float* h,
                    uint32_t* n_cells, uint32_t cap, uint32_t* status,
-                   void* ws, hipStream_t s)
+                   void* ws, hipStream_t s, const float* per_min = nullptr,
+                   const float* per_L = nullptr)
 {
     const unsigned nb =
         std::min<unsigned>(MM_MAXB, blocks_for(N, MM_BS * 8));
@@ -308,28 +343,29 @@ void linklist_grid(int dims, const float* r, uint32_t N, float* r_min,
                            N, part);
         hipLaunchKernelGGL(grid_final<2>, dim3(1), dim3(MM_BS), 0, s, part,
                            nb, r_min, r_max, support, h, n_cells, cap,
-                           status);
+                           status, per_min, per_L);
     } else {
         hipLaunchKernelGGL(minmax_partial<3>, dim3(nb), dim3(MM_BS), 0, s, r,
                            N, part);
         hipLaunchKernelGGL(grid_final<3>, dim3(1), dim3(MM_BS), 0, s, part,
                            nb, r_min, r_max, support, h, n_cells, cap,
-                           status);
+                           status, per_min, per_L);
     }
 }
 
 void linklist_icell(int dims, uint32_t* icell, const float* r, uint32_t N,
                     const float* r_min, const float* support, const float* h,
                     const uint32_t* n_cells, const uint32_t* status,
-                    hipStream_t s)
+                    hipStream_t s, const float* per_min = nullptr,
+                    const float* per_L = nullptr)
 {
     const dim3 g(blocks_for(N, 256)), b(256);
     if (dims == 2)
         hipLaunchKernelGGL(icell_k<2>, g, b, 0, s, icell, r, N, r_min,
-                           support, h, n_cells, status);
+                           support, h, n_cells, status, per_min, per_L);
     else
         hipLaunchKernelGGL(icell_k<3>, g, b, 0, s, icell, r, N, r_min,
-                           support, h, n_cells, status);
+                           support, h, n_cells, status, per_min, per_L);
 }
 
 void linklist_ihoc(const uint32_t* icell, uint32_t* ihoc, uint32_t N,
@@ -347,11 +383,13 @@ void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
                   float* r_max, const float* support, const float* h,
                   uint32_t* n_cells, uint32_t* icell, uint32_t* id_unsorted,
                   uint32_t* id_sorted, uint32_t* ihoc, uint32_t cap,
-                  uint32_t* status, void* ws, hipStream_t s)
+                  uint32_t* status, void* ws, hipStream_t s,
+                  const float* per_min, const float* per_L)
 {
     linklist_grid(dims, r, N, r_min, r_max, support, h, n_cells, cap, status,
-                  ws, s);
-    linklist_icell(dims, icell, r, N, r_min, support, h, n_cells, status, s);
+                  ws, s, per_min, per_L);
+    linklist_icell(dims, icell, r, N, r_min, support, h, n_cells, status, s,
+                   per_min, per_L);
     const size_t mm = (size_t)MM_MAXB * 2 * 4 * sizeof(float);
     void* sws = (char*)ws + ((mm + 255) & ~(size_t)255);
     // icell < n_cells.w <= cap, so sorting on the bits of cap gives the
@@ -423,7 +461,7 @@ AQ_API int aq_linklist_noalloc(int dims, const float* r, uint32_t N,
                    "aq_linklist_noalloc: scratch smaller than the workspace");
         linklist_all(dims, r, N, r_min, r_max, support, h, n_cells, icell,
                      id_unsorted, id_sorted, ihoc, ihoc_capacity, status,
-                     scratch_ptr(), S(stream));
+                     scratch_ptr(), S(stream), nullptr, nullptr);
     });
 }
 
@@ -488,7 +526,7 @@ AQ_API int aq_linklist(int dims, const float* r, uint32_t N, float* r_min,
                    "aq_linklist: null pointer");
         linklist_all(dims, r, N, r_min, r_max, support, h, n_cells, icell,
                      id_unsorted, id_sorted, ihoc, ihoc_capacity, status,
-                     workspace, S(stream));
+                     workspace, S(stream), nullptr, nullptr);
     });
 }
 

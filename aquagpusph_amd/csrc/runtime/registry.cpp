@@ -26,6 +26,13 @@ void launch_eos(const uint32_t*, const int32_t*, const float*, float*,
                 const float*, uint32_t, const float*, const float*,
                 hipStream_t);
 void launch_binormal(int, const float*, float*, float*, uint32_t, hipStream_t);
+void launch_tait_eos(const uint32_t*, const int32_t*, const float*, float*,
+                     const float*, uint32_t, const float*, const float*,
+                     const float*, hipStream_t);
+void launch_bi_interp_shepard_tait(const uint32_t*, const int32_t*,
+                                   const float*, float*, float*, const float*,
+                                   uint32_t, const float*, const float*,
+                                   const float*, hipStream_t);
 void launch_rates(int, const uint32_t*, const int32_t*, const float*,
                   const float*, const float*, float*, float*, const float*,
                   uint32_t, const float*, hipStream_t);
@@ -183,7 +190,7 @@ void launch_lapp(int, const int32_t*, const float*, const float*,
 void launch_sensors(int, const int32_t*, const float*, const float*, float*,
                     float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
-                    aq_defs, hipStream_t);
+                    aq_defs, hipStream_t, const float*);
 void launch_bi_shepard_compute(int, const int32_t*, const float*,
                                const float*, const float*, const float*,
                                const float*, float*, uint32_t,
@@ -272,6 +279,16 @@ void launch_bie_bounce(int, const int32_t*, const float*, const float*,
                        const float*, const uint32_t*, const uint32_t*,
                        const uint32_t*, const uint32_t*, float, float,
                        hipStream_t);
+void launch_periodic_wrap(int, float*, uint32_t, const float*, const float*,
+                          hipStream_t);
+void launch_les_gradu_cells(int, const int32_t*, const float*, const float*,
+                            const float*, const float*, float*,
+                            const CellArgs&, aq_defs, float, const float*,
+                            hipStream_t);
+void launch_les_interactions_cells(int, const int32_t*, const float*,
+                                   const float*, const float*, const float*,
+                                   const float*, float*, const CellArgs&,
+                                   aq_defs, float, const float*, hipStream_t);
 void launch_elastic_bounce(int, const int32_t*, const float*, const float*,
                            float*, float*, uint32_t, const float*,
                            const float*, const uint32_t*, const uint32_t*,
@@ -408,6 +425,20 @@ static const std::map<std::string, KernelSpec>& table()
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_eos(U(0), I(1), F(2), F(3), F(4), S.N, F(5), F(6), s);
             } } },
+        // Tait EOS (basic/tait.xml; no counterpart script in the reference)
+        { "basic/EOS_Tait.cl:entry",
+          { { "iset", "imove", "rho", "p", "refd", "cs", "p0", "tait_gamma" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_tait_eos(U(0), I(1), F(2), F(3), F(4), S.N, F(5), F(6),
+                                F(7), s);
+            } } },
+        { "cfd/Boundary/BI/InterpolationShepard_Tait.cl:entry",
+          { { "iset", "imove", "shepard", "rho", "p", "refd", "cs", "p0",
+              "tait_gamma" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_bi_interp_shepard_tait(U(0), I(1), F(2), F(3), F(4),
+                                              F(5), S.N, F(6), F(7), F(8), s);
+            } } },
         { "basic/Binormal.cl:entry",
           { { "normal", "tangent", "binormal" },
             [](Sim& S, Var* const* a, hipStream_t s) {
@@ -450,7 +481,8 @@ static const std::map<std::string, KernelSpec>& table()
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
                 launch_sensors(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
-                               S.N, U(6), U(7), U(8), S.d_status, S.kdefs, s);
+                               S.N, U(6), U(7), U(8), S.d_status, S.kdefs, s,
+                               S.periodic_L());
             } } },
         { "cfd/SensorsRenormalization.cl:entry",
           { { "imove", "shepard", "u", "rho", "p" },
@@ -668,9 +700,19 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "u", "rho", "m", "grad_u", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_les_gradu(S.dims, I(0), F(1), F(2), F(3), F(4), F(5),
-                                 S.N, U(6), U(7), U(8), S.kdefs,
-                                 support_threshold(S.kdefs.H), s);
+                if (S.clist.usable(a[6], a[7]) && a[8] == S.clist.n_cells)
+                    launch_les_gradu_cells(S.dims, I(0), F(1), F(2), F(3),
+                                           F(4), F(5), S.cell_args(),
+                                           S.kdefs,
+                                           support_threshold(S.kdefs.H),
+                                           S.periodic_L(), s);
+                else if (S.periodic_L())
+                    throw Error(AQ_EUNSUP, "periodic box: cfd/LES/GradU.cl "
+                                           "needs the cell list");
+                else
+                    launch_les_gradu(S.dims, I(0), F(1), F(2), F(3), F(4),
+                                     F(5), S.N, U(6), U(7), U(8), S.kdefs,
+                                     support_threshold(S.kdefs.H), s);
             } } },
         { "cfd/LES/Features.cl:entry",
           { { "imove", "grad_u", "les_in", "hfac" },
@@ -706,10 +748,20 @@ static const std::map<std::string, KernelSpec>& table()
           { { "imove", "r", "u", "rho", "m", "nu_t", "dudt", "icell", "ihoc",
               "n_cells" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                launch_les_interactions(S.dims, I(0), F(1), F(2), F(3), F(4),
-                                        F(5), F(6), S.N, U(7), U(8), U(9),
-                                        S.kdefs, support_threshold(S.kdefs.H),
-                                        s);
+                if (S.clist.usable(a[7], a[8]) && a[9] == S.clist.n_cells)
+                    launch_les_interactions_cells(
+                        S.dims, I(0), F(1), F(2), F(3), F(4), F(5), F(6),
+                        S.cell_args(), S.kdefs, support_threshold(S.kdefs.H),
+                        S.periodic_L(), s);
+                else if (S.periodic_L())
+                    throw Error(AQ_EUNSUP, "periodic box: "
+                                           "cfd/LES/Interactions.cl needs the "
+                                           "cell list");
+                else
+                    launch_les_interactions(S.dims, I(0), F(1), F(2), F(3),
+                                            F(4), F(5), F(6), S.N, U(7), U(8),
+                                            U(9), S.kdefs,
+                                            support_threshold(S.kdefs.H), s);
             } } },
         // ---- semi-implicit midpoint scheme ----------------------------------
         { "basic/time_scheme/midpoint.cl:predictor",
@@ -1005,6 +1057,11 @@ static const std::map<std::string, KernelSpec>& table()
                                   S.N, F(6), U(7), U(8), U(9), S.d_status,
                                   def_or(S, "__DR_FACTOR__", 0.5f),
                                   def_or(S, "__MIN_BOUND_DIST__", 0.f), s);
+            } } },
+        { "cfd/Boundary/Periodic/Wrap.cl:entry",
+          { { "r_in", "periodic_min", "periodic_L" },
+            [](Sim& S, Var* const* a, hipStream_t s) {
+                launch_periodic_wrap(S.dims, F(0), S.N, F(1), F(2), s);
             } } },
         { "cfd/Boundary/ElasticBounce.cl:entry",
           { { "imove", "r", "normal", "u", "dudt", "dr", "dt", "icell", "ihoc",

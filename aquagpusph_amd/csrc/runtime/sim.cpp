@@ -300,9 +300,53 @@ void Sim::setup()
                     "__KAHAN_SUMS__ needs the fused cell-parallel walk (cfd "
                     "Shepard + cfd interactions + cfd lap p in a row, no "
                     "AQ_WALK=thread or AQ_NO_FUSION)");
+    check_periodic();
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
     is_setup = true;
+}
+
+// A periodic box (periodic_L declared): every neighbour walk of the
+// pipeline must have a periodic form.  Those are the fused cell walk (cfd
+// Shepard + cfd interactions + cfd lap p, with the cell list) and the
+// cell-parallel LES gradient and viscous term, and the sensors; any other
+// kernel that walks the link-list is refused.  Periodic boxes hold fluid
+// particles (and sensors) only.
+void Sim::check_periodic()
+{
+    if (!vars.get("periodic_L")) return;
+    static const std::set<std::string> fused = {
+        "cfd/Shepard.cl:entry", "cfd/Interactions.cl:entry",
+        "cfd/deltaSPH.cl:lapp"
+    };
+    static const std::set<std::string> cells = {
+        "cfd/LES/GradU.cl:entry", "cfd/LES/Interactions.cl:entry"
+    };
+    // per-particle walks with a periodic form
+    static const std::set<std::string> part = { "cfd/Sensors.cl:entry" };
+    for (auto& t : tools) {
+        const std::string k = t->key();
+        if (k.empty() || t->type() != "kernel") continue;
+        const KernelSpec* spec = find_kernel(k);
+        if (!spec) continue;
+        bool walks = false;
+        for (auto& a : spec->args) walks = walks || a == "ihoc";
+        if (!walks) continue;
+        if (fused.count(k)) {
+            if (!(fusion.active && clist.enabled &&
+                  (fusion.anchor == t.get() || fusion.absorbed.count(t.get()))))
+                throw Error(AQ_EUNSUP,
+                            "periodic box: tool \"" + t->name() +
+                                "\" walks the neighbours outside the fused "
+                                "cell-parallel walk (AQ_NO_FUSION, "
+                                "AQ_WALK=thread or a tool in between)");
+            continue;
+        }
+        if (cells.count(k) && clist.enabled) continue;
+        if (part.count(k)) continue;
+        throw Error(AQ_EUNSUP, "periodic box: tool \"" + t->name() + "\" (" +
+                                   k + ") has no periodic neighbour walk");
+    }
 }
 
 // Kernel tools that neither move particles nor change imove, r, u, rho, m, p
@@ -453,7 +497,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks,
                              int segw, int flagged, int math,
-                             unsigned ranges, int fast_div, hipStream_t s);
+                             unsigned ranges, int fast_div, hipStream_t s,
+                             const float* per);
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -538,12 +583,15 @@ void Sim::run_fused(hipStream_t s)
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
             f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges,
-            f.fast_div ? 1 : 0, s);
+            f.fast_div ? 1 : 0, s, periodic_L());
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }
     if (f.math == 2)
         throw Error(AQ_EUNSUP, "__KAHAN_SUMS__: the cell list is not usable "
+                               "for this walk");
+    if (periodic_L())
+        throw Error(AQ_EUNSUP, "periodic box: the cell list is not usable "
                                "for this walk");
     launch_fluid_pass(dims, f.shep, f.lapp, (const int32_t*)P(f.imove), fr,
                       (const float*)P(f.u), (const float*)P(f.rho),
@@ -711,6 +759,10 @@ void Sim::check_status()
             ? std::string()
             : " in the step ending at iter = " + std::to_string(st[1]) +
                   " (the particles are kept as they were before it)";
+    if (st[0] & 4u)
+        throw Error(AQ_EINVAL,
+                    "link-list: a periodic axis of the box (periodic_L) is "
+                    "shorter than 3 cells of support * h" + when);
     if (st[0] & 1u)
         throw Error(AQ_ENOMEM,
                     "link-list: the cell grid outgrew the ihoc capacity "

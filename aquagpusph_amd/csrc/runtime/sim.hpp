@@ -359,6 +359,13 @@ class Sim {
     void* ptr(const std::string& name);
     void* ptr(Var* v);
     int assert_id(const std::string& name);
+    // device address of periodic_L when the case declares a periodic box
+    // (Presets/cfd/periodic.xml), nullptr otherwise
+    const float* periodic_L()
+    {
+        Var* v = vars.get("periodic_L");
+        return v ? (const float*)ptr(v) : nullptr;
+    }
     // one step of the pipeline in order, following while/if/end; `hook` is
     // called around each executed tool (index, false before / true after).
     // With stop_on_assert (eager steps only: it synchronises the stream),
@@ -372,6 +379,7 @@ class Sim {
     void stamp_status(hipStream_t s);
 
   private:
+    void check_periodic();
     size_t control(size_t pc, hipStream_t s);
     void match_scopes();
     void step_segments(int nsteps);

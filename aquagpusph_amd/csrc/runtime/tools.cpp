@@ -18,7 +18,8 @@ void linklist_all(int dims, const float* r, uint32_t N, float* r_min,
                   float* r_max, const float* support, const float* h,
                   uint32_t* n_cells, uint32_t* icell, uint32_t* id_unsorted,
                   uint32_t* id_sorted, uint32_t* ihoc, uint32_t cap,
-                  uint32_t* status, void* ws, hipStream_t s);
+                  uint32_t* status, void* ws, hipStream_t s,
+                  const float* per_min, const float* per_L);
 size_t linklist_workspace(uint32_t N);
 size_t cell_list_workspace(uint32_t N);
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
@@ -455,6 +456,19 @@ class LinkListTool : public Tool {
         h_ = &S.vars.at("h");
         if (in_->type.base != 'f' || !in_->is_array())
             throw Error(AQ_EINVAL, "link-list input must be a vec array");
+        // a periodic box (Presets/cfd/periodic.xml): periodic_min and
+        // periodic_L, the axes with L > 0 are periodic
+        per_min_ = S.vars.get("periodic_min");
+        per_L_ = S.vars.get("periodic_L");
+        if (per_min_ || per_L_) {
+            for (Var* v : { per_min_, per_L_ })
+                if (!v || v->is_array() || v->type.base != 'f' ||
+                    v->type.ncomp < S.dims)
+                    throw Error(AQ_EINVAL,
+                                "link-list \"" + name_ +
+                                    "\": periodic_min and periodic_L must "
+                                    "both be vec variables");
+        }
         ws_.alloc(linklist_workspace(S.N));
     }
     // ihoc capacity: the grid of the domain box when domain_min/domain_max
@@ -526,7 +540,9 @@ class LinkListTool : public Tool {
                      (uint32_t*)S.ptr(nc_), (uint32_t*)icell_->dev,
                      (uint32_t*)perm_->dev, (uint32_t*)inv_->dev,
                      (uint32_t*)ihoc_->dev, (uint32_t)ihoc_->length,
-                     S.d_status, ws_.p, s);
+                     S.d_status, ws_.p, s,
+                     per_min_ ? (const float*)S.ptr(per_min_) : nullptr,
+                     per_L_ ? (const float*)S.ptr(per_L_) : nullptr);
         // the non-empty cell list of the cell-parallel walks
         CellList& c = S.clist;
         if (c.enabled && c.icell == icell_ && c.ihoc == ihoc_) {
@@ -560,6 +576,7 @@ class LinkListTool : public Tool {
   private:
     ToolDesc d_;
     Var *in_, *mn_, *mx_, *ihoc_, *icell_, *nc_, *perm_, *inv_, *sup_, *h_;
+    Var *per_min_ = nullptr, *per_L_ = nullptr;
     DevBuf ws_;
     size_t cap_ = 0;
 };

@@ -374,6 +374,74 @@ __global__ void eos_k(const uint32_t* __restrict__ iset,
     p[i] = (*p0p) + cs * cs * (rho[i] - refd[iset[i]]);
 }
 
+// Tait equation of state (no counterpart in the reference, whose only
+// liquid EOS is the linear one of basic/EOS.cl): with B = rho0 cs^2 / gamma,
+//   p = p0 + B ((rho / rho0)^gamma - 1),
+// for the particles basic/EOS.cl:35 computes (imove > 0 or imove == -1).
+// An integral gamma in [1, 16] is raised by binary powering in a fixed
+// order (the same products on the host), any other through powf.
+__device__ __forceinline__ float tait_pow(float x, float g)
+{
+    const float gi = rintf(g);
+    if (gi == g && gi >= 1.f && gi <= 16.f) {
+        unsigned n = (unsigned)gi;
+        float r = 1.f, b = x;
+        bool first = true;
+        while (n) {
+            if (n & 1u) {
+                r = first ? b : r * b;
+                first = false;
+            }
+            n >>= 1;
+            if (n) b = b * b;
+        }
+        return r;
+    }
+    return powf(x, g);
+}
+
+__global__ void tait_eos_k(const uint32_t* __restrict__ iset,
+                           const int* __restrict__ imove,
+                           const float* __restrict__ rho, float* __restrict__ p,
+                           const float* __restrict__ refd, uint32_t N,
+                           const float* csp, const float* p0p,
+                           const float* gammap)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int im = imove[i];
+    if ((im <= 0) && (im != -1)) return;
+    const float cs = *csp, g = *gammap;
+    const float rho0 = refd[iset[i]];
+    const float B = rho0 * cs * cs / g;
+    p[i] = (*p0p) + B * (tait_pow(rho[i] / rho0, g) - 1.f);
+}
+
+// cfd/Boundary/BI/InterpolationShepard.cl:48-76 with the inverse of the
+// Tait EOS for the element density:
+//   rho = rho0 ((p - p0) / B + 1)^(1 / gamma)
+__global__ void bi_interp_shepard_tait_k(const uint32_t* __restrict__ iset,
+                                         const int* __restrict__ imove,
+                                         const float* __restrict__ shepard,
+                                         float* __restrict__ rho,
+                                         float* __restrict__ p,
+                                         const float* __restrict__ refd,
+                                         uint32_t N, const float* csp,
+                                         const float* p0p, const float* gammap)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    if (imove[i] != -3) return;
+    float s = shepard[i];
+    if (s < 1.0E-6f) s = 1.f;
+    const float pi = p[i] / s;
+    p[i] = pi;
+    const float cs = *csp, g = *gammap;
+    const float rho0 = refd[iset[i]];
+    const float B = rho0 * cs * cs / g;
+    rho[i] = rho0 * powf((pi - (*p0p)) / B + 1.f, 1.f / g);
+}
+
 // basic/Binormal.cl:37-53
 template <int D>
 __global__ void binormal_k(const float* __restrict__ normal,
@@ -848,7 +916,7 @@ __global__ void __launch_bounds__(BS)
     sensors_k(const int* __restrict__ imove, const float* __restrict__ r,
               const float* __restrict__ m, float* __restrict__ u,
               float* __restrict__ rho, float* __restrict__ p, LLArgs ll,
-              float H, float CONW, float smax)
+              float H, float CONW, float smax, const float* per)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ll.N) return;
@@ -857,21 +925,28 @@ __global__ void __launch_bounds__(BS)
     const f3 r_i = ldv<D>(r, i);
     f3 uu = { 0.f, 0.f, 0.f };
     float rr = 0.f, pp = 0.f;
-    walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
-            ll.ihoc, [&](uint32_t j) {
-                const f3 r_ij = sub<D>(ldv<D>(r, j), r_i);
-                const float s = dot<D>(r_ij, r_ij);
-                if (s > smax) return;
-                if (i == j) return;
-                if (imove[j] != 1) return;
-                const float q = sqrtf(s) / H;
-                const float rho_j = rho[j];
-                const float w_ij = kernelW<D>(q) * CONW * m[j] / rho_j;
-                const f3 u_j = ldv<D>(u, j);
-                axpy<D>(uu, w_ij, u_j);
-                rr += rho_j * w_ij;
-                pp += p[j] * w_ij;
-            });
+    auto body = [&](uint32_t j, const f3& r_j) {
+        const f3 r_ij = sub<D>(r_j, r_i);
+        const float s = dot<D>(r_ij, r_ij);
+        if (s > smax) return;
+        if (i == j) return;
+        if (imove[j] != 1) return;
+        const float q = sqrtf(s) / H;
+        const float rho_j = rho[j];
+        const float w_ij = kernelW<D>(q) * CONW * m[j] / rho_j;
+        const f3 u_j = ldv<D>(u, j);
+        axpy<D>(uu, w_ij, u_j);
+        rr += rho_j * w_ij;
+        pp += p[j] * w_ij;
+    };
+    if (per)  // a periodic box: wrapped cells, images at r + shift
+        walk_per<D>(ll.icell[i], ll.N, ll.n_cells, per, ll.icell, ll.ihoc,
+                    [&](uint32_t j, const f3& sh) {
+                        body(j, per_shift<D>(ldv<D>(r, j), sh, per));
+                    });
+    else
+        walk<D>(ll.icell[i], ll.N, ll.n_cells[0], ll.n_cells[1], ll.icell,
+                ll.ihoc, [&](uint32_t j) { body(j, ldv<D>(r, j)); });
     stxyz<D>(u, i, uu);
     rho[i] = rr;
     p[i] = pp;
@@ -1307,6 +1382,23 @@ void launch_eos(const uint32_t* iset, const int32_t* imove, const float* rho,
     hipLaunchKernelGGL(eos_k, AQ_GRID(N), iset, imove, rho, p, refd, N, cs,
                        p0);
 }
+void launch_tait_eos(const uint32_t* iset, const int32_t* imove,
+                     const float* rho, float* p, const float* refd, uint32_t N,
+                     const float* cs, const float* p0, const float* gamma,
+                     hipStream_t s)
+{
+    hipLaunchKernelGGL(tait_eos_k, AQ_GRID(N), iset, imove, rho, p, refd, N,
+                       cs, p0, gamma);
+}
+void launch_bi_interp_shepard_tait(const uint32_t* iset, const int32_t* imove,
+                                   const float* shepard, float* rho, float* p,
+                                   const float* refd, uint32_t N,
+                                   const float* cs, const float* p0,
+                                   const float* gamma, hipStream_t s)
+{
+    hipLaunchKernelGGL(bi_interp_shepard_tait_k, AQ_GRID(N), iset, imove,
+                       shepard, rho, p, refd, N, cs, p0, gamma);
+}
 void launch_binormal(int dims, const float* normal, float* tangent,
                      float* binormal, uint32_t N, hipStream_t s)
 {
@@ -1481,11 +1573,11 @@ void launch_sensors(int dims, const int32_t* imove, const float* r,
                     const float* m, float* u, float* rho, float* p,
                     uint32_t N, const uint32_t* icell, const uint32_t* ihoc,
                     const uint32_t* n_cells, const uint32_t* status,
-                    aq_defs d, hipStream_t s)
+                    aq_defs d, hipStream_t s, const float* per)
 {
     AQ_DISPATCH(dims, sensors_k, AQ_GRID(N), imove, r, m, u, rho, p,
                 mkll(icell, ihoc, n_cells, status, N), d.H, d.CONW,
-                support_threshold(d.H));
+                support_threshold(d.H), per);
 }
 void launch_bi_shepard_compute(int dims, const int32_t* imove, const float* r,
                                const float* normal, const float* tangent,
@@ -1892,7 +1984,7 @@ AQ_API int aq_sensors(int dims, const int32_t* imove, const float* r,
         if (!N) return;
         AQ_PTRS(imove, r, m, u, rho, p, icell, ihoc, n_cells);
         launch_sensors(dims, imove, r, m, u, rho, p, N, icell, ihoc, n_cells,
-                       nullptr, defs, S(stream));
+                       nullptr, defs, S(stream), nullptr);
     });
 }
 

@@ -66,6 +66,9 @@ struct WalkArgs {
     uint32_t N;
     uint32_t flagged;  // cell_end carries AQ_CELL_FLUID / AQ_CELL_ELEM flags
     uint32_t ranges;   // 1: one global counter; NRANGE: per-XCD ranges
+    // periodic_L of a periodic box (the walks instantiated with PER wrap
+    // their neighbour cells), nullptr otherwise
+    const float* per = nullptr;
 };
 
 // Next cell (index into heads) for the calling wave, n_heads when all are
@@ -96,19 +99,26 @@ __device__ __forceinline__ uint32_t next_cell(const WalkArgs& w,
 // Shared state of one wavefront: neighbour-cell table, the candidate chunk
 // being tested and the two segment slots (candidate indices, accept bits;
 // SEGW 32-bit words of bits per lane and slot).
-template <int SEGW> struct WaveLds {
+template <int SEGW, bool PER = false> struct WaveLds {
     uint32_t start[27];
     uint32_t cum[28];
     float4 stage[64];
     uint32_t cand[2][SEGW * 32];
     uint32_t bits[2][SEGW][64];
+    // periodic box: the shift of each neighbour cell's particles
+    float4 shift[PER ? 27 : 1];
 };
+
+// Candidate words of the periodic walks: the particle index in the low
+// bits and the neighbour-table entry (0..26) in the top 5.
+constexpr uint32_t PER_KL_SHIFT = 27;
+constexpr uint32_t PER_J_MASK = (1u << PER_KL_SHIFT) - 1u;
 
 // Candidate table of cell c_i: start and prefix count of each neighbour
 // cell in walk order.  Cells known to hold none of the `want` kinds
 // (AQ_CELL_FLUID / AQ_CELL_ELEM) count as empty: they have no candidate the
 // caller accepts, so the sums are unchanged.
-template <int D, class Lds>
+template <int D, bool PER = false, class Lds>
 __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
                                                     uint32_t c_i, Lds& L,
                                                     int lane, uint32_t want)
@@ -127,8 +137,15 @@ __device__ __forceinline__ uint32_t neighbour_table(const WalkArgs& w,
             cj = (lane / 3) % 3 - 1;
             ck = lane % 3 - 1;
         }
-        const uint32_t c = c_i + (uint32_t)ci + (uint32_t)cj * nx +
-                           (uint32_t)ck * nx * ny;
+        uint32_t c;
+        if constexpr (PER) {
+            f3 sh;
+            c = per_neighbour<D>(c_i, ci, cj, ck, w.n_cells, w.per, sh);
+            L.shift[lane] = make_float4(sh.x, sh.y, sh.z, 0.f);
+        } else {
+            c = c_i + (uint32_t)ci + (uint32_t)cj * nx +
+                (uint32_t)ck * nx * ny;
+        }
         const uint32_t s = w.ihoc[c];
         uint32_t n = 0;
         if (s < w.N) {
@@ -194,15 +211,18 @@ __device__ __forceinline__ void wave_box(const f3& r_i, bool act, f3& bmin,
 // rounded subtraction, square and sum per axis in the same order, and
 // rounding is monotone, so it never exceeds a lane's s: the prefilter
 // drops only candidates every lane would reject.
-template <int D, int SEGW, class Pair>
-__device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
+// pair(j, kl): kl is the neighbour-table entry the candidate came from
+// (its shift in a periodic walk; 0 otherwise).
+template <int D, int SEGW, bool PER, class Pair>
+__device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                                           uint32_t total, const f3& r_i,
                                           bool act, const f3& bmin,
                                           const f3& bmax, float smax,
                                           const float* __restrict__ r,
                                           const int* __restrict__ imove,
                                           int kind, bool flag_el,
-                                          bool& el_near, Pair pair)
+                                          bool& el_near, Pair pair,
+                                          const float* __restrict__ per)
 {
     constexpr uint32_t SEG = SEGW * 32;
     const uint32_t nsegs = (total + SEG - 1) / SEG;
@@ -241,7 +261,11 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             if (bw != 0) {
                 const uint32_t b = __builtin_clz(bw);
                 bw ^= 0x80000000u >> b;
-                pair(cand[((nw - 1) % RING) * 32 + b]);
+                const uint32_t cw = cand[((nw - 1) % RING) * 32 + b];
+                if constexpr (PER)
+                    pair(cw & PER_J_MASK, cw >> PER_KL_SHIFT);
+                else
+                    pair(cw, 0u);
             }
         }
         if (tested >= nsegs) break;  // and no lane has a candidate left
@@ -257,7 +281,11 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
             if (cb + lane < nseg) {
                 while (t >= L.cum[kl + 1]) kl++;
                 j = L.start[kl] + (t - L.cum[kl]);
-                const f3 rj = ldv_o<D>(r, j);
+                f3 rj = ldv_o<D>(r, j);
+                if constexpr (PER) {
+                    const float4 sh = L.shift[kl];
+                    rj = per_shift<D>(rj, f3{ sh.x, sh.y, sh.z }, per);
+                }
                 const int im_j = ldi_o(imove, j);
                 is_el = (im_j == -3);
                 take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
@@ -287,7 +315,8 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW>& L, int lane,
                     __builtin_amdgcn_mbcnt_lo((uint32_t)tm, 0u));
                 if (take) {
                     L.stage[rank] = st;
-                    L.cand[slot][cb + rank] = j;
+                    L.cand[slot][cb + rank] =
+                        PER ? (j | ((uint32_t)kl << PER_KL_SHIFT)) : j;
                 }
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_s_waitcnt(0);
@@ -537,7 +566,8 @@ __device__ float fast_threshold(const uint32_t* __restrict__ rec, float H,
     return ok ? (float)T : 0.f;
 }
 
-template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, int MATH>
+template <int D, int SHEP, bool LAPP, int SEGW, int LAPF, int MATH,
+          bool PER = false>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     fluid_pass_cells_k(const int* __restrict__ imove,
                        const float* __restrict__ r,
@@ -555,10 +585,19 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                        float smax, float iH, const uint32_t* __restrict__ rec)
 {
     constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
-    __shared__ WaveLds<SEGW> lds[WC_WAVES];
+    __shared__ WaveLds<SEGW, PER> lds[WC_WAVES];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    WaveLds<SEGW>& L = lds[wv];
+    WaveLds<SEGW, PER>& L = lds[wv];
+    // a candidate's position, at its nearest image in a periodic box
+    auto rjof = [&](uint32_t j, uint32_t kl) {
+        f3 v = ldv_o<D>(r, j);
+        if constexpr (PER) {
+            const float4 sh = L.shift[kl];
+            v = per_shift<D>(v, f3{ sh.x, sh.y, sh.z }, w.per);
+        }
+        return v;
+    };
     if (w.status && (*w.status & 1u)) return;
     const uint32_t n_heads = uni(*w.n_heads);
     const float eps = 1E-8f * H;
@@ -587,7 +626,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             const uint32_t i = i0 + lane;
             const bool act = (i < cend) && (imove[i] == 1);
             if (__ballot(act) == 0) continue;  // no fluid particle here
-            const uint32_t total = neighbour_table<D>(
+            const uint32_t total = neighbour_table<D, PER>(
                 w, c_i, L, lane,
                 AQ_CELL_FLUID | (flag_el ? AQ_CELL_ELEM : 0u));
             f3 r_i = { 0.f, 0.f, 0.f }, u_i = { 0.f, 0.f, 0.f };
@@ -669,8 +708,8 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             // div_nr / sqrt_nr where the per-pair bounds hold (s == 0 or
             // s >= 2^-96, udr == 0 or |udr| >= T), the IEEE sequences for
             // the other pairs
-            auto pair_fd = [&](uint32_t j) {
-                const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+            auto pair_fd = [&](uint32_t j, uint32_t kl) {
+                const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                 const float s = dot<D>(r_ij, r_ij);
                 const float qf = div_nr(sqrt_nr(s), RH);
                 const float m_j = ldf_o(m, j);
@@ -687,12 +726,12 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     terms(std::false_type{}, r_ij, sqrtf(s) / H, m_j, rho_j,
                           p_j, u_ij, udr);
             };
-            auto pair_ieee = [&](uint32_t j) {
+            auto pair_ieee = [&](uint32_t j, uint32_t kl) {
                     if constexpr (FAST) {
                         // OpenCL C default accuracy: reciprocal-based
                         // division, hardware square root, contracted
                         // multiply-adds
-                        const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+                        const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                         const float s = fdot<D>(r_ij, r_ij);
                         const float q = __builtin_amdgcn_sqrtf(s) * iH;
                         const float m_j = ldf_o(m, j);
@@ -719,7 +758,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                         if (LAPP) lp = fmaf(p_j - p_i, f_ij * irho_j, lp);
                         return;
                     }
-                    const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+                    const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                     const float s = dot<D>(r_ij, r_ij);
                     const float q = sqrtf(s) / H;
                     const float m_j = ldf_o(m, j);
@@ -734,26 +773,27 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 if (fastdiv) {
                     ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
                                        smax, r, imove, 0, flag_el, nearw,
-                                       pair_fd);
+                                       pair_fd, w.per);
                 } else {
                     ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
                                        smax, r, imove, 0, flag_el, nearw,
-                                       pair_ieee);
+                                       pair_ieee, w.per);
                 }
             } else {
                 ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax, smax,
-                                   r, imove, 0, flag_el, nearw, pair_ieee);
+                                   r, imove, 0, flag_el, nearw, pair_ieee,
+                                   w.per);
             }
             // ---- wall elements (boundary-integral Shepard) ------------------
             if (SHEP == 2 && __ballot(nearw) != 0) {
                 bool self_added = false;
                 bool unused = false;
                 const uint32_t total_el =
-                    neighbour_table<D>(w, c_i, L, lane, AQ_CELL_ELEM);
+                    neighbour_table<D, PER>(w, c_i, L, lane, AQ_CELL_ELEM);
                 ring_walk<D, SEGW>(
                     L, lane, total_el, r_i, act && nearw, bmin, bmax, smax,
-                    r, imove, 1, false, unused, [&](uint32_t j) {
-                        const f3 r_ij = sub<D>(ldv_o<D>(r, j), r_i);
+                    r, imove, 1, false, unused, [&](uint32_t j, uint32_t kl) {
+                        const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                         const float s = dot<D>(r_ij, r_ij);
                         const float q = sqrtf(s) / H;
                         const f3 n_j = ldv_o<D>(normal, j);
@@ -778,7 +818,8 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                         const float r_b = fabsf(dot<D>(r_ij, b_j));
                         sh += r_n * CONW * kernelS_P<D>(q) * area_j +
                               kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
-                    });
+                    },
+                    w.per);
             }
             if (act) {
                 if constexpr (MATH == 2) {
@@ -820,15 +861,17 @@ template <bool NRQ> __device__ __forceinline__ float qof(float s, float H)
     else return sqrtf(s) / H;
 }
 
-template <int D, int SEGW, class Pol>
+// PER: a periodic box; the policy's pair_p<NRQ>(i, j, shift, r_i, st) gets
+// the shift of j's neighbour cell.
+template <int D, int SEGW, class Pol, bool PER = false>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     cell_kernel_k(Pol pol, const int* __restrict__ imove,
                   const float* __restrict__ r, WalkArgs w, float smax,
                   int nrq)
 {
-    __shared__ WaveLds<SEGW> lds[WC_WAVES];
+    __shared__ WaveLds<SEGW, PER> lds[WC_WAVES];
     const int lane = threadIdx.x & 63;
-    WaveLds<SEGW>& L = lds[threadIdx.x >> 6];
+    WaveLds<SEGW, PER>& L = lds[threadIdx.x >> 6];
     if (w.status && (*w.status & 1u)) return;
     const uint32_t n_heads = uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
@@ -847,7 +890,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             const bool act = (i < cend) && pol.owner(i, imove[i]);
             if (__ballot(act) == 0) continue;
             if (!table) {
-                total = neighbour_table<D>(
+                total = neighbour_table<D, PER>(
                     w, c_i, L, lane,
                     Pol::KIND ? AQ_CELL_ELEM : AQ_CELL_FLUID);
                 table = true;
@@ -861,18 +904,32 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             bool unused = false;
             f3 bmin, bmax;
             wave_box<D>(r_i, act, bmin, bmax);
+            auto call = [&](auto nrqc, uint32_t j, uint32_t kl) {
+                constexpr bool NRQ = decltype(nrqc)::value;
+                if constexpr (PER) {
+                    const float4 sh = L.shift[kl];
+                    pol.template pair_p<NRQ>(i, j, f3{ sh.x, sh.y, sh.z },
+                                             r_i, st);
+                } else {
+                    pol.template pair<NRQ>(i, j, r_i, st);
+                }
+            };
             if (nrq)
                 ring_walk<D, SEGW>(
                     L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
-                    Pol::KIND, false, unused, [&](uint32_t j) {
-                        pol.template pair<true>(i, j, r_i, st);
-                    });
+                    Pol::KIND, false, unused,
+                    [&](uint32_t j, uint32_t kl) {
+                        call(std::true_type{}, j, kl);
+                    },
+                    w.per);
             else
                 ring_walk<D, SEGW>(
                     L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
-                    Pol::KIND, false, unused, [&](uint32_t j) {
-                        pol.template pair<false>(i, j, r_i, st);
-                    });
+                    Pol::KIND, false, unused,
+                    [&](uint32_t j, uint32_t kl) {
+                        call(std::false_type{}, j, kl);
+                    },
+                    w.per);
             if (act) pol.finish(i, st);
             __builtin_amdgcn_wave_barrier();
         }
@@ -1074,6 +1131,147 @@ template <int D> struct PolBiInteractions {
     }
 };
 
+// ---- LES closure pair kernels (cfd/LES/GradU.cl, cfd/LES/Interactions.cl
+// of Presets/cfd/LES.xml; no counterpart in the reference) -------------------
+// The SPH velocity gradient of the fluid particles,
+//   G_i = sum_j m_j / rho_j (u_j - u_i) (x) F_ij r_ij,
+// grad_u[a n + b] = du_a / dx_b (row stride n = 2 in 2D, 4 in 3D), summed
+// in the walk order; pair_p: the same at a periodic image.  The non-fluid
+// particles' rows are cleared by the launcher.
+template <int D> struct PolLesGradU {
+    static constexpr int KIND = 0;
+    static constexpr uint32_t OWNERS = AQ_CELL_FLUID;
+    static constexpr int NR = (D == 2) ? 2 : 4;
+    const float *r_, *u, *rho, *m;
+    float* grad_u;
+    float H, CONF;
+    const float* per;
+    struct State {
+        f3 u_i;
+        float G[D * D];
+    };
+    __device__ bool owner(uint32_t, int im) const { return im == 1; }
+    __device__ void init(uint32_t i, const f3&, State& st) const
+    {
+        st.u_i = ldv<D>(u, i);
+#pragma unroll
+        for (int k = 0; k < D * D; k++) st.G[k] = 0.f;
+    }
+    __device__ void add(uint32_t j, const f3& r_j, const f3& r_i, float q,
+                        State& st) const
+    {
+        const f3 r_ij = sub<D>(r_j, r_i);
+        const float w = kernelF<D>(q) * CONF * ldf_o(m, j) / ldf_o(rho, j);
+        const f3 du = sub<D>(ldv_o<D>(u, j), st.u_i);
+        const float dv[3] = { du.x, du.y, du.z };
+        const float rv[3] = { r_ij.x, r_ij.y, r_ij.z };
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++)
+                st.G[a * D + b] += dv[a] * (w * rv[b]);
+    }
+    template <bool NRQ>
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        if (i == j) return;
+        const f3 r_j = ldv_o<D>(r_, j);
+        const f3 r_ij = sub<D>(r_j, r_i);
+        add(j, r_j, r_i, qof<NRQ>(dot<D>(r_ij, r_ij), H), st);
+    }
+    template <bool NRQ>
+    __device__ void pair_p(uint32_t i, uint32_t j, const f3& sh,
+                           const f3& r_i, State& st) const
+    {
+        if (i == j) return;
+        const f3 r_j = per_shift<D>(ldv_o<D>(r_, j), sh, per);
+        const f3 r_ij = sub<D>(r_j, r_i);
+        add(j, r_j, r_i, qof<NRQ>(dot<D>(r_ij, r_ij), H), st);
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        constexpr int S = (D == 2) ? 4 : 16;
+        float o[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) o[k] = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int b = 0; b < D; b++) o[a * NR + b] = st.G[a * D + b];
+#pragma unroll
+        for (int k = 0; k < S; k++) grad_u[(size_t)i * S + k] = o[k];
+    }
+};
+
+// The eddy-viscosity term in the variable-viscosity Morris form,
+//   dudt_i += sum_j F_ij m_j (rho_i nu_i + rho_j nu_j) / (rho_i rho_j)
+//             (u_j - u_i),
+// summed in the walk order and added to dudt at the end.
+template <int D> struct PolLesInteractions {
+    static constexpr int KIND = 0;
+    static constexpr uint32_t OWNERS = AQ_CELL_FLUID;
+    const float *r_, *u, *rho, *m, *nu_t;
+    float* dudt;
+    float H, CONF;
+    const float* per;
+    struct State {
+        f3 u_i, acc;
+        float rho_i, mu_i;
+    };
+    __device__ bool owner(uint32_t, int im) const { return im == 1; }
+    __device__ void init(uint32_t i, const f3&, State& st) const
+    {
+        st.u_i = ldv<D>(u, i);
+        st.rho_i = rho[i];
+        st.mu_i = st.rho_i * nu_t[i];
+        st.acc = f3{ 0.f, 0.f, 0.f };
+    }
+    __device__ void add(uint32_t j, float q, State& st) const
+    {
+        const float rho_j = ldf_o(rho, j);
+        const float f = kernelF<D>(q) * CONF * ldf_o(m, j);
+        const float sc =
+            f * (st.mu_i + rho_j * ldf_o(nu_t, j)) / (st.rho_i * rho_j);
+        axpy<D>(st.acc, sc, sub<D>(ldv_o<D>(u, j), st.u_i));
+    }
+    template <bool NRQ>
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
+                         State& st) const
+    {
+        if (i == j) return;
+        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        add(j, qof<NRQ>(dot<D>(r_ij, r_ij), H), st);
+    }
+    template <bool NRQ>
+    __device__ void pair_p(uint32_t i, uint32_t j, const f3& sh,
+                           const f3& r_i, State& st) const
+    {
+        if (i == j) return;
+        const f3 r_ij = sub<D>(per_shift<D>(ldv_o<D>(r_, j), sh, per), r_i);
+        add(j, qof<NRQ>(dot<D>(r_ij, r_ij), H), st);
+    }
+    __device__ void finish(uint32_t i, const State& st) const
+    {
+        const f3 d = ldv<D>(dudt, i);
+        stxyz<D>(dudt, i, f3{ d.x + st.acc.x, d.y + st.acc.y,
+                              d.z + st.acc.z });
+    }
+};
+
+// grad_u of the particles that are not fluid: zero (the per-particle
+// kernel writes zeros for them)
+template <int D>
+__global__ void les_gradu_clear_k(const int* __restrict__ imove,
+                                  float* __restrict__ grad_u, uint32_t N)
+{
+    constexpr int S = (D == 2) ? 4 : 16;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || imove[i] == 1) return;
+#pragma unroll
+    for (int k = 0; k < S; k++) grad_u[(size_t)i * S + k] = 0.f;
+}
+
 // Clears the NRANGE range counters.
 __global__ void zero_u32_k(uint32_t* p, uint32_t* rec)
 {
@@ -1128,8 +1326,12 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
                              int flagged, int math, unsigned ranges,
-                             int fast_div, hipStream_t s)
+                             int fast_div, hipStream_t s, const float* per)
 {
+    // a periodic box has no walls: the boundary-integral Shepard pass is
+    // not built for it
+    AQ_REQUIRE(!per || shep != 2,
+               "periodic box: boundary-integral walls are not supported");
     // candidate loads use 32-bit byte offsets (16 B per particle at most)
     AQ_REQUIRE(N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     // the hand-out counters live after the non-empty cell count
@@ -1150,18 +1352,25 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     }
     WalkArgs w{ icell,  ihoc,   n_cells, cell_end, heads,
                 n_heads, next, status, N,       (uint32_t)flagged,
-                ranges ? std::min<unsigned>(ranges, NRANGE) : walk_ranges() };
-#define AQ_FC2(DD, SH, LP, SW, LF)                                             \
+                ranges ? std::min<unsigned>(ranges, NRANGE) : walk_ranges(),
+                per };
+#define AQ_FC3(DD, SH, LP, SW, LF, PR)                                         \
     do {                                                                       \
-        auto* ker = math == 1   ? fluid_pass_cells_k<DD, SH, LP, SW, LF, 1>    \
-                    : math == 2 ? fluid_pass_cells_k<DD, SH, LP, 8, LF, 2>     \
-                                : fluid_pass_cells_k<DD, SH, LP, SW, LF, 0>;   \
+        auto* ker = math == 1                                                  \
+                        ? fluid_pass_cells_k<DD, SH, LP, SW, LF, 1, PR>        \
+                    : math == 2 ? fluid_pass_cells_k<DD, SH, LP, 8, LF, 2, PR> \
+                                : fluid_pass_cells_k<DD, SH, LP, SW, LF, 0, PR>; \
         const unsigned nb = nblocks ? nblocks : resident_blocks((const void*)ker, \
                                                     64 * WC_WAVES);            \
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \
                            u, rho, m, p, normal, tangent, binormal, shepard,   \
                            grad_p, lap_u, div_u, lap_p, near, w, d.H, d.CONW,  \
                            d.CONF, d.DIMS, smax, 1.f / d.H, rec);              \
+    } while (0)
+#define AQ_FC2(DD, SH, LP, SW, LF)                                             \
+    do {                                                                       \
+        if (per && SH != 2) AQ_FC3(DD, SH, LP, SW, LF, (SH != 2));             \
+        else AQ_FC3(DD, SH, LP, SW, LF, false);                                \
     } while (0)
 #define AQ_FC(DD, SH, LP, SW)                                                  \
     do {                                                                       \
@@ -1186,13 +1395,15 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
 #undef AQ_FC_D
 #undef AQ_FC
 #undef AQ_FC2
+#undef AQ_FC3
 }
 
 
 // ---- launchers of the generic cell kernels ---------------------------------
-template <int D, class Pol>
+template <int D, class Pol, bool PER = false>
 static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
-                         const CellArgs& c, float smax, hipStream_t s)
+                         const CellArgs& c, float smax, hipStream_t s,
+                         const float* per = nullptr)
 {
     AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     uint32_t* next = const_cast<uint32_t*>(c.n_heads) + RSTRIDE;
@@ -1200,8 +1411,8 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
                        (uint32_t*)nullptr);
     WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
                 c.n_heads, next,     c.status,  c.N,        c.flagged,
-                walk_ranges() };
-    auto* ker = cell_kernel_k<D, 8, Pol>;
+                walk_ranges(), per };
+    auto* ker = cell_kernel_k<D, 8, Pol, PER>;
     const unsigned nb = resident_blocks((const void*)ker, 64 * WC_WAVES);
     const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
@@ -1271,6 +1482,49 @@ void launch_bi_interactions_cells(int dims, const int32_t* imove,
         PolBiInteractions<3> q{ r, normal, u, rho, m, p, grad_p, div_u,
                                 near, d.H, d.CONW };
         launch_cells<3>(q, imove, r, c, smax, s);
+    }
+}
+
+void launch_les_gradu_cells(int dims, const int32_t* imove, const float* r,
+                            const float* u, const float* rho, const float* m,
+                            float* grad_u, const CellArgs& c, aq_defs d,
+                            float smax, const float* per, hipStream_t s)
+{
+    const dim3 g(blocks_for(c.N, 256)), b(256);
+    if (dims == 2) {
+        hipLaunchKernelGGL(les_gradu_clear_k<2>, g, b, 0, s, imove, grad_u,
+                           c.N);
+        PolLesGradU<2> p{ r, u, rho, m, grad_u, d.H, d.CONF, per };
+        if (per) launch_cells<2, PolLesGradU<2>, true>(p, imove, r, c, smax, s, per);
+        else launch_cells<2>(p, imove, r, c, smax, s);
+    } else {
+        hipLaunchKernelGGL(les_gradu_clear_k<3>, g, b, 0, s, imove, grad_u,
+                           c.N);
+        PolLesGradU<3> p{ r, u, rho, m, grad_u, d.H, d.CONF, per };
+        if (per) launch_cells<3, PolLesGradU<3>, true>(p, imove, r, c, smax, s, per);
+        else launch_cells<3>(p, imove, r, c, smax, s);
+    }
+}
+
+void launch_les_interactions_cells(int dims, const int32_t* imove,
+                                   const float* r, const float* u,
+                                   const float* rho, const float* m,
+                                   const float* nu_t, float* dudt,
+                                   const CellArgs& c, aq_defs d, float smax,
+                                   const float* per, hipStream_t s)
+{
+    if (dims == 2) {
+        PolLesInteractions<2> p{ r, u, rho, m, nu_t, dudt, d.H, d.CONF, per };
+        if (per)
+            launch_cells<2, PolLesInteractions<2>, true>(p, imove, r, c, smax,
+                                                         s, per);
+        else launch_cells<2>(p, imove, r, c, smax, s);
+    } else {
+        PolLesInteractions<3> p{ r, u, rho, m, nu_t, dudt, d.H, d.CONF, per };
+        if (per)
+            launch_cells<3, PolLesInteractions<3>, true>(p, imove, r, c, smax,
+                                                         s, per);
+        else launch_cells<3>(p, imove, r, c, smax, s);
     }
 }
 

@@ -69,6 +69,26 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
     {                                                                          \
         icell<D, R>(ic, r, N, rmin, support, h, nc);                           \
     }                                                                          \
+    ORC_EXPORT int orc_n_cells_per_##SR##_##D##d(                              \
+        const R* rmin, const R* rmax, const R* pmin, const R* L,               \
+        R cell_length, usize* nc, R* rmin_out)                                 \
+    {                                                                          \
+        return n_cells_per<D, R>(rmin, rmax, pmin, L, cell_length, nc,         \
+                                 rmin_out)                                     \
+                   ? 1                                                         \
+                   : 0;                                                        \
+    }                                                                          \
+    ORC_EXPORT void orc_icell_per_##SR##_##D##d(                               \
+        usize* ic, const R* r, usize N, const R* rmin, R support, R h,         \
+        const usize* nc, const R* pmin, const R* L)                            \
+    {                                                                          \
+        icell_per<D, R>(ic, r, N, rmin, support, h, nc, pmin, L);              \
+    }                                                                          \
+    ORC_EXPORT void orc_periodic_wrap_##SR##_##D##d(R* r, usize N,             \
+                                                    const R* pmin, const R* L) \
+    {                                                                          \
+        periodic_wrap<D, R>(r, N, pmin, L);                                    \
+    }                                                                          \
     ORC_EXPORT void orc_predictor_##SR##_##D##d(                               \
         const int* imove, const R* r, const R* u, const R* dudt, const R* rho, \
         const R* drhodt, R* r_in, R* u_in, R* dudt_in, R* rho_in,             \
@@ -96,6 +116,19 @@ ORC_EXPORT void orc_neighs_3d(const int* imove, usize* n_neighs, usize limit,
                                           R p0)                               \
     {                                                                          \
         eos<R>(iset, imove, rho, p, refd, N, cs, p0);                          \
+    }                                                                          \
+    ORC_EXPORT void orc_tait_eos_##SR##_##D##d(                                \
+        const unsigned* iset, const int* imove, const R* rho, R* p,            \
+        const R* refd, usize N, R cs, R p0, R gamma)                           \
+    {                                                                          \
+        tait_eos<R>(iset, imove, rho, p, refd, N, cs, p0, gamma);              \
+    }                                                                          \
+    ORC_EXPORT void orc_bi_interp_shepard_tait_##SR##_##D##d(                  \
+        const unsigned* iset, const int* imove, const R* shepard_, R* rho,     \
+        R* p, const R* refd, usize N, R cs, R p0, R gamma)                     \
+    {                                                                          \
+        bi_interp_shepard_tait<R>(iset, imove, shepard_, rho, p, refd, N, cs,  \
+                                  p0, gamma);                                  \
     }                                                                          \
     ORC_EXPORT void orc_binormal_##SR##_##D##d(const R* normal, R* tangent,   \
                                                R* binormal_, usize N)         \
@@ -625,4 +658,13 @@ ORC_EXPORT void orc_clmath_ulp(int op, unsigned lo, unsigned hi,
     }
     out[0] = best;
     out[1] = arg;
+}
+
+// The periodic box of the oracle's walks: lengths along x, y, z (0: that
+// axis is not periodic; all 0: no periodic box).
+ORC_EXPORT void orc_set_periodic(double Lx, double Ly, double Lz)
+{
+    orc::g_perL[0] = Lx;
+    orc::g_perL[1] = Ly;
+    orc::g_perL[2] = Lz;
 }

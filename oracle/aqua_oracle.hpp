@@ -256,6 +256,70 @@ inline void walk(usize c_i, usize N, const usize* nc, const usize* icell,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Periodic box (Presets/cfd/periodic.xml; no counterpart in the reference,
+// whose periodic domains are plane portals): the lengths along the periodic
+// axes, 0 along the others, set by orc_set_periodic for the walks below.
+// Along a periodic axis the grid tiles the box with n cells (no margins),
+// the neighbour cell c + o wraps modulo n and its particles are seen at
+// r + shift, shift = -L (wrapped below 0) or +L (above n - 1), added only
+// along the periodic axes.
+// ---------------------------------------------------------------------------
+inline double g_perL[3] = { 0.0, 0.0, 0.0 };
+inline bool per_on() { return g_perL[0] > 0 || g_perL[1] > 0 || g_perL[2] > 0; }
+
+// walk<D> handing each candidate's position (at its periodic image) to
+// f(j, r_j); without a periodic box the plain walk and r[j]
+template <int D, typename R, class F>
+inline void walk_r(usize c_i, usize N, const usize* nc, const usize* icell,
+                   const usize* ihoc, const R* r, F f)
+{
+    const int VS = Dim<D>::VS;
+    auto pos = [&](usize j) {
+        V<D, R> v;
+        for (int k = 0; k < 3; k++)
+            v.c[k] = (k < D) ? r[(size_t)j * VS + k] : R(0);
+        return v;
+    };
+    if (!per_on()) {
+        walk<D>(c_i, N, nc, icell, ihoc, [&](usize j) { f(j, pos(j)); });
+        return;
+    }
+    const int q[3] = { (int)(c_i % nc[0]), (int)((c_i / nc[0]) % nc[1]),
+                       D == 3 ? (int)(c_i / (nc[0] * nc[1])) : 0 };
+    for (int ci = -1; ci <= 1; ci++)
+        for (int cj = -1; cj <= 1; cj++)
+            for (int ck = (D == 3 ? -1 : 0); ck <= (D == 3 ? 1 : 0); ck++) {
+                const int o[3] = { ci, cj, ck };
+                int v[3];
+                R sh[3] = { R(0), R(0), R(0) };
+                for (int k = 0; k < 3; k++) {
+                    v[k] = q[k] + o[k];
+                    if (k < D && g_perL[k] > 0) {
+                        const int n = (int)nc[k];
+                        const R L = R(g_perL[k]);
+                        if (v[k] < 0) {
+                            v[k] += n;
+                            sh[k] = -L;
+                        } else if (v[k] >= n) {
+                            v[k] -= n;
+                            sh[k] = L;
+                        }
+                    }
+                }
+                const usize c_j = (usize)v[0] + (usize)v[1] * nc[0] +
+                                  (D == 3 ? (usize)v[2] * nc[0] * nc[1] : 0u);
+                usize j = ihoc[c_j];
+                while ((j < N) && (icell[j] == c_j)) {
+                    V<D, R> rj = pos(j);
+                    for (int k = 0; k < D; k++)
+                        if (g_perL[k] > 0) rj.c[k] = rj.c[k] + sh[k];
+                    f(j, rj);
+                    j++;
+                }
+            }
+}
+
 // float -> unsigned conversion with the AMDGPU v_cvt_u32_f32 saturation
 // (NaN and negatives -> 0, too large -> UINT_MAX); the OpenCL cast is
 // implementation-defined outside [0, 2^32).
@@ -536,6 +600,62 @@ void eos(const unsigned* iset, const int* imove, const R* rho, R* p,
     }
 }
 
+// Tait EOS (no counterpart in the reference; the linear EOS above is its
+// only liquid law): p = p0 + B ((rho / rho0)^gamma - 1), B = rho0 cs^2 /
+// gamma, for the particles of basic/EOS.cl:35.  An integral gamma in
+// [1, 16] is raised by binary powering in the device's order of products.
+template <typename R> R tait_pow(R x, R g)
+{
+    const R gi = std::nearbyint(g);
+    if (gi == g && gi >= R(1) && gi <= R(16)) {
+        unsigned n = (unsigned)gi;
+        R r = R(1), b = x;
+        bool first = true;
+        while (n) {
+            if (n & 1u) {
+                r = first ? b : r * b;
+                first = false;
+            }
+            n >>= 1;
+            if (n) b = b * b;
+        }
+        return r;
+    }
+    return std::pow(x, g);
+}
+
+template <typename R>
+void tait_eos(const unsigned* iset, const int* imove, const R* rho, R* p,
+              const R* refd, usize N, R cs, R p0, R gamma)
+{
+#pragma omp parallel for schedule(static)
+    for (long long ii = 0; ii < (long long)N; ii++) {
+        const size_t i = (size_t)ii;
+        if ((imove[i] <= 0) && (imove[i] != -1)) continue;
+        const R rho0 = refd[iset[i]];
+        const R B = rho0 * cs * cs / gamma;
+        p[i] = p0 + B * (tait_pow(rho[i] / rho0, gamma) - R(1));
+    }
+}
+
+// cfd/Boundary/BI/InterpolationShepard.cl:48-76 with the Tait inverse for
+// the element density, rho = rho0 ((p - p0) / B + 1)^(1 / gamma)
+template <typename R>
+void bi_interp_shepard_tait(const unsigned* iset, const int* imove,
+                            const R* shepard, R* rho, R* p, const R* refd,
+                            usize N, R cs, R p0, R gamma)
+{
+    for (usize i = 0; i < N; i++) {
+        if (imove[i] != -3) continue;
+        R s = shepard[i];
+        if (s < R(1.0E-6f)) s = R(1);
+        p[i] /= s;
+        const R rho0 = refd[iset[i]];
+        const R B = rho0 * cs * cs / gamma;
+        rho[i] = rho0 * std::pow((p[i] - p0) / B + R(1), R(1) / gamma);
+    }
+}
+
 // basic/Binormal.cl:37-53
 template <int D, typename R>
 void binormal(const R* normal, R* tangent, R* binormal, usize N)
@@ -687,6 +807,72 @@ void sensors_renorm(const int* imove, const R* shepard, R* u, R* rho, R* p,
     }
 }
 
+// The grid and the cell hash of a periodic box (periodic_min pmin,
+// lengths L; the axes with L == 0 as in n_cells / icell above): n =
+// trunc(L / cell_length) cells along a periodic axis (at least 3), the cell
+// coordinate min(trunc((r - pmin) n / L), n - 1).
+template <int D, typename R>
+bool n_cells_per(const R* rmin, const R* rmax, const R* pmin, const R* L,
+                 R cell_length, usize* nc, R* rmin_out)
+{
+    uint64_t n[3] = { 1, 1, 1 };
+    bool ok = true;
+    for (int k = 0; k < D; k++) {
+        if (L[k] > R(0)) {
+            n[k] = (uint64_t)cvt_u32(L[k] / cell_length);
+            ok = ok && n[k] >= 3;
+            rmin_out[k] = pmin[k];
+            continue;
+        }
+        const R v = (rmax[k] - rmin[k]) / cell_length;
+        n[k] = (uint64_t)cvt_u32(v) + 6;
+        rmin_out[k] = rmin[k];
+    }
+    nc[0] = (usize)n[0];
+    nc[1] = (usize)n[1];
+    nc[2] = (usize)n[2];
+    nc[3] = (usize)(n[0] * n[1] * n[2]);
+    return ok;
+}
+
+template <int D, typename R>
+void icell_per(usize* ic, const R* r, usize N, const R* rmin, R support, R h,
+               const usize* nc, const R* pmin, const R* L)
+{
+    const int VS = Dim<D>::VS;
+    const R idist = R(1) / (support * h);
+    for (usize i = 0; i < N; i++) {
+        usize q[3] = { 0, 0, 0 };
+        for (int k = 0; k < D; k++) {
+            const R x = r[(size_t)i * VS + k];
+            if (L[k] > R(0)) {
+                const R inv = R((R)nc[k]) / L[k];
+                q[k] = std::min<usize>(cvt_u32((x - pmin[k]) * inv),
+                                       nc[k] - 1u);
+            } else {
+                q[k] = cvt_u32((x - rmin[k]) * idist) + 2u;
+            }
+        }
+        ic[i] = q[0] + q[1] * nc[0] + (D == 3 ? q[2] * nc[0] * nc[1] : 0u);
+    }
+}
+
+// cfd periodic wrap: r back into [pmin, pmin + L) along the periodic axes
+// (one wrap: a particle moves less than L in a step)
+template <int D, typename R>
+void periodic_wrap(R* r, usize N, const R* pmin, const R* L)
+{
+    const int VS = Dim<D>::VS;
+    for (usize i = 0; i < N; i++)
+        for (int k = 0; k < D; k++) {
+            if (!(L[k] > R(0))) continue;
+            R x = r[(size_t)i * VS + k];
+            if (x < pmin[k]) x = x + L[k];
+            else if (x >= pmin[k] + L[k]) x = x - L[k];
+            r[(size_t)i * VS + k] = x;
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Pair kernels.  H, CONW, CONF are the preset definitions after the
 // reference's "%#G" formatting (CalcServer.cpp:241-260).
@@ -713,9 +899,10 @@ void shepard(const int* imove, const R* r, const R* rho, const R* m,
         if ((imove[i] < -3) || ((imove[i] > 0) && excl(i))) continue;
         const V<D, R> r_i = ld<D, R>(r, i);
         R acc = shep[i];
-        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+        walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                        [&](usize j, const V<D, R>& r_j) {
             if (excl(j)) return;
-            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> r_ij = sub<D, R>(r_j, r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             acc += kernelW<D, R>(q) * CONW * m[j] / rho[j];
@@ -755,10 +942,11 @@ void interactions(const int* imove, const R* r, const R* u, const R* rho,
         V<D, R> gp = ld<D, R>(grad_p, i), lu = ld<D, R>(lap_u, i);
         R du = div_u[i];
         R cg[3] = { 0, 0, 0 }, cl[3] = { 0, 0, 0 }, cd = 0;
-        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+        walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                        [&](usize j, const V<D, R>& r_j) {
             if (i == j) return;
             if (imove[j] != 1) return;
-            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> r_ij = sub<D, R>(r_j, r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             const R rho_j = rho[j];
@@ -813,9 +1001,10 @@ void lapp(const int* imove, const R* r, const R* rho, const R* m, const R* p,
         const V<D, R> r_i = ld<D, R>(r, i);
         const R p_i = p[i];
         R acc = lap_p[i], c = 0;
-        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+        walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                        [&](usize j, const V<D, R>& r_j) {
             if ((i == j) || (imove[j] != 1)) return;
-            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> r_ij = sub<D, R>(r_j, r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             const R f_ij = kernelF<D, R>(q) * CONF * m[j] / rho[j];
@@ -836,10 +1025,11 @@ void sensors(const int* imove, const R* r, const R* m, R* u, R* rho, R* p,
         if (imove[i] != 0) continue;
         const V<D, R> r_i = ld<D, R>(r, i);
         R uu[3] = { 0, 0, 0 }, rr = 0, pp = 0;
-        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+        walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                     [&](usize j, const V<D, R>& r_j) {
             if (i == j) return;
             if (imove[j] != 1) return;
-            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> r_ij = sub<D, R>(r_j, r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             const R rho_j = rho[j];
@@ -1969,9 +2159,10 @@ void les_gradu(const int* imove, const R* r, const R* u, const R* rho,
         R G[16] = { 0 };
         if (imove[i] == 1) {
             const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
-            walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+            walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                        [&](usize j, const V<D, R>& r_j) {
                 if (j == i || imove[j] != 1) return;
-                const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+                const V<D, R> r_ij = sub<D, R>(r_j, r_i);
                 const R q = length<D, R>(r_ij) / H;
                 if (q >= R(2)) return;
                 const R w = kernelF<D, R>(q) * CONF * m[j] / rho[j];
@@ -1996,9 +2187,10 @@ void les_interactions(const int* imove, const R* r, const R* u, const R* rho,
         const V<D, R> r_i = ld<D, R>(r, i), u_i = ld<D, R>(u, i);
         const R rho_i = rho[i], mu_i = rho_i * nu_t[i];
         R acc[3] = { 0, 0, 0 };
-        walk<D>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, [&](usize j) {
+        walk_r<D, R>(ll.icell[i], N, ll.nc, ll.icell, ll.ihoc, r,
+                        [&](usize j, const V<D, R>& r_j) {
             if (j == i || imove[j] != 1) return;
-            const V<D, R> r_ij = sub<D, R>(ld<D, R>(r, j), r_i);
+            const V<D, R> r_ij = sub<D, R>(r_j, r_i);
             const R q = length<D, R>(r_ij) / H;
             if (q >= R(2)) return;
             const R rho_j = rho[j];

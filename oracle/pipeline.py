@@ -65,7 +65,8 @@ class OracleSim:
                  freeslip_iset=None, bie=False, dr_factor=0.5,
                  min_bound_dist=0.0, delta_corr=None, energy=False,
                  midpoint=None, portals=None, precision=False,
-                 excitation=None, les=None, elastic_bounce=None):
+                 excitation=None, les=None, elastic_bounce=None,
+                 tait_gamma=None, periodic=None):
         self.dims = case["dims"]
         self.vs = 2 if self.dims == 2 else 4
         self.dtype = np.dtype(dtype)
@@ -100,6 +101,23 @@ class OracleSim:
             eb.update(elastic_bounce)
             elastic_bounce = eb
         self.elastic_bounce = elastic_bounce
+        # basic/tait.xml: the Tait EOS with this gamma (and its inverse for
+        # the boundary-integral wall density)
+        self.tait_gamma = tait_gamma
+        # Presets/cfd/periodic.xml: {"min": box origin, "L": lengths (0 for
+        # an axis that is not periodic)}; walls are not modelled with it
+        self.periodic = None
+        if periodic is not None:
+            if portals or bie or midpoint is not None:
+                raise ValueError("the periodic box is modelled with improved "
+                                 "Euler and no walls or portals")
+            vs = 2 if case["dims"] == 2 else 4
+            pm = np.zeros(vs)
+            pl = np.zeros(vs)
+            pm[:case["dims"]] = periodic["min"][:case["dims"]]
+            pl[:case["dims"]] = periodic["L"][:case["dims"]]
+            # kept in the working precision (the kernels take pointers)
+            self.periodic = (pm.astype(self.dtype), pl.astype(self.dtype))
         if precision and (midpoint is not None or dtype != np.float32):
             raise ValueError("the precision mode is a float32 improved-Euler "
                              "model")
@@ -358,14 +376,32 @@ class OracleSim:
         f, L, N = self.f, lib(), self.N
         self._k("minmax_vec")(_p(f[r_name]), ctypes.c_uint(N),
                               _p(self.r_min), _p(self.r_max))
-        self._k("n_cells")(_p(self.r_min), _p(self.r_max),
-                           self._r(self.cell_length), _p(self.n_cells))
-        nw = int(self.n_cells[3])
-        if self.ihoc.size < nw:
-            self.ihoc = np.zeros(nw, np.uint32)
-        self._k("icell")(_p(f["icell"]), _p(f[r_name]), ctypes.c_uint(N),
-                         _p(self.r_min), self._r(self.support),
-                         self._r(self.h), _p(self.n_cells))
+        if self.periodic is not None:
+            pm, pl = self.periodic
+            rmin = self.r_min.copy()
+            ok = self._k("n_cells_per")(_p(self.r_min), _p(self.r_max),
+                                        _p(pm), _p(pl),
+                                        self._r(self.cell_length),
+                                        _p(self.n_cells), _p(rmin))
+            if not ok:
+                raise ValueError("periodic box under 3 cells of support * h")
+            self.r_min[...] = rmin
+            nw = int(self.n_cells[3])
+            if self.ihoc.size < nw:
+                self.ihoc = np.zeros(nw, np.uint32)
+            self._k("icell_per")(_p(f["icell"]), _p(f[r_name]),
+                                 ctypes.c_uint(N), _p(self.r_min),
+                                 self._r(self.support), self._r(self.h),
+                                 _p(self.n_cells), _p(pm), _p(pl))
+        else:
+            self._k("n_cells")(_p(self.r_min), _p(self.r_max),
+                               self._r(self.cell_length), _p(self.n_cells))
+            nw = int(self.n_cells[3])
+            if self.ihoc.size < nw:
+                self.ihoc = np.zeros(nw, np.uint32)
+            self._k("icell")(_p(f["icell"]), _p(f[r_name]), ctypes.c_uint(N),
+                             _p(self.r_min), self._r(self.support),
+                             self._r(self.h), _p(self.n_cells))
         bits = L.orc_radix_key_bits(ctypes.c_ulonglong(nw))
         L.orc_stable_sort(_p(f["icell"]), _p(f["id_unsorted"]),
                           _p(f["id_sorted"]), ctypes.c_uint(N),
@@ -511,10 +547,16 @@ class OracleSim:
             k("bi_interp")(_p(f["imove"]), _p(f["r"]), _p(f["m"]),
                            _p(f["rho"]), _p(f["grad_p"]), _p(f["p"]), n, *ll,
                            H, CONW)
-            k("bi_interp_shepard")(_p(f["iset"]), _p(f["imove"]),
-                                   _p(f["shepard"]), _p(f["rho"]), _p(f["p"]),
-                                   _p(self.sets["refd"]), n, R(self.cs),
-                                   R(self.p0))
+            if self.tait_gamma is None:
+                k("bi_interp_shepard")(_p(f["iset"]), _p(f["imove"]),
+                                       _p(f["shepard"]), _p(f["rho"]),
+                                       _p(f["p"]), _p(self.sets["refd"]), n,
+                                       R(self.cs), R(self.p0))
+            else:
+                k("bi_interp_shepard_tait")(
+                    _p(f["iset"]), _p(f["imove"]), _p(f["shepard"]),
+                    _p(f["rho"]), _p(f["p"]), _p(self.sets["refd"]), n,
+                    R(self.cs), R(self.p0), R(self.tait_gamma))
             k("bi_interactions")(_p(f["imove"]), _p(f["r"]), _p(f["normal"]),
                                  _p(f["u"]), _p(f["rho"]), _p(f["m"]),
                                  _p(f["p"]), _p(f["grad_p"]), _p(f["div_u"]),
@@ -570,8 +612,7 @@ class OracleSim:
                         _p(self.domain_min), _p(self.domain_max))
         self.link_list("r_in")
         self.reorder()
-        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
-                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        self._eos()
         k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
         ll = self._ll()
         if o["delta"]:
@@ -593,9 +634,7 @@ class OracleSim:
             if mp["r"]:
                 k("midpoint_r")(_p(f["imove"]), _p(f["r_in"]), _p(f["r"]),
                                 _p(f["u"]), n, R(self.dt))
-            k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]),
-                     _p(f["p"]), _p(self.sets["refd"]), n, R(self.cs),
-                     R(self.p0))
+            self._eos()
             self._forces(ll, reinit_lap_p=False)
             if mp["residual_max"] is not None:
                 res = np.zeros(N, self.dtype)
@@ -652,9 +691,34 @@ class OracleSim:
         self.t = self.dtype.type(float(self.t) + float(self.dt))
         self.iter += 1
 
+    def _eos(self):
+        """basic/EOS.cl (or the Tait law of basic/tait.xml) on the sorted
+        fields."""
+        f, n, k, R = self.f, ctypes.c_uint(self.N), self._k, self._r
+        args = (_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
+                _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        if self.tait_gamma is None:
+            k("eos")(*args)
+        else:
+            k("tait_eos")(*args, R(self.tait_gamma))
+
     def step(self):
         if self.opts["midpoint"] is not None:
             return self._step_midpoint()
+        if self.periodic is None:
+            return self._step()
+        L = self.periodic[1]
+        lib().orc_set_periodic(ctypes.c_double(float(L[0])),
+                               ctypes.c_double(float(L[1])),
+                               ctypes.c_double(float(L[2]) if L.size > 2
+                                               else 0.0))
+        try:
+            return self._step()
+        finally:
+            lib().orc_set_periodic(ctypes.c_double(0.0), ctypes.c_double(0.0),
+                                   ctypes.c_double(0.0))
+
+    def _step(self):
         f, N, o = self.f, self.N, self.opts
         k, R, ll = self._k, self._r, None
         n = ctypes.c_uint(N)
@@ -674,10 +738,13 @@ class OracleSim:
             k("domain")(_p(f["imove"]), _p(f["r_in"]), _p(f["u_in"]),
                         _p(f["dudt_in"]), _p(f["m"]), n,
                         _p(self.domain_min), _p(self.domain_max))
+        if self.periodic is not None:
+            # "cfd periodic wrap", before the link-list
+            k("periodic_wrap")(_p(f["r_in"]), n, _p(self.periodic[0]),
+                               _p(self.periodic[1]))
         self.link_list("r_in")
         self.reorder()
-        k("eos")(_p(f["iset"]), _p(f["imove"]), _p(f["rho"]), _p(f["p"]),
-                 _p(self.sets["refd"]), n, R(self.cs), R(self.p0))
+        self._eos()
         k("binormal")(_p(f["normal"]), _p(f["tangent"]), _p(f["binormal"]), n)
         if o["gp"]:
             # "Backup associations" + "Sort associations" (GP/Sort.cl)

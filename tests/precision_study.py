@@ -1,7 +1,8 @@
 """Precision study: the 2D dam break (tank 1.61 x 0.6 m, water column
-0.6 x 0.3 m, dr = 3 mm, 20,000 fluid particles, BI walls, delta-SPH,
-improved Euler with the variable time step) run three ways from the same
-float32 start:
+0.6 x 0.3 m, dr = 3 mm, 20,000 fluid particles) or the 3D one (tank
+1.6 x 0.6 x 0.6 m, column 0.6 x 0.6 x 0.3 m, --dims 3 --dr DR), BI walls,
+delta-SPH, improved Euler with the variable time step, run three ways from
+the same float32 start:
 
   plain      the GPU pipeline in float32;
   precision  the GPU pipeline with Presets/precision.xml (compensated walk
@@ -13,7 +14,8 @@ Every `every` steps it records, per run: the time, the surge front (largest
 x of the fluid), the fluid mass, the mechanical energy sum m (|u|^2/2 + g y)
 and, against fp64, the RMS and largest position and velocity differences.
 
-    python tests/precision_study.py [--steps 2000] [--every 100] [--out F]
+    python tests/precision_study.py [--dims 2|3] [--dr DR] [--steps 2000]
+                                    [--every 100] [--out F]
 
 prints one JSON line (and writes it to --out).  tests/test_precision_gpu.py
 runs a short version and checks that the precision run stays closer to
@@ -55,9 +57,9 @@ def _state(r, u, m, imove, ids, n, g):
                 e_kin=float(ek))
 
 
-def run_study(steps=2000, every=100, dr=0.003, log=None):
+def run_study(steps=2000, every=100, dr=0.003, log=None, dims=2):
     from aquagpusph_amd.sim import Simulation
-    case = cases.dam_break(2, dr)
+    case = cases.dam_break(dims, dr)
     # every run starts from the same float32 values
     for k in ("r", "normal", "tangent", "u", "dudt", "rho", "drhodt", "m"):
         case[k] = np.asarray(case[k]).astype(np.float32).astype(np.float64)
@@ -66,10 +68,10 @@ def run_study(steps=2000, every=100, dr=0.003, log=None):
     tmp = tempfile.mkdtemp(prefix="aq_prec_")
     sims = {
         "plain": Simulation(casefiles.write_case(
-            case, os.path.join(tmp, "plain")), 2),
+            case, os.path.join(tmp, "plain")), dims),
         "precision": Simulation(casefiles.write_case(
             case, os.path.join(tmp, "prec"),
-            presets=casefiles.PBI_PRESETS + (PRECISION,)), 2),
+            presets=casefiles.PBI_PRESETS + (PRECISION,)), dims),
     }
     ref = op.OracleSim(case, dtype=np.float64)
     rows = []
@@ -124,8 +126,10 @@ def run_study(steps=2000, every=100, dr=0.003, log=None):
     e0 = rows[0]["fp64"]["e_mech"]
     return {
         "study": "precision",
-        "workload": "2D dam break, tank 1.61x0.6 m, column 0.6x0.3 m, "
-                    "dr=%g m, BI walls, delta-SPH, improved Euler, "
+        "workload": ("2D dam break, tank 1.61x0.6 m, column 0.6x0.3 m, "
+                     if dims == 2 else
+                     "3D dam break, tank 1.6x0.6x0.6 m, column 0.6x0.6x0.3 "
+                     "m, ") + "dr=%g m, BI walls, delta-SPH, improved Euler, "
                     "variable dt" % dr,
         "n_particles": n, "n_fluid": int(case["n_fluid"]), "steps": steps,
         "reference": "oracle float64 instantiation on the host (no float64 "
@@ -140,8 +144,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--dims", type=int, default=2)
+    ap.add_argument("--dr", type=float, default=0.003)
     a = ap.parse_args()
-    res = run_study(a.steps, a.every,
+    res = run_study(a.steps, a.every, dr=a.dr, dims=a.dims,
                     log=lambda s: print(s, file=sys.stderr, flush=True))
     line = json.dumps(res)
     if a.out:

@@ -548,6 +548,34 @@ def test_elastic_bounce_matches_oracle(cuda, tmp_path, dims, nl, steps, mbd):
     sim.close()
 
 
+TAIT_PRESETS = casefiles.PBI_PRESETS + ("resources/Presets/basic/tait.xml",)
+
+
+@pytest.mark.parametrize("dims,nl,steps", [(2, 25, 2), (3, 8, 2),
+                                           (2, 25, 12), (3, 8, 8)])
+def test_tait_eos_matches_oracle(cuda, tmp_path, dims, nl, steps):
+    """basic/tait.xml (gamma = 7) on the sheared P-BI dam break (initial
+    density from the hydrostatic pressure through the same law): the fluid
+    pressure by the same binary powering on both sides, the wall density
+    by the Tait inverse (powf against the host pow: a few ulp); every field
+    within 1e-5 of its largest magnitude after two steps and 1e-4 after
+    8-12, and the pressure differs from the linear EOS run."""
+    case = _sheared(cases.dam_break(dims, 0.3 / nl, tait_gamma=7.0))
+    xml = casefiles.write_case(case, str(tmp_path), presets=TAIT_PRESETS)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, tait_gamma=7.0)
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4)
+    print(dims, steps, errs)
+    lin = op.OracleSim(case)
+    for _ in range(steps):
+        lin.step()
+    assert not np.array_equal(lin.f["p"], o.f["p"])
+    sim.close()
+
+
 @pytest.mark.parametrize("dims,mode,steps", [(2, "smag", 2), (3, "smag", 2),
                                            (2, "mlp", 2), (3, "mlp", 2),
                                            (2, "mlp", 10)])
@@ -587,6 +615,96 @@ def test_les_presets_match_oracle(cuda, tmp_path, dims, mode, steps):
     assert errs["nu_t"] <= (2e-3 if mode == "mlp" else tol), errs
     assert errs["grad_u"] <= tol, errs
     assert np.abs(o.f["nu_t"]).max() > 0
+    sim.close()
+
+
+@pytest.mark.parametrize("dims,n,steps", [(2, 32, 2), (3, 16, 2),
+                                          (2, 32, 20), (3, 16, 10)])
+def test_periodic_box_matches_oracle(cuda, tmp_path, dims, n, steps):
+    """Presets/cfd/periodic.xml: decaying turbulence in a box periodic
+    along every axis (no walls), the positions wrapped before the
+    link-list, the fused cell walk over wrapped cells at shifted positions;
+    every field bit-identical to the oracle's periodic walk (pinned against
+    explicit periodic images in test_oracle_periodic_cpu.py) after 2 and
+    10-20 steps, with particles crossing the box faces."""
+    case = cases.periodic_box(dims, n)
+    # a uniform drift carries particles through the faces: the lattice is
+    # moved to within 1e-4 of the faces the drift crosses
+    drift = np.array([3.0, -2.0, 1.0][:dims])
+    off = np.sign(drift) * (0.5 / n - 1e-4)
+    case["r"][:, :dims] = (case["r"][:, :dims] + off) % 1.0
+    case["u"][:, :dims] += drift
+    xml = casefiles.write_case(case, str(tmp_path),
+                               presets=casefiles.PPER_PRESETS)
+    sim = Simulation(xml, dims)
+    o = op.OracleSim(case, bi=False, domain=False,
+                     periodic=case["params"]["periodic"])
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    errs = _compare(sim, o, 1e-5 if steps <= 2 else 1e-4)
+    print(dims, n, steps, errs)
+    assert max(errs.values()) == 0.0, errs
+    x = sim.get("r")[:, :dims]
+    assert np.all((x >= 0.0) & (x <= 1.0))
+    ids = o.f["id"].astype(np.int64)
+    moved = np.abs(o.f["r"][:, :dims] - case["r"][ids, :dims]) > 0.5
+    assert moved.any(), float(o.t)
+    sim.close()
+
+
+@pytest.mark.parametrize("dims,mode,steps", [(3, "smag", 2), (3, "mlp", 2),
+                                             (3, "mlp", 8), (2, "smag", 6)])
+def test_periodic_les_matches_oracle(cuda, tmp_path, dims, mode, steps):
+    """The LES closure (Presets/cfd/LES.xml with a given network, or
+    LES-smagorinsky.xml) in the periodic box: the cell-parallel LES
+    gradient and viscous term over the wrapped cells, against the oracle
+    (the velocity gradient to 1e-5, the network's eddy viscosity to 2e-3
+    as in the walled case, every field to 1e-5 after two steps and 1e-4
+    after 6-8)."""
+    from oracle import mlp as omlp
+    case = cases.periodic_box(dims, 16 if dims == 3 else 32)
+    pres = casefiles.PPER_PRESETS + (
+        "resources/Presets/cfd/LES%s.xml"
+        % ("" if mode == "mlp" else "-smagorinsky"),)
+    xml = casefiles.write_case(case, str(tmp_path), presets=pres,
+                               extra_xml=MORRIS)
+    sim = Simulation(xml, dims)
+    if mode == "mlp":
+        P = omlp.init(5)
+        P[omlp.OFF_W3:omlp.OFF_B3] *= 0.1
+        P[omlp.OFF_B3] = 0.05
+        sim.set("les_params", P)
+        les = {"params": P, "hdr": case["params"]["hfac"]}
+    else:
+        les = {"Cs": 0.15}
+    o = op.OracleSim(case, bi=False, domain=False, lap_form=2, les=les,
+                     periodic=case["params"]["periodic"])
+    for _ in range(steps):
+        o.step()
+    sim.step(steps)
+    tol = 1e-5 if steps <= 2 else 1e-4
+    errs = _compare(sim, o, tol)
+    for k in ("grad_u", "nu_t"):
+        errs[k] = relerr(sim.get(k), o.f[k])
+    print(dims, mode, steps, errs)
+    assert errs["nu_t"] <= (2e-3 if mode == "mlp" else tol), errs
+    assert errs["grad_u"] <= tol, errs
+    if mode == "smag":
+        assert max(errs.values()) == 0.0, errs
+    sim.close()
+
+
+def test_periodic_box_too_small_is_refused(cuda, tmp_path):
+    """A periodic axis shorter than 3 cells of support * h cannot be walked
+    (wrapped neighbour cells would repeat): the link-list flags it and the
+    step reports it."""
+    case = cases.periodic_box(3, 8)  # 2 cells of 2h per axis
+    xml = casefiles.write_case(case, str(tmp_path),
+                               presets=casefiles.PPER_PRESETS)
+    sim = Simulation(xml, 3)
+    with pytest.raises(_lib.AquaError, match="periodic"):
+        sim.step(1)
     sim.close()
 
 
