@@ -239,32 +239,45 @@ def decaying_box(n, seed=0, k0=4.0, urms=1.0, nu=1e-4, cs=20.0, refd=1.0,
     return case
 
 
-def solenoidal_field(dims, n, seed=0, k0=4.0, urms=1.0):
+def solenoidal_field(dims, n, seed=0, k0=4.0, urms=1.0, kmax=None):
     """A seeded random divergence-free velocity on the n^dims periodic
     lattice of a unit box, with the energy spectrum of the decaying
     turbulence benchmark, E(k) ~ k^4 exp(-2 (k/k0)^2) (k in units of
-    2 pi), scaled to the RMS speed urms.  Built in Fourier space: random
-    complex amplitudes of modulus sqrt(E(k) / (shell measure)), projected
-    onto the plane normal to k (the spectral form of div u = 0), and the
-    real part of the inverse FFT.  Returns an (n^dims, dims) array in the
-    lattice order of np.meshgrid(..., indexing="ij")."""
+    2 pi), scaled to the RMS speed urms.  Built in Fourier space over the
+    wavenumbers |k_i| <= kmax (default ceil(4 k0), where E has fallen by
+    e^-32 from its peak): random complex amplitudes of modulus
+    sqrt(E(k) / (shell measure)), drawn in a fixed order, projected onto
+    the plane normal to k (the spectral form of div u = 0); the field is
+    the real part of their Fourier series.  The draws do not depend on n,
+    so lattices with n > 2 kmax sample one and the same field (a coarse
+    and a fine run start from the same flow); smaller lattices keep the
+    modes they resolve (|k_i| < n / 2).  Returns an (n^dims, dims) array in
+    the lattice order of np.meshgrid(..., indexing="ij")."""
     rng = np.random.default_rng(seed)
-    k1 = np.fft.fftfreq(n, d=1.0 / n)  # integer wavenumbers
-    K = np.stack(np.meshgrid(*([k1] * dims), indexing="ij"), axis=0)
-    kk = np.sqrt(np.sum(K ** 2, axis=0))
+    K = int(math.ceil(4.0 * k0)) if kmax is None else int(kmax)
+    ks = np.arange(-K, K + 1)
+    Kv = np.stack(np.meshgrid(*([ks] * dims), indexing="ij"), axis=0)
+    kk = np.sqrt(np.sum(Kv ** 2, axis=0))
     shell = 2.0 * math.pi * kk if dims == 2 else 4.0 * math.pi * kk ** 2
     E = kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2)
     amp = np.zeros_like(kk)
-    # no Nyquist modes (their derivative is not represented on the lattice)
-    nz = (kk > 0) & np.all(np.abs(K) < n / 2, axis=0)
+    nz = kk > 0
     amp[nz] = np.sqrt(E[nz] / shell[nz])
     U = (rng.normal(size=(dims,) + kk.shape) +
          1j * rng.normal(size=(dims,) + kk.shape)) * amp
     kd = np.where(nz, kk, 1.0)
-    proj = np.sum(U * K, axis=0) / kd ** 2
-    U = U - K * proj
-    u = np.stack([np.real(np.fft.ifftn(U[a])) for a in range(dims)], -1)
-    u = u.reshape(-1, dims)
+    U = U - Kv * (np.sum(U * Kv, axis=0) / kd ** 2)
+    # at the cell-centred lattice points x = (i + 1/2) / n: each mode's
+    # phase at the half-spacing offset
+    U = U * np.exp(1j * math.pi * np.sum(Kv, axis=0) / n)
+    # onto the lattice's FFT grid: the resolved modes (no Nyquist mode)
+    keep = np.all(np.abs(Kv) < n / 2, axis=0)
+    G = np.zeros((dims,) + (n,) * dims, complex)
+    idx = tuple(np.where(keep, Kv[a] % n, 0) for a in range(dims))
+    for a in range(dims):
+        np.add.at(G[a], tuple(ix[keep] for ix in idx), U[a][keep])
+    u = np.stack([np.real(np.fft.ifftn(G[a])) * n ** dims
+                  for a in range(dims)], -1).reshape(-1, dims)
     u -= u.mean(axis=0)
     return u * (urms / math.sqrt(np.mean(np.sum(u ** 2, axis=1))))
 

