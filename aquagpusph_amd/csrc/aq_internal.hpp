@@ -48,6 +48,17 @@ class Error : public std::runtime_error {
 #define AQ_CELL_END 0x0FFFFFFFu   // cell_end bits holding the end index
 #define AQ_CELL_FLUID 0x40000000u // the cell holds fluid (imove == 1)
 #define AQ_CELL_ELEM 0x80000000u  // the cell holds wall elements (imove == -3)
+#define AQ_CELL_OTHER 0x20000000u // the cell holds other particles (sensors,
+                                  // ghosts, mirrors: imove not 1 nor -3)
+// Word of the n_heads buffer holding the number of cells in the owner-cell
+// list (cells flagged AQ_CELL_ELEM or AQ_CELL_OTHER; the list holds their
+// first particle), filled with the kind flags.
+#define AQ_CL_NOCELLS 512u
+#define AQ_CL_WORDS 1024u  // size of the n_heads buffer in words
+// Hand-out counters of the walks launched on the side stream beside the
+// fused walk (8 counters, 32 words apart, from this word on); the walks of
+// the main stream use the counters from word 32.
+#define AQ_CL_SIDE 544u
 struct CellArgs {
     const uint32_t* icell;
     const uint32_t* ihoc;
@@ -58,6 +69,13 @@ struct CellArgs {
     const uint32_t* status;
     uint32_t N;
     uint32_t flagged;  // cell_end carries the kind flags
+    // owner-cell list (first particle of every cell holding wall elements
+    // or other non-fluid particles, counted in n_heads[AQ_CL_NOCELLS]);
+    // nullptr: the element kernels visit every non-empty cell
+    const uint32_t* ocells = nullptr;
+    // word of the n_heads buffer where the walk's hand-out counters start
+    // (0: word 32, the main stream's)
+    uint32_t next_off = 0;
 };
 
 // Optional aids of the per-particle walk kernels, supplied by the runtime:

@@ -260,34 +260,62 @@ __global__ void cell_list_k(const uint32_t* __restrict__ head,
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     if (head[i]) heads[pos[i]] = i;
-    if (i == N - 1) *n_heads = pos[i] + head[i];
+    if (i == N - 1) {
+        *n_heads = pos[i] + head[i];
+        n_heads[AQ_CL_NOCELLS] = 0;  // the owner-cell list is refilled next
+    }
 }
 
 // Particle kinds present in each non-empty cell, in the top bits of
-// cell_end (AQ_CELL_FLUID: imove == 1, AQ_CELL_ELEM: imove == -3; cell ends
-// stay below 2^28).  imove is read through perm (id_unsorted): the cell list
-// is built right after the sort, before the fields are reordered.
+// cell_end (AQ_CELL_FLUID: imove == 1, AQ_CELL_ELEM: imove == -3,
+// AQ_CELL_OTHER: any other imove; cell ends stay below 2^28).  imove is
+// read through perm (id_unsorted): the cell list is built right after the
+// sort, before the fields are reordered.  The cells with elements or other
+// particles are appended to the owner-cell list as (first particle, cell,
+// cell_end with the flags) triplets (one atomic per wavefront, so the list
+// is ordered within each wavefront's 64 cells and the blocks of 64 in no
+// set order).
 __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
-                             const uint32_t* __restrict__ n_heads,
+                             uint32_t* __restrict__ n_heads,
                              const uint32_t* __restrict__ icell,
                              const uint32_t* __restrict__ perm,
                              const int* __restrict__ imove,
                              uint32_t* __restrict__ cell_end,
-                             const uint32_t* status)
+                             const uint32_t* status,
+                             uint32_t* __restrict__ ocells)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *n_heads) return;
-    if (status && (*status & 1u)) return;
-    const uint32_t start = heads[k];
-    const uint32_t c = icell[start];
-    const uint32_t end = cell_end[c];
-    uint32_t f = 0;
-    for (uint32_t i = start; i < end; i++) {
-        const int im = imove[perm[i]];
-        f |= (im == 1) ? AQ_CELL_FLUID : 0u;
-        f |= (im == -3) ? AQ_CELL_ELEM : 0u;
+    const uint32_t nh = *n_heads;
+    if (status && (*status & 1u)) return;  // uniform
+    uint32_t f = 0, start = 0, c = 0, end = 0;
+    if (k < nh) {
+        start = heads[k];
+        c = icell[start];
+        end = cell_end[c];
+        for (uint32_t i = start; i < end; i++) {
+            const int im = imove[perm[i]];
+            f |= (im == 1) ? AQ_CELL_FLUID
+                           : ((im == -3) ? AQ_CELL_ELEM : AQ_CELL_OTHER);
+        }
+        cell_end[c] = end | f;
     }
-    cell_end[c] = end | f;
+    if (!ocells) return;
+    const bool own = (f & (AQ_CELL_ELEM | AQ_CELL_OTHER)) != 0u;
+    const uint64_t bm = __ballot(own);
+    if (bm == 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == __builtin_ctzll(bm))
+        base = atomicAdd(n_heads + AQ_CL_NOCELLS, (uint32_t)__popcll(bm));
+    base = __shfl(base, __builtin_ctzll(bm));
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+    if (own) {
+        const uint32_t o = 3u * (base + rank);
+        ocells[o] = start;
+        ocells[o + 1] = c;
+        ocells[o + 2] = end | f;
+    }
 }
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
@@ -302,11 +330,12 @@ size_t cell_list_workspace(uint32_t N)
 
 // Non-empty cells of a sorted icell: heads[k] = first particle of the k-th
 // non-empty cell, *n_heads their number, cell_end[c] its end and, when
-// imove and perm are given, the kinds of particle it holds.
+// imove and perm are given, the kinds of particle it holds and (with
+// ocells, n_heads of AQ_CL_WORDS words) the owner-cell list.
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
-               hipStream_t s)
+               hipStream_t s, uint32_t* ocells)
 {
     const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
     uint32_t* head = (uint32_t*)ws;
@@ -319,7 +348,7 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
     hipLaunchKernelGGL(cell_list_k, g, b, 0, s, head, pos, N, heads, n_heads);
     if (perm && imove)
         hipLaunchKernelGGL(cell_kinds_k, g, b, 0, s, heads, n_heads, icell,
-                           perm, imove, cell_end, status);
+                           perm, imove, cell_end, status, ocells);
 }
 
 // ---- host drivers ---------------------------------------------------------

@@ -201,6 +201,12 @@ AQ_API int aq_sim_profile_tools(aq_sim* s, int nsteps, double* ms)
             if (!F.ev1) AQ_HIP(hipEventCreate(&F.ev1));
         }
         if (S.vars.host_dirty()) S.vars.push(S.stream);
+        // every tool timed on its own: no element chain beside the walk
+        struct SideOff {
+            bool& b;
+            SideOff(bool& x) : b(x) { b = true; }
+            ~SideOff() { b = false; }
+        } side_off_(S.side_off);
         for (int k = 0; k < nsteps; k++) {
             F.timing = F.active;
             size_t used = 0;

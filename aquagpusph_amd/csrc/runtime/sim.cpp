@@ -1,4 +1,5 @@
 // The runtime: variables + definitions + tool pipeline (CalcServer.cpp).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -51,9 +52,13 @@ Sim::~Sim()
     }
     tools.clear();
     for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
-                     (void*)clist.heads, (void*)clist.n_heads, clist.ws,
+                     (void*)clist.heads, (void*)clist.ocells,
+                     (void*)clist.n_heads, clist.ws,
                      (void*)sel_list, (void*)sel_count })
         if (b) (void)hipFree(b);
+    if (fusion.ev_fork) (void)hipEventDestroy(fusion.ev_fork);
+    if (fusion.ev_join) (void)hipEventDestroy(fusion.ev_join);
+    if (fusion.side_stream) (void)hipStreamDestroy(fusion.side_stream);
     if (fusion.ev0) (void)hipEventDestroy(fusion.ev0);
     if (fusion.ev1) (void)hipEventDestroy(fusion.ev1);
     if (d_status) (void)hipFree(d_status);
@@ -285,9 +290,13 @@ void Sim::setup()
     clist.n_cells = vars.get("n_cells");
     // boundary-integral element kernels: cell-parallel unless
     // AQ_BI_WALK=thread asks for the per-particle forms (over compacted
-    // particle lists while the cell list is on)
+    // particle lists while the cell list is on); the kernels whose owners
+    // are wall elements or other non-fluid particles walk only the cells
+    // holding such particles (the owner-cell list) unless AQ_BI_WALK=cells
+    // asks them to visit every cell
     const char* bw = getenv("AQ_BI_WALK");
     bi_cells = clist.enabled && !(bw && strcmp(bw, "thread") == 0);
+    bi_group = bi_cells && !(bw && strcmp(bw, "cells") == 0);
     if (clist.enabled && N && !sel_list) {
         AQ_HIP(hipMalloc(&sel_list, (size_t)N * sizeof(uint32_t)));
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
@@ -351,6 +360,12 @@ void Sim::check_periodic()
 
 // Kernel tools that neither move particles nor change imove, r, u, rho, m, p
 // of fluid particles: they may sit between fused tools.
+static bool getenv_true(const char* name)
+{
+    const char* e = getenv(name);
+    return e && *e && strcmp(e, "0") != 0;
+}
+
 static bool fusion_transparent(const Tool& t)
 {
     if (t.type() == "dummy" || t.type().compare(0, 7, "report_") == 0)
@@ -455,6 +470,52 @@ void Sim::plan_fusion()
         AQ_HIP(hipMalloc(&fusion.near_buf, N));
         AQ_HIP(hipMemset(fusion.near_buf, 1, N));
     }
+    // the boundary-integral element chain beside the walk (BI walls with
+    // the near flags): BI LapU, GradP, Interpolation and InterpolationShepard
+    // between the anchor and BI Interactions, and between the anchor and BI
+    // Interactions only the fused tools, dummies and the sensors (which
+    // read fluid fields and write sensor fields)
+    if (fusion.shep == 2 && fusion.near && !getenv_true("AQ_NO_OVERLAP")) {
+        static const char* keys[] = {
+            "cfd/Boundary/BI/LapU.cl:freeslip",
+            "cfd/Boundary/BI/GradP.cl:freeslip",
+            "cfd/Boundary/BI/Interpolation.cl:entry",
+            "cfd/Boundary/BI/InterpolationShepard.cl:entry" };
+        std::vector<int> at;
+        bool ok = true;
+        for (const char* k : keys) {
+            const int i = find(k);
+            ok = ok && i > anchor && i < iB && !tools[i]->once() &&
+                 (at.empty() || i > at.back());
+            at.push_back(i);
+        }
+        for (int k = anchor + 1; ok && k < iB; k++) {
+            const Tool& t = *tools[k];
+            if (std::find(at.begin(), at.end(), k) != at.end()) continue;
+            if (k == iI || (fusion.lapp && k == iL)) continue;
+            if ((t.type() == "dummy" || t.type().compare(0, 7, "report_") == 0)
+                && !t.once())
+                continue;
+            if (t.type() == "kernel" && !t.once() &&
+                (t.key() == "cfd/Sensors.cl:entry" ||
+                 t.key() == "cfd/SensorsRenormalization.cl:entry"))
+                continue;
+            ok = false;
+        }
+        if (ok) {
+            for (int i : at) {
+                fusion.side.push_back(tools[i].get());
+                fusion.side_set.insert(tools[i].get());
+            }
+            fusion.join = tools[iB].get();
+            AQ_HIP(hipStreamCreateWithFlags(&fusion.side_stream,
+                                            hipStreamNonBlocking));
+            AQ_HIP(hipEventCreateWithFlags(&fusion.ev_fork,
+                                           hipEventDisableTiming));
+            AQ_HIP(hipEventCreateWithFlags(&fusion.ev_join,
+                                           hipEventDisableTiming));
+        }
+    }
     if (const char* wb = getenv("AQ_WALK_BLOCKS"))
         fusion.walk_blocks = (unsigned)std::max(0, atoi(wb));
     if (const char* ws = getenv("AQ_WALK_SEGW"))
@@ -498,7 +559,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              aq_defs d, float smax, unsigned nblocks,
                              int segw, int flagged, int math,
                              unsigned ranges, int fast_div, hipStream_t s,
-                             const float* per);
+                             const float* per, unsigned reserve_per_cu);
+unsigned side_wgs_per_cu();
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
                     const uint32_t*, const uint32_t*, const uint32_t*,
@@ -522,7 +584,16 @@ CellArgs Sim::cell_args()
                      clist.n_heads,
                      d_status,
                      N,
-                     clist.kinds ? 1u : 0u };
+                     clist.kinds ? 1u : 0u,
+                     (bi_group && clist.kinds) ? clist.ocells : nullptr,
+                     in_side ? AQ_CL_SIDE : 0u };
+}
+
+void Sim::join_side(hipStream_t s)
+{
+    if (!fusion.side_launched) return;
+    AQ_HIP(hipStreamWaitEvent(s, fusion.ev_join, 0));
+    fusion.side_launched = false;
 }
 
 PartWalk Sim::part_walk(const Var* ic, const Var* ih, const Var* nc)
@@ -544,8 +615,29 @@ void Sim::run_fused(hipStream_t s)
     auto P = [&](Var* v) { return v ? ptr(v) : nullptr; };
     const float* fr = (const float*)P(f.r);
     const PartWalk pw = part_walk(f.icell, f.ihoc, f.n_cells);
+    // the boundary-integral element chain on the side stream, beside the
+    // walk: the non-fluid part of the Shepard tool, then the side tools
+    if (f.shep == 2 && !f.side.empty() && !side_off && bi_cells &&
+        clist.usable(f.icell, f.ihoc)) {
+        hipStream_t ss = f.side_stream;
+        AQ_HIP(hipEventRecord(f.ev_fork, s));
+        AQ_HIP(hipStreamWaitEvent(ss, f.ev_fork, 0));
+        struct InSide {
+            bool& b;
+            InSide(bool& x) : b(x) { b = true; }
+            ~InSide() { b = false; }
+        } guard_(in_side);
+        launch_bi_shepard_compute_cells(
+            dims, (const int32_t*)P(f.imove), fr, (const float*)P(f.normal),
+            (const float*)P(f.tangent), (const float*)P(f.binormal),
+            (const float*)P(f.m), (float*)P(f.shepard), cell_args(), kdefs,
+            support_threshold(kdefs.H), 1, ss);
+        for (Tool* t : f.side) t->launch_kernel(*this, ss);
+        AQ_HIP(hipEventRecord(f.ev_join, ss));
+        f.side_launched = true;
+    }
     // non-fluid part of the Shepard tool first (sensors, elements)
-    if (f.shep == 2 && bi_cells && clist.usable(f.icell, f.ihoc))
+    else if (f.shep == 2 && bi_cells && clist.usable(f.icell, f.ihoc))
         launch_bi_shepard_compute_cells(
             dims, (const int32_t*)P(f.imove), fr, (const float*)P(f.normal),
             (const float*)P(f.tangent), (const float*)P(f.binormal),
@@ -583,7 +675,8 @@ void Sim::run_fused(hipStream_t s)
             clist.n_heads,
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
             f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges,
-            f.fast_div ? 1 : 0, s, periodic_L());
+            f.fast_div ? 1 : 0, s, periodic_L(),
+            f.side_launched ? side_wgs_per_cu() : 0u);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }
@@ -685,6 +778,7 @@ void Sim::run_tools(hipStream_t s,
         }
         pc++;
     }
+    join_side(s);  // a chain not joined by its tool (never left in flight)
     stamp_status(s);
 }
 

@@ -120,6 +120,9 @@ class Tool {
     virtual bool graph_safe() const { return true; }
     // registry key of kernel tools ("" for other tools)
     virtual std::string key() const { return ""; }
+    // enqueue the tool's own work on `s`, outside the fusion plan (the
+    // kernel tools launched on the side stream)
+    virtual void launch_kernel(Sim& S, hipStream_t s) { execute(S, s); }
     // +1 for the scope openers (while, if), -1 for end (Tool.hpp
     // scope_modifier); such tools are run by the pipeline, not execute()
     virtual int scope() const { return 0; }
@@ -251,6 +254,18 @@ struct FusionPlan {
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double walk_ms = -1.0;  // mean of the last profile
+    // Boundary-integral element chain beside the walk: the non-fluid part
+    // of BI Shepard, then the `side` tools (BI LapU, GradP, Interpolation,
+    // InterpolationShepard) run on side_stream while the fused walk runs on
+    // the main stream; the main stream waits for them before `join` (BI
+    // Interactions).  They read the fluid fields the walk only reads and
+    // write element fields the walk and the tools in between never read.
+    std::set<const Tool*> side_set;
+    std::vector<Tool*> side;
+    const Tool* join = nullptr;
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool side_launched = false;  // this step's chain is in flight
 };
 
 // ---- non-empty cell list of the cell-parallel walks ---------------------------
@@ -265,6 +280,7 @@ struct CellList {
     uint32_t* cell_end = nullptr;
     size_t cap = 0;  // cells cell_end holds
     uint32_t* heads = nullptr;
+    uint32_t* ocells = nullptr;   // owner-cell triplets (cell_kinds_k)
     uint32_t* n_heads = nullptr;  // [0] count, [32..] hand-out counters
     void* ws = nullptr;
     bool usable(const Var* ic, const Var* ih) const
@@ -340,11 +356,18 @@ class Sim {
     FusionPlan fusion;
     CellList clist;
     bool bi_cells = false;  // boundary-integral element kernels cell-parallel
+    bool bi_group = false;  // ... over the owner-cell list
+    // no side-stream chain (per-tool profiling), and the side-stream
+    // launches in progress (cell_args then hands out the side counters)
+    bool side_off = false;
+    bool in_side = false;
     // scratch of the per-particle walk kernels' compacted particle lists
     // (with the cell-parallel walks on): N indices and one counter
     uint32_t* sel_list = nullptr;
     uint32_t* sel_count = nullptr;
     void run_fused(hipStream_t s);
+    // the main stream waits for the side chain (no-op if none in flight)
+    void join_side(hipStream_t s);
     CellArgs cell_args();
     // aids of a per-particle walk kernel bound to these link-list variables
     PartWalk part_walk(const Var* icell, const Var* ihoc, const Var* n_cells);

@@ -25,7 +25,7 @@ size_t cell_list_workspace(uint32_t N);
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
-               hipStream_t s);
+               hipStream_t s, uint32_t* ocells);
 void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                 unsigned key_bits, void* workspace, hipStream_t s);
 size_t radix_sort_workspace(uint32_t n);
@@ -415,7 +415,16 @@ class KernelTool : public Tool {
                 return;
             }
             if (S.fusion.absorbed.count(this)) return;
+            if (S.fusion.side_launched) {
+                // launched beside the walk by the anchor
+                if (S.fusion.side_set.count(this)) return;
+                if (S.fusion.join == this) S.join_side(s);
+            }
         }
+        spec_->launch(S, bound_.data(), s);
+    }
+    void launch_kernel(Sim& S, hipStream_t s) override
+    {
         spec_->launch(S, bound_.data(), s);
     }
     std::string key() const override { return key_; }
@@ -552,14 +561,20 @@ class LinkListTool : public Tool {
                 c.cell_end = nullptr;
                 AQ_HIP(hipMalloc(&c.cell_end, ihoc_->length * 4));
                 c.cap = ihoc_->length;
+                // owner-cell triplets: at most one per non-empty cell
+                if (c.ocells) AQ_HIP(hipFree(c.ocells));
+                c.ocells = nullptr;
+                AQ_HIP(hipMalloc(&c.ocells,
+                                 std::min<size_t>(S.N, c.cap) * 12));
             }
             if (!c.heads) {
                 AQ_HIP(hipMalloc(&c.heads, (size_t)S.N * 4));
                 // [0] count, then the walks' hand-out counters (one
-                // 128-byte line each, from word 32)
-                AQ_HIP(hipMalloc(&c.n_heads, 2048));
+                // 128-byte line each, from word 32), the range record and
+                // the owner-cell count (word AQ_CL_NOCELLS)
+                AQ_HIP(hipMalloc(&c.n_heads, AQ_CL_WORDS * 4));
                 AQ_HIP(hipMalloc(&c.ws, cell_list_workspace(S.N)));
-                AQ_HIP(hipMemset(c.n_heads, 0, 2048));
+                AQ_HIP(hipMemset(c.n_heads, 0, AQ_CL_WORDS * 4));
             }
             Var* im = S.vars.get("imove");
             const bool kinds = im && im->is_array() && im->length >= S.N &&
@@ -567,7 +582,8 @@ class LinkListTool : public Tool {
             cell_list((const uint32_t*)icell_->dev, S.N, c.cell_end, c.heads,
                       c.n_heads, S.d_status, c.ws,
                       kinds ? (const uint32_t*)perm_->dev : nullptr,
-                      kinds ? (const int*)im->dev : nullptr, s);
+                      kinds ? (const int*)im->dev : nullptr, s,
+                      kinds ? c.ocells : nullptr);
             c.kinds = kinds;
             c.filled = true;
         }

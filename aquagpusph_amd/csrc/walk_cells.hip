@@ -213,7 +213,17 @@ __device__ __forceinline__ void wave_box(const f3& r_i, bool act, f3& bmin,
 // drops only candidates every lane would reject.
 // pair(j, kl): kl is the neighbour-table entry the candidate came from
 // (its shift in a periodic walk; 0 otherwise).
-template <int D, int SEGW, bool PER, class Pair>
+//
+// NB > 1 (batched consume, not periodic): `pair` has load(j), returning
+// the candidate's fields, and use(j, fields); each step takes up to NB
+// accepted candidates of the lane's current bit word, issues all their
+// loads, then uses them in increasing candidate order.  A lane's pair
+// terms form a chain of dependent loads (next accepted candidate, its
+// fields, the terms), so a walk with few lanes busy (wall cells) is bound
+// by the memory latency of that chain; batching takes one latency per NB
+// candidates instead of one per candidate, with the same terms in the same
+// order.
+template <int D, int SEGW, int NB = 1, bool PER, class Pair>
 __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                                           uint32_t total, const f3& r_i,
                                           bool act, const f3& bmin,
@@ -258,35 +268,51 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
         // the test sits at the bottom of the consume loop, so the pair sums
         // leave it from where they are updated (no copies at the back edge)
         for (bool more = go(); more; more = go()) {
-            if (bw != 0) {
-                const uint32_t b = __builtin_clz(bw);
-                bw ^= 0x80000000u >> b;
-                const uint32_t cw = cand[((nw - 1) % RING) * 32 + b];
-                if constexpr (PER)
-                    pair(cw & PER_J_MASK, cw >> PER_KL_SHIFT);
-                else
-                    pair(cw, 0u);
+            if constexpr (NB == 1) {
+                if (bw != 0) {
+                    const uint32_t b = __builtin_clz(bw);
+                    bw ^= 0x80000000u >> b;
+                    const uint32_t cw = cand[((nw - 1) % RING) * 32 + b];
+                    if constexpr (PER)
+                        pair(cw & PER_J_MASK, cw >> PER_KL_SHIFT);
+                    else
+                        pair(cw, 0u);
+                }
+            } else {
+                static_assert(!PER, "batched consume: not for periodic walks");
+                uint32_t jb[NB];
+                int nb = 0;
+#pragma unroll
+                for (int k = 0; k < NB; k++) {
+                    jb[k] = 0;
+                    if (bw != 0) {
+                        const uint32_t b = __builtin_clz(bw);
+                        bw ^= 0x80000000u >> b;
+                        jb[k] = cand[((nw - 1) % RING) * 32 + b];
+                        nb = k + 1;
+                    }
+                }
+                decltype(pair.load(0u)) cd[NB];
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+                    if (k < nb) cd[k] = pair.load(jb[k]);
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+                    if (k < nb) pair.use(jb[k], cd[k]);
             }
         }
         if (tested >= nsegs) break;  // and no lane has a candidate left
-        // refill the oldest slot
+        // refill the oldest slot: chunk(cb, ...) tests the candidates
+        // cb .. cb + 63 of the segment (lane: candidate cb + lane, when
+        // `have`, at j, rj, of kind im_j)
         const uint32_t seg = tested * SEG;
         const uint32_t nseg = min(SEG, total - seg);
         const int slot = tested & 1;
-        for (uint32_t cb = 0; cb < nseg; cb += 64) {
-            const uint32_t t = seg + cb + lane;
+        auto chunk = [&](uint32_t cb, bool have, uint32_t j, f3 rj,
+                         int im_j) {
             float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
             bool take = false, is_el = false;
-            uint32_t j = 0;
-            if (cb + lane < nseg) {
-                while (t >= L.cum[kl + 1]) kl++;
-                j = L.start[kl] + (t - L.cum[kl]);
-                f3 rj = ldv_o<D>(r, j);
-                if constexpr (PER) {
-                    const float4 sh = L.shift[kl];
-                    rj = per_shift<D>(rj, f3{ sh.x, sh.y, sh.z }, per);
-                }
-                const int im_j = ldi_o(imove, j);
+            if (have) {
                 is_el = (im_j == -3);
                 take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
                 if (take) {
@@ -357,6 +383,56 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
             L.bits[slot][cb >> 5][lane] = lo;
             if (nseg - cb > 32) L.bits[slot][(cb >> 5) + 1][lane] = hi;
             __builtin_amdgcn_wave_barrier();
+        };
+        if constexpr (NB > 1) {
+            // the whole segment's candidate loads issued together
+            constexpr uint32_t NCH = SEG / 64;
+            uint32_t pj[NCH];
+            f3 prj[NCH];
+            int pim[NCH];
+#pragma unroll
+            for (uint32_t ch = 0; ch < NCH; ch++) {
+                pj[ch] = 0;
+                const uint32_t t = seg + ch * 64 + lane;
+                if (ch * 64 + lane < nseg) {
+                    while (t >= L.cum[kl + 1]) kl++;
+                    pj[ch] = L.start[kl] + (t - L.cum[kl]);
+                }
+            }
+#pragma unroll
+            for (uint32_t ch = 0; ch < NCH; ch++) {
+                prj[ch] = f3{ 0.f, 0.f, 0.f };
+                pim[ch] = 0;
+                if (ch * 64 + lane < nseg) {
+                    prj[ch] = ldv_o<D>(r, pj[ch]);
+                    pim[ch] = ldi_o(imove, pj[ch]);
+                }
+            }
+#pragma unroll
+            for (uint32_t ch = 0; ch < NCH; ch++) {
+                if (ch * 64 >= nseg) break;
+                chunk(ch * 64, ch * 64 + lane < nseg, pj[ch], prj[ch],
+                      pim[ch]);
+            }
+        } else {
+            for (uint32_t cb = 0; cb < nseg; cb += 64) {
+                const uint32_t t = seg + cb + lane;
+                uint32_t j = 0;
+                f3 rj = { 0.f, 0.f, 0.f };
+                int im_j = 0;
+                const bool have = cb + lane < nseg;
+                if (have) {
+                    while (t >= L.cum[kl + 1]) kl++;
+                    j = L.start[kl] + (t - L.cum[kl]);
+                    rj = ldv_o<D>(r, j);
+                    if constexpr (PER) {
+                        const float4 sh = L.shift[kl];
+                        rj = per_shift<D>(rj, f3{ sh.x, sh.y, sh.z }, per);
+                    }
+                    im_j = ldi_o(imove, j);
+                }
+                chunk(cb, have, j, rj, im_j);
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_s_waitcnt(0);
@@ -861,26 +937,57 @@ template <bool NRQ> __device__ __forceinline__ float qof(float s, float H)
     else return sqrtf(s) / H;
 }
 
+// Candidate fields loaded ahead of their use (NB > 1): the policy's
+// Cand load(j) and use<NRQ>(i, j, cand, r_i, st), which together are its
+// pair<NRQ>(i, j, r_i, st).
+template <class Pol, bool NRQ> struct BatchPair {
+    const Pol& pol;
+    uint32_t i;
+    const f3& r_i;
+    typename Pol::State& st;
+    __device__ __forceinline__ typename Pol::Cand load(uint32_t j) const
+    {
+        return pol.load(j);
+    }
+    __device__ __forceinline__ void use(uint32_t j,
+                                        const typename Pol::Cand& c) const
+    {
+        pol.template use<NRQ>(i, j, c, r_i, st);
+    }
+};
+
 // PER: a periodic box; the policy's pair_p<NRQ>(i, j, shift, r_i, st) gets
-// the shift of j's neighbour cell.
-template <int D, int SEGW, class Pol, bool PER = false>
+// the shift of j's neighbour cell.  NB > 1: batched consume (ring_walk).
+// olist: walk the cells it lists (the owner-cell list of (first particle,
+// cell, cell_end) triplets, counted in n_heads[AQ_CL_NOCELLS]) instead of
+// every non-empty cell.
+template <int D, int SEGW, class Pol, bool PER = false, int NB = 1>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     cell_kernel_k(Pol pol, const int* __restrict__ imove,
                   const float* __restrict__ r, WalkArgs w, float smax,
-                  int nrq)
+                  int nrq, const uint32_t* __restrict__ olist)
 {
     __shared__ WaveLds<SEGW, PER> lds[WC_WAVES];
     const int lane = threadIdx.x & 63;
     WaveLds<SEGW, PER>& L = lds[threadIdx.x >> 6];
     if (w.status && (*w.status & 1u)) return;
-    const uint32_t n_heads = uni(*w.n_heads);
+    const uint32_t n_heads =
+        olist ? uni(w.n_heads[AQ_CL_NOCELLS]) : uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
     while (true) {
         const uint32_t hk = next_cell(w, n_heads, lane, range);
         if (hk >= n_heads) break;
-        const uint32_t cstart = uni(w.heads[hk]);
-        const uint32_t c_i = uni(w.icell[cstart]);
-        const uint32_t ce = uni(w.cell_end[c_i]);
+        uint32_t cstart, c_i, ce;
+        if (olist) {
+            // (first particle, cell, cell_end) triplets: one round trip
+            cstart = uni(olist[3 * hk]);
+            c_i = uni(olist[3 * hk + 1]);
+            ce = uni(olist[3 * hk + 2]);
+        } else {
+            cstart = uni(w.heads[hk]);
+            c_i = uni(w.icell[cstart]);
+            ce = uni(w.cell_end[c_i]);
+        }
         const uint32_t cend = ce & AQ_CELL_END;
         if (w.flagged && Pol::OWNERS && !(ce & Pol::OWNERS)) continue;
         bool table = false;
@@ -914,7 +1021,18 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     pol.template pair<NRQ>(i, j, r_i, st);
                 }
             };
-            if (nrq)
+            if constexpr (NB > 1) {
+                if (nrq)
+                    ring_walk<D, SEGW, NB>(
+                        L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                        Pol::KIND, false, unused,
+                        BatchPair<Pol, true>{ pol, i, r_i, st }, w.per);
+                else
+                    ring_walk<D, SEGW, NB>(
+                        L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                        Pol::KIND, false, unused,
+                        BatchPair<Pol, false>{ pol, i, r_i, st }, w.per);
+            } else if (nrq)
                 ring_walk<D, SEGW>(
                     L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
                     Pol::KIND, false, unused,
@@ -936,9 +1054,315 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative cell kernel: Q lanes per owner.
+//
+// A wall cell holds few owners (the 16 elements of a 3D wall plane in a cell
+// of side 2h at h = 2 dr), so a wavefront with one owner per lane leaves
+// most lanes idle, and the few wet wall cells (about a fifth) each form a
+// long chain of candidate tests and pair terms.  Here a pass takes up to
+// OW = 64 / Q owners of the cell and gives each Q lanes (lane q OW + o
+// serves owner o):
+//   test     every chunk of 64 candidates is staged as in ring_walk; lane
+//            (q, o) tests the q-th 64/Q of the chunk's survivors against
+//            owner o, and the Q pieces are assembled into the owner's
+//            accept bits, which all Q lanes hold;
+//   consume  in each round the Q lanes of an owner take its next Q accepted
+//            candidates, lane q evaluates the q-th one's term (Pol::term, a
+//            pure function of the owner and the candidate), the terms go
+//            through LDS and lane 0 of the owner adds them to the
+//            accumulator in candidate order (Pol::add).
+// Every owner's accumulator thus takes the terms of the one-lane walk in the
+// same order: the sums are bit-identical.
+// ---------------------------------------------------------------------------
+template <int D, int SEGW, int Q, class Term, class TermFn, class AddFn>
+__device__ __forceinline__ void coop_walk(WaveLds<SEGW, false>& L,
+                                          Term* tsl, int lane,
+                                          uint32_t total, const f3& r_i,
+                                          bool act, const f3& bmin,
+                                          const f3& bmax, float smax,
+                                          const float* __restrict__ r,
+                                          const int* __restrict__ imove,
+                                          int kind, TermFn term, AddFn add)
+{
+    static_assert(Q == 1 || Q == 2 || Q == 4, "lanes per owner");
+    constexpr int OW = 64 / Q;   // owners per pass
+    constexpr int TPL = 64 / Q;  // survivors tested per lane and chunk
+    constexpr uint32_t SEG = SEGW * 32;
+    constexpr uint32_t RING = 2 * SEGW;
+    const int q = lane / OW, o = lane % OW;
+    const uint32_t nsegs = (total + SEG - 1) / SEG;
+    const uint32_t nwords = (total + 31) >> 5;
+    const uint32_t* bits = &L.bits[0][0][0];
+    const uint32_t* cand = &L.cand[0][0];
+    int kl = 0;
+    uint32_t nw = 0, bw = 0, tested = 0;
+    auto advance = [&]() {
+        const uint32_t tw = min(tested * SEGW, nwords);
+        while (bw == 0 && nw < tw) {
+            bw = bits[(nw % RING) * 64 + lane];
+            nw++;
+        }
+    };
+    auto go = [&]() {
+        advance();
+        if (tested < nsegs &&
+            (tested < 2 ||
+             __ballot(bw != 0 && nw <= (tested - 1) * SEGW) == 0))
+            return false;
+        return __ballot(bw != 0) != 0;
+    };
+    while (true) {
+        for (bool more = go(); more; more = go()) {
+            // one round: the next Q accepted candidates of every owner (the
+            // Q lanes of an owner hold the same bits and step alike)
+            uint32_t myj = 0;
+            int n = 0;
+            bool mine = false;
+#pragma unroll
+            for (int k = 0; k < Q; k++) {
+                if (bw == 0) advance();
+                if (bw != 0) {
+                    const uint32_t b = __builtin_clz(bw);
+                    bw ^= 0x80000000u >> b;
+                    const uint32_t jj = cand[((nw - 1) % RING) * 32 + b];
+                    if (k == q) {
+                        myj = jj;
+                        mine = true;
+                    }
+                    n = k + 1;
+                }
+            }
+            Term t{};
+            if (mine) t = term(myj);
+            if constexpr (Q == 1) {
+                if (mine) add(t);
+            } else {
+                tsl[lane] = t;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+                if (q == 0) {
+#pragma unroll
+                    for (int k = 0; k < Q; k++)
+                        if (k < n) add(tsl[k * OW + o]);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (tested >= nsegs) break;  // and no lane has a candidate left
+        // refill the oldest slot
+        const uint32_t seg = tested * SEG;
+        const uint32_t nseg = min(SEG, total - seg);
+        const int slot = tested & 1;
+        for (uint32_t cb = 0; cb < nseg; cb += 64) {
+            const uint32_t t = seg + cb + lane;
+            float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
+            bool take = false;
+            uint32_t j = 0;
+            if (cb + lane < nseg) {
+                while (t >= L.cum[kl + 1]) kl++;
+                j = L.start[kl] + (t - L.cum[kl]);
+                const f3 rj = ldv_o<D>(r, j);
+                const int im_j = ldi_o(imove, j);
+                take = kind ? (im_j == -3) : (im_j == 1);
+                if (take) {
+                    const float dx =
+                        fmaxf(fmaxf(bmin.x - rj.x, rj.x - bmax.x), 0.f);
+                    const float dy =
+                        fmaxf(fmaxf(bmin.y - rj.y, rj.y - bmax.y), 0.f);
+                    float d2 = dx * dx;
+                    d2 = d2 + dy * dy;
+                    if constexpr (D == 3) {
+                        const float dz =
+                            fmaxf(fmaxf(bmin.z - rj.z, rj.z - bmax.z), 0.f);
+                        d2 = d2 + dz * dz;
+                    }
+                    take = !(d2 > smax);
+                }
+                st = make_float4(rj.x, rj.y, rj.z, 0.f);
+            }
+            const uint64_t tm = __ballot(take);
+            const uint32_t cnt = (uint32_t)__popcll(tm);
+            uint32_t lo = 0, hi = 0;
+            if (cnt != 0) {
+                // compact the survivors into stage[0 .. cnt), in order
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(tm >> 32),
+                    __builtin_amdgcn_mbcnt_lo((uint32_t)tm, 0u));
+                if (take) {
+                    L.stage[rank] = st;
+                    L.cand[slot][cb + rank] = j;
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+                // lane (q, o): survivors q TPL .. q TPL + TPL - 1
+                const uint32_t k0 = (uint32_t)(q * TPL);
+                const uint32_t kn =
+                    cnt > k0 ? min(cnt - k0, (uint32_t)TPL) : 0u;
+                uint64_t piece = 0;
+                for (uint32_t k = 0; k < kn; k++) {
+                    const float4 c = L.stage[k0 + k];
+                    const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
+                    const float s = dot<D>(r_ij, r_ij);
+                    piece = (piece << 1) | (!(s > smax) ? 1u : 0u);
+                }
+                if (kn == 0u || !act) piece = 0;
+                else if (kn < (uint32_t)TPL) piece <<= (uint32_t)TPL - kn;
+                // the owner's accept bits of the whole chunk, MSB first
+                uint64_t mask = 0;
+#pragma unroll
+                for (int qq = 0; qq < Q; qq++) {
+                    const uint64_t pc =
+                        (Q == 1) ? piece
+                                 : (uint64_t)(uint32_t)__shfl(
+                                       (int)(uint32_t)piece, qq * OW + o);
+                    mask |= pc << (64 - TPL * (qq + 1));
+                }
+                lo = (uint32_t)(mask >> 32);
+                hi = (uint32_t)mask;
+            }
+            L.bits[slot][cb >> 5][lane] = lo;
+            if (nseg - cb > 32) L.bits[slot][(cb >> 5) + 1][lane] = hi;
+            __builtin_amdgcn_wave_barrier();
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0);
+        tested++;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Owners of each listed cell in passes of up to 64 / Q; the policy splits
+// its pair into load(j) (the candidate's fields), term<NRQ>(i, j, cand,
+// r_i, st) (pure) and add(st, term), applied in candidate order.  olist as
+// in cell_kernel_k.
+template <int D, int SEGW, int Q, class Pol>
+__global__ void __launch_bounds__(64 * WC_WAVES)
+    cell_coop_k(Pol pol, const int* __restrict__ imove,
+                const float* __restrict__ r, WalkArgs w, float smax, int nrq,
+                const uint32_t* __restrict__ olist)
+{
+    constexpr int OW = 64 / Q;
+    using Term = typename Pol::Term;
+    __shared__ WaveLds<SEGW, false> lds[WC_WAVES];
+    __shared__ uint32_t owns[WC_WAVES][OW];
+    __shared__ Term tsl[WC_WAVES][64];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int q = lane / OW, o = lane % OW;
+    WaveLds<SEGW, false>& L = lds[wv];
+    if (w.status && (*w.status & 1u)) return;
+    const uint32_t n_heads =
+        olist ? uni(w.n_heads[AQ_CL_NOCELLS]) : uni(*w.n_heads);
+    uint32_t range = blockIdx.x % w.ranges;
+    while (true) {
+        const uint32_t hk = next_cell(w, n_heads, lane, range);
+        if (hk >= n_heads) break;
+        uint32_t cstart, c_i, ce;
+        if (olist) {
+            cstart = uni(olist[3 * hk]);
+            c_i = uni(olist[3 * hk + 1]);
+            ce = uni(olist[3 * hk + 2]);
+        } else {
+            cstart = uni(w.heads[hk]);
+            c_i = uni(w.icell[cstart]);
+            ce = uni(w.cell_end[c_i]);
+        }
+        const uint32_t cend = ce & AQ_CELL_END;
+        if (w.flagged && Pol::OWNERS && !(ce & Pol::OWNERS)) continue;
+        bool table = false;
+        uint32_t total = 0;
+        uint32_t pos = cstart;  // scan cursor (uniform)
+        while (pos < cend) {
+            // collect up to OW owners
+            uint32_t nf = 0;
+            while (nf < (uint32_t)OW && pos < cend) {
+                const uint32_t ii = pos + (uint32_t)lane;
+                const bool own = ii < cend && pol.owner(ii, imove[ii]);
+                const uint64_t bm = __ballot(own);
+                const uint32_t c = (uint32_t)__popcll(bm);
+                const uint32_t room = (uint32_t)OW - nf;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(bm >> 32),
+                    __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+                if (own && rank < room) owns[wv][nf + rank] = ii;
+                if (c <= room) {
+                    nf += c;
+                    pos += 64;
+                } else {
+                    // resume at the first owner not taken
+                    uint64_t m = bm;
+                    for (uint32_t k = 0; k < room; k++) m &= m - 1ull;
+                    pos += (uint32_t)__builtin_ctzll(m);
+                    nf = OW;
+                }
+            }
+            if (nf == 0) break;
+            if (!table) {
+                total = neighbour_table<D, false>(
+                    w, c_i, L, lane,
+                    Pol::KIND ? AQ_CELL_ELEM : AQ_CELL_FLUID);
+                table = true;
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_s_waitcnt(0);
+            const bool act = (uint32_t)o < nf;
+            const uint32_t i = act ? owns[wv][o] : 0u;
+            typename Pol::State st{};
+            f3 r_i = { 0.f, 0.f, 0.f };
+            if (act) {
+                r_i = ldv<D>(r, i);
+                pol.init(i, r_i, st);
+            }
+            f3 bmin, bmax;
+            wave_box<D>(r_i, act, bmin, bmax);
+            auto add = [&](const Term& t) { pol.add(st, t); };
+            if (nrq)
+                coop_walk<D, SEGW, Q>(
+                    L, &tsl[wv][0], lane, total, r_i, act, bmin, bmax, smax,
+                    r, imove, Pol::KIND,
+                    [&](uint32_t j) {
+                        return pol.template term<true>(i, j, pol.load(j),
+                                                       r_i, st);
+                    },
+                    add);
+            else
+                coop_walk<D, SEGW, Q>(
+                    L, &tsl[wv][0], lane, total, r_i, act, bmin, bmax, smax,
+                    r, imove, Pol::KIND,
+                    [&](uint32_t j) {
+                        return pol.template term<false>(i, j, pol.load(j),
+                                                        r_i, st);
+                    },
+                    add);
+            if (act && q == 0) pol.finish(i, st);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// The element-owner policies below split each pair into load(j) (the
+// candidate's fields), term<NRQ>(i, j, cand, r_i, st) (a pure function of
+// the owner's state and the candidate) and add(st, term); pair and use
+// apply them in that order, so every walk form adds the same terms.
+#define AQ_POL_PAIR_FROM_TERM                                                  \
+    template <bool NRQ>                                                        \
+    __device__ void use(uint32_t i, uint32_t j, const Cand& c,                 \
+                        const f3& r_i, State& st) const                        \
+    {                                                                          \
+        add(st, term<NRQ>(i, j, c, r_i, st));                                  \
+    }                                                                          \
+    template <bool NRQ>                                                        \
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i, State& st)     \
+        const                                                                  \
+    {                                                                          \
+        use<NRQ>(i, j, load(j), r_i, st);                                      \
+    }
+
 // cfd/Boundary/BI/Shepard.cl:61-150 (compute) for the non-fluid particles
 // (skip_fluid) or all of them
 template <int D> struct PolBiShepard {
+    static constexpr int BATCH = 4;  // candidates loaded together
     static constexpr int KIND = 1;
     static constexpr uint32_t OWNERS = 0;  // elements, sensors, ...
     const int* imove;
@@ -950,6 +1374,17 @@ template <int D> struct PolBiShepard {
         float acc;
         bool self_added;
     };
+    struct Cand {
+        f3 r, n, t, b;
+        float area;
+    };
+    // kind 0: add v; 1: the particle's own element (i itself, or a
+    // coplanar element within half a spacing), which takes 1/2 once;
+    // 2: nothing
+    struct Term {
+        float v;
+        uint32_t kind;
+    };
     __device__ bool owner(uint32_t, int im) const
     {
         return !((im < -3) || (im > 1)) && !(skip_fluid && im == 1);
@@ -959,43 +1394,42 @@ template <int D> struct PolBiShepard {
         st.acc = 1.f;
         st.self_added = false;
     }
-    template <bool NRQ>
-    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
-                         State& st) const
+    __device__ Cand load(uint32_t j) const
     {
-        if (i == j) {
-            if (!st.self_added) {
-                st.self_added = true;
-                st.acc -= 0.5f;
-            }
-            return;
-        }
+        return Cand{ ldv_o<D>(r_, j), ldv_o<D>(normal, j),
+                     ldv_o<D>(tangent, j), ldv_o<D>(binormal, j),
+                     ldf_o(m, j) };
+    }
+    template <bool NRQ>
+    __device__ Term term(uint32_t i, uint32_t j, const Cand& c,
+                         const f3& r_i, const State&) const
+    {
+        if (i == j) return Term{ 0.f, 1u };
         const float eps = 1E-8f * H;
-        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        const f3 r_ij = sub<D>(c.r, r_i);
         const float s = dot<D>(r_ij, r_ij);
         const float q = qof<NRQ>(s, H);
-        const f3 n_j = ldv_o<D>(normal, j);
-        const f3 t_j = ldv_o<D>(tangent, j);
-        const f3 b_j = ldv_o<D>(binormal, j);
-        const float r_n = dot<D>(r_ij, n_j);
-        const float area_j = ldf_o(m, j);
+        const float r_n = dot<D>(r_ij, c.n);
+        const float r_t = fabsf(dot<D>(r_ij, c.t));
+        const float r_b = fabsf(dot<D>(r_ij, c.b));
         if ((r_n > -eps) && (r_n < eps)) {
-            if (!st.self_added) {
-                const float r_t = fabsf(dot<D>(r_ij, t_j));
-                const float r_b = fabsf(dot<D>(r_ij, b_j));
-                const float dr = 0.55f * pow_cl(area_j, 1.f / (DIMS - 1.f));
-                if ((r_t <= dr) && (r_b <= dr)) {
-                    st.self_added = true;
-                    st.acc -= 0.5f;
-                }
-            }
-            return;
+            const float dr = 0.55f * pow_cl(c.area, 1.f / (DIMS - 1.f));
+            return Term{ 0.f, ((r_t <= dr) && (r_b <= dr)) ? 1u : 2u };
         }
-        const float r_t = fabsf(dot<D>(r_ij, t_j));
-        const float r_b = fabsf(dot<D>(r_ij, b_j));
-        st.acc += r_n * CONW * kernelS_P<D>(q) * area_j +
-                  kernelS_D<D>(fabsf(r_n), r_t, r_b, area_j);
+        return Term{ r_n * CONW * kernelS_P<D>(q) * c.area +
+                         kernelS_D<D>(fabsf(r_n), r_t, r_b, c.area),
+                     0u };
     }
+    __device__ void add(State& st, const Term& t) const
+    {
+        if (t.kind == 0u) {
+            st.acc += t.v;
+        } else if (t.kind == 1u && !st.self_added) {
+            st.self_added = true;
+            st.acc -= 0.5f;
+        }
+    }
+    AQ_POL_PAIR_FROM_TERM
     __device__ void finish(uint32_t i, const State& st) const
     {
         shepard[i] = st.acc;
@@ -1005,6 +1439,7 @@ template <int D> struct PolBiShepard {
 
 // cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
 template <int D> struct PolBiLapU {
+    static constexpr int BATCH = 4;  // candidates loaded together
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
     const float *r_, *u, *rho, *m;
@@ -1014,32 +1449,58 @@ template <int D> struct PolBiLapU {
     struct State {
         f3 u_i, lu;
     };
+    struct Cand {
+        f3 r, u;
+        float rho, m;
+    };
+    struct Term {
+        f3 v;  // the rounded products of the axpy
+        uint32_t use;
+    };
     __device__ bool owner(uint32_t, int im) const { return im == -3; }
     __device__ void init(uint32_t i, const f3&, State& st) const
     {
         st.u_i = ldv<D>(u, i);
         st.lu = ldv<D>(lap_u, i);
     }
+    __device__ Cand load(uint32_t j) const
+    {
+        return Cand{ ldv_o<D>(r_, j), ldv_o<D>(u, j), ldf_o(rho, j),
+                     ldf_o(m, j) };
+    }
     template <bool NRQ>
-    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
-                         State& st) const
+    __device__ Term term(uint32_t i, uint32_t j, const Cand& c,
+                         const f3& r_i, const State& st) const
     {
         constexpr float CLEARY = (D == 2) ? 8.f : 10.f;
-        if (i == j) return;
-        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        if (i == j) return Term{ f3{ 0.f, 0.f, 0.f }, 0u };
+        const f3 r_ij = sub<D>(c.r, r_i);
         const float s = dot<D>(r_ij, r_ij);
         const float q = qof<NRQ>(s, H);
-        const float rho_j = ldf_o(rho, j);
-        const float f_ij = kernelF<D>(q) * CONF * ldf_o(m, j);
-        const f3 u_ij = sub<D>(ldv_o<D>(u, j), st.u_i);
+        const float rho_j = c.rho;
+        const float f_ij = kernelF<D>(q) * CONF * c.m;
+        const f3 u_ij = sub<D>(c.u, st.u_i);
+        float sc;
+        f3 v;
         if (lap == AQ_LAP_MONAGHAN) {
             const float udr = dot<D>(u_ij, r_ij);
             const float r2 = (q * q + 0.01f) * H * H;
-            axpy<D>(st.lu, f_ij * CLEARY * udr / (r2 * rho_j), r_ij);
+            sc = f_ij * CLEARY * udr / (r2 * rho_j);
+            v = r_ij;
         } else {
-            axpy<D>(st.lu, f_ij * 2.f / rho_j, u_ij);
+            sc = f_ij * 2.f / rho_j;
+            v = u_ij;
         }
+        return Term{ f3{ sc * v.x, sc * v.y, sc * v.z }, 1u };
     }
+    __device__ void add(State& st, const Term& t) const
+    {
+        if (!t.use) return;
+        st.lu.x += t.v.x;
+        st.lu.y += t.v.y;
+        if constexpr (D == 3) st.lu.z += t.v.z;
+    }
+    AQ_POL_PAIR_FROM_TERM
     __device__ void finish(uint32_t i, const State& st) const
     {
         stxyz<D>(lap_u, i, st.lu);
@@ -1048,6 +1509,7 @@ template <int D> struct PolBiLapU {
 
 // cfd/Boundary/BI/Interpolation.cl:54-107
 template <int D> struct PolBiInterp {
+    static constexpr int BATCH = 4;  // candidates loaded together
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
     const float *r_, *m, *rho, *grad_p;
@@ -1057,6 +1519,11 @@ template <int D> struct PolBiInterp {
         f3 gp_i;
         float acc;
     };
+    struct Cand {
+        f3 r;
+        float m, rho, p;
+    };
+    using Term = float;
     __device__ bool owner(uint32_t, int im) const { return im == -3; }
     __device__ void init(uint32_t i, const f3&, State& st) const
     {
@@ -1067,26 +1534,34 @@ template <int D> struct PolBiInterp {
         st.gp_i.z = rho_i * st.gp_i.z;
         st.acc = 0.f;
     }
-    template <bool NRQ>
-    __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
-                         State& st) const
+    __device__ Cand load(uint32_t j) const
     {
-        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        return Cand{ ldv_o<D>(r_, j), ldf_o(m, j), ldf_o(rho, j),
+                     ldf_o(p, j) };
+    }
+    template <bool NRQ>
+    __device__ Term term(uint32_t, uint32_t, const Cand& c, const f3& r_i,
+                         const State& st) const
+    {
+        const f3 r_ij = sub<D>(c.r, r_i);
         const float s = dot<D>(r_ij, r_ij);
         const float q = qof<NRQ>(s, H);
-        const float w_ij =
-            kernelW<D>(q) * CONW * ldf_o(m, j) / ldf_o(rho, j);
-        st.acc += (ldf_o(p, j) - dot<D>(st.gp_i, r_ij)) * w_ij;
+        const float w_ij = kernelW<D>(q) * CONW * c.m / c.rho;
+        return (c.p - dot<D>(st.gp_i, r_ij)) * w_ij;
     }
+    __device__ void add(State& st, const Term& t) const { st.acc += t; }
+    AQ_POL_PAIR_FROM_TERM
     __device__ void finish(uint32_t i, const State& st) const
     {
         p[i] = st.acc;
     }
 };
+#undef AQ_POL_PAIR_FROM_TERM
 
 // cfd/Boundary/BI/Interactions.cl:51-133, for the fluid particles with an
 // element inside the support (near flags of the fused walk) or all of them
 template <int D> struct PolBiInteractions {
+    static constexpr int BATCH = 4;  // candidates loaded together
     static constexpr int KIND = 1;
     static constexpr uint32_t OWNERS = AQ_CELL_FLUID;
     const float *r_, *normal, *u, *rho, *m, *p;
@@ -1109,17 +1584,32 @@ template <int D> struct PolBiInteractions {
         st.gp = ldv<D>(grad_p, i);
         st.dv = div_u[i];
     }
+    struct Cand {
+        f3 r, n, u;
+        float area, p;
+    };
+    __device__ Cand load(uint32_t j) const
+    {
+        return Cand{ ldv_o<D>(r_, j), ldv_o<D>(normal, j), ldv_o<D>(u, j),
+                     ldf_o(m, j), ldf_o(p, j) };
+    }
     template <bool NRQ>
-    __device__ void pair(uint32_t, uint32_t j, const f3& r_i,
+    __device__ void pair(uint32_t i, uint32_t j, const f3& r_i,
                          State& st) const
     {
-        const f3 r_ij = sub<D>(ldv_o<D>(r_, j), r_i);
+        use<NRQ>(i, j, load(j), r_i, st);
+    }
+    template <bool NRQ>
+    __device__ void use(uint32_t, uint32_t, const Cand& c, const f3& r_i,
+                        State& st) const
+    {
+        const f3 r_ij = sub<D>(c.r, r_i);
         const float s = dot<D>(r_ij, r_ij);
         const float q = qof<NRQ>(s, H);
-        const f3 n_j = ldv_o<D>(normal, j);
-        const float area_j = ldf_o(m, j);
-        const float p_j = ldf_o(p, j);
-        const f3 du = sub<D>(ldv_o<D>(u, j), st.u_i);
+        const f3 n_j = c.n;
+        const float area_j = c.area;
+        const float p_j = c.p;
+        const f3 du = sub<D>(c.u, st.u_i);
         const float w_ij = kernelW<D>(q) * CONW * area_j;
         axpy<D>(st.gp, (st.p_i + p_j) / st.rho_i * w_ij, n_j);
         st.dv += st.rho_i * dot<D>(du, n_j) * w_ij;
@@ -1313,6 +1803,52 @@ unsigned resident_blocks(const void* ker, int block)
     return nb;
 }
 
+// Compute units of the current device (cached).
+static unsigned cu_count()
+{
+    static std::mutex mu;
+    static std::map<int, unsigned> cache;
+    int dev = 0;
+    AQ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
+    int ncu = 0;
+    AQ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                                 dev));
+    return cache[dev] = (unsigned)std::max(1, ncu);
+}
+
+// Workgroups per compute unit the walks of the side stream take
+// (AQ_SIDE_WGS, default 1); the fused walk beside them leaves them that
+// room, so both stay resident together.
+unsigned side_wgs_per_cu()
+{
+    static const unsigned n = [] {
+        const char* e = getenv("AQ_SIDE_WGS");
+        const int v = e ? atoi(e) : 1;
+        return (unsigned)std::min(4, std::max(0, v));
+    }();
+    return n;
+}
+
+// A resident grid less `reserve` workgroups per compute unit (one per
+// compute unit at least).
+static unsigned reserved_grid(unsigned resident, unsigned reserve)
+{
+    const unsigned ncu = cu_count();
+    const unsigned cut = reserve * ncu;
+    return resident > cut + ncu ? resident - cut : std::min(resident, ncu);
+}
+
+// Grid of a walk: the resident grid, or the side stream's share.
+static unsigned walk_grid(const void* ker, const CellArgs& c)
+{
+    const unsigned res = resident_blocks(ker, 64 * WC_WAVES);
+    if (c.next_off != AQ_CL_SIDE) return res;
+    return std::min(res, std::max(1u, side_wgs_per_cu()) * cu_count());
+}
+
 void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const int32_t* imove, const float* r,
                              const float* u, const float* rho, const float* m,
@@ -1326,7 +1862,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              const uint32_t* n_heads, const uint32_t* status,
                              aq_defs d, float smax, unsigned nblocks, int segw,
                              int flagged, int math, unsigned ranges,
-                             int fast_div, hipStream_t s, const float* per)
+                             int fast_div, hipStream_t s, const float* per,
+                             unsigned reserve_per_cu)
 {
     // a periodic box has no walls: the boundary-integral Shepard pass is
     // not built for it
@@ -1360,8 +1897,11 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                         ? fluid_pass_cells_k<DD, SH, LP, SW, LF, 1, PR>        \
                     : math == 2 ? fluid_pass_cells_k<DD, SH, LP, 8, LF, 2, PR> \
                                 : fluid_pass_cells_k<DD, SH, LP, SW, LF, 0, PR>; \
-        const unsigned nb = nblocks ? nblocks : resident_blocks((const void*)ker, \
-                                                    64 * WC_WAVES);            \
+        const unsigned nb =                                                    \
+            nblocks ? nblocks                                                  \
+                    : reserved_grid(resident_blocks((const void*)ker,          \
+                                                    64 * WC_WAVES),            \
+                                    reserve_per_cu);                           \
         hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, imove, r, \
                            u, rho, m, p, normal, tangent, binormal, shepard,   \
                            grad_p, lap_u, div_u, lap_p, near, w, d.H, d.CONW,  \
@@ -1400,23 +1940,64 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
 
 
 // ---- launchers of the generic cell kernels ---------------------------------
-template <int D, class Pol, bool PER = false>
+template <int D, class Pol, bool PER = false, int NB = 1>
 static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
                          const CellArgs& c, float smax, hipStream_t s,
-                         const float* per = nullptr)
+                         const float* per = nullptr,
+                         const uint32_t* olist = nullptr)
 {
     AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
-    uint32_t* next = const_cast<uint32_t*>(c.n_heads) + RSTRIDE;
+    uint32_t* next = const_cast<uint32_t*>(c.n_heads) +
+                     (c.next_off ? c.next_off : RSTRIDE);
     hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next,
                        (uint32_t*)nullptr);
     WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
                 c.n_heads, next,     c.status,  c.N,        c.flagged,
                 walk_ranges(), per };
-    auto* ker = cell_kernel_k<D, 8, Pol, PER>;
-    const unsigned nb = resident_blocks((const void*)ker, 64 * WC_WAVES);
+    auto* ker = cell_kernel_k<D, 8, Pol, PER, NB>;
+    const unsigned nb = walk_grid((const void*)ker, c);
     const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
-                       r, w, smax, nrq);
+                       r, w, smax, nrq, olist);
+}
+
+// The boundary-integral element kernels: batched consume; the kernels whose
+// owners are wall elements or other non-fluid particles (owner_cells) walk
+// the owner-cell list when the link-list built it.
+// Kernels whose owners are wall elements or other non-fluid particles
+// (owner_cells) run the cooperative walk over the owner-cell list when the
+// link-list built it (4 lanes per owner); the others, and all of them
+// without the list, the one-owner-per-lane walk with batched loads.
+template <int D, class Pol>
+static void launch_bi_cells(const Pol& pol, const int32_t* imove,
+                            const float* r, const CellArgs& c, float smax,
+                            hipStream_t s)
+{
+    launch_cells<D, Pol, false, Pol::BATCH>(pol, imove, r, c, smax, s);
+}
+
+template <int D, class Pol>
+static void launch_bi_coop(const Pol& pol, const int32_t* imove,
+                           const float* r, const CellArgs& c, float smax,
+                           hipStream_t s, bool owner_cells)
+{
+    if (!(owner_cells && c.flagged && c.ocells)) {
+        launch_bi_cells<D>(pol, imove, r, c, smax, s);
+        return;
+    }
+    AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
+    uint32_t* next = const_cast<uint32_t*>(c.n_heads) +
+                     (c.next_off ? c.next_off : RSTRIDE);
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next,
+                       (uint32_t*)nullptr);
+    WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
+                c.n_heads, next,     c.status,  c.N,        c.flagged,
+                walk_ranges(), nullptr };
+    auto* ker = cell_coop_k<D, 8, 4, Pol>;
+    const unsigned nb = walk_grid((const void*)ker, c);
+    const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
+    hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
+                       r, w, smax, nrq, c.ocells);
 }
 
 void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
@@ -1427,14 +2008,15 @@ void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
                                      aq_defs d, float smax, int skip_fluid,
                                      hipStream_t s)
 {
+    // with the fluid among the owners (skip_fluid == 0) every cell owns
     if (dims == 2) {
         PolBiShepard<2> p{ imove, normal, tangent, binormal, m, shepard,
                            d.H,   d.CONW, d.DIMS,  skip_fluid, r };
-        launch_cells<2>(p, imove, r, c, smax, s);
+        launch_bi_coop<2>(p, imove, r, c, smax, s, skip_fluid != 0);
     } else {
         PolBiShepard<3> p{ imove, normal, tangent, binormal, m, shepard,
                            d.H,   d.CONW, d.DIMS,  skip_fluid, r };
-        launch_cells<3>(p, imove, r, c, smax, s);
+        launch_bi_coop<3>(p, imove, r, c, smax, s, skip_fluid != 0);
     }
 }
 
@@ -1445,10 +2027,10 @@ void launch_bi_lapu_cells(int dims, const int32_t* imove, const float* r,
 {
     if (dims == 2) {
         PolBiLapU<2> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
-        launch_cells<2>(p, imove, r, c, smax, s);
+        launch_bi_coop<2>(p, imove, r, c, smax, s, true);
     } else {
         PolBiLapU<3> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
-        launch_cells<3>(p, imove, r, c, smax, s);
+        launch_bi_coop<3>(p, imove, r, c, smax, s, true);
     }
 }
 
@@ -1459,10 +2041,10 @@ void launch_bi_interp_cells(int dims, const int32_t* imove, const float* r,
 {
     if (dims == 2) {
         PolBiInterp<2> q{ r, m, rho, grad_p, p, d.H, d.CONW };
-        launch_cells<2>(q, imove, r, c, smax, s);
+        launch_bi_coop<2>(q, imove, r, c, smax, s, true);
     } else {
         PolBiInterp<3> q{ r, m, rho, grad_p, p, d.H, d.CONW };
-        launch_cells<3>(q, imove, r, c, smax, s);
+        launch_bi_coop<3>(q, imove, r, c, smax, s, true);
     }
 }
 
@@ -1477,11 +2059,11 @@ void launch_bi_interactions_cells(int dims, const int32_t* imove,
     if (dims == 2) {
         PolBiInteractions<2> q{ r, normal, u, rho, m, p, grad_p, div_u,
                                 near, d.H, d.CONW };
-        launch_cells<2>(q, imove, r, c, smax, s);
+        launch_bi_cells<2>(q, imove, r, c, smax, s);
     } else {
         PolBiInteractions<3> q{ r, normal, u, rho, m, p, grad_p, div_u,
                                 near, d.H, d.CONW };
-        launch_cells<3>(q, imove, r, c, smax, s);
+        launch_bi_cells<3>(q, imove, r, c, smax, s);
     }
 }
 

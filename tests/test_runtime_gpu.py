@@ -825,7 +825,10 @@ def test_graph_replay_equals_eager(cuda, tmp_path, dims):
     (3, True, "cells", "cells", "1"),
     (2, True, "cells", "thread", "0"),
     (3, True, "cells", "thread", "0"),
-    (3, True, "cells", "thread", "1")])
+    (3, True, "cells", "thread", "1"),
+    (2, True, "cells", "list", "0"),
+    (3, True, "cells", "list", "0"),
+    (3, True, "cells", "list", "1")])
 def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, bi_walk,
                                  no_fusion, monkeypatch):
     """The fused neighbour walk (Shepard + interactions + lap p + near-wall
@@ -851,6 +854,31 @@ def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, bi_walk,
         for f in ("r", "u", "rho", "p", "shepard", "grad_p", "lap_u",
                   "div_u", "lap_p", "dudt", "drhodt"):
             np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+
+
+@pytest.mark.parametrize("dims,div", [(2, 60), (3, 16)])
+def test_owner_cell_list_matches_all_cells(cuda, tmp_path, dims, div,
+                                           monkeypatch):
+    """The boundary-integral element kernels walking the owner-cell list
+    (cells with wall elements or sensors; default) equal the same kernels
+    visiting every non-empty cell (AQ_BI_WALK=cells) bit for bit on a dam
+    break with sensors, whose wall cells give candidate lists of several
+    segments."""
+    pos, nrm = cases.dam_break_sensors(dims)
+    case = cases.with_sensors(cases.dam_break(dims, 0.3 / div), pos, nrm)
+    xml = casefiles.write_case(case, str(tmp_path))
+    monkeypatch.setenv("AQ_BI_WALK", "cells")
+    a = Simulation(xml, dims)
+    monkeypatch.setenv("AQ_BI_WALK", "list")
+    b = Simulation(xml, dims)
+    for _ in range(2):
+        a.step(3)
+        b.step(3, graph=True)
+        for f in ("r", "u", "rho", "p", "shepard", "grad_p", "lap_u",
+                  "div_u", "dudt", "drhodt"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    a.close()
+    b.close()
 
 
 def _dense_case(dims, n, seed):
