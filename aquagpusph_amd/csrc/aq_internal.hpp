@@ -54,6 +54,12 @@ class Error : public std::runtime_error {
 // list (cells flagged AQ_CELL_ELEM or AQ_CELL_OTHER; the list holds their
 // first particle), filled with the kind flags.
 #define AQ_CL_NOCELLS 512u
+// Word of the n_heads buffer counting the wet owner cells (owner cells with
+// fluid in their neighbourhood; a second list of the same triplets).
+#define AQ_CL_NWCELLS 513u
+// Word of the n_heads buffer counting the near cells (cells with a fluid
+// particle within the support of a wall element, listed by the fused walk).
+#define AQ_CL_NNCELLS 514u
 #define AQ_CL_WORDS 1024u  // size of the n_heads buffer in words
 // Hand-out counters of the walks launched on the side stream beside the
 // fused walk (8 counters, 32 words apart, from this word on); the walks of
@@ -73,6 +79,8 @@ struct CellArgs {
     // or other non-fluid particles, counted in n_heads[AQ_CL_NOCELLS]);
     // nullptr: the element kernels visit every non-empty cell
     const uint32_t* ocells = nullptr;
+    const uint32_t* wcells = nullptr;  // the wet owner cells
+    const uint32_t* nlist = nullptr;   // the near cells (fused walk)
     // word of the n_heads buffer where the walk's hand-out counters start
     // (0: word 32, the main stream's)
     uint32_t next_off = 0;

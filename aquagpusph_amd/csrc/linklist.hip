@@ -262,7 +262,9 @@ __global__ void cell_list_k(const uint32_t* __restrict__ head,
     if (head[i]) heads[pos[i]] = i;
     if (i == N - 1) {
         *n_heads = pos[i] + head[i];
-        n_heads[AQ_CL_NOCELLS] = 0;  // the owner-cell list is refilled next
+        // the owner-cell lists are refilled next
+        n_heads[AQ_CL_NOCELLS] = 0;
+        n_heads[AQ_CL_NWCELLS] = 0;
     }
 }
 
@@ -318,6 +320,64 @@ __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
     }
 }
 
+// Wet owner cells: the owner cells (ocells) with fluid in their 3^d
+// neighbourhood, appended to wcells as the same triplets (count at
+// n_heads[AQ_CL_NWCELLS]).  The element kernels whose candidates are fluid
+// (BI LapU, BI Interpolation) have no term to add in the other cells.
+template <int D>
+__global__ void owner_wet_k(const uint32_t* __restrict__ ocells,
+                            uint32_t* __restrict__ n_heads,
+                            const uint32_t* __restrict__ ihoc,
+                            const uint32_t* __restrict__ cell_end,
+                            const uint32_t* __restrict__ n_cells, uint32_t N,
+                            uint32_t* __restrict__ wcells,
+                            const uint32_t* status)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (status && (*status & 1u)) return;  // uniform
+    const uint32_t n = n_heads[AQ_CL_NOCELLS];
+    if (blockIdx.x * blockDim.x >= n) return;  // uniform per block
+    bool wet = false;
+    uint32_t t0 = 0, t1 = 0, t2 = 0;
+    if (m < n) {
+        t0 = ocells[3 * m];
+        t1 = ocells[3 * m + 1];
+        t2 = ocells[3 * m + 2];
+        const uint32_t nx = n_cells[0], ny = n_cells[1];
+        constexpr int NC = (D == 2) ? 9 : 27;
+        for (int k = 0; k < NC && !wet; k++) {
+            int ci, cj, ck;
+            if (D == 2) {
+                ci = k / 3 - 1;
+                cj = k % 3 - 1;
+                ck = 0;
+            } else {
+                ci = k / 9 - 1;
+                cj = (k / 3) % 3 - 1;
+                ck = k % 3 - 1;
+            }
+            const uint32_t c = t1 + (uint32_t)ci + (uint32_t)cj * nx +
+                               (uint32_t)ck * nx * ny;
+            if (ihoc[c] < N && (cell_end[c] & AQ_CELL_FLUID)) wet = true;
+        }
+    }
+    const uint64_t bm = __ballot(wet);
+    if (bm == 0) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == __builtin_ctzll(bm))
+        base = atomicAdd(n_heads + AQ_CL_NWCELLS, (uint32_t)__popcll(bm));
+    base = __shfl(base, __builtin_ctzll(bm));
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+    if (wet) {
+        const uint32_t o = 3u * (base + rank);
+        wcells[o] = t0;
+        wcells[o + 1] = t1;
+        wcells[o + 2] = t2;
+    }
+}
+
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
                     uint32_t* sums, hipStream_t s);
 size_t scan_workspace(uint32_t m);
@@ -335,7 +395,8 @@ size_t cell_list_workspace(uint32_t N)
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
-               hipStream_t s, uint32_t* ocells)
+               hipStream_t s, uint32_t* ocells, uint32_t* wcells, int dims,
+               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap)
 {
     const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
     uint32_t* head = (uint32_t*)ws;
@@ -349,6 +410,15 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
     if (perm && imove)
         hipLaunchKernelGGL(cell_kinds_k, g, b, 0, s, heads, n_heads, icell,
                            perm, imove, cell_end, status, ocells);
+    if (perm && imove && ocells && wcells && ocap) {
+        const dim3 gw(blocks_for(ocap, 256));
+        if (dims == 2)
+            hipLaunchKernelGGL(owner_wet_k<2>, gw, b, 0, s, ocells, n_heads,
+                               ihoc, cell_end, n_cells, N, wcells, status);
+        else
+            hipLaunchKernelGGL(owner_wet_k<3>, gw, b, 0, s, ocells, n_heads,
+                               ihoc, cell_end, n_cells, N, wcells, status);
+    }
 }
 
 // ---- host drivers ---------------------------------------------------------

@@ -53,6 +53,7 @@ Sim::~Sim()
     tools.clear();
     for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
                      (void*)clist.heads, (void*)clist.ocells,
+                     (void*)clist.wcells, (void*)clist.nlist,
                      (void*)clist.n_heads, clist.ws,
                      (void*)sel_list, (void*)sel_count })
         if (b) (void)hipFree(b);
@@ -470,12 +471,15 @@ void Sim::plan_fusion()
         AQ_HIP(hipMalloc(&fusion.near_buf, N));
         AQ_HIP(hipMemset(fusion.near_buf, 1, N));
     }
-    // the boundary-integral element chain beside the walk (BI walls with
-    // the near flags): BI LapU, GradP, Interpolation and InterpolationShepard
-    // between the anchor and BI Interactions, and between the anchor and BI
-    // Interactions only the fused tools, dummies and the sensors (which
-    // read fluid fields and write sensor fields)
-    if (fusion.shep == 2 && fusion.near && !getenv_true("AQ_NO_OVERLAP")) {
+    // the boundary-integral element chain beside the walk (opt-in,
+    // AQ_BI_OVERLAP=1; BI walls with the near flags): BI LapU, GradP,
+    // Interpolation and InterpolationShepard between the anchor and BI
+    // Interactions, and between the anchor and BI Interactions only the
+    // fused tools, dummies and the sensors (which read fluid fields and
+    // write sensor fields).  On the 3D dam break it does not pay: the walk,
+    // bound by VALU issue at four wavefronts per SIMD, loses more to the
+    // slots it leaves the chain than the chain's 1 ms saves.
+    if (fusion.shep == 2 && fusion.near && getenv_true("AQ_BI_OVERLAP")) {
         static const char* keys[] = {
             "cfd/Boundary/BI/LapU.cl:freeslip",
             "cfd/Boundary/BI/GradP.cl:freeslip",
@@ -559,7 +563,8 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              aq_defs d, float smax, unsigned nblocks,
                              int segw, int flagged, int math,
                              unsigned ranges, int fast_div, hipStream_t s,
-                             const float* per, unsigned reserve_per_cu);
+                             const float* per, unsigned reserve_per_cu,
+                             uint32_t* nlist);
 unsigned side_wgs_per_cu();
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
@@ -586,6 +591,11 @@ CellArgs Sim::cell_args()
                      N,
                      clist.kinds ? 1u : 0u,
                      (bi_group && clist.kinds) ? clist.ocells : nullptr,
+                     (bi_group && clist.kinds) ? clist.wcells : nullptr,
+                     // filled by the fused walk of the same step
+                     (bi_group && clist.kinds && fusion.active && fusion.near)
+                         ? clist.nlist
+                         : nullptr,
                      in_side ? AQ_CL_SIDE : 0u };
 }
 
@@ -676,7 +686,8 @@ void Sim::run_fused(hipStream_t s)
             d_status, kdefs, support_threshold(kdefs.H), f.walk_blocks,
             f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges,
             f.fast_div ? 1 : 0, s, periodic_L(),
-            f.side_launched ? side_wgs_per_cu() : 0u);
+            f.side_launched ? side_wgs_per_cu() : 0u,
+            (bi_group && clist.kinds) ? clist.nlist : nullptr);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

@@ -281,6 +281,8 @@ struct CellList {
     size_t cap = 0;  // cells cell_end holds
     uint32_t* heads = nullptr;
     uint32_t* ocells = nullptr;   // owner-cell triplets (cell_kinds_k)
+    uint32_t* wcells = nullptr;   // wet owner cells (owner_wet_k)
+    uint32_t* nlist = nullptr;    // near cells (the fused walk)
     uint32_t* n_heads = nullptr;  // [0] count, [32..] hand-out counters
     void* ws = nullptr;
     bool usable(const Var* ic, const Var* ih) const

@@ -25,7 +25,8 @@ size_t cell_list_workspace(uint32_t N);
 void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
-               hipStream_t s, uint32_t* ocells);
+               hipStream_t s, uint32_t* ocells, uint32_t* wcells, int dims,
+               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap);
 void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                 unsigned key_bits, void* workspace, hipStream_t s);
 size_t radix_sort_workspace(uint32_t n);
@@ -562,10 +563,11 @@ class LinkListTool : public Tool {
                 AQ_HIP(hipMalloc(&c.cell_end, ihoc_->length * 4));
                 c.cap = ihoc_->length;
                 // owner-cell triplets: at most one per non-empty cell
-                if (c.ocells) AQ_HIP(hipFree(c.ocells));
-                c.ocells = nullptr;
-                AQ_HIP(hipMalloc(&c.ocells,
-                                 std::min<size_t>(S.N, c.cap) * 12));
+                for (uint32_t** l : { &c.ocells, &c.wcells, &c.nlist }) {
+                    if (*l) AQ_HIP(hipFree(*l));
+                    *l = nullptr;
+                    AQ_HIP(hipMalloc(l, std::min<size_t>(S.N, c.cap) * 12));
+                }
             }
             if (!c.heads) {
                 AQ_HIP(hipMalloc(&c.heads, (size_t)S.N * 4));
@@ -583,7 +585,10 @@ class LinkListTool : public Tool {
                       c.n_heads, S.d_status, c.ws,
                       kinds ? (const uint32_t*)perm_->dev : nullptr,
                       kinds ? (const int*)im->dev : nullptr, s,
-                      kinds ? c.ocells : nullptr);
+                      kinds ? c.ocells : nullptr, c.wcells, S.dims,
+                      (const uint32_t*)ihoc_->dev,
+                      (const uint32_t*)S.ptr(nc_),
+                      std::min<size_t>(S.N, c.cap));
             c.kinds = kinds;
             c.filled = true;
         }

@@ -69,6 +69,10 @@ struct WalkArgs {
     // periodic_L of a periodic box (the walks instantiated with PER wrap
     // their neighbour cells), nullptr otherwise
     const float* per = nullptr;
+    // near-cell list the fused walk fills: (first, cell, cell_end)
+    // triplets of the cells with a fluid particle near a wall element,
+    // counted in n_heads[AQ_CL_NNCELLS]; nullptr: not built
+    uint32_t* nlist = nullptr;
 };
 
 // Next cell (index into heads) for the calling wave, n_heads when all are
@@ -698,6 +702,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
         const uint32_t ce = uni(w.cell_end[c_i]);
         const uint32_t cend = ce & AQ_CELL_END;
         if (w.flagged && !(ce & AQ_CELL_FLUID)) continue;  // no fluid here
+        bool cell_near = false;  // some fluid particle of the cell is near
         for (uint32_t i0 = cstart; i0 < cend; i0 += 64) {
             const uint32_t i = i0 + lane;
             const bool act = (i < cend) && (imove[i] == 1);
@@ -911,7 +916,15 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 if (SHEP) shepard[i] = sh;
                 if (near) near[i] = nearw ? 1 : 0;
             }
+            if (w.nlist) cell_near = cell_near || __ballot(act && nearw) != 0;
             __builtin_amdgcn_wave_barrier();
+        }
+        if (w.nlist && cell_near && lane == 0) {
+            uint32_t* cnt = const_cast<uint32_t*>(w.n_heads) + AQ_CL_NNCELLS;
+            const uint32_t k = atomicAdd(cnt, 1u);
+            w.nlist[3 * k] = cstart;
+            w.nlist[3 * k + 1] = c_i;
+            w.nlist[3 * k + 2] = ce;
         }
     }
 }
@@ -958,21 +971,22 @@ template <class Pol, bool NRQ> struct BatchPair {
 
 // PER: a periodic box; the policy's pair_p<NRQ>(i, j, shift, r_i, st) gets
 // the shift of j's neighbour cell.  NB > 1: batched consume (ring_walk).
-// olist: walk the cells it lists (the owner-cell list of (first particle,
-// cell, cell_end) triplets, counted in n_heads[AQ_CL_NOCELLS]) instead of
-// every non-empty cell.
+// olist: walk the cells it lists ((first particle, cell, cell_end)
+// triplets, counted in n_heads[count_word]: the owner-cell lists, the
+// near-cell list) instead of every non-empty cell.
 template <int D, int SEGW, class Pol, bool PER = false, int NB = 1>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     cell_kernel_k(Pol pol, const int* __restrict__ imove,
                   const float* __restrict__ r, WalkArgs w, float smax,
-                  int nrq, const uint32_t* __restrict__ olist)
+                  int nrq, const uint32_t* __restrict__ olist,
+                  uint32_t count_word)
 {
     __shared__ WaveLds<SEGW, PER> lds[WC_WAVES];
     const int lane = threadIdx.x & 63;
     WaveLds<SEGW, PER>& L = lds[threadIdx.x >> 6];
     if (w.status && (*w.status & 1u)) return;
     const uint32_t n_heads =
-        olist ? uni(w.n_heads[AQ_CL_NOCELLS]) : uni(*w.n_heads);
+        olist ? uni(w.n_heads[count_word]) : uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
     while (true) {
         const uint32_t hk = next_cell(w, n_heads, lane, range);
@@ -1075,7 +1089,10 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
 // Every owner's accumulator thus takes the terms of the one-lane walk in the
 // same order: the sums are bit-identical.
 // ---------------------------------------------------------------------------
-template <int D, int SEGW, int Q, class Term, class TermFn, class AddFn>
+// B: candidates per lane and round (Q B per owner), whose loads are issued
+// together; tsl holds B x 64 terms.
+template <int D, int SEGW, int Q, int B, class Term, class LoadFn,
+          class TermFn, class AddFn>
 __device__ __forceinline__ void coop_walk(WaveLds<SEGW, false>& L,
                                           Term* tsl, int lane,
                                           uint32_t total, const f3& r_i,
@@ -1083,9 +1100,11 @@ __device__ __forceinline__ void coop_walk(WaveLds<SEGW, false>& L,
                                           const f3& bmax, float smax,
                                           const float* __restrict__ r,
                                           const int* __restrict__ imove,
-                                          int kind, TermFn term, AddFn add)
+                                          int kind, LoadFn load, TermFn term,
+                                          AddFn add)
 {
     static_assert(Q == 1 || Q == 2 || Q == 4, "lanes per owner");
+    static_assert(B >= 1 && B <= 4, "candidates per lane and round");
     constexpr int OW = 64 / Q;   // owners per pass
     constexpr int TPL = 64 / Q;  // survivors tested per lane and chunk
     constexpr uint32_t SEG = SEGW * 32;
@@ -1114,56 +1133,92 @@ __device__ __forceinline__ void coop_walk(WaveLds<SEGW, false>& L,
     };
     while (true) {
         for (bool more = go(); more; more = go()) {
-            // one round: the next Q accepted candidates of every owner (the
-            // Q lanes of an owner hold the same bits and step alike)
-            uint32_t myj = 0;
-            int n = 0;
-            bool mine = false;
+            // one round: the next Q B accepted candidates of every owner
+            // (the Q lanes of an owner hold the same bits and step alike);
+            // lane q takes the candidates k = q + Q b
+            uint32_t myj[B];
+            bool mine[B];
 #pragma unroll
-            for (int k = 0; k < Q; k++) {
+            for (int b = 0; b < B; b++) {
+                myj[b] = 0;
+                mine[b] = false;
+            }
+            int n = 0;
+#pragma unroll
+            for (int k = 0; k < Q * B; k++) {
                 if (bw == 0) advance();
                 if (bw != 0) {
                     const uint32_t b = __builtin_clz(bw);
                     bw ^= 0x80000000u >> b;
                     const uint32_t jj = cand[((nw - 1) % RING) * 32 + b];
-                    if (k == q) {
-                        myj = jj;
-                        mine = true;
+                    if (k % Q == q) {
+                        myj[k / Q] = jj;
+                        mine[k / Q] = true;
                     }
                     n = k + 1;
                 }
             }
-            Term t{};
-            if (mine) t = term(myj);
-            if constexpr (Q == 1) {
-                if (mine) add(t);
+            decltype(load(0u)) cd[B];
+#pragma unroll
+            for (int b = 0; b < B; b++)
+                if (mine[b]) cd[b] = load(myj[b]);
+            if constexpr (Q == 1 && B == 1) {
+                if (mine[0]) add(term(myj[0], cd[0]));
             } else {
-                tsl[lane] = t;
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    Term t{};
+                    if (mine[b]) t = term(myj[b], cd[b]);
+                    tsl[b * 64 + lane] = t;
+                }
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_s_waitcnt(0);
                 if (q == 0) {
 #pragma unroll
-                    for (int k = 0; k < Q; k++)
-                        if (k < n) add(tsl[k * OW + o]);
+                    for (int k = 0; k < Q * B; k++)
+                        if (k < n) add(tsl[(k / Q) * 64 + (k % Q) * OW + o]);
                 }
                 __builtin_amdgcn_wave_barrier();
             }
         }
         if (tested >= nsegs) break;  // and no lane has a candidate left
-        // refill the oldest slot
+        // refill the oldest slot: the segment's candidate loads are issued
+        // together, then its chunks of 64 are tested
         const uint32_t seg = tested * SEG;
         const uint32_t nseg = min(SEG, total - seg);
         const int slot = tested & 1;
-        for (uint32_t cb = 0; cb < nseg; cb += 64) {
-            const uint32_t t = seg + cb + lane;
+        constexpr uint32_t NCH = SEG / 64;
+        uint32_t pj[NCH];
+        f3 prj[NCH];
+        int pim[NCH];
+#pragma unroll
+        for (uint32_t ch = 0; ch < NCH; ch++) {
+            pj[ch] = 0;
+            const uint32_t t = seg + ch * 64 + lane;
+            if (ch * 64 + lane < nseg) {
+                while (t >= L.cum[kl + 1]) kl++;
+                pj[ch] = L.start[kl] + (t - L.cum[kl]);
+            }
+        }
+#pragma unroll
+        for (uint32_t ch = 0; ch < NCH; ch++) {
+            prj[ch] = f3{ 0.f, 0.f, 0.f };
+            pim[ch] = 0;
+            if (ch * 64 + lane < nseg) {
+                prj[ch] = ldv_o<D>(r, pj[ch]);
+                pim[ch] = ldi_o(imove, pj[ch]);
+            }
+        }
+#pragma unroll
+        for (uint32_t ch = 0; ch < NCH; ch++) {
+            const uint32_t cb = ch * 64;
+            if (cb >= nseg) break;
             float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
             bool take = false;
-            uint32_t j = 0;
+            const uint32_t j = pj[ch];
             if (cb + lane < nseg) {
-                while (t >= L.cum[kl + 1]) kl++;
-                j = L.start[kl] + (t - L.cum[kl]);
-                const f3 rj = ldv_o<D>(r, j);
-                const int im_j = ldi_o(imove, j);
+                const f3 rj = prj[ch];
+                const int im_j = pim[ch];
                 take = kind ? (im_j == -3) : (im_j == 1);
                 if (take) {
                     const float dx =
@@ -1235,25 +1290,26 @@ __device__ __forceinline__ void coop_walk(WaveLds<SEGW, false>& L,
 // Owners of each listed cell in passes of up to 64 / Q; the policy splits
 // its pair into load(j) (the candidate's fields), term<NRQ>(i, j, cand,
 // r_i, st) (pure) and add(st, term), applied in candidate order.  olist as
-// in cell_kernel_k.
+// in cell_kernel_k, its count in n_heads[count_word].
 template <int D, int SEGW, int Q, class Pol>
 __global__ void __launch_bounds__(64 * WC_WAVES)
     cell_coop_k(Pol pol, const int* __restrict__ imove,
                 const float* __restrict__ r, WalkArgs w, float smax, int nrq,
-                const uint32_t* __restrict__ olist)
+                const uint32_t* __restrict__ olist, uint32_t count_word)
 {
     constexpr int OW = 64 / Q;
     using Term = typename Pol::Term;
+    constexpr int B = Pol::COOP_B;
     __shared__ WaveLds<SEGW, false> lds[WC_WAVES];
     __shared__ uint32_t owns[WC_WAVES][OW];
-    __shared__ Term tsl[WC_WAVES][64];
+    __shared__ Term tsl[WC_WAVES][B * 64];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int q = lane / OW, o = lane % OW;
     WaveLds<SEGW, false>& L = lds[wv];
     if (w.status && (*w.status & 1u)) return;
     const uint32_t n_heads =
-        olist ? uni(w.n_heads[AQ_CL_NOCELLS]) : uni(*w.n_heads);
+        olist ? uni(w.n_heads[count_word]) : uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
     while (true) {
         const uint32_t hk = next_cell(w, n_heads, lane, range);
@@ -1317,22 +1373,22 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             f3 bmin, bmax;
             wave_box<D>(r_i, act, bmin, bmax);
             auto add = [&](const Term& t) { pol.add(st, t); };
+            auto load = [&](uint32_t j) { return pol.load(j); };
+            using Cand = typename Pol::Cand;
             if (nrq)
-                coop_walk<D, SEGW, Q>(
+                coop_walk<D, SEGW, Q, B>(
                     L, &tsl[wv][0], lane, total, r_i, act, bmin, bmax, smax,
-                    r, imove, Pol::KIND,
-                    [&](uint32_t j) {
-                        return pol.template term<true>(i, j, pol.load(j),
-                                                       r_i, st);
+                    r, imove, Pol::KIND, load,
+                    [&](uint32_t j, const Cand& c) {
+                        return pol.template term<true>(i, j, c, r_i, st);
                     },
                     add);
             else
-                coop_walk<D, SEGW, Q>(
+                coop_walk<D, SEGW, Q, B>(
                     L, &tsl[wv][0], lane, total, r_i, act, bmin, bmax, smax,
-                    r, imove, Pol::KIND,
-                    [&](uint32_t j) {
-                        return pol.template term<false>(i, j, pol.load(j),
-                                                        r_i, st);
+                    r, imove, Pol::KIND, load,
+                    [&](uint32_t j, const Cand& c) {
+                        return pol.template term<false>(i, j, c, r_i, st);
                     },
                     add);
             if (act && q == 0) pol.finish(i, st);
@@ -1363,6 +1419,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
 // (skip_fluid) or all of them
 template <int D> struct PolBiShepard {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr int COOP_B = 1;  // per lane and round (coop walk)
     static constexpr int KIND = 1;
     static constexpr uint32_t OWNERS = 0;  // elements, sensors, ...
     const int* imove;
@@ -1440,6 +1497,7 @@ template <int D> struct PolBiShepard {
 // cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
 template <int D> struct PolBiLapU {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
     const float *r_, *u, *rho, *m;
@@ -1510,6 +1568,7 @@ template <int D> struct PolBiLapU {
 // cfd/Boundary/BI/Interpolation.cl:54-107
 template <int D> struct PolBiInterp {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
     const float *r_, *m, *rho, *grad_p;
@@ -1763,11 +1822,13 @@ __global__ void les_gradu_clear_k(const int* __restrict__ imove,
 }
 
 // Clears the NRANGE range counters.
-__global__ void zero_u32_k(uint32_t* p, uint32_t* rec)
+__global__ void zero_u32_k(uint32_t* p, uint32_t* rec,
+                           uint32_t* extra = nullptr)
 {
     if (threadIdx.x < NRANGE) p[threadIdx.x * RSTRIDE] = 0u;
     if (rec && threadIdx.x < NREC)
         rec[threadIdx.x * RSTRIDE] = rec_init[threadIdx.x];
+    if (extra && threadIdx.x == 0) *extra = 0u;
 }
 
 // Cell hand-out ranges: NRANGE (one per XCD) unless AQ_WALK_RANGES=1.
@@ -1863,7 +1924,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              aq_defs d, float smax, unsigned nblocks, int segw,
                              int flagged, int math, unsigned ranges,
                              int fast_div, hipStream_t s, const float* per,
-                             unsigned reserve_per_cu)
+                             unsigned reserve_per_cu, uint32_t* nlist)
 {
     // a periodic box has no walls: the boundary-integral Shepard pass is
     // not built for it
@@ -1877,7 +1938,11 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     uint32_t* rec =
         (fast_div && math != 1) ? const_cast<uint32_t*>(n_heads) + RREC
                                 : nullptr;
-    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next, rec);
+    // the near-cell list is filled from the walk's near flags
+    if (!near || shep != 2) nlist = nullptr;
+    hipLaunchKernelGGL(zero_u32_k, dim3(1), dim3(64), 0, s, next, rec,
+                       nlist ? const_cast<uint32_t*>(n_heads) + AQ_CL_NNCELLS
+                             : nullptr);
     if (rec && N) {
         const unsigned nb = std::min(512u, (N + 1023u) / 1024u);
         if (dims == 2)
@@ -1890,7 +1955,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
     WalkArgs w{ icell,  ihoc,   n_cells, cell_end, heads,
                 n_heads, next, status, N,       (uint32_t)flagged,
                 ranges ? std::min<unsigned>(ranges, NRANGE) : walk_ranges(),
-                per };
+                per, nlist };
 #define AQ_FC3(DD, SH, LP, SW, LF, PR)                                         \
     do {                                                                       \
         auto* ker = math == 1                                                  \
@@ -1944,7 +2009,8 @@ template <int D, class Pol, bool PER = false, int NB = 1>
 static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
                          const CellArgs& c, float smax, hipStream_t s,
                          const float* per = nullptr,
-                         const uint32_t* olist = nullptr)
+                         const uint32_t* olist = nullptr,
+                         uint32_t count_word = AQ_CL_NOCELLS)
 {
     AQ_REQUIRE(c.N < (1u << 28), "cell-parallel walk: more than 2^28 particles");
     uint32_t* next = const_cast<uint32_t*>(c.n_heads) +
@@ -1958,7 +2024,7 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
     const unsigned nb = walk_grid((const void*)ker, c);
     const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
-                       r, w, smax, nrq, olist);
+                       r, w, smax, nrq, olist, count_word);
 }
 
 // The boundary-integral element kernels: batched consume; the kernels whose
@@ -1976,12 +2042,18 @@ static void launch_bi_cells(const Pol& pol, const int32_t* imove,
     launch_cells<D, Pol, false, Pol::BATCH>(pol, imove, r, c, smax, s);
 }
 
+// list: 0 every non-empty cell (one owner per lane), 1 the owner cells,
+// 2 the wet owner cells (cooperative walk)
 template <int D, class Pol>
 static void launch_bi_coop(const Pol& pol, const int32_t* imove,
                            const float* r, const CellArgs& c, float smax,
-                           hipStream_t s, bool owner_cells)
+                           hipStream_t s, int list)
 {
-    if (!(owner_cells && c.flagged && c.ocells)) {
+    const uint32_t* ol = !c.flagged ? nullptr
+                         : list == 1 ? c.ocells
+                         : list == 2 ? c.wcells
+                                     : nullptr;
+    if (!ol) {
         launch_bi_cells<D>(pol, imove, r, c, smax, s);
         return;
     }
@@ -1997,7 +2069,18 @@ static void launch_bi_coop(const Pol& pol, const int32_t* imove,
     const unsigned nb = walk_grid((const void*)ker, c);
     const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
     hipLaunchKernelGGL(ker, dim3(nb), dim3(64 * WC_WAVES), 0, s, pol, imove,
-                       r, w, smax, nrq, c.ocells);
+                       r, w, smax, nrq, ol,
+                       list == 1 ? AQ_CL_NOCELLS : AQ_CL_NWCELLS);
+}
+
+// p = 0 for the wall elements (imove == -3): the value BI Interpolation
+// gives the elements without fluid in range, which the walk over the wet
+// owner cells does not visit
+__global__ void elem_zero_k(const int* __restrict__ imove,
+                            float* __restrict__ p, uint32_t N)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N && imove[i] == -3) p[i] = 0.f;
 }
 
 void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
@@ -2012,11 +2095,11 @@ void launch_bi_shepard_compute_cells(int dims, const int32_t* imove,
     if (dims == 2) {
         PolBiShepard<2> p{ imove, normal, tangent, binormal, m, shepard,
                            d.H,   d.CONW, d.DIMS,  skip_fluid, r };
-        launch_bi_coop<2>(p, imove, r, c, smax, s, skip_fluid != 0);
+        launch_bi_coop<2>(p, imove, r, c, smax, s, skip_fluid ? 1 : 0);
     } else {
         PolBiShepard<3> p{ imove, normal, tangent, binormal, m, shepard,
                            d.H,   d.CONW, d.DIMS,  skip_fluid, r };
-        launch_bi_coop<3>(p, imove, r, c, smax, s, skip_fluid != 0);
+        launch_bi_coop<3>(p, imove, r, c, smax, s, skip_fluid ? 1 : 0);
     }
 }
 
@@ -2027,10 +2110,10 @@ void launch_bi_lapu_cells(int dims, const int32_t* imove, const float* r,
 {
     if (dims == 2) {
         PolBiLapU<2> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
-        launch_bi_coop<2>(p, imove, r, c, smax, s, true);
+        launch_bi_coop<2>(p, imove, r, c, smax, s, 2);
     } else {
         PolBiLapU<3> p{ r, u, rho, m, lap_u, d.H, d.CONF, d.lap_formulation };
-        launch_bi_coop<3>(p, imove, r, c, smax, s, true);
+        launch_bi_coop<3>(p, imove, r, c, smax, s, 2);
     }
 }
 
@@ -2039,12 +2122,15 @@ void launch_bi_interp_cells(int dims, const int32_t* imove, const float* r,
                             const float* grad_p, float* p, const CellArgs& c,
                             aq_defs d, float smax, hipStream_t s)
 {
+    if (c.flagged && c.wcells)
+        hipLaunchKernelGGL(elem_zero_k, dim3(blocks_for(c.N, 256)), dim3(256),
+                           0, s, imove, p, c.N);
     if (dims == 2) {
         PolBiInterp<2> q{ r, m, rho, grad_p, p, d.H, d.CONW };
-        launch_bi_coop<2>(q, imove, r, c, smax, s, true);
+        launch_bi_coop<2>(q, imove, r, c, smax, s, 2);
     } else {
         PolBiInterp<3> q{ r, m, rho, grad_p, p, d.H, d.CONW };
-        launch_bi_coop<3>(q, imove, r, c, smax, s, true);
+        launch_bi_coop<3>(q, imove, r, c, smax, s, 2);
     }
 }
 
@@ -2056,14 +2142,20 @@ void launch_bi_interactions_cells(int dims, const int32_t* imove,
                                   const uint8_t* near, const CellArgs& c,
                                   aq_defs d, float smax, hipStream_t s)
 {
+    // with the near flags, only the cells the fused walk listed as near
+    const uint32_t* ol = (near && c.flagged) ? c.nlist : nullptr;
     if (dims == 2) {
         PolBiInteractions<2> q{ r, normal, u, rho, m, p, grad_p, div_u,
                                 near, d.H, d.CONW };
-        launch_bi_cells<2>(q, imove, r, c, smax, s);
+        launch_cells<2, PolBiInteractions<2>, false,
+                     PolBiInteractions<2>::BATCH>(q, imove, r, c, smax, s,
+                                                  nullptr, ol, AQ_CL_NNCELLS);
     } else {
         PolBiInteractions<3> q{ r, normal, u, rho, m, p, grad_p, div_u,
                                 near, d.H, d.CONW };
-        launch_bi_cells<3>(q, imove, r, c, smax, s);
+        launch_cells<3, PolBiInteractions<3>, false,
+                     PolBiInteractions<3>::BATCH>(q, imove, r, c, smax, s,
+                                                  nullptr, ol, AQ_CL_NNCELLS);
     }
 }
 
