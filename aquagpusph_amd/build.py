@@ -26,6 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
+          "-Wno-pass-failed",
           "-Wno-unused-variable", "-I" + os.path.join(ROOT, "include")]
 
 

@@ -43,6 +43,11 @@ __device__ __forceinline__ uint32_t uni(uint32_t x)
 {
     return __builtin_amdgcn_readfirstlane(x);
 }
+// a wave-uniform float, moved to a scalar register
+__device__ __forceinline__ float uni_f(float x)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
 
 // Cells are handed out in NRANGE contiguous ranges of the non-empty cell
 // list, one per XCD: workgroups are dispatched to the XCDs round-robin, so
@@ -192,8 +197,9 @@ __device__ __forceinline__ void wave_box(const f3& r_i, bool act, f3& bmin,
             hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], o));
         }
     }
-    bmin = f3{ lo[0], lo[1], D == 3 ? lo[2] : 0.f };
-    bmax = f3{ hi[0], hi[1], D == 3 ? hi[2] : 0.f };
+    // uniform: held in scalar registers
+    bmin = f3{ uni_f(lo[0]), uni_f(lo[1]), D == 3 ? uni_f(lo[2]) : 0.f };
+    bmax = f3{ uni_f(hi[0]), uni_f(hi[1]), D == 3 ? uni_f(hi[2]) : 0.f };
 }
 
 // Walk the accepted candidates of one particle per lane.  `kind` selects
@@ -227,7 +233,20 @@ __device__ __forceinline__ void wave_box(const f3& r_i, bool act, f3& bmin,
 // by the memory latency of that chain; batching takes one latency per NB
 // candidates instead of one per candidate, with the same terms in the same
 // order.
-template <int D, int SEGW, int NB = 1, bool PER, class Pair>
+//
+// LOOSE (the fused fluid walk, not periodic): the per-lane test runs on
+// coordinates relative to the box centre c, s' = |r_i - c|^2 + |r_j - c|^2
+// - 2 (r_i - c).(r_j - c), with |r_j - c|^2 computed once per staged
+// candidate: 5 operations per lane and candidate instead of 8.  s' is
+// within 64 ulp of R^2 (R bounds |r - c| for every owner and survivor:
+// the half diagonal of the box plus the support radius) of the reference's
+// s, so the test s' <= smax + 2^-14 R^2 accepts every candidate the exact
+// test accepts, and a few within a relative 2^-14 R^2 / smax beyond; the
+// consume repeats the exact test (s <= smax) before adding a pair, so the
+// sums are unchanged.  The staged record's w carries |r_j - c|^2 with the
+// sign bit set for wall elements.
+template <int D, int SEGW, int NB = 1, bool LOOSE = false, bool PER,
+          class Pair>
 __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                                           uint32_t total, const f3& r_i,
                                           bool act, const f3& bmin,
@@ -246,6 +265,23 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
     constexpr uint32_t RING = 2 * SEGW;
     const uint32_t* bits = &L.bits[0][0][0];
     const uint32_t* cand = &L.cand[0][0];
+    static_assert(!(LOOSE && PER), "relative test: not for periodic walks");
+    // LOOSE: the reference point, the owner's relative position and the
+    // widened threshold
+    f3 cen = { 0.f, 0.f, 0.f }, rio = { 0.f, 0.f, 0.f };
+    float ai = 0.f, smaxl = smax;
+    if constexpr (LOOSE) {
+        if (bmin.x <= bmax.x) {
+            cen = f3{ uni_f(0.5f * (bmin.x + bmax.x)),
+                      uni_f(0.5f * (bmin.y + bmax.y)),
+                      D == 3 ? uni_f(0.5f * (bmin.z + bmax.z)) : 0.f };
+            const f3 hd = sub<D>(bmax, cen);
+            const float rb = sqrtf(dot<D>(hd, hd)) * 1.0001f + sqrtf(smax);
+            smaxl = uni_f(smax + rb * rb * 0x1p-14f);
+        }
+        rio = sub<D>(r_i, cen);
+        ai = dot<D>(rio, rio);
+    }
     int kl = 0;          // per-lane cursor: neighbour cell of a staged candidate
     uint32_t nw = 0;     // next word to load (the current one is nw - 1)
     uint32_t bw = 0;     // remaining bits of the current word
@@ -333,8 +369,30 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                     }
                     take = !(d2 > smax);
                 }
-                st = make_float4(rj.x, rj.y, rj.z, is_el ? 1.f : 0.f);
+                if constexpr (LOOSE) {
+                    const f3 rjo = sub<D>(rj, cen);
+                    float bj = rjo.x * rjo.x;
+                    bj = __builtin_fmaf(rjo.y, rjo.y, bj);
+                    if constexpr (D == 3) bj = __builtin_fmaf(rjo.z, rjo.z, bj);
+                    st = make_float4(rjo.x, rjo.y, rjo.z, is_el ? -bj : bj);
+                } else {
+                    st = make_float4(rj.x, rj.y, rj.z, is_el ? 1.f : 0.f);
+                }
             }
+            // the per-lane test of a staged candidate
+            auto accept = [&](const float4& c) {
+                if constexpr (LOOSE) {
+                    float t = rio.x * c.x;
+                    t = __builtin_fmaf(rio.y, c.y, t);
+                    if constexpr (D == 3) t = __builtin_fmaf(rio.z, c.z, t);
+                    const float s = __builtin_fmaf(-2.f, t, ai + fabsf(c.w));
+                    return !(s > smaxl);
+                } else {
+                    const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
+                    const float s = dot<D>(r_ij, r_ij);
+                    return !(s > smax);
+                }
+            };
             const uint64_t tm = __ballot(take);
             const uint32_t cnt = (uint32_t)__popcll(tm);
             uint32_t lo = 0, hi = 0;
@@ -352,22 +410,13 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint32_t c1 = cnt < 32u ? cnt : 32u;
 #pragma unroll 8
-                for (uint32_t k = 0; k < c1; k++) {
-                    const float4 c = L.stage[k];
-                    const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
-                    const float s = dot<D>(r_ij, r_ij);
-                    lo = (lo << 1) | (!(s > smax) ? 1u : 0u);
-                }
+                for (uint32_t k = 0; k < c1; k++)
+                    lo = (lo << 1) | (accept(L.stage[k]) ? 1u : 0u);
                 if (c1 < 32u) lo <<= 32u - c1;
                 if (cnt > 32u) {
 #pragma unroll 8
-                    for (uint32_t k = 32; k < cnt; k++) {
-                        const float4 c = L.stage[k];
-                        const f3 r_ij =
-                            sub<D>(f3{ c.x, c.y, c.z }, r_i);
-                        const float s = dot<D>(r_ij, r_ij);
-                        hi = (hi << 1) | (!(s > smax) ? 1u : 0u);
-                    }
+                    for (uint32_t k = 32; k < cnt; k++)
+                        hi = (hi << 1) | (accept(L.stage[k]) ? 1u : 0u);
                     if (cnt < 64u) hi <<= 64u - cnt;
                 }
                 if (!act) lo = hi = 0;
@@ -375,7 +424,9 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
                     // wall elements in range: flagged, then dropped
                     uint32_t el = 0, eh = 0;
                     for (uint32_t k = 0; k < cnt; k++) {
-                        if (L.stage[k].w == 0.f) continue;
+                        const float w = L.stage[k].w;
+                        if (LOOSE ? !(__float_as_uint(w) >> 31) : w == 0.f)
+                            continue;
                         if (k < 32u) el |= 0x80000000u >> k;
                         else eh |= 0x80000000u >> (k - 32u);
                     }
@@ -789,11 +840,20 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             // div_nr / sqrt_nr where the per-pair bounds hold (s == 0 or
             // s >= 2^-96, udr == 0 or |udr| >= T), the IEEE sequences for
             // the other pairs
+            // The relative-coordinate test (LOOSE) accepts a thin shell
+            // beyond the support.  Such a candidate is given zero mass here
+            // (the reference's test, s <= smax, as a select rather than a
+            // branch, so every load of the pair is issued at once): its
+            // terms are then +-0 (finite fields), and adding +-0 leaves the
+            // sums unchanged, since they are never -0 (they start at +0, or
+            // 1, and an exact cancellation rounds to +0).
+            constexpr bool LOOSE = (MATH != 1) && !PER;
             auto pair_fd = [&](uint32_t j, uint32_t kl) {
                 const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                 const float s = dot<D>(r_ij, r_ij);
                 const float qf = div_nr(sqrt_nr(s), RH);
-                const float m_j = ldf_o(m, j);
+                float m_j = ldf_o(m, j);
+                if (LOOSE) m_j = (s > smax) ? 0.f : m_j;
                 const float rho_j = ldf_o(rho, j);
                 const float p_j = ldf_o(p, j);
                 const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
@@ -842,7 +902,8 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                     const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                     const float s = dot<D>(r_ij, r_ij);
                     const float q = sqrtf(s) / H;
-                    const float m_j = ldf_o(m, j);
+                    float m_j = ldf_o(m, j);
+                    if (LOOSE) m_j = (s > smax) ? 0.f : m_j;
                     const float rho_j = ldf_o(rho, j);
                     const float p_j = ldf_o(p, j);
                     const f3 u_ij = sub<D>(ldv_o<D>(u, j), u_i);
@@ -852,13 +913,13 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             };
             if constexpr (MATH != 1) {
                 if (fastdiv) {
-                    ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
-                                       smax, r, imove, 0, flag_el, nearw,
-                                       pair_fd, w.per);
+                    ring_walk<D, SEGW, 1, LOOSE>(
+                        L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                        0, flag_el, nearw, pair_fd, w.per);
                 } else {
-                    ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax,
-                                       smax, r, imove, 0, flag_el, nearw,
-                                       pair_ieee, w.per);
+                    ring_walk<D, SEGW, 1, LOOSE>(
+                        L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
+                        0, flag_el, nearw, pair_ieee, w.per);
                 }
             } else {
                 ring_walk<D, SEGW>(L, lane, total, r_i, act, bmin, bmax, smax,
@@ -871,11 +932,12 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 bool unused = false;
                 const uint32_t total_el =
                     neighbour_table<D, PER>(w, c_i, L, lane, AQ_CELL_ELEM);
-                ring_walk<D, SEGW>(
+                ring_walk<D, SEGW, 1, LOOSE>(
                     L, lane, total_el, r_i, act && nearw, bmin, bmax, smax,
                     r, imove, 1, false, unused, [&](uint32_t j, uint32_t kl) {
                         const f3 r_ij = sub<D>(rjof(j, kl), r_i);
                         const float s = dot<D>(r_ij, r_ij);
+                        if (LOOSE && s > smax) return;
                         const float q = sqrtf(s) / H;
                         const f3 n_j = ldv_o<D>(normal, j);
                         const f3 t_j = ldv_o<D>(tangent, j);
