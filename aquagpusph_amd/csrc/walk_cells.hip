@@ -1373,8 +1373,19 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     const uint32_t n_heads =
         olist ? uni(w.n_heads[count_word]) : uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
+    // static hand-out (Pol::STATIC_CELLS): wave v takes the cells
+    // v, v + W, v + 2W, ... (no atomic per cell), for lists whose cells cost
+    // alike
+    const uint32_t nwv = gridDim.x * WC_WAVES;
+    uint32_t hs = blockIdx.x * WC_WAVES + (uint32_t)wv;
     while (true) {
-        const uint32_t hk = next_cell(w, n_heads, lane, range);
+        uint32_t hk;
+        if (Pol::STATIC_CELLS) {
+            hk = hs;
+            hs += nwv;
+        } else {
+            hk = next_cell(w, n_heads, lane, range);
+        }
         if (hk >= n_heads) break;
         uint32_t cstart, c_i, ce;
         if (olist) {
@@ -1481,6 +1492,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
 // (skip_fluid) or all of them
 template <int D> struct PolBiShepard {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr bool STATIC_CELLS = true;  // coop walk hand-out
     static constexpr int COOP_B = 1;  // per lane and round (coop walk)
     static constexpr int KIND = 1;
     static constexpr uint32_t OWNERS = 0;  // elements, sensors, ...
@@ -1526,15 +1538,15 @@ template <int D> struct PolBiShepard {
         if (i == j) return Term{ 0.f, 1u };
         const float eps = 1E-8f * H;
         const f3 r_ij = sub<D>(c.r, r_i);
-        const float s = dot<D>(r_ij, r_ij);
-        const float q = qof<NRQ>(s, H);
         const float r_n = dot<D>(r_ij, c.n);
         const float r_t = fabsf(dot<D>(r_ij, c.t));
         const float r_b = fabsf(dot<D>(r_ij, c.b));
         if ((r_n > -eps) && (r_n < eps)) {
+            // a coplanar element (most candidates on a flat wall): no q
             const float dr = 0.55f * pow_cl(c.area, 1.f / (DIMS - 1.f));
             return Term{ 0.f, ((r_t <= dr) && (r_b <= dr)) ? 1u : 2u };
         }
+        const float q = qof<NRQ>(dot<D>(r_ij, r_ij), H);
         return Term{ r_n * CONW * kernelS_P<D>(q) * c.area +
                          kernelS_D<D>(fabsf(r_n), r_t, r_b, c.area),
                      0u };
@@ -1559,6 +1571,7 @@ template <int D> struct PolBiShepard {
 // cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
 template <int D> struct PolBiLapU {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr bool STATIC_CELLS = false;  // coop walk hand-out
     static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
@@ -1630,6 +1643,7 @@ template <int D> struct PolBiLapU {
 // cfd/Boundary/BI/Interpolation.cl:54-107
 template <int D> struct PolBiInterp {
     static constexpr int BATCH = 4;  // candidates loaded together
+    static constexpr bool STATIC_CELLS = false;  // coop walk hand-out
     static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
