@@ -8,6 +8,7 @@
 // capacity, so the whole link-list can be replayed from a HIP graph.  An
 // overflowing grid sets bit 0 of a sticky status word and the later kernels
 // skip their work; the caller grows the capacity and re-runs.
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -378,6 +379,77 @@ __global__ void owner_wet_k(const uint32_t* __restrict__ ocells,
     }
 }
 
+// Blocked order of the non-empty cells for the fused walk: the cells are
+// bucketed by blocks of BLK^d grid cells (8^3 in 3D, 16^2 in 2D) into
+// bcells, as (first particle, cell, cell_end) triplets, so the cells handed
+// out together lie in one compact region and the neighbourhoods in flight
+// on one XCD share its L2 (the row-major cell list puts a few long rows of
+// cells in flight instead).  Within a block the order is arbitrary; the
+// walk's sums do not depend on the order in which cells are visited.
+template <int D> struct Blk {
+    static constexpr uint32_t B = (D == 2) ? 16u : 8u;
+};
+template <int D>
+__device__ __forceinline__ uint32_t cell_block(uint32_t c,
+                                               const uint32_t* n_cells,
+                                               uint32_t nbmax)
+{
+    constexpr uint32_t B = Blk<D>::B;
+    const uint32_t nx = n_cells[0], ny = n_cells[1];
+    const uint32_t cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+    const uint32_t nbx = (nx + B - 1) / B, nby = (ny + B - 1) / B;
+    const uint32_t b = (cz / B * nby + cy / B) * nbx + cx / B;
+    return b < nbmax ? b : nbmax - 1u;
+}
+
+template <int D>
+__global__ void blk_count_k(const uint32_t* __restrict__ heads,
+                            const uint32_t* __restrict__ n_heads,
+                            const uint32_t* __restrict__ icell,
+                            const uint32_t* __restrict__ n_cells,
+                            uint32_t* __restrict__ bcount, uint32_t nbmax,
+                            const uint32_t* status)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (status && (*status & 1u)) return;
+    if (k >= *n_heads) return;
+    atomicAdd(bcount + cell_block<D>(icell[heads[k]], n_cells, nbmax), 1u);
+}
+
+// bcount counts down to zero again, ready for the next step
+template <int D>
+__global__ void blk_scatter_k(const uint32_t* __restrict__ heads,
+                              const uint32_t* __restrict__ n_heads,
+                              const uint32_t* __restrict__ icell,
+                              const uint32_t* __restrict__ n_cells,
+                              const uint32_t* __restrict__ cell_end,
+                              uint32_t* __restrict__ bcount,
+                              const uint32_t* __restrict__ bbase,
+                              uint32_t nbmax, uint32_t* __restrict__ bcells,
+                              const uint32_t* status)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (status && (*status & 1u)) return;
+    if (k >= *n_heads) return;
+    const uint32_t start = heads[k];
+    const uint32_t c = icell[start];
+    const uint32_t b = cell_block<D>(c, n_cells, nbmax);
+    const uint32_t pos = bbase[b] + atomicSub(bcount + b, 1u) - 1u;
+    bcells[3 * pos] = start;
+    bcells[3 * pos + 1] = c;
+    bcells[3 * pos + 2] = cell_end[c];
+}
+
+// Blocks of the blocked cell order a grid of `cap` cells can hold (a bound
+// on the product of ceil(n_i / B) over the axes).
+size_t cell_blocks_bound(int dims, size_t cap)
+{
+    const double c = (double)cap;
+    if (dims == 2) return (size_t)(c / 256.0 + 2.0 * std::sqrt(c) / 16.0 + 2.0);
+    return (size_t)(c / 512.0 + std::pow(c, 2.0 / 3.0) * 3.0 / 64.0 +
+                    std::cbrt(c) * 3.0 / 8.0 + 2.0);
+}
+
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
                     uint32_t* sums, hipStream_t s);
 size_t scan_workspace(uint32_t m);
@@ -396,7 +468,9 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
                hipStream_t s, uint32_t* ocells, uint32_t* wcells, int dims,
-               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap)
+               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap,
+               uint32_t* bcells, uint32_t* bcount, uint32_t* bbase,
+               uint32_t* bsums, uint32_t nbmax)
 {
     const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
     uint32_t* head = (uint32_t*)ws;
@@ -418,6 +492,25 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
         else
             hipLaunchKernelGGL(owner_wet_k<3>, gw, b, 0, s, ocells, n_heads,
                                ihoc, cell_end, n_cells, N, wcells, status);
+    }
+    if (bcells && ocap && nbmax) {
+        // the blocked order (after the kind flags, which it copies)
+        const dim3 gb(blocks_for(ocap, 256));
+        if (dims == 2)
+            hipLaunchKernelGGL(blk_count_k<2>, gb, b, 0, s, heads, n_heads,
+                               icell, n_cells, bcount, nbmax, status);
+        else
+            hipLaunchKernelGGL(blk_count_k<3>, gb, b, 0, s, heads, n_heads,
+                               icell, n_cells, bcount, nbmax, status);
+        exclusive_scan(bcount, bbase, nbmax, bsums, s);
+        if (dims == 2)
+            hipLaunchKernelGGL(blk_scatter_k<2>, gb, b, 0, s, heads, n_heads,
+                               icell, n_cells, cell_end, bcount, bbase, nbmax,
+                               bcells, status);
+        else
+            hipLaunchKernelGGL(blk_scatter_k<3>, gb, b, 0, s, heads, n_heads,
+                               icell, n_cells, cell_end, bcount, bbase, nbmax,
+                               bcells, status);
     }
 }
 

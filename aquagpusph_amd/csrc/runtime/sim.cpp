@@ -54,6 +54,8 @@ Sim::~Sim()
     for (void* b : { (void*)fusion.near_buf, (void*)clist.cell_end,
                      (void*)clist.heads, (void*)clist.ocells,
                      (void*)clist.wcells, (void*)clist.nlist,
+                     (void*)clist.bcells, (void*)clist.bcount,
+                     (void*)clist.bbase, (void*)clist.bsums,
                      (void*)clist.n_heads, clist.ws,
                      (void*)sel_list, (void*)sel_count })
         if (b) (void)hipFree(b);
@@ -298,6 +300,10 @@ void Sim::setup()
     const char* bw = getenv("AQ_BI_WALK");
     bi_cells = clist.enabled && !(bw && strcmp(bw, "thread") == 0);
     bi_group = bi_cells && !(bw && strcmp(bw, "cells") == 0);
+    // the fused walk hands its cells out in blocked order unless
+    // AQ_WALK_ORDER=rows asks for the row-major cell list
+    const char* wo = getenv("AQ_WALK_ORDER");
+    clist.blocked = clist.enabled && !(wo && strcmp(wo, "rows") == 0);
     if (clist.enabled && N && !sel_list) {
         AQ_HIP(hipMalloc(&sel_list, (size_t)N * sizeof(uint32_t)));
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
@@ -564,7 +570,7 @@ void launch_fluid_pass_cells(int dims, int shep, bool lapp,
                              int segw, int flagged, int math,
                              unsigned ranges, int fast_div, hipStream_t s,
                              const float* per, unsigned reserve_per_cu,
-                             uint32_t* nlist);
+                             uint32_t* nlist, const uint32_t* blist);
 unsigned side_wgs_per_cu();
 void launch_shepard(int, const int32_t*, const float*, const float*,
                     const float*, float*, uint32_t, const uint32_t*,
@@ -687,7 +693,8 @@ void Sim::run_fused(hipStream_t s)
             f.walk_segw, clist.kinds ? 1 : 0, f.math, f.walk_ranges,
             f.fast_div ? 1 : 0, s, periodic_L(),
             f.side_launched ? side_wgs_per_cu() : 0u,
-            (bi_group && clist.kinds) ? clist.nlist : nullptr);
+            (bi_group && clist.kinds) ? clist.nlist : nullptr,
+            clist.blocked ? clist.bcells : nullptr);
         if (f.timing) AQ_HIP(hipEventRecord(f.ev1, s));
         return;
     }

@@ -283,6 +283,12 @@ struct CellList {
     uint32_t* ocells = nullptr;   // owner-cell triplets (cell_kinds_k)
     uint32_t* wcells = nullptr;   // wet owner cells (owner_wet_k)
     uint32_t* nlist = nullptr;    // near cells (the fused walk)
+    // the non-empty cells in blocked order (the fused walk's hand-out),
+    // with the per-block counters, bases and scan sums
+    bool blocked = false;
+    uint32_t* bcells = nullptr;
+    uint32_t *bcount = nullptr, *bbase = nullptr, *bsums = nullptr;
+    uint32_t nbmax = 0;
     uint32_t* n_heads = nullptr;  // [0] count, [32..] hand-out counters
     void* ws = nullptr;
     bool usable(const Var* ic, const Var* ih) const

@@ -26,7 +26,11 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
                uint32_t* heads, uint32_t* n_heads, const uint32_t* status,
                void* ws, const uint32_t* perm, const int* imove,
                hipStream_t s, uint32_t* ocells, uint32_t* wcells, int dims,
-               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap);
+               const uint32_t* ihoc, const uint32_t* n_cells, size_t ocap,
+               uint32_t* bcells, uint32_t* bcount, uint32_t* bbase,
+               uint32_t* bsums, uint32_t nbmax);
+size_t cell_blocks_bound(int dims, size_t cap);
+size_t scan_workspace(uint32_t m);
 void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
                 unsigned key_bits, void* workspace, hipStream_t s);
 size_t radix_sort_workspace(uint32_t n);
@@ -563,11 +567,23 @@ class LinkListTool : public Tool {
                 AQ_HIP(hipMalloc(&c.cell_end, ihoc_->length * 4));
                 c.cap = ihoc_->length;
                 // owner-cell triplets: at most one per non-empty cell
-                for (uint32_t** l : { &c.ocells, &c.wcells, &c.nlist }) {
+                for (uint32_t** l : { &c.ocells, &c.wcells, &c.nlist,
+                                      &c.bcells }) {
                     if (*l) AQ_HIP(hipFree(*l));
                     *l = nullptr;
                     AQ_HIP(hipMalloc(l, std::min<size_t>(S.N, c.cap) * 12));
                 }
+                // the blocked order's per-block counters (zero between
+                // steps) and bases
+                for (uint32_t** l : { &c.bcount, &c.bbase, &c.bsums }) {
+                    if (*l) AQ_HIP(hipFree(*l));
+                    *l = nullptr;
+                }
+                c.nbmax = (uint32_t)cell_blocks_bound(S.dims, c.cap);
+                AQ_HIP(hipMalloc(&c.bcount, (size_t)c.nbmax * 4));
+                AQ_HIP(hipMemset(c.bcount, 0, (size_t)c.nbmax * 4));
+                AQ_HIP(hipMalloc(&c.bbase, (size_t)c.nbmax * 4));
+                AQ_HIP(hipMalloc(&c.bsums, scan_workspace(c.nbmax) + 256));
             }
             if (!c.heads) {
                 AQ_HIP(hipMalloc(&c.heads, (size_t)S.N * 4));
@@ -588,7 +604,9 @@ class LinkListTool : public Tool {
                       kinds ? c.ocells : nullptr, c.wcells, S.dims,
                       (const uint32_t*)ihoc_->dev,
                       (const uint32_t*)S.ptr(nc_),
-                      std::min<size_t>(S.N, c.cap));
+                      std::min<size_t>(S.N, c.cap),
+                      c.blocked ? c.bcells : nullptr, c.bcount, c.bbase,
+                      c.bsums, c.nbmax);
             c.kinds = kinds;
             c.filled = true;
         }
