@@ -81,6 +81,7 @@ struct CellArgs {
     const uint32_t* ocells = nullptr;
     const uint32_t* wcells = nullptr;  // the wet owner cells
     const uint32_t* nlist = nullptr;   // the near cells (fused walk)
+    const uint32_t* blist = nullptr;   // every non-empty cell, blocked order
     // word of the n_heads buffer where the walk's hand-out counters start
     // (0: word 32, the main stream's)
     uint32_t next_off = 0;

@@ -602,6 +602,7 @@ CellArgs Sim::cell_args()
                      (bi_group && clist.kinds && fusion.active && fusion.near)
                          ? clist.nlist
                          : nullptr,
+                     (clist.blocked && clist.kinds) ? clist.bcells : nullptr,
                      in_side ? AQ_CL_SIDE : 0u };
 }
 

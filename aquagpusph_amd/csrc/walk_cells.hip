@@ -501,30 +501,7 @@ __device__ __forceinline__ void ring_walk(WaveLds<SEGW, PER>& L, int lane,
     __builtin_amdgcn_wave_barrier();
 }
 
-// ---------------------------------------------------------------------------
-// Stream walk: ring_walk with the survivors stored contiguously.
-//
-// ring_walk keeps each staged chunk's survivors at their chunk's candidate
-// positions (about half of every 64-entry chunk stays empty) and refills a
-// whole segment once every lane has left the oldest: its 2 SEGW-word ring
-// holds about 256 survivors, of which the lanes may be up to one segment
-// apart.  The accepted candidates of different lanes of a cell lie up to
-// about 800 list positions apart (the owners near the cell's -x face accept
-// from the x - 1 neighbour cells at the list's head, those near its +x face
-// from the x + 1 cells at its tail), so the lanes run out of tested bits at
-// different times and the consume loop idles a third of its lanes (a model
-// of an interior lattice cell: 354 consume steps for 251 pairs per lane).
-// Here survivor s of the walk sits at ring entry s % (64 SEGW): the ring
-// holds 512 survivors, each chunk is staged as soon as its three words are
-// free (no lane still reads them), and the lanes consume whenever no chunk
-// can be staged (the model: 252 steps).  A lane's bits of the word being
-// filled stay in a register until the word is complete; the last, partial
-// word is written once the list is staged.  Every lane meets its accepted
-// candidates in increasing candidate order, as in ring_walk.
-// ---------------------------------------------------------------------------
-#ifndef AQ_STREAM_BATCH
-#define AQ_STREAM_BATCH 4
-#endif
+// The fused walk's form of ring_walk (one candidate per consume step).
 template <int D, int SEGW, bool LOOSE = false, bool PER, class Pair>
 __device__ __forceinline__ void ring_walk_l(WaveLds<SEGW, PER>& L, int lane,
                                             uint32_t total, const f3& r_i,
@@ -539,215 +516,6 @@ __device__ __forceinline__ void ring_walk_l(WaveLds<SEGW, PER>& L, int lane,
     ring_walk<D, SEGW, 1, LOOSE>(L, lane, total, r_i, act, bmin, bmax, smax,
                                  r, imove, kind, flag_el, el_near, pair, per);
 }
-#ifndef AQ_FLUID_STREAM
-#define FLUID_WALK ring_walk_l
-#else
-#define FLUID_WALK stream_walk
-#endif
-template <int D, int SEGW, bool LOOSE = false, bool PER, class Pair>
-__device__ __forceinline__ void stream_walk(WaveLds<SEGW, PER>& L, int lane,
-                                            uint32_t total, const f3& r_i,
-                                            bool act, const f3& bmin,
-                                            const f3& bmax, float smax,
-                                            const float* __restrict__ r,
-                                            const int* __restrict__ imove,
-                                            int kind, bool flag_el,
-                                            bool& el_near, Pair pair,
-                                            const float* __restrict__ per)
-{
-    constexpr uint32_t RW = 2 * SEGW;  // ring words (32 survivors each)
-    uint32_t* bitr = &L.bits[0][0][0];  // word w: bitr[(w % RW) 64 + lane]
-    uint32_t* candr = &L.cand[0][0];    // survivor s: candr[s % (32 RW)]
-    static_assert(!(LOOSE && PER), "relative test: not for periodic walks");
-    f3 cen = { 0.f, 0.f, 0.f }, rio = { 0.f, 0.f, 0.f };
-    float ai = 0.f, smaxl = smax;
-    if constexpr (LOOSE) {
-        if (bmin.x <= bmax.x) {
-            cen = f3{ uni_f(0.5f * (bmin.x + bmax.x)),
-                      uni_f(0.5f * (bmin.y + bmax.y)),
-                      D == 3 ? uni_f(0.5f * (bmin.z + bmax.z)) : 0.f };
-            const f3 hd = sub<D>(bmax, cen);
-            const float rb = sqrtf(dot<D>(hd, hd)) * 1.0001f + sqrtf(smax);
-            smaxl = uni_f(smax + rb * rb * 0x1p-14f);
-        }
-        rio = sub<D>(r_i, cen);
-        ai = dot<D>(rio, rio);
-    }
-    int kl = 0;            // per-lane cursor: neighbour cell of a candidate
-    uint32_t cpos = 0;     // next candidate to stage (uniform)
-    uint32_t fill = 0;     // survivors staged (uniform)
-    uint32_t wavail = 0;   // words the lanes may read (uniform)
-    uint32_t lo_w = 0;     // no lane reads a word below this (uniform)
-    uint32_t pend = 0;     // this lane's bits of the word being filled
-    uint32_t nw = 0;       // next word to read (the current one is nw - 1)
-    uint32_t bw = 0;       // remaining bits of the current word
-    bool eln = false;
-    auto advance = [&]() {
-        while (bw == 0 && nw < wavail) {
-            bw = bitr[(nw % RW) * 64 + lane];
-            nw++;
-        }
-    };
-    // room for k more chunks (a chunk may touch three words: fill >> 5 ..
-    // (fill + 63) >> 5)
-    auto room = [&](uint32_t k) {
-        return cpos < total && ((fill + 64u * k - 1u) >> 5) < lo_w + RW;
-    };
-    // consume while some lane has a candidate, until STREAM_BATCH chunks
-    // can be staged at once
-    auto go = [&]() {
-        advance();
-        while (lo_w < wavail && __ballot(bw != 0 && nw - 1u == lo_w) == 0)
-            lo_w++;
-        if (room(AQ_STREAM_BATCH)) return false;
-        return __ballot(bw != 0) != 0;
-    };
-    while (true) {
-        // the test sits at the bottom of the consume loop, so the pair sums
-        // leave it from where they are updated (no copies at the back edge)
-        for (bool more = go(); more; more = go()) {
-            if (bw != 0) {
-                const uint32_t b = __builtin_clz(bw);
-                bw ^= 0x80000000u >> b;
-                const uint32_t cw = candr[((nw - 1) % RW) * 32 + b];
-                if constexpr (PER)
-                    pair(cw & PER_J_MASK, cw >> PER_KL_SHIFT);
-                else
-                    pair(cw, 0u);
-            }
-        }
-        if (!room(1)) break;  // every candidate staged and consumed
-        do {
-            // stage and test the candidates cpos .. cpos + 63
-            const uint32_t t = cpos + lane;
-            float4 st = make_float4(0.f, 0.f, 0.f, 0.f);
-            bool take = false, is_el = false;
-            uint32_t j = 0;
-            if (t < total) {
-                while (t >= L.cum[kl + 1]) kl++;
-                j = L.start[kl] + (t - L.cum[kl]);
-                f3 rj = ldv_o<D>(r, j);
-                if constexpr (PER) {
-                    const float4 sh = L.shift[kl];
-                    rj = per_shift<D>(rj, f3{ sh.x, sh.y, sh.z }, per);
-                }
-                const int im_j = ldi_o(imove, j);
-                is_el = (im_j == -3);
-                take = kind ? is_el : (im_j == 1 || (flag_el && is_el));
-                if (take) {
-                    const float dx =
-                        fmaxf(fmaxf(bmin.x - rj.x, rj.x - bmax.x), 0.f);
-                    const float dy =
-                        fmaxf(fmaxf(bmin.y - rj.y, rj.y - bmax.y), 0.f);
-                    float d2 = dx * dx;
-                    d2 = d2 + dy * dy;
-                    if constexpr (D == 3) {
-                        const float dz =
-                            fmaxf(fmaxf(bmin.z - rj.z, rj.z - bmax.z), 0.f);
-                        d2 = d2 + dz * dz;
-                    }
-                    take = !(d2 > smax);
-                }
-                if constexpr (LOOSE) {
-                    const f3 rjo = sub<D>(rj, cen);
-                    float bj = rjo.x * rjo.x;
-                    bj = __builtin_fmaf(rjo.y, rjo.y, bj);
-                    if constexpr (D == 3) bj = __builtin_fmaf(rjo.z, rjo.z, bj);
-                    st = make_float4(rjo.x, rjo.y, rjo.z, is_el ? -bj : bj);
-                } else {
-                    st = make_float4(rj.x, rj.y, rj.z, is_el ? 1.f : 0.f);
-                }
-            }
-            auto accept = [&](const float4& c) {
-                if constexpr (LOOSE) {
-                    float tt = rio.x * c.x;
-                    tt = __builtin_fmaf(rio.y, c.y, tt);
-                    if constexpr (D == 3) tt = __builtin_fmaf(rio.z, c.z, tt);
-                    const float s = __builtin_fmaf(-2.f, tt, ai + fabsf(c.w));
-                    return !(s > smaxl);
-                } else {
-                    const f3 r_ij = sub<D>(f3{ c.x, c.y, c.z }, r_i);
-                    const float s = dot<D>(r_ij, r_ij);
-                    return !(s > smax);
-                }
-            };
-            const uint64_t tm = __ballot(take);
-            const uint32_t cnt = (uint32_t)__popcll(tm);
-            uint32_t lo = 0, hi = 0;
-            if (cnt != 0) {
-                // the survivors go to stage[0 .. cnt) and to the ring entries
-                // fill .. fill + cnt - 1, in order
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(tm >> 32),
-                    __builtin_amdgcn_mbcnt_lo((uint32_t)tm, 0u));
-                if (take) {
-                    L.stage[rank] = st;
-                    candr[(fill + rank) % (32 * RW)] =
-                        PER ? (j | ((uint32_t)kl << PER_KL_SHIFT)) : j;
-                }
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_s_waitcnt(0);
-                const uint32_t c1 = cnt < 32u ? cnt : 32u;
-    #pragma unroll 8
-                for (uint32_t k = 0; k < c1; k++)
-                    lo = (lo << 1) | (accept(L.stage[k]) ? 1u : 0u);
-                if (c1 < 32u) lo <<= 32u - c1;
-                if (cnt > 32u) {
-    #pragma unroll 8
-                    for (uint32_t k = 32; k < cnt; k++)
-                        hi = (hi << 1) | (accept(L.stage[k]) ? 1u : 0u);
-                    if (cnt < 64u) hi <<= 64u - cnt;
-                }
-                if (!act) lo = hi = 0;
-                if (!kind && __ballot(take && is_el) != 0) {
-                    // wall elements in range: flagged, then dropped
-                    uint32_t el = 0, eh = 0;
-                    for (uint32_t k = 0; k < cnt; k++) {
-                        const float w = L.stage[k].w;
-                        if (LOOSE ? !(__float_as_uint(w) >> 31) : w == 0.f)
-                            continue;
-                        if (k < 32u) el |= 0x80000000u >> k;
-                        else eh |= 0x80000000u >> (k - 32u);
-                    }
-                    eln = eln || ((lo & el) | (hi & eh)) != 0;
-                    lo &= ~el;
-                    hi &= ~eh;
-                }
-                // append the cnt bits (lo, hi; MSB first) after the o bits of
-                // the word being filled
-                const uint32_t o = fill & 31u;
-                const uint32_t w0 = pend | (o ? (lo >> o) : lo);
-                const uint32_t nb = o + cnt;
-                if (nb >= 32u) {
-                    bitr[((fill >> 5) % RW) * 64 + lane] = w0;
-                    const uint32_t w1 =
-                        (o ? (lo << (32u - o)) : 0u) | (o ? (hi >> o) : hi);
-                    if (nb >= 64u) {
-                        bitr[(((fill >> 5) + 1u) % RW) * 64 + lane] = w1;
-                        pend = o ? (hi << (32u - o)) : 0u;
-                    } else {
-                        pend = w1;
-                    }
-                } else {
-                    pend = w0;
-                }
-                fill += cnt;
-            }
-            cpos += 64;
-            wavail = fill >> 5;
-            if (cpos >= total && (fill & 31u)) {
-                // the list is staged: its last word is final
-                bitr[((fill >> 5) % RW) * 64 + lane] = pend;
-                wavail = (fill + 31u) >> 5;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_s_waitcnt(0);
-        } while (room(1));  // fill the ring before consuming again
-    }
-    el_near = eln;
-    __builtin_amdgcn_wave_barrier();
-}
-
 // ---------------------------------------------------------------------------
 // Fused fluid walk (same terms as fluid_pass_k in sph.hip).
 //   MATH 0: IEEE-754 terms and sums, bit-identical to the oracle;
@@ -1171,16 +939,16 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
             };
             if constexpr (MATH != 1) {
                 if (fastdiv) {
-                    FLUID_WALK<D, SEGW, LOOSE>(
+                    ring_walk_l<D, SEGW, LOOSE>(
                         L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
                         0, flag_el, nearw, pair_fd, w.per);
                 } else {
-                    FLUID_WALK<D, SEGW, LOOSE>(
+                    ring_walk_l<D, SEGW, LOOSE>(
                         L, lane, total, r_i, act, bmin, bmax, smax, r, imove,
                         0, flag_el, nearw, pair_ieee, w.per);
                 }
             } else {
-                FLUID_WALK<D, SEGW, false>(L, lane, total, r_i, act, bmin,
+                ring_walk_l<D, SEGW, false>(L, lane, total, r_i, act, bmin,
                                            bmax, smax, r, imove, 0, flag_el,
                                            nearw, pair_ieee, w.per);
             }
@@ -1190,7 +958,7 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
                 bool unused = false;
                 const uint32_t total_el =
                     neighbour_table<D, PER>(w, c_i, L, lane, AQ_CELL_ELEM);
-                FLUID_WALK<D, SEGW, LOOSE>(
+                ring_walk_l<D, SEGW, LOOSE>(
                     L, lane, total_el, r_i, act && nearw, bmin, bmax, smax,
                     r, imove, 1, false, unused, [&](uint32_t j, uint32_t kl) {
                         const f3 r_ij = sub<D>(rjof(j, kl), r_i);
@@ -2355,6 +2123,11 @@ static void launch_cells(const Pol& pol, const int32_t* imove, const float* r,
     WalkArgs w{ c.icell,   c.ihoc,   c.n_cells, c.cell_end, c.heads,
                 c.n_heads, next,     c.status,  c.N,        c.flagged,
                 walk_ranges(), per };
+    if (!olist && c.blist) {
+        // every non-empty cell, in blocked order (n_heads[0] of them)
+        olist = c.blist;
+        count_word = 0;
+    }
     auto* ker = cell_kernel_k<D, 8, Pol, PER, NB>;
     const unsigned nb = walk_grid((const void*)ker, c);
     const int nrq = (pol.H >= 0x1p-40f && pol.H <= 0x1p40f) ? 1 : 0;
