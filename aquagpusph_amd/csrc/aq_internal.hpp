@@ -50,10 +50,23 @@ class Error : public std::runtime_error {
 #define AQ_CELL_ELEM 0x80000000u  // the cell holds wall elements (imove == -3)
 #define AQ_CELL_OTHER 0x20000000u // the cell holds other particles (sensors,
                                   // ghosts, mirrors: imove not 1 nor -3)
-// Word of the n_heads buffer holding the number of cells in the owner-cell
-// list (cells flagged AQ_CELL_ELEM or AQ_CELL_OTHER; the list holds their
-// first particle), filled with the kind flags.
+// Word of the n_heads buffer holding the number of entries in the
+// owner-cell list (cells flagged AQ_CELL_ELEM or AQ_CELL_OTHER, as (first
+// particle, cell, end particle with the cell's flags) triplets), filled with
+// the kind flags.  A cell with more than AQ_OWNER_CHUNK non-fluid particles
+// (a wall edge or corner) is listed as several entries, each a run of its
+// particles holding at most AQ_OWNER_CHUNK of them, so no entry takes more
+// than one pass of the cooperative element walks (16 owners) and no single
+// heavy cell lengthens their tail.
 #define AQ_CL_NOCELLS 512u
+#define AQ_OWNER_CHUNK 16u
+// Capacity of the owner-cell lists (entries) for a bound `cells` on the
+// non-empty cells and N particles: one entry per cell plus one per
+// AQ_OWNER_CHUNK non-fluid particles past the first chunk of a cell.
+inline size_t aq_owner_list_cap(size_t cells, size_t N)
+{
+    return cells + N / AQ_OWNER_CHUNK + 1;
+}
 // Word of the n_heads buffer counting the wet owner cells (owner cells with
 // fluid in their neighbourhood; a second list of the same triplets).
 #define AQ_CL_NWCELLS 513u

@@ -275,9 +275,12 @@ __global__ void cell_list_k(const uint32_t* __restrict__ head,
 // read through perm (id_unsorted): the cell list is built right after the
 // sort, before the fields are reordered.  The cells with elements or other
 // particles are appended to the owner-cell list as (first particle, cell,
-// cell_end with the flags) triplets (one atomic per wavefront, so the list
-// is ordered within each wavefront's 64 cells and the blocks of 64 in no
-// set order).
+// end particle with the cell's flags) triplets: the cells of at most
+// AQ_OWNER_CHUNK non-fluid particles as one entry (one atomic per
+// wavefront, so the list is ordered within each wavefront's 64 cells and
+// the blocks of 64 in no set order), the others as one entry per run of
+// particles ending at every AQ_OWNER_CHUNK-th non-fluid one (the last run
+// ends at the cell's end), reserved with one atomic per cell.
 __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
                              uint32_t* __restrict__ n_heads,
                              const uint32_t* __restrict__ icell,
@@ -290,7 +293,7 @@ __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nh = *n_heads;
     if (status && (*status & 1u)) return;  // uniform
-    uint32_t f = 0, start = 0, c = 0, end = 0;
+    uint32_t f = 0, start = 0, c = 0, end = 0, nonf = 0;
     if (k < nh) {
         start = heads[k];
         c = icell[start];
@@ -299,12 +302,33 @@ __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
             const int im = imove[perm[i]];
             f |= (im == 1) ? AQ_CELL_FLUID
                            : ((im == -3) ? AQ_CELL_ELEM : AQ_CELL_OTHER);
+            nonf += (im != 1) ? 1u : 0u;
         }
         cell_end[c] = end | f;
     }
     if (!ocells) return;
     const bool own = (f & (AQ_CELL_ELEM | AQ_CELL_OTHER)) != 0u;
-    const uint64_t bm = __ballot(own);
+    if (own && nonf > AQ_OWNER_CHUNK) {
+        // runs of AQ_OWNER_CHUNK non-fluid particles
+        const uint32_t nch = (nonf + AQ_OWNER_CHUNK - 1u) / AQ_OWNER_CHUNK;
+        uint32_t e = atomicAdd(n_heads + AQ_CL_NOCELLS, nch);
+        const uint32_t elast = e + nch - 1u;
+        uint32_t s0 = start, cnt = 0;
+        for (uint32_t i = start; i < end && e < elast; i++) {
+            if (imove[perm[i]] == 1) continue;
+            if (++cnt < AQ_OWNER_CHUNK) continue;
+            ocells[3 * e] = s0;
+            ocells[3 * e + 1] = c;
+            ocells[3 * e + 2] = (i + 1u) | f;
+            e++;
+            s0 = i + 1u;
+            cnt = 0;
+        }
+        ocells[3 * e] = s0;
+        ocells[3 * e + 1] = c;
+        ocells[3 * e + 2] = end | f;
+    }
+    const uint64_t bm = __ballot(own && nonf <= AQ_OWNER_CHUNK);
     if (bm == 0) return;
     const int lane = threadIdx.x & 63;
     uint32_t base = 0;
@@ -313,7 +337,7 @@ __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
     base = __shfl(base, __builtin_ctzll(bm));
     const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
         (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-    if (own) {
+    if (own && nonf <= AQ_OWNER_CHUNK) {
         const uint32_t o = 3u * (base + rank);
         ocells[o] = start;
         ocells[o + 1] = c;
@@ -321,8 +345,8 @@ __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
     }
 }
 
-// Wet owner cells: the owner cells (ocells) with fluid in their 3^d
-// neighbourhood, appended to wcells as the same triplets (count at
+// Wet owner cells: the owner-list entries (ocells) whose cell has fluid in
+// its 3^d neighbourhood, appended to wcells as the same triplets (count at
 // n_heads[AQ_CL_NWCELLS]).  The element kernels whose candidates are fluid
 // (BI LapU, BI Interpolation) have no term to add in the other cells.
 template <int D>
@@ -485,7 +509,7 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
         hipLaunchKernelGGL(cell_kinds_k, g, b, 0, s, heads, n_heads, icell,
                            perm, imove, cell_end, status, ocells);
     if (perm && imove && ocells && wcells && ocap) {
-        const dim3 gw(blocks_for(ocap, 256));
+        const dim3 gw(blocks_for(aq_owner_list_cap(ocap, N), 256));
         if (dims == 2)
             hipLaunchKernelGGL(owner_wet_k<2>, gw, b, 0, s, ocells, n_heads,
                                ihoc, cell_end, n_cells, N, wcells, status);

@@ -566,12 +566,17 @@ class LinkListTool : public Tool {
                 c.cell_end = nullptr;
                 AQ_HIP(hipMalloc(&c.cell_end, ihoc_->length * 4));
                 c.cap = ihoc_->length;
-                // owner-cell triplets: at most one per non-empty cell
+                // cell triplets: at most one per non-empty cell, and the
+                // owner lists one more per AQ_OWNER_CHUNK non-fluid
+                // particles of the cells split into runs
+                const size_t ncap = std::min<size_t>(S.N, c.cap);
                 for (uint32_t** l : { &c.ocells, &c.wcells, &c.nlist,
                                       &c.bcells }) {
                     if (*l) AQ_HIP(hipFree(*l));
                     *l = nullptr;
-                    AQ_HIP(hipMalloc(l, std::min<size_t>(S.N, c.cap) * 12));
+                    const bool own = l == &c.ocells || l == &c.wcells;
+                    AQ_HIP(hipMalloc(
+                        l, (own ? aq_owner_list_cap(ncap, S.N) : ncap) * 12));
                 }
                 // the blocked order's per-block counters (zero between
                 // steps) and bases
