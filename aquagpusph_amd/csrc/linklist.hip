@@ -236,10 +236,12 @@ __global__ void ihoc_heads(const uint32_t* __restrict__ icell,
 
 // ---- cell ranges for the cell-parallel walks --------------------------------
 // head[i] = 1 when i starts a run of equal icell; cell_end[c] = one past the
-// last particle of cell c (only written for non-empty cells).
+// last particle of cell c (only written for non-empty cells); kword (the
+// per-cell kind words cell_list_k accumulates) cleared.
 __global__ void cell_heads_k(const uint32_t* __restrict__ icell, uint32_t N,
                              uint32_t* __restrict__ head,
                              uint32_t* __restrict__ cell_end,
+                             uint32_t* __restrict__ kword,
                              const uint32_t* status)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,100 +250,163 @@ __global__ void cell_heads_k(const uint32_t* __restrict__ icell, uint32_t N,
         head[i] = 0;
         return;
     }
+    if (kword) kword[i] = 0;
     const uint32_t c = icell[i];
     head[i] = (i == 0 || icell[i - 1] != c) ? 1u : 0u;
     if (i == N - 1 || icell[i + 1] != c) cell_end[c] = i + 1;
 }
 
+// heads[k] = first particle of the k-th non-empty cell.  With kword: the
+// kinds of particle of each non-empty cell (AQ_CELL_FLUID / ELEM / OTHER
+// bits) or-ed into kword[k] and its number of non-fluid particles added to
+// the low bits (below 2^28), one pair of atomics per run of a cell's
+// particles within a wavefront; imove is read through perm (id_unsorted):
+// the cell list is built right after the sort, before the fields are
+// reordered.
 __global__ void cell_list_k(const uint32_t* __restrict__ head,
                             const uint32_t* __restrict__ pos, uint32_t N,
                             uint32_t* __restrict__ heads,
-                            uint32_t* __restrict__ n_heads)
+                            uint32_t* __restrict__ n_heads,
+                            const uint32_t* __restrict__ perm,
+                            const int* __restrict__ imove,
+                            uint32_t* __restrict__ kword,
+                            const uint32_t* status)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    if (head[i]) heads[pos[i]] = i;
-    if (i == N - 1) {
-        *n_heads = pos[i] + head[i];
-        // the owner-cell lists are refilled next
-        n_heads[AQ_CL_NOCELLS] = 0;
-        n_heads[AQ_CL_NWCELLS] = 0;
+    const bool valid = i < N;
+    uint32_t hd = 0, ps = 0;
+    if (valid) {
+        hd = head[i];
+        ps = pos[i];
+        if (hd) heads[ps] = i;
+        if (i == N - 1) {
+            *n_heads = ps + hd;
+            // the owner-cell lists are refilled next
+            n_heads[AQ_CL_NOCELLS] = 0;
+            n_heads[AQ_CL_NWCELLS] = 0;
+        }
     }
+    if (!kword || (status && (*status & 1u))) return;  // uniform
+    const int lane = threadIdx.x & 63;
+    const int im = valid ? imove[perm[i]] : 1;
+    const uint64_t vm = __ballot(valid);
+    // a run starts at a cell's first particle or at the wavefront's first
+    const uint64_t lead = __ballot(valid && (hd || lane == 0));
+    const uint64_t fl = __ballot(valid && im == 1);
+    const uint64_t el = __ballot(valid && im == -3);
+    const uint64_t ot = __ballot(valid && im != 1 && im != -3);
+    if (!valid || !((lead >> lane) & 1ull)) return;
+    const uint64_t after = lane == 63 ? 0ull : lead >> (lane + 1);
+    const int ln = after ? lane + 1 + __builtin_ctzll(after) : 64;
+    const uint64_t run =
+        (ln == 64 ? ~0ull : (1ull << ln) - 1ull) & ~((1ull << lane) - 1ull) & vm;
+    uint32_t f = 0;
+    if (fl & run) f |= AQ_CELL_FLUID;
+    if (el & run) f |= AQ_CELL_ELEM;
+    if (ot & run) f |= AQ_CELL_OTHER;
+    const uint32_t k = ps + hd - 1u;
+    atomicOr(kword + k, f);
+    const uint32_t nonf = (uint32_t)__popcll((el | ot) & run);
+    if (nonf) atomicAdd(kword + k, nonf);
 }
 
-// Particle kinds present in each non-empty cell, in the top bits of
-// cell_end (AQ_CELL_FLUID: imove == 1, AQ_CELL_ELEM: imove == -3,
-// AQ_CELL_OTHER: any other imove; cell ends stay below 2^28).  imove is
-// read through perm (id_unsorted): the cell list is built right after the
-// sort, before the fields are reordered.  The cells with elements or other
-// particles are appended to the owner-cell list as (first particle, cell,
-// end particle with the cell's flags) triplets: the cells of at most
-// AQ_OWNER_CHUNK non-fluid particles as one entry (one atomic per
-// wavefront, so the list is ordered within each wavefront's 64 cells and
-// the blocks of 64 in no set order), the others as one entry per run of
-// particles ending at every AQ_OWNER_CHUNK-th non-fluid one (the last run
-// ends at the cell's end), reserved with one atomic per cell.
+// Reserves nent consecutive entries for each thread of the block in a list
+// counted by *counter, with one atomic per block (the lists' counters are
+// single words every block adds to); returns the thread's first entry.
+// Every thread of the block calls it; blockDim.x is a multiple of 64, at
+// most 1024.
+__device__ uint32_t block_reserve(uint32_t nent, uint32_t* counter)
+{
+    __shared__ uint32_t wtot[16];
+    __shared__ uint32_t bbase;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t x = nent;  // inclusive scan over the wavefront
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wtot[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int v = 0; v < (int)(blockDim.x >> 6); v++) {
+            const uint32_t w = wtot[v];
+            wtot[v] = t;
+            t += w;
+        }
+        bbase = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    return bbase + wtot[wv] + x - nent;
+}
+
+// Kind flags of each non-empty cell into the top bits of cell_end
+// (AQ_CELL_FLUID: imove == 1, AQ_CELL_ELEM: imove == -3, AQ_CELL_OTHER: any
+// other imove; cell ends stay below 2^28), from the kind words of
+// cell_list_k.  The cells with elements or other particles are appended to
+// the owner-cell list as (first particle, cell, end particle with the cell's
+// flags) triplets, one atomic per block (the list is ordered within a
+// block's cells, the blocks in no set order): the cells of at most
+// AQ_OWNER_CHUNK non-fluid particles as one entry, the others as one entry
+// per run of particles ending at every AQ_OWNER_CHUNK-th non-fluid one (the
+// last run ends at the cell's end).
 __global__ void cell_kinds_k(const uint32_t* __restrict__ heads,
                              uint32_t* __restrict__ n_heads,
                              const uint32_t* __restrict__ icell,
                              const uint32_t* __restrict__ perm,
                              const int* __restrict__ imove,
                              uint32_t* __restrict__ cell_end,
+                             const uint32_t* __restrict__ kword,
                              const uint32_t* status,
                              uint32_t* __restrict__ ocells)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nh = *n_heads;
-    if (status && (*status & 1u)) return;  // uniform
+    if (status && (*status & 1u)) return;           // uniform
+    if (blockIdx.x * blockDim.x >= nh) return;      // uniform per block
+    constexpr uint32_t CH = AQ_OWNER_CHUNK;
     uint32_t f = 0, start = 0, c = 0, end = 0, nonf = 0;
     if (k < nh) {
         start = heads[k];
         c = icell[start];
         end = cell_end[c];
-        for (uint32_t i = start; i < end; i++) {
-            const int im = imove[perm[i]];
-            f |= (im == 1) ? AQ_CELL_FLUID
-                           : ((im == -3) ? AQ_CELL_ELEM : AQ_CELL_OTHER);
-            nonf += (im != 1) ? 1u : 0u;
-        }
+        const uint32_t kw = kword[k];
+        f = kw & (AQ_CELL_FLUID | AQ_CELL_ELEM | AQ_CELL_OTHER);
+        nonf = kw & AQ_CELL_END;
         cell_end[c] = end | f;
     }
     if (!ocells) return;
     const bool own = (f & (AQ_CELL_ELEM | AQ_CELL_OTHER)) != 0u;
-    if (own && nonf > AQ_OWNER_CHUNK) {
-        // runs of AQ_OWNER_CHUNK non-fluid particles
-        const uint32_t nch = (nonf + AQ_OWNER_CHUNK - 1u) / AQ_OWNER_CHUNK;
-        uint32_t e = atomicAdd(n_heads + AQ_CL_NOCELLS, nch);
-        const uint32_t elast = e + nch - 1u;
-        uint32_t s0 = start, cnt = 0;
-        for (uint32_t i = start; i < end && e < elast; i++) {
-            if (imove[perm[i]] == 1) continue;
-            if (++cnt < AQ_OWNER_CHUNK) continue;
-            ocells[3 * e] = s0;
-            ocells[3 * e + 1] = c;
-            ocells[3 * e + 2] = (i + 1u) | f;
-            e++;
-            s0 = i + 1u;
-            cnt = 0;
-        }
-        ocells[3 * e] = s0;
-        ocells[3 * e + 1] = c;
-        ocells[3 * e + 2] = end | f;
+    const uint32_t nent = !own ? 0u : (nonf > CH ? (nonf + CH - 1u) / CH : 1u);
+    const uint32_t e0 = block_reserve(nent, n_heads + AQ_CL_NOCELLS);
+    if (own && nonf <= CH) {
+        ocells[3 * e0] = start;
+        ocells[3 * e0 + 1] = c;
+        ocells[3 * e0 + 2] = end | f;
     }
-    const uint64_t bm = __ballot(own && nonf <= AQ_OWNER_CHUNK);
-    if (bm == 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == __builtin_ctzll(bm))
-        base = atomicAdd(n_heads + AQ_CL_NOCELLS, (uint32_t)__popcll(bm));
-    base = __shfl(base, __builtin_ctzll(bm));
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
-    if (own && nonf <= AQ_OWNER_CHUNK) {
-        const uint32_t o = 3u * (base + rank);
-        ocells[o] = start;
-        ocells[o + 1] = c;
-        ocells[o + 2] = end | f;
+    // the cells split into runs: each lane scans its cell, 16 particles a
+    // round trip; run m ends at the (m + 1) CH-th non-fluid particle
+    if (own && nonf > CH) {
+        const uint32_t elast = e0 + nent - 1u;
+        uint32_t e = e0, cnt = 0;
+        ocells[3 * e] = start;
+        ocells[3 * e + 1] = c;
+        for (uint32_t p0 = start; p0 < end && e < elast; p0 += 16) {
+            int im[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++)
+                im[u] = p0 + u < end ? imove[perm[p0 + u]] : 1;
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                if (im[u] == 1 || e >= elast || ++cnt < CH) continue;
+                ocells[3 * e + 2] = (p0 + u + 1u) | f;
+                e++;
+                ocells[3 * e] = p0 + u + 1u;
+                ocells[3 * e + 1] = c;
+                cnt = 0;
+            }
+        }
+        ocells[3 * elast + 2] = end | f;
     }
 }
 
@@ -386,17 +451,9 @@ __global__ void owner_wet_k(const uint32_t* __restrict__ ocells,
             if (ihoc[c] < N && (cell_end[c] & AQ_CELL_FLUID)) wet = true;
         }
     }
-    const uint64_t bm = __ballot(wet);
-    if (bm == 0) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == __builtin_ctzll(bm))
-        base = atomicAdd(n_heads + AQ_CL_NWCELLS, (uint32_t)__popcll(bm));
-    base = __shfl(base, __builtin_ctzll(bm));
-    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+    const uint32_t e = block_reserve(wet ? 1u : 0u, n_heads + AQ_CL_NWCELLS);
     if (wet) {
-        const uint32_t o = 3u * (base + rank);
+        const uint32_t o = 3u * e;
         wcells[o] = t0;
         wcells[o + 1] = t1;
         wcells[o + 2] = t2;
@@ -426,6 +483,26 @@ __device__ __forceinline__ uint32_t cell_block(uint32_t c,
     return b < nbmax ? b : nbmax - 1u;
 }
 
+// Runs of consecutive cells of one block within a wavefront (the cells come
+// in row-major order, so a row crosses a block every B cells): the lanes
+// below nv hold cells; lane l is in the run led by lane *lead_lane, of
+// *len cells (set on the leading lane only).
+__device__ __forceinline__ bool block_run(uint32_t b, int lane, uint32_t nv,
+                                          int* lead_lane, uint32_t* len)
+{
+    const uint32_t prev = __shfl_up(b, 1);
+    const bool valid = (uint32_t)lane < nv;
+    const bool lead = valid && (lane == 0 || prev != b);
+    const uint64_t L = __ballot(lead);
+    const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+    *lead_lane = 63 - __builtin_clzll((L & upto) | 1ull);
+    const uint64_t after = lane == 63 ? 0ull : L >> (lane + 1);
+    const uint32_t ln = after ? (uint32_t)(lane + 1 + __builtin_ctzll(after))
+                              : 64u;
+    *len = (ln < nv ? ln : nv) - (uint32_t)lane;
+    return lead;
+}
+
 template <int D>
 __global__ void blk_count_k(const uint32_t* __restrict__ heads,
                             const uint32_t* __restrict__ n_heads,
@@ -435,9 +512,17 @@ __global__ void blk_count_k(const uint32_t* __restrict__ heads,
                             const uint32_t* status)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (status && (*status & 1u)) return;
-    if (k >= *n_heads) return;
-    atomicAdd(bcount + cell_block<D>(icell[heads[k]], n_cells, nbmax), 1u);
+    if (status && (*status & 1u)) return;            // uniform
+    const uint32_t nh = *n_heads;
+    const uint32_t k0 = k - (uint32_t)(threadIdx.x & 63);
+    if (k0 >= nh) return;                            // uniform per wavefront
+    const int lane = threadIdx.x & 63;
+    const uint32_t nv = nh - k0 < 64u ? nh - k0 : 64u;
+    const uint32_t b =
+        k < nh ? cell_block<D>(icell[heads[k]], n_cells, nbmax) : 0u;
+    int ll;
+    uint32_t len;
+    if (block_run(b, lane, nv, &ll, &len)) atomicAdd(bcount + b, len);
 }
 
 // bcount counts down to zero again, ready for the next step
@@ -453,12 +538,26 @@ __global__ void blk_scatter_k(const uint32_t* __restrict__ heads,
                               const uint32_t* status)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (status && (*status & 1u)) return;
-    if (k >= *n_heads) return;
-    const uint32_t start = heads[k];
-    const uint32_t c = icell[start];
-    const uint32_t b = cell_block<D>(c, n_cells, nbmax);
-    const uint32_t pos = bbase[b] + atomicSub(bcount + b, 1u) - 1u;
+    if (status && (*status & 1u)) return;            // uniform
+    const uint32_t nh = *n_heads;
+    const uint32_t k0 = k - (uint32_t)(threadIdx.x & 63);
+    if (k0 >= nh) return;                            // uniform per wavefront
+    const int lane = threadIdx.x & 63;
+    const uint32_t nv = nh - k0 < 64u ? nh - k0 : 64u;
+    uint32_t start = 0, c = 0, b = 0;
+    if (k < nh) {
+        start = heads[k];
+        c = icell[start];
+        b = cell_block<D>(c, n_cells, nbmax);
+    }
+    int ll;
+    uint32_t len, base = 0;
+    // a run takes the top len places still free in its block
+    if (block_run(b, lane, nv, &ll, &len))
+        base = bbase[b] + atomicSub(bcount + b, len) - len;
+    base = __shfl(base, ll);
+    if (k >= nh) return;
+    const uint32_t pos = base + (uint32_t)(lane - ll);
     bcells[3 * pos] = start;
     bcells[3 * pos + 1] = c;
     bcells[3 * pos + 2] = cell_end[c];
@@ -480,7 +579,7 @@ size_t scan_workspace(uint32_t m);
 
 size_t cell_list_workspace(uint32_t N)
 {
-    return (((size_t)N * 4 + 255) & ~(size_t)255) * 2 + scan_workspace(N) +
+    return (((size_t)N * 4 + 255) & ~(size_t)255) * 3 + scan_workspace(N) +
            256;
 }
 
@@ -499,22 +598,28 @@ void cell_list(const uint32_t* icell, uint32_t N, uint32_t* cell_end,
     const size_t a = ((size_t)N * 4 + 255) & ~(size_t)255;
     uint32_t* head = (uint32_t*)ws;
     uint32_t* pos = (uint32_t*)((char*)ws + a);
-    uint32_t* sums = (uint32_t*)((char*)ws + 2 * a);
+    uint32_t* kword = (perm && imove) ? (uint32_t*)((char*)ws + 2 * a)
+                                      : nullptr;
+    uint32_t* sums = (uint32_t*)((char*)ws + 3 * a);
     const dim3 g(blocks_for(N, 256)), b(256);
     hipLaunchKernelGGL(cell_heads_k, g, b, 0, s, icell, N, head, cell_end,
-                       status);
+                       kword, status);
     exclusive_scan(head, pos, N, sums, s);
-    hipLaunchKernelGGL(cell_list_k, g, b, 0, s, head, pos, N, heads, n_heads);
-    if (perm && imove)
-        hipLaunchKernelGGL(cell_kinds_k, g, b, 0, s, heads, n_heads, icell,
-                           perm, imove, cell_end, status, ocells);
+    hipLaunchKernelGGL(cell_list_k, g, b, 0, s, head, pos, N, heads, n_heads,
+                       perm, imove, kword, status);
+    // 1024-thread blocks: the list appends take one atomic per block
+    const dim3 b4(1024);
+    if (kword)
+        hipLaunchKernelGGL(cell_kinds_k, dim3(blocks_for(N, 1024)), b4, 0, s,
+                           heads, n_heads, icell, perm, imove, cell_end,
+                           kword, status, ocells);
     if (perm && imove && ocells && wcells && ocap) {
-        const dim3 gw(blocks_for(aq_owner_list_cap(ocap, N), 256));
+        const dim3 gw(blocks_for(aq_owner_list_cap(ocap, N), 1024));
         if (dims == 2)
-            hipLaunchKernelGGL(owner_wet_k<2>, gw, b, 0, s, ocells, n_heads,
+            hipLaunchKernelGGL(owner_wet_k<2>, gw, b4, 0, s, ocells, n_heads,
                                ihoc, cell_end, n_cells, N, wcells, status);
         else
-            hipLaunchKernelGGL(owner_wet_k<3>, gw, b, 0, s, ocells, n_heads,
+            hipLaunchKernelGGL(owner_wet_k<3>, gw, b4, 0, s, ocells, n_heads,
                                ihoc, cell_end, n_cells, N, wcells, status);
     }
     if (bcells && ocap && nbmax) {
