@@ -13,6 +13,9 @@ namespace aq {
 
 void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
              uint32_t n, hipStream_t s);
+void permute_many(const void* const* src, void* const* dst,
+                  const size_t* esize, int nf, const uint32_t* perm,
+                  uint32_t n, hipStream_t s);
 void launch_predictor(int, const int32_t*, const float*, const float*,
                       const float*, const float*, const float*, float*, float*,
                       float*, float*, float*, uint32_t, const float*,
@@ -341,6 +344,27 @@ static void gather(Sim& S, Var* src, Var* dst, const uint32_t* perm,
     permute(src->dev, dst->dev, src->type.elem_bytes(), perm, S.N, s);
 }
 
+// The gathers (a[k] -> a[k + 1] for even k < 2 nf) of a sort stage in one
+// pass over the permutation
+static void gather_many(Sim& S, Var* const* a, int nf, const uint32_t* perm,
+                        hipStream_t s)
+{
+    const void* src[8];
+    void* dst[8];
+    size_t es[8];
+    AQ_REQUIRE(nf <= 8, "sort stage: more than 8 fields");
+    if (S.no_batch) {
+        for (int f = 0; f < nf; f++) gather(S, a[2 * f], a[2 * f + 1], perm, s);
+        return;
+    }
+    for (int f = 0; f < nf; f++) {
+        src[f] = a[2 * f]->dev;
+        dst[f] = a[2 * f + 1]->dev;
+        es[f] = a[2 * f]->type.elem_bytes();
+    }
+    permute_many(src, dst, es, nf, perm, S.N, s);
+}
+
 // __GP_LAPU_MODEL__ names another definition (__GP_LAPU_TAKEDA__ = 1 or
 // __GP_LAPU_U0M__ = 2, cfd/GP.xml): follow the chain to the number.
 static int gp_lapu_model(Sim& S)
@@ -407,17 +431,13 @@ static const std::map<std::string, KernelSpec>& table()
               "r", "normal_in", "normal", "tangent_in", "tangent",
               "id_unsorted" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                const uint32_t* perm = U(12);
-                for (int k = 0; k < 12; k += 2)
-                    gather(S, a[k], a[k + 1], perm, s);
+                gather_many(S, a, 6, U(12), s);
             } } },
         { "basic/Sort.cl:stage2",
           { { "rho_in", "rho", "m_in", "m", "u_in", "u", "dudt", "dudt_in",
               "drhodt", "drhodt_in", "id_unsorted" },
             [](Sim& S, Var* const* a, hipStream_t s) {
-                const uint32_t* perm = U(10);
-                for (int k = 0; k < 10; k += 2)
-                    gather(S, a[k], a[k + 1], perm, s);
+                gather_many(S, a, 5, U(10), s);
             } } },
         // ---- basic ----------------------------------------------------------
         { "basic/EOS.cl:entry",

@@ -309,6 +309,7 @@ void Sim::setup()
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
     }
     plan_fusion();
+    plan_batches();
     // the compensated sums live in the fused cell walk only
     if (problem.is_defined("__KAHAN_SUMS__") &&
         !(fusion.active && clist.enabled))
@@ -320,6 +321,55 @@ void Sim::setup()
     vars.push(stream);
     AQ_HIP(hipStreamSynchronize(stream));
     is_setup = true;
+}
+
+static bool getenv_true(const char* name);
+
+// Runs of consecutive set tools, and of copy tools, each in one launch
+// (Tool::batch_kind), unless AQ_NO_BATCH is set.  A run ends at a tool of
+// another kind (control tools included, so the pipeline never jumps into
+// one), at a `once` tool or at a tool whose arrays meet what the run writes
+// (or that writes what it reads), and holds at most 8 tools.
+void Sim::plan_batches()
+{
+    for (auto& t : tools) {
+        t->batch_next.clear();
+        t->batch_member = false;
+    }
+    no_batch = getenv_true("AQ_NO_BATCH");
+    if (no_batch) return;
+    size_t a = 0;
+    while (a < tools.size()) {
+        const int k = tools[a]->batch_kind();
+        if (!k || tools[a]->once()) {
+            a++;
+            continue;
+        }
+        std::vector<const void*> rd, wr;
+        tools[a]->batch_io(rd, wr);
+        size_t b = a + 1;
+        while (b < tools.size() && b - a < 8 &&
+               tools[b]->batch_kind() == k && !tools[b]->once()) {
+            std::vector<const void*> r2, w2;
+            tools[b]->batch_io(r2, w2);
+            auto meets = [](const std::vector<const void*>& x,
+                            const std::vector<const void*>& y) {
+                for (const void* p : x)
+                    if (std::find(y.begin(), y.end(), p) != y.end())
+                        return true;
+                return false;
+            };
+            if (meets(r2, wr) || meets(w2, wr) || meets(w2, rd)) break;
+            rd.insert(rd.end(), r2.begin(), r2.end());
+            wr.insert(wr.end(), w2.begin(), w2.end());
+            b++;
+        }
+        for (size_t i = a + 1; i < b; i++) {
+            tools[a]->batch_next.push_back(tools[i].get());
+            tools[i]->batch_member = true;
+        }
+        a = b;
+    }
 }
 
 // A periodic box (periodic_L declared): every neighbour walk of the

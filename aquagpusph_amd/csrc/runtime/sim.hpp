@@ -126,6 +126,23 @@ class Tool {
     // +1 for the scope openers (while, if), -1 for end (Tool.hpp
     // scope_modifier); such tools are run by the pipeline, not execute()
     virtual int scope() const { return 0; }
+    // Runs of consecutive tools of one batch kind (1 set, 2 copy; 0 none)
+    // run as one launch: the first tool's execute() does the whole run
+    // (batch_next: the tools after it), the others' execute() nothing.
+    // Runs never cross a control tool, where the pipeline may jump, and
+    // hold no `once` tool.
+    virtual int batch_kind() const { return 0; }
+    // the arrays a batchable tool reads and writes: a run holds no tool
+    // that reads or writes what an earlier tool of the run writes, or
+    // writes what an earlier one reads
+    virtual void batch_io(std::vector<const void*>& rd,
+                          std::vector<const void*>& wr) const
+    {
+        (void)rd;
+        (void)wr;
+    }
+    std::vector<Tool*> batch_next;
+    bool batch_member = false;
     const std::string& name() const { return name_; }
     const std::string& type() const { return type_; }
     bool once() const { return once_; }
@@ -369,6 +386,9 @@ class Sim {
     // launches in progress (cell_args then hands out the side counters)
     bool side_off = false;
     bool in_side = false;
+    // AQ_NO_BATCH: set / copy tools and the sort stages' gathers one launch
+    // each
+    bool no_batch = false;
     // scratch of the per-particle walk kernels' compacted particle lists
     // (with the cell-parallel walks on): N indices and one counter
     uint32_t* sel_list = nullptr;
@@ -415,6 +435,7 @@ class Sim {
     void match_scopes();
     void step_segments(int nsteps);
     void plan_fusion();
+    void plan_batches();
     void make_defs();
     void load_sets();
 };
@@ -429,6 +450,23 @@ void load_fastascii(Sim& S, const std::string& path, size_t offset, size_t n,
 void vm_launch(const VmOp* ops, uint8_t* arena, uint32_t* status,
                int32_t* fail_id, hipStream_t s);
 void status_stamp(uint32_t* status, const uint32_t* iter, hipStream_t s);
+// Batched fills (dst[f][i * stored[f] + k] = pat[f][k], i < n[f]) and
+// whole-word copies (words[f] words from src[f] to dst[f]), up to 8 each
+struct FillBatch {
+    void* dst[8];
+    const uint32_t* pat[8];
+    int stored[8];
+    size_t n[8];
+    int cnt = 0;
+};
+struct CopyBatch {
+    const void* src[8];
+    void* dst[8];
+    size_t words[8];
+    int cnt = 0;
+};
+void fill_batch(const FillBatch& b, hipStream_t s);
+void copy_batch(const CopyBatch& b, hipStream_t s);
 
 }  // namespace rt
 }  // namespace aq

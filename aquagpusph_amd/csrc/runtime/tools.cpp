@@ -749,11 +749,33 @@ class SetTool : public Tool {
     }
     void execute(Sim&, hipStream_t s) override
     {
-        fill_launch(v_->dev, (const uint32_t*)pat_.p,
-                    (int)(v_->type.elem_bytes() / 4), v_->length, s);
+        if (batch_member) return;  // filled by the run's first set tool
+        if (batch_next.empty()) {
+            fill_launch(v_->dev, (const uint32_t*)pat_.p,
+                        (int)(v_->type.elem_bytes() / 4), v_->length, s);
+            return;
+        }
+        FillBatch b;
+        add_to(b);
+        for (Tool* t : batch_next) static_cast<SetTool*>(t)->add_to(b);
+        fill_batch(b, s);
+    }
+    int batch_kind() const override { return 1; }
+    void batch_io(std::vector<const void*>&,
+                  std::vector<const void*>& wr) const override
+    {
+        wr.push_back(v_->dev);
     }
 
   private:
+    void add_to(FillBatch& b) const
+    {
+        b.dst[b.cnt] = v_->dev;
+        b.pat[b.cnt] = (const uint32_t*)pat_.p;
+        b.stored[b.cnt] = (int)(v_->type.elem_bytes() / 4);
+        b.n[b.cnt] = v_->length;
+        b.cnt++;
+    }
     ToolDesc d_;
     Var* v_;
     DevBuf pat_;
@@ -940,13 +962,43 @@ class CopyTool : public Tool {
     }
     void execute(Sim&, hipStream_t s) override
     {
-        const size_t b = std::min(in_->bytes, out_->bytes);
-        if (b)
-            AQ_HIP(hipMemcpyAsync(out_->dev, in_->dev, b,
-                                  hipMemcpyDeviceToDevice, s));
+        if (batch_member) return;  // copied by the run's first copy tool
+        if (batch_next.empty()) {
+            const size_t b = std::min(in_->bytes, out_->bytes);
+            if (b)
+                AQ_HIP(hipMemcpyAsync(out_->dev, in_->dev, b,
+                                      hipMemcpyDeviceToDevice, s));
+            return;
+        }
+        CopyBatch b;
+        add_to(b);
+        for (Tool* t : batch_next) static_cast<CopyTool*>(t)->add_to(b);
+        copy_batch(b, s);
+    }
+    // whole words between distinct arrays (the batched copy copies words,
+    // and a run's copies must not read what another writes)
+    int batch_kind() const override
+    {
+        return (std::min(in_->bytes, out_->bytes) % 4 == 0 &&
+                in_->dev != out_->dev)
+                   ? 2
+                   : 0;
+    }
+    void batch_io(std::vector<const void*>& rd,
+                  std::vector<const void*>& wr) const override
+    {
+        rd.push_back(in_->dev);
+        wr.push_back(out_->dev);
     }
 
   private:
+    void add_to(CopyBatch& b) const
+    {
+        b.src[b.cnt] = in_->dev;
+        b.dst[b.cnt] = out_->dev;
+        b.words[b.cnt] = std::min(in_->bytes, out_->bytes) / 4;
+        b.cnt++;
+    }
     ToolDesc d_;
     Var *in_, *out_;
 };

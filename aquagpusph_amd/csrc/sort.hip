@@ -13,6 +13,8 @@
 //                   wavefront ballots (match-any), rows are processed in
 //                   order, so equal digits keep their input order (stable).
 // No kernel waits on another workgroup, so every wave always finishes.
+#include <algorithm>
+
 #include "aq_internal.hpp"
 
 namespace aq {
@@ -230,31 +232,25 @@ void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
     w += align256((size_t)m * 4);
     uint32_t* sums = (uint32_t*)w;
     const int passes = key_bits == 0 ? 1 : (int)((key_bits + 7) / 8);
-    uint32_t *kin = keys, *kout = kb, *vin = nullptr, *vout = vb;
+    // The keys alternate between kb (even passes: the first cannot write
+    // keys in place) and keys; the values between vb and perm, chosen so
+    // that the last pass writes perm (the first pass reads no values).
+    uint32_t *kin = keys, *vin = nullptr, *kout = kb, *vout = nullptr;
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
+        kout = (p % 2 == 0) ? kb : keys;
+        vout = ((passes - 1 - p) % 2 == 0) ? perm : vb;
         hipLaunchKernelGGL(rs_hist, dim3(nt), dim3(RS_BS), 0, s, kin, n,
                            shift, counts, nt);
         exclusive_scan(counts, counts, m, sums, s);
         hipLaunchKernelGGL(rs_scatter, dim3(nt), dim3(RS_BS), 0, s, kin, vin,
                            kout, vout, n, shift, counts, nt);
-        if (p == 0) {
-            // first pass wrote (keys, iota) to the buffers; continue there
-            vin = vout;
-            vout = perm;
-            kin = kout;
-            kout = keys;
-        } else {
-            std::swap(kin, kout);
-            std::swap(vin, vout);
-        }
+        kin = kout;
+        vin = vout;
     }
-    // after the loop the sorted data is in (kin, vin)
+    // the sorted keys are in kb after an odd number of passes
     if (kin != keys)
         AQ_HIP(hipMemcpyAsync(keys, kin, (size_t)n * 4,
-                              hipMemcpyDeviceToDevice, s));
-    if (vin != perm)
-        AQ_HIP(hipMemcpyAsync(perm, vin, (size_t)n * 4,
                               hipMemcpyDeviceToDevice, s));
     if (inv)
         hipLaunchKernelGGL(inverse_perm, dim3(blocks_for(n, 256)), dim3(256),
@@ -264,6 +260,84 @@ void radix_sort(uint32_t* keys, uint32_t* perm, uint32_t* inv, uint32_t n,
 struct alignas(16) Words8 {
     uint4 a, b;
 };
+
+// Several gathers by one permutation in one pass (the perm entry read
+// once): dst_f[i] = src_f[perm[i]] for the fields f < n, of 4, 8 or 16
+// bytes per element.  Every field's loads are issued before its stores.
+constexpr int PERM_BATCH = 8;
+struct PermBatch {
+    const void* src[PERM_BATCH];
+    void* dst[PERM_BATCH];
+    int esize[PERM_BATCH];
+    int n;
+};
+
+__global__ void permute_batch_k(PermBatch b, const uint32_t* __restrict__ perm,
+                                uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = perm[i];
+    uint4 v[PERM_BATCH];
+#pragma unroll
+    for (int f = 0; f < PERM_BATCH; f++) {
+        if (f >= b.n) break;
+        const int es = b.esize[f];
+        if (es == 16) {
+            v[f] = ((const uint4*)b.src[f])[p];
+        } else if (es == 8) {
+            const uint2 t = ((const uint2*)b.src[f])[p];
+            v[f] = make_uint4(t.x, t.y, 0u, 0u);
+        } else {
+            v[f] = make_uint4(((const uint32_t*)b.src[f])[p], 0u, 0u, 0u);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < PERM_BATCH; f++) {
+        if (f >= b.n) break;
+        const int es = b.esize[f];
+        if (es == 16)
+            ((uint4*)b.dst[f])[i] = v[f];
+        else if (es == 8)
+            ((uint2*)b.dst[f])[i] = make_uint2(v[f].x, v[f].y);
+        else
+            ((uint32_t*)b.dst[f])[i] = v[f].x;
+    }
+}
+
+// Gathers (src[f] -> dst[f], esize[f] bytes per element) by one
+// permutation.  Fields of 4, 8 or 16 bytes, none of whose destinations is
+// another field's source, go through permute_batch_k, up to PERM_BATCH at
+// a time; the others one by one, in order after them.
+void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
+             uint32_t n, hipStream_t s);
+void permute_many(const void* const* src, void* const* dst,
+                  const size_t* esize, int nf, const uint32_t* perm,
+                  uint32_t n, hipStream_t s)
+{
+    if (n == 0 || nf <= 0) return;
+    bool ok = true;
+    for (int f = 0; f < nf && ok; f++) {
+        ok = (esize[f] == 4 || esize[f] == 8 || esize[f] == 16) &&
+             src[f] != dst[f];
+        for (int g = 0; g < nf && ok; g++) ok = dst[f] != src[g];
+    }
+    if (!ok) {
+        for (int f = 0; f < nf; f++) permute(src[f], dst[f], esize[f], perm, n, s);
+        return;
+    }
+    for (int f0 = 0; f0 < nf; f0 += PERM_BATCH) {
+        PermBatch b{};
+        b.n = std::min(nf - f0, PERM_BATCH);
+        for (int f = 0; f < b.n; f++) {
+            b.src[f] = src[f0 + f];
+            b.dst[f] = dst[f0 + f];
+            b.esize[f] = (int)esize[f0 + f];
+        }
+        hipLaunchKernelGGL(permute_batch_k, dim3(blocks_for(n, 256)),
+                           dim3(256), 0, s, b, perm, n);
+    }
+}
 
 void permute(const void* src, void* dst, size_t esize, const uint32_t* perm,
              uint32_t n, hipStream_t s)

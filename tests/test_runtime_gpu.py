@@ -1005,6 +1005,75 @@ def test_reference_radixsort_case(cuda, tmp_path):
     assert int(sim.get("iter")) == 2
 
 
+def test_batched_set_and_copy_runs(cuda, tmp_path):
+    """Consecutive set tools, and consecutive copy tools, run as one launch
+    each; a run ends where a tool meets an array the run writes (a copy of
+    a copy, a second set of one array), so every result is the one the
+    tools give in order."""
+    d = np.loadtxt(os.path.join(GOLDEN, "radixsort_3d_particles.dat"))
+    xml = _fixture_case(tmp_path, """
+    <Variables>
+        <Variable length="N" name="f" type="size_t*" />
+        <Variable length="N" name="a" type="size_t*" />
+        <Variable length="N" name="b" type="size_t*" />
+        <Variable length="N" name="c" type="float*" />
+        <Variable length="N" name="e" type="vec*" />
+        <Variable length="N" name="r1" type="vec*" />
+        <Variable length="N" name="r2" type="vec*" />
+        <Variable length="1" name="h" type="float" value="0.1"/>
+    </Variables>
+    <Tools>
+        <Tool action="add" name="copy f a" type="copy" in="f" out="a" />
+        <Tool action="add" name="copy a b" type="copy" in="a" out="b" />
+        <Tool action="add" name="copy r r1" type="copy" in="r" out="r1" />
+        <Tool action="add" name="set c" type="set" in="c" value="7.5" />
+        <Tool action="add" name="set e one" type="set" in="e" value="VEC_ONE" />
+        <Tool action="add" name="set e zero" type="set" in="e" value="VEC_ZERO" />
+        <Tool action="add" name="set r2" type="set" in="r2" value="VEC_ONE" />
+        <Tool action="add" name="copy r1 r2" type="copy" in="r1" out="r2" />
+        <Tool action="add" name="iter += 1" type="set_scalar" in="iter" value="iter + 1"/>
+    </Tools>""", 3, d, "r, f")
+    sim = Simulation(xml, 3)
+    for graph in (False, True):
+        sim.step(1, graph=graph)
+        f = sim.get("f")
+        np.testing.assert_array_equal(sim.get("a"), f)
+        np.testing.assert_array_equal(sim.get("b"), f)
+        np.testing.assert_array_equal(sim.get("r1"), sim.get("r"))
+        np.testing.assert_array_equal(sim.get("r2"), sim.get("r"))
+        assert np.all(sim.get("c") == np.float32(7.5))
+        assert np.all(sim.get("e") == 0.0)
+    sim.close()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_batched_launches_are_bit_identical(cuda, tmp_path, dims,
+                                            monkeypatch):
+    """The P-BI dam break with its set and copy runs batched and the sort
+    stages' gathers in one pass each gives the state of one launch per tool
+    and field (AQ_NO_BATCH=1) bit for bit, the sort's backup and unsorted
+    fields included."""
+    pos, nrm = cases.dam_break_sensors(dims)
+    case = cases.with_sensors(
+        cases.dam_break(dims, 0.3 / (40 if dims == 2 else 12)), pos, nrm)
+    xml = casefiles.write_case(case, str(tmp_path))
+    monkeypatch.setenv("AQ_NO_BATCH", "1")
+    a = Simulation(xml, dims)
+    monkeypatch.delenv("AQ_NO_BATCH")
+    b = Simulation(xml, dims)
+    for graph in (False, True):
+        a.step(3)
+        b.step(3, graph=graph)
+        for f in ("r", "u", "rho", "p", "m", "id", "iset", "imove", "normal",
+                  "tangent", "id_in", "iset_in", "imove_in", "normal_in",
+                  "tangent_in", "m_in", "dudt", "dudt_in", "drhodt",
+                  "drhodt_in", "shepard", "grad_p", "lap_u", "div_u",
+                  "lap_p"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    a.close()
+    b.close()
+
+
 def test_reference_reduction_case(cuda, tmp_path):
     """The reference tests/3D/Reduction tool set: vec sum (tol 1e-2) and max
     (exact) of its fixture."""
