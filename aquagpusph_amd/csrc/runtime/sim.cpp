@@ -325,11 +325,12 @@ void Sim::setup()
 
 static bool getenv_true(const char* name);
 
-// Runs of consecutive set tools, and of copy tools, each in one launch
-// (Tool::batch_kind), unless AQ_NO_BATCH is set.  A run ends at a tool of
-// another kind (control tools included, so the pipeline never jumps into
-// one), at a `once` tool or at a tool whose arrays meet what the run writes
-// (or that writes what it reads), and holds at most 8 tools.
+// Runs of consecutive set tools, of copy tools, each in one launch
+// (Tool::batch_kind), and of set_scalar tools, unless AQ_NO_BATCH is set.
+// A run ends at a tool of another kind (control tools included, so the
+// pipeline never jumps into one), at a `once` tool or at a tool whose
+// arrays meet what the run writes (or that writes what it reads), and holds
+// at most 8 tools; dummy tools in between are passed over.
 void Sim::plan_batches()
 {
     for (auto& t : tools) {
@@ -347,9 +348,13 @@ void Sim::plan_batches()
         }
         std::vector<const void*> rd, wr;
         tools[a]->batch_io(rd, wr);
-        size_t b = a + 1;
-        while (b < tools.size() && b - a < 8 &&
-               tools[b]->batch_kind() == k && !tools[b]->once()) {
+        size_t b = a + 1, n = 1;
+        while (b < tools.size() && n < 8) {
+            if (tools[b]->batch_transparent()) {
+                b++;
+                continue;
+            }
+            if (tools[b]->batch_kind() != k || tools[b]->once()) break;
             std::vector<const void*> r2, w2;
             tools[b]->batch_io(r2, w2);
             auto meets = [](const std::vector<const void*>& x,
@@ -362,12 +367,12 @@ void Sim::plan_batches()
             if (meets(r2, wr) || meets(w2, wr) || meets(w2, rd)) break;
             rd.insert(rd.end(), r2.begin(), r2.end());
             wr.insert(wr.end(), w2.begin(), w2.end());
+            tools[a]->batch_next.push_back(tools[b].get());
+            tools[b]->batch_member = true;
             b++;
+            n++;
         }
-        for (size_t i = a + 1; i < b; i++) {
-            tools[a]->batch_next.push_back(tools[i].get());
-            tools[i]->batch_member = true;
-        }
+        if (!tools[a]->batch_next.empty()) tools[a]->batch_setup(*this);
         a = b;
     }
 }

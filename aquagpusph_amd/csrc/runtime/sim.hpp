@@ -126,12 +126,17 @@ class Tool {
     // +1 for the scope openers (while, if), -1 for end (Tool.hpp
     // scope_modifier); such tools are run by the pipeline, not execute()
     virtual int scope() const { return 0; }
-    // Runs of consecutive tools of one batch kind (1 set, 2 copy; 0 none)
+    // Runs of consecutive tools of one batch kind (1 set, 2 copy, 3 set_scalar;
+// 0 none)
     // run as one launch: the first tool's execute() does the whole run
     // (batch_next: the tools after it), the others' execute() nothing.
     // Runs never cross a control tool, where the pipeline may jump, and
     // hold no `once` tool.
     virtual int batch_kind() const { return 0; }
+    // tools that do nothing (dummy markers) neither end a run nor join it
+    virtual bool batch_transparent() const { return false; }
+    // called on a run's first tool once the run is formed
+    virtual void batch_setup(Sim& S) { (void)S; }
     // the arrays a batchable tool reads and writes: a run holds no tool
     // that reads or writes what an earlier tool of the run writes, or
     // writes what an earlier one reads

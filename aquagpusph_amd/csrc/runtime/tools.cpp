@@ -138,6 +138,7 @@ class DummyTool : public Tool {
   public:
     using Tool::Tool;
     void execute(Sim&, hipStream_t) override {}
+    bool batch_transparent() const override { return true; }
 };
 
 // ---- reports: Reports/Screen.cpp, TabFile.cpp, Performance.cpp -----------------
@@ -811,15 +812,35 @@ class VmTool : public Tool {
         prog_.alloc(p.ops.size() * sizeof(VmOp));
         AQ_HIP(hipMemcpy(prog_.p, p.ops.data(), p.ops.size() * sizeof(VmOp),
                          hipMemcpyHostToDevice));
+        ops_ = p.ops;
     }
     void execute(Sim& S, hipStream_t s) override
     {
-        vm_launch((const VmOp*)prog_.p, S.vars.dev_arena(), S.d_status,
+        if (batch_member) return;  // run by the first tool of its run
+        const DevBuf& prog = batch_next.empty() ? prog_ : batch_prog_;
+        vm_launch((const VmOp*)prog.p, S.vars.dev_arena(), S.d_status,
                   S.d_fail_id, s);
+    }
+    // the run's programs one after the other (each leaves the stack empty,
+    // so the next starts as it would alone)
+    void batch_setup(Sim&) override
+    {
+        std::vector<VmOp> all;
+        auto append = [&](const std::vector<VmOp>& o) {
+            for (const VmOp& op : o)
+                if (op.code != VM_END) all.push_back(op);
+        };
+        append(ops_);
+        for (Tool* t : batch_next) append(static_cast<VmTool*>(t)->ops_);
+        all.push_back(VmOp{ VM_END, 0, 0.0 });
+        batch_prog_.alloc(all.size() * sizeof(VmOp));
+        AQ_HIP(hipMemcpy(batch_prog_.p, all.data(), all.size() * sizeof(VmOp),
+                         hipMemcpyHostToDevice));
     }
 
   protected:
-    DevBuf prog_;
+    DevBuf prog_, batch_prog_;
+    std::vector<VmOp> ops_;
 };
 
 class SetScalarTool : public VmTool {
@@ -849,6 +870,7 @@ class SetScalarTool : public VmTool {
         p.end();
         upload(p);
     }
+    int batch_kind() const override { return 3; }
 
   private:
     ToolDesc d_;

@@ -178,11 +178,33 @@ __global__ void __launch_bounds__(SC_BS)
     }
 }
 
+// single workgroup: exclusive scan of a short array in chunks of 1024
+// (in == out allowed: a chunk is read before any of it is written)
+__global__ void __launch_bounds__(1024)
+    scan_one(const uint32_t* in, uint32_t* out, uint32_t m)
+{
+    __shared__ uint32_t ws[16];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < m; b += 1024) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = (i < m) ? in[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, ws, &tot);
+        if (i < m) out[i] = carry + ex;
+        carry += tot;
+    }
+}
+
 size_t scan_workspace(uint32_t m) { return (size_t)blocks_for(m, SC_TILE) * 4; }
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m,
                     uint32_t* sums, hipStream_t s)
 {
+    if (m <= 16384) {
+        // one launch for short arrays (the blocked cell order's counts)
+        if (m) hipLaunchKernelGGL(scan_one, dim3(1), dim3(1024), 0, s, in, out, m);
+        return;
+    }
     const unsigned nb = blocks_for(m, SC_TILE);
     hipLaunchKernelGGL(scan_reduce, dim3(nb), dim3(SC_BS), 0, s, in, m, sums);
     hipLaunchKernelGGL(scan_top, dim3(1), dim3(1024), 0, s, sums, nb);

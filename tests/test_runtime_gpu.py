@@ -1006,10 +1006,11 @@ def test_reference_radixsort_case(cuda, tmp_path):
 
 
 def test_batched_set_and_copy_runs(cuda, tmp_path):
-    """Consecutive set tools, and consecutive copy tools, run as one launch
-    each; a run ends where a tool meets an array the run writes (a copy of
-    a copy, a second set of one array), so every result is the one the
-    tools give in order."""
+    """Consecutive set tools, consecutive copy tools and consecutive
+    set_scalar tools (dummies between them aside) run as one launch each;
+    a run ends where a tool meets an array the run writes (a copy of a
+    copy, a second set of one array), so every result is the one the tools
+    give in order."""
     d = np.loadtxt(os.path.join(GOLDEN, "radixsort_3d_particles.dat"))
     xml = _fixture_case(tmp_path, """
     <Variables>
@@ -1032,6 +1033,9 @@ def test_batched_set_and_copy_runs(cuda, tmp_path):
         <Tool action="add" name="set r2" type="set" in="r2" value="VEC_ONE" />
         <Tool action="add" name="copy r1 r2" type="copy" in="r1" out="r2" />
         <Tool action="add" name="iter += 1" type="set_scalar" in="iter" value="iter + 1"/>
+        <Tool action="add" name="marker" type="dummy"/>
+        <Tool action="add" name="h = 2 h" type="set_scalar" in="h" value="2 * h"/>
+        <Tool action="add" name="h = h + iter" type="set_scalar" in="h" value="h + iter"/>
     </Tools>""", 3, d, "r, f")
     sim = Simulation(xml, 3)
     for graph in (False, True):
@@ -1043,16 +1047,21 @@ def test_batched_set_and_copy_runs(cuda, tmp_path):
         np.testing.assert_array_equal(sim.get("r2"), sim.get("r"))
         assert np.all(sim.get("c") == np.float32(7.5))
         assert np.all(sim.get("e") == 0.0)
+    # three set_scalar tools (one run, a dummy between), each reading what
+    # the one before stored: h = 2 (2 0.1 + 1) + 2 after two steps
+    assert int(sim.get("iter")) == 2
+    np.testing.assert_allclose(float(sim.get("h")), 2 * (2 * 0.1 + 1) + 2,
+                               rtol=1e-6)
     sim.close()
 
 
 @pytest.mark.parametrize("dims", [2, 3])
 def test_batched_launches_are_bit_identical(cuda, tmp_path, dims,
                                             monkeypatch):
-    """The P-BI dam break with its set and copy runs batched and the sort
-    stages' gathers in one pass each gives the state of one launch per tool
-    and field (AQ_NO_BATCH=1) bit for bit, the sort's backup and unsorted
-    fields included."""
+    """The P-BI dam break with its set, copy and set_scalar runs batched and
+    the sort stages' gathers in one pass each gives the state of one launch
+    per tool and field (AQ_NO_BATCH=1) bit for bit, the sort's backup and
+    unsorted fields and the time scalars included."""
     pos, nrm = cases.dam_break_sensors(dims)
     case = cases.with_sensors(
         cases.dam_break(dims, 0.3 / (40 if dims == 2 else 12)), pos, nrm)
@@ -1068,7 +1077,7 @@ def test_batched_launches_are_bit_identical(cuda, tmp_path, dims,
                   "tangent", "id_in", "iset_in", "imove_in", "normal_in",
                   "tangent_in", "m_in", "dudt", "dudt_in", "drhodt",
                   "drhodt_in", "shepard", "grad_p", "lap_u", "div_u",
-                  "lap_p"):
+                  "lap_p", "t", "dt", "iter"):
             np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
     a.close()
     b.close()
