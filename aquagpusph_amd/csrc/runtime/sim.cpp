@@ -309,6 +309,7 @@ void Sim::setup()
         AQ_HIP(hipMalloc(&sel_count, sizeof(uint32_t)));
     }
     plan_fusion();
+    plan_tail();
     plan_batches();
     // the compensated sums live in the fused cell walk only
     if (problem.is_defined("__KAHAN_SUMS__") &&
@@ -596,6 +597,17 @@ void Sim::plan_fusion()
 
 }  // namespace rt
 
+void launch_step_tail(int dims, bool apply, bool delta_sph,
+                      const uint32_t* iset, const int32_t* imove,
+                      const float* shepard, float* grad_p, float* lap_u,
+                      float* div_u, float* dudt, float* drhodt,
+                      const float* visc_dyn, const float* g,
+                      const float* lap_p, const float* refd,
+                      const float* delta, float* r, float* u, float* rho,
+                      const float* dudt_in, const float* drhodt_in,
+                      uint32_t N, const float* dt, const uint32_t* status,
+                      hipStream_t s);
+
 void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
                        const float* r, const float* u, const float* rho,
                        const float* m, const float* p, const float* normal,
@@ -679,6 +691,72 @@ PartWalk Sim::part_walk(const Var* ic, const Var* ih, const Var* nc)
         pw.flagged = clist.kinds ? 1u : 0u;
     }
     return pw;
+}
+
+// The step's per-particle tail in one kernel (launch_step_tail): cfd Rates
+// and the improved-Euler corrector, with BI Shepard apply before them and
+// the delta-SPH rate between them when present, each tool once, in that
+// order, with only dummy tools between (so no other tool reads or writes
+// their fields in between).  AQ_NO_FUSION turns it off.
+void Sim::plan_tail()
+{
+    tail = TailPlan();
+    const char* env = getenv("AQ_NO_FUSION");
+    if (env && *env && strcmp(env, "0") != 0) return;
+    auto find = [&](const std::string& k) -> int {
+        int at = -1;
+        for (size_t i = 0; i < tools.size(); i++)
+            if (tools[i]->key() == k) {
+                if (at >= 0) return -2;
+                at = (int)i;
+            }
+        return at;
+    };
+    const int iA = find("cfd/Boundary/BI/Shepard.cl:apply");
+    const int iR = find("cfd/Rates.cl:entry");
+    const int iD = find("cfd/deltaSPH.cl:deltaSPH");
+    const int iC = find("basic/time_scheme/improved_euler.cl:corrector");
+    if (iR < 0 || iC < 0 || iA == -2 || iD == -2) return;
+    std::vector<int> chain;
+    if (iA >= 0) chain.push_back(iA);
+    chain.push_back(iR);
+    if (iD >= 0) chain.push_back(iD);
+    chain.push_back(iC);
+    for (size_t k = 0; k < chain.size(); k++) {
+        if (tools[chain[k]]->once()) return;
+        if (k == 0) continue;
+        if (chain[k] <= chain[k - 1]) return;
+        for (int t = chain[k - 1] + 1; t < chain[k]; t++)
+            if (tools[t]->type() != "dummy") return;
+    }
+    for (const char* n : { "iset", "imove", "grad_p", "lap_u", "div_u",
+                           "dudt", "drhodt", "visc_dyn", "g", "r", "u", "rho",
+                           "dudt_in", "drhodt_in", "dt" })
+        if (!vars.get(n)) return;
+    if (iA >= 0 && !vars.get("shepard")) return;
+    if (iD >= 0 && !(vars.get("lap_p") && vars.get("refd") &&
+                     vars.get("delta")))
+        return;
+    tail.apply = iA >= 0;
+    tail.delta = iD >= 0;
+    tail.anchor = tools[chain[0]].get();
+    for (size_t k = 1; k < chain.size(); k++)
+        tail.absorbed.insert(tools[chain[k]].get());
+}
+
+void Sim::run_tail(hipStream_t s)
+{
+    auto F = [&](const char* n) { return (float*)ptr(vars.get(n)); };
+    launch_step_tail(dims, tail.apply, tail.delta,
+                     (const uint32_t*)ptr(vars.get("iset")),
+                     (const int32_t*)ptr(vars.get("imove")),
+                     tail.apply ? F("shepard") : nullptr, F("grad_p"),
+                     F("lap_u"), F("div_u"), F("dudt"), F("drhodt"),
+                     F("visc_dyn"), F("g"), tail.delta ? F("lap_p") : nullptr,
+                     tail.delta ? F("refd") : nullptr,
+                     tail.delta ? F("delta") : nullptr, F("r"), F("u"),
+                     F("rho"), F("dudt_in"), F("drhodt_in"), N, F("dt"),
+                     d_status, s);
 }
 
 void Sim::run_fused(hipStream_t s)

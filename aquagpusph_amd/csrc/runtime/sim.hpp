@@ -384,6 +384,14 @@ class Sim {
     void save_now();
     bool is_setup = false;
     FusionPlan fusion;
+    // The per-particle tail of the step (BI Shepard apply, cfd Rates, the
+    // delta-SPH rate, the improved-Euler corrector) in one kernel when
+    // those tools follow one another with only dummy tools between
+    struct TailPlan {
+        const Tool* anchor = nullptr;
+        std::set<const Tool*> absorbed;
+        bool apply = false, delta = false;
+    } tail;
     CellList clist;
     bool bi_cells = false;  // boundary-integral element kernels cell-parallel
     bool bi_group = false;  // ... over the owner-cell list
@@ -399,6 +407,8 @@ class Sim {
     uint32_t* sel_list = nullptr;
     uint32_t* sel_count = nullptr;
     void run_fused(hipStream_t s);
+    void plan_tail();
+    void run_tail(hipStream_t s);
     // the main stream waits for the side chain (no-op if none in flight)
     void join_side(hipStream_t s);
     CellArgs cell_args();

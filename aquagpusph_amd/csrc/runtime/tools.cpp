@@ -415,6 +415,11 @@ class KernelTool : public Tool {
     }
     void execute(Sim& S, hipStream_t s) override
     {
+        if (S.tail.anchor == this) {
+            S.run_tail(s);
+            return;
+        }
+        if (S.tail.absorbed.count(this)) return;
         if (S.fusion.active) {
             if (S.fusion.anchor == this) {
                 S.run_fused(s);

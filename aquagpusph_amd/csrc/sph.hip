@@ -643,6 +643,107 @@ __global__ void bi_shepard_apply_k(const int* __restrict__ imove,
     div_u[i] /= s;
 }
 
+// The per-particle tail of the step in one pass: BI Shepard apply
+// (APPLY), cfd Rates, the delta-SPH rate (DELTA) and the improved-Euler
+// corrector, which run one after the other and each touch their own
+// particle only.  Each part is the expression of its own kernel above
+// (bi_shepard_apply_k, rates_k, delta_rate_k, corrector_k), in the same
+// order, on the values the part before produced (kept in registers instead
+// of read back), so the results are those of the four launches bit for bit.
+template <int D, bool APPLY, bool DELTA>
+__global__ void step_tail_k(const uint32_t* __restrict__ iset,
+                            const int* __restrict__ imove,
+                            const float* __restrict__ shepard,
+                            float* __restrict__ grad_p,
+                            float* __restrict__ lap_u,
+                            float* __restrict__ div_u,
+                            float* __restrict__ dudt,
+                            float* __restrict__ drhodt,
+                            const float* __restrict__ visc_dyn,
+                            const float* g, const float* __restrict__ lap_p,
+                            const float* __restrict__ refd,
+                            const float* __restrict__ delta,
+                            float* __restrict__ r, float* __restrict__ u,
+                            float* __restrict__ rho,
+                            const float* __restrict__ dudt_in,
+                            const float* __restrict__ drhodt_in, uint32_t N,
+                            const float* dtp, const uint32_t* status)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    // every operand is loaded up front (one round trip; the rows of the
+    // particles a part skips are read and left unused)
+    const int im = imove[i];
+    const bool stopped = status && *status;
+    const uint32_t st = iset[i];
+    float gp[VS], lu[VS], du_st[VS], du_in[VS], uu[VS], rr[VS];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        gp[k] = grad_p[a];
+        lu[k] = lap_u[a];
+        du_st[k] = dudt[a];
+        du_in[k] = dudt_in[a];
+        uu[k] = u[a];
+        rr[k] = r[a];
+    }
+    float dv = div_u[i];
+    const float sh = APPLY ? shepard[i] : 1.f;
+    const float lp = DELTA ? lap_p[i] : 0.f;
+    const float rho_i = rho[i];
+    const float dr_st = drhodt[i];
+    const float dr_in = drhodt_in[i];
+    const float dt = *dtp;
+    const bool fluid = im == 1;
+    float du[VS];
+    float dr = dr_st;
+#pragma unroll
+    for (int k = 0; k < VS; k++) du[k] = du_st[k];
+    if (fluid) {
+        if (APPLY) {
+#pragma unroll
+            for (int k = 0; k < VS; k++) {
+                gp[k] /= sh;
+                grad_p[(size_t)i * VS + k] = gp[k];
+            }
+#pragma unroll
+            for (int k = 0; k < VS; k++) {
+                lu[k] /= sh;
+                lap_u[(size_t)i * VS + k] = lu[k];
+            }
+            dv /= sh;
+            div_u[i] = dv;
+        }
+        const float mu = visc_dyn[st];
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            du[k] = -gp[k] + mu * lu[k] + g[k];
+            dudt[(size_t)i * VS + k] = du[k];
+        }
+        dr = -dv;
+        if (DELTA) {
+            const float delta_f = delta[st] * dt * rho_i / refd[st];
+            dr += delta_f * lp;
+        }
+        drhodt[i] = dr;
+    }
+    // the corrector: a failure flagged earlier leaves the state as it was
+    if (stopped) return;
+    if (im > 0) {
+        const float DT = 0.5f * dt;
+        const float DT2 = DT * DT;
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            const size_t a = (size_t)i * VS + k;
+            const float da = du[k] - du_in[k];
+            u[a] = uu[k] + DT * da;
+            r[a] = rr[k] + DT2 * da;
+        }
+        rho[i] = rho_i + DT * (dr - dr_in);
+    }
+}
+
 // cfd/Boundary/BI/GradP.cl:55-83 (freeslip)
 template <int D>
 __global__ void bi_gradp_k(const uint32_t* __restrict__ iset,
@@ -1506,6 +1607,35 @@ void launch_sensors_renorm(int dims, const int32_t* imove,
 {
     AQ_DISPATCH(dims, sensors_renorm_k, AQ_GRID(N), imove, shepard, u, rho, p,
                 N);
+}
+void launch_step_tail(int dims, bool apply, bool delta_sph,
+                      const uint32_t* iset, const int32_t* imove,
+                      const float* shepard, float* grad_p, float* lap_u,
+                      float* div_u, float* dudt, float* drhodt,
+                      const float* visc_dyn, const float* g,
+                      const float* lap_p, const float* refd,
+                      const float* delta, float* r, float* u, float* rho,
+                      const float* dudt_in, const float* drhodt_in,
+                      uint32_t N, const float* dt, const uint32_t* status,
+                      hipStream_t s)
+{
+#define AQ_TAIL(DD, AP, DL)                                                    \
+    hipLaunchKernelGGL((step_tail_k<DD, AP, DL>), AQ_GRID(N), iset, imove,    \
+                       shepard, grad_p, lap_u, div_u, dudt, drhodt, visc_dyn,  \
+                       g, lap_p, refd, delta, r, u, rho, dudt_in, drhodt_in,   \
+                       N, dt, status)
+    if (dims == 2) {
+        if (apply && delta_sph) AQ_TAIL(2, true, true);
+        else if (apply) AQ_TAIL(2, true, false);
+        else if (delta_sph) AQ_TAIL(2, false, true);
+        else AQ_TAIL(2, false, false);
+    } else {
+        if (apply && delta_sph) AQ_TAIL(3, true, true);
+        else if (apply) AQ_TAIL(3, true, false);
+        else if (delta_sph) AQ_TAIL(3, false, true);
+        else AQ_TAIL(3, false, false);
+    }
+#undef AQ_TAIL
 }
 void launch_bi_shepard_apply(int dims, const int32_t* imove,
                              const float* shepard, float* grad_p,
