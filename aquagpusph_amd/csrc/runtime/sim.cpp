@@ -608,6 +608,13 @@ void launch_step_tail(int dims, bool apply, bool delta_sph,
                       uint32_t N, const float* dt, const uint32_t* status,
                       hipStream_t s);
 
+void launch_step_head(int dims, int32_t* imove, const float* r,
+                      const float* u, const float* dudt, const float* rho,
+                      const float* drhodt, float* r_in, float* u_in,
+                      float* dudt_in, float* rho_in, float* drhodt_in,
+                      float* m, uint32_t N, const float* dt,
+                      const uint32_t* status, const float* dmin,
+                      const float* dmax, hipStream_t s);
 void launch_fluid_pass(int dims, int shep, bool lapp, const int32_t* imove,
                        const float* r, const float* u, const float* rho,
                        const float* m, const float* p, const float* normal,
@@ -701,8 +708,37 @@ PartWalk Sim::part_walk(const Var* ic, const Var* ih, const Var* nc)
 void Sim::plan_tail()
 {
     tail = TailPlan();
+    head = TailPlan();
     const char* env = getenv("AQ_NO_FUSION");
     if (env && *env && strcmp(env, "0") != 0) return;
+    {
+        // the head: predictor, then Domain, with only dummies between
+        int iP = -1, iM = -1, nP = 0, nM = 0;
+        for (size_t i = 0; i < tools.size(); i++) {
+            const std::string k = tools[i]->key();
+            if (k == "basic/time_scheme/improved_euler.cl:predictor") {
+                iP = (int)i;
+                nP++;
+            }
+            if (k == "basic/Domain.cl:entry") {
+                iM = (int)i;
+                nM++;
+            }
+        }
+        bool ok = nP == 1 && nM == 1 && iP < iM && !tools[iP]->once() &&
+                  !tools[iM]->once();
+        for (int t = iP + 1; ok && t < iM; t++)
+            ok = tools[t]->type() == "dummy";
+        for (const char* n : { "imove", "r", "u", "dudt", "rho", "drhodt",
+                               "r_in", "u_in", "dudt_in", "rho_in",
+                               "drhodt_in", "dt", "m", "domain_min",
+                               "domain_max" })
+            ok = ok && vars.get(n);
+        if (ok) {
+            head.anchor = tools[iP].get();
+            head.absorbed.insert(tools[iM].get());
+        }
+    }
     auto find = [&](const std::string& k) -> int {
         int at = -1;
         for (size_t i = 0; i < tools.size(); i++)
@@ -742,6 +778,15 @@ void Sim::plan_tail()
     tail.anchor = tools[chain[0]].get();
     for (size_t k = 1; k < chain.size(); k++)
         tail.absorbed.insert(tools[chain[k]].get());
+}
+
+void Sim::run_head(hipStream_t s)
+{
+    auto F = [&](const char* n) { return (float*)ptr(vars.get(n)); };
+    launch_step_head(dims, (int32_t*)ptr(vars.get("imove")), F("r"), F("u"),
+                     F("dudt"), F("rho"), F("drhodt"), F("r_in"), F("u_in"),
+                     F("dudt_in"), F("rho_in"), F("drhodt_in"), F("m"), N,
+                     F("dt"), d_status, F("domain_min"), F("domain_max"), s);
 }
 
 void Sim::run_tail(hipStream_t s)

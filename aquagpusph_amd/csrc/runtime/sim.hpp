@@ -392,6 +392,9 @@ class Sim {
         std::set<const Tool*> absorbed;
         bool apply = false, delta = false;
     } tail;
+    // The head of the step (improved-Euler predictor, then Domain) in one
+    // kernel, on the same terms
+    TailPlan head;
     CellList clist;
     bool bi_cells = false;  // boundary-integral element kernels cell-parallel
     bool bi_group = false;  // ... over the owner-cell list
@@ -409,6 +412,7 @@ class Sim {
     void run_fused(hipStream_t s);
     void plan_tail();
     void run_tail(hipStream_t s);
+    void run_head(hipStream_t s);
     // the main stream waits for the side chain (no-op if none in flight)
     void join_side(hipStream_t s);
     CellArgs cell_args();

@@ -419,7 +419,12 @@ class KernelTool : public Tool {
             S.run_tail(s);
             return;
         }
-        if (S.tail.absorbed.count(this)) return;
+        if (S.head.anchor == this) {
+            S.run_head(s);
+            return;
+        }
+        if (S.tail.absorbed.count(this) || S.head.absorbed.count(this))
+            return;
         if (S.fusion.active) {
             if (S.fusion.anchor == this) {
                 S.run_fused(s);

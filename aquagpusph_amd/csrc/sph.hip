@@ -359,6 +359,75 @@ __global__ void domain_k(int* __restrict__ imove, float* __restrict__ r_in,
     }
 }
 
+// The head of the step in one pass: the improved-Euler predictor, then
+// basic/Domain.cl on the positions it predicts.  Each part is the
+// expression of its own kernel (predictor_k, domain_k) in the same order;
+// a particle the domain check sends out gets the domain kernel's values
+// in place of the predicted ones, as it would overwrite them, so the
+// results are those of the two launches bit for bit.
+template <int D>
+__global__ void step_head_k(int* __restrict__ imove,
+                            const float* __restrict__ r,
+                            const float* __restrict__ u,
+                            const float* __restrict__ dudt,
+                            const float* __restrict__ rho,
+                            const float* __restrict__ drhodt,
+                            float* __restrict__ r_in, float* __restrict__ u_in,
+                            float* __restrict__ dudt_in,
+                            float* __restrict__ rho_in,
+                            float* __restrict__ drhodt_in,
+                            float* __restrict__ m, uint32_t N,
+                            const float* dtp, const uint32_t* status,
+                            const float* dmin, const float* dmax)
+{
+    constexpr int VS = (D == 2) ? 2 : 4;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int im = imove[i];
+    float DT = *dtp;
+    // a failure flagged earlier: predict no motion (the state is kept)
+    if ((im <= 0) || (status && *status)) DT = 0.f;
+    const float hdt2 = 0.5f * DT * DT;
+    float ai[VS], ui[VS], ri[VS];
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        const float a_ = dudt[a];
+        const float u_ = u[a];
+        ai[k] = a_;
+        ui[k] = u_ + DT * a_;
+        ri[k] = r[a] + DT * u_ + hdt2 * a_;
+    }
+    const float dr_ = drhodt[i];
+    drhodt_in[i] = dr_;
+    rho_in[i] = rho[i] + DT * dr_;
+    bool out = false;
+    if (im > -255) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const float c = ri[k];
+            if (isnan(c) || isinf(c) || c < dmin[k] || c > dmax[k]) out = true;
+        }
+    }
+    if (out) {
+        imove[i] = -256;
+        m[i] = 0.f;
+#pragma unroll
+        for (int k = 0; k < VS; k++) {
+            ui[k] = 0.f;
+            ai[k] = 0.f;
+            ri[k] = dmax[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < VS; k++) {
+        const size_t a = (size_t)i * VS + k;
+        dudt_in[a] = ai[k];
+        u_in[a] = ui[k];
+        r_in[a] = ri[k];
+    }
+}
+
 // basic/EOS.cl:57-73
 __global__ void eos_k(const uint32_t* __restrict__ iset,
                       const int* __restrict__ imove,
@@ -1468,6 +1537,18 @@ void launch_euler_corrector(int dims, const int32_t* imove, float* r,
 {
     AQ_DISPATCH(dims, euler_corrector_k, AQ_GRID(N), imove, r, u, dudt, rho,
                 drhodt, N, dt);
+}
+void launch_step_head(int dims, int32_t* imove, const float* r,
+                      const float* u, const float* dudt, const float* rho,
+                      const float* drhodt, float* r_in, float* u_in,
+                      float* dudt_in, float* rho_in, float* drhodt_in,
+                      float* m, uint32_t N, const float* dt,
+                      const uint32_t* status, const float* dmin,
+                      const float* dmax, hipStream_t s)
+{
+    AQ_DISPATCH(dims, step_head_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
+                r_in, u_in, dudt_in, rho_in, drhodt_in, m, N, dt, status,
+                dmin, dmax);
 }
 void launch_domain(int dims, int32_t* imove, float* r_in, float* u_in,
                    float* dudt_in, float* m, uint32_t N, const float* dmin,

@@ -928,6 +928,34 @@ def _dense_case(dims, n, seed):
                 params=params)
 
 
+@pytest.mark.parametrize("dims,n", [(2, 4000), (3, 12000)])
+def test_step_head_and_tail_passes_are_bit_identical(cuda, tmp_path, dims, n,
+                                                     monkeypatch):
+    """The predictor + Domain pass and the Rates + delta-SPH rate +
+    corrector pass give the state of their separate launches
+    (AQ_NO_FUSION=1) bit for bit, with particles leaving the domain (their
+    imove, mass and backed-up fields set by the Domain part)."""
+    case = _dense_case(dims, n, 5)
+    case["params"]["domain_max"] = [0.97] * dims
+    presets = tuple(p for p in casefiles.PBI_PRESETS
+                    if not p.endswith("BI.xml"))
+    xml = casefiles.write_case(case, str(tmp_path), presets=presets)
+    monkeypatch.setenv("AQ_NO_FUSION", "1")
+    a = Simulation(xml, dims)
+    monkeypatch.setenv("AQ_NO_FUSION", "0")
+    b = Simulation(xml, dims)
+    for graph in (False, True):
+        a.step(2)
+        b.step(2, graph=graph)
+        for f in ("imove", "m", "r", "u", "rho", "r_in", "u_in", "dudt_in",
+                  "rho_in", "drhodt_in", "dudt", "drhodt", "grad_p",
+                  "lap_u", "div_u", "lap_p"):
+            np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
+    assert np.count_nonzero(b.get("imove") == -256) > 0
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("dims,n", [(2, 6000), (3, 20000)])
 def test_cell_walk_dense_cells(cuda, tmp_path, dims, n, monkeypatch):
     """Cells holding more particles than a wavefront and neighbourhoods of
