@@ -608,6 +608,18 @@ void launch_step_tail(int dims, bool apply, bool delta_sph,
                       uint32_t N, const float* dt, const uint32_t* status,
                       hipStream_t s);
 
+void launch_sort_chain(int dims, const uint32_t* perm, uint32_t N,
+                       const uint32_t* id_in, uint32_t* id,
+                       const uint32_t* iset_in, uint32_t* iset,
+                       const int* imove_in, int* imove, const float* r_in,
+                       float* r, const float* normal_in, float* normal,
+                       const float* tangent_in, float* tangent,
+                       const float* rho_in, float* rho, const float* m_in,
+                       float* m, const float* u_in, float* u,
+                       const float* dudt, float* dudt_in,
+                       const float* drhodt, float* drhodt_in, float* p,
+                       const float* refd, const float* cs, const float* p0,
+                       float* binormal, hipStream_t s);
 void launch_step_head(int dims, int32_t* imove, const float* r,
                       const float* u, const float* dudt, const float* rho,
                       const float* drhodt, float* r_in, float* u_in,
@@ -709,8 +721,50 @@ void Sim::plan_tail()
 {
     tail = TailPlan();
     head = TailPlan();
+    sortc = TailPlan();
     const char* env = getenv("AQ_NO_FUSION");
     if (env && *env && strcmp(env, "0") != 0) return;
+    {
+        // the reorder chain: Sort stage 1, stage 2, EOS, Binormal, each
+        // once, in that order, with only dummies between
+        const char* keys[4] = { "basic/Sort.cl:stage1", "basic/Sort.cl:stage2",
+                                "basic/EOS.cl:entry",
+                                "basic/Binormal.cl:entry" };
+        int at[4] = { -1, -1, -1, -1 }, cnt[4] = { 0, 0, 0, 0 };
+        for (size_t i = 0; i < tools.size(); i++)
+            for (int k = 0; k < 4; k++)
+                if (tools[i]->key() == keys[k]) {
+                    at[k] = (int)i;
+                    cnt[k]++;
+                }
+        bool ok = true;
+        for (int k = 0; k < 4 && ok; k++) {
+            ok = cnt[k] == 1 && !tools[at[k]]->once() &&
+                 (k == 0 || at[k] > at[k - 1]);
+            for (int t = k ? at[k - 1] + 1 : at[k]; ok && k && t < at[k]; t++)
+                ok = tools[t]->type() == "dummy";
+        }
+        const size_t vb = (dims == 2 ? 2 : 4) * 4;
+        const std::pair<const char*, size_t> need[] = {
+            { "id_unsorted", 4 }, { "id_in", 4 }, { "id", 4 },
+            { "iset_in", 4 }, { "iset", 4 }, { "imove_in", 4 },
+            { "imove", 4 }, { "r_in", vb }, { "r", vb },
+            { "normal_in", vb }, { "normal", vb }, { "tangent_in", vb },
+            { "tangent", vb }, { "rho_in", 4 }, { "rho", 4 }, { "m_in", 4 },
+            { "m", 4 }, { "u_in", vb }, { "u", vb }, { "dudt", vb },
+            { "dudt_in", vb }, { "drhodt", 4 }, { "drhodt_in", 4 },
+            { "p", 4 }, { "binormal", vb } };
+        for (auto& nv : need) {
+            Var* v = vars.get(nv.first);
+            ok = ok && v && v->is_array() && v->length >= N &&
+                 v->type.elem_bytes() == nv.second;
+        }
+        for (const char* n : { "refd", "cs", "p0" }) ok = ok && vars.get(n);
+        if (ok) {
+            sortc.anchor = tools[at[0]].get();
+            for (int k = 1; k < 4; k++) sortc.absorbed.insert(tools[at[k]].get());
+        }
+    }
     {
         // the head: predictor, then Domain, with only dummies between
         int iP = -1, iM = -1, nP = 0, nM = 0;
@@ -778,6 +832,19 @@ void Sim::plan_tail()
     tail.anchor = tools[chain[0]].get();
     for (size_t k = 1; k < chain.size(); k++)
         tail.absorbed.insert(tools[chain[k]].get());
+}
+
+void Sim::run_sort_chain(hipStream_t s)
+{
+    auto F = [&](const char* n) { return (float*)ptr(vars.get(n)); };
+    auto U = [&](const char* n) { return (uint32_t*)ptr(vars.get(n)); };
+    launch_sort_chain(dims, U("id_unsorted"), N, U("id_in"), U("id"),
+                      U("iset_in"), U("iset"), (const int*)U("imove_in"),
+                      (int*)U("imove"), F("r_in"), F("r"), F("normal_in"),
+                      F("normal"), F("tangent_in"), F("tangent"), F("rho_in"),
+                      F("rho"), F("m_in"), F("m"), F("u_in"), F("u"),
+                      F("dudt"), F("dudt_in"), F("drhodt"), F("drhodt_in"),
+                      F("p"), F("refd"), F("cs"), F("p0"), F("binormal"), s);
 }
 
 void Sim::run_head(hipStream_t s)

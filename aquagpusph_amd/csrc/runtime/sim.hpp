@@ -395,6 +395,8 @@ class Sim {
     // The head of the step (improved-Euler predictor, then Domain) in one
     // kernel, on the same terms
     TailPlan head;
+    // The reorder (Sort stage 1 and 2), EOS and Binormal in one kernel
+    TailPlan sortc;
     CellList clist;
     bool bi_cells = false;  // boundary-integral element kernels cell-parallel
     bool bi_group = false;  // ... over the owner-cell list
@@ -413,6 +415,7 @@ class Sim {
     void plan_tail();
     void run_tail(hipStream_t s);
     void run_head(hipStream_t s);
+    void run_sort_chain(hipStream_t s);
     // the main stream waits for the side chain (no-op if none in flight)
     void join_side(hipStream_t s);
     CellArgs cell_args();

@@ -423,7 +423,12 @@ class KernelTool : public Tool {
             S.run_head(s);
             return;
         }
-        if (S.tail.absorbed.count(this) || S.head.absorbed.count(this))
+        if (S.sortc.anchor == this) {
+            S.run_sort_chain(s);
+            return;
+        }
+        if (S.tail.absorbed.count(this) || S.head.absorbed.count(this) ||
+            S.sortc.absorbed.count(this))
             return;
         if (S.fusion.active) {
             if (S.fusion.anchor == this) {

@@ -18,6 +18,8 @@
 #include <unordered_map>
 
 #include "aq_device.hpp"
+#include <type_traits>
+
 #include "aq_internal.hpp"
 
 namespace aq {
@@ -425,6 +427,69 @@ __global__ void step_head_k(int* __restrict__ imove,
         dudt_in[a] = ai[k];
         u_in[a] = ui[k];
         r_in[a] = ri[k];
+    }
+}
+
+// The reorder and what follows it in one pass: basic/Sort.cl stage 1 and
+// stage 2 (gathers by id_unsorted: id, iset, imove, r, normal, tangent;
+// rho, m, u, and dudt, drhodt into their _in copies), then basic/EOS.cl and
+// basic/Binormal.cl on the gathered values.  Each part is the expression
+// of its own kernel (permute_k, eos_k, binormal_k) on the values the
+// gathers left in registers, so the results are those of the separate
+// launches bit for bit.
+template <int D>
+__global__ void sort_chain_k(
+    const uint32_t* __restrict__ perm, uint32_t N,
+    const uint32_t* __restrict__ id_in, uint32_t* __restrict__ id,
+    const uint32_t* __restrict__ iset_in, uint32_t* __restrict__ iset,
+    const int* __restrict__ imove_in, int* __restrict__ imove,
+    const float* __restrict__ r_in, float* __restrict__ r,
+    const float* __restrict__ normal_in, float* __restrict__ normal,
+    const float* __restrict__ tangent_in, float* __restrict__ tangent,
+    const float* __restrict__ rho_in, float* __restrict__ rho,
+    const float* __restrict__ m_in, float* __restrict__ m,
+    const float* __restrict__ u_in, float* __restrict__ u,
+    const float* __restrict__ dudt, float* __restrict__ dudt_in,
+    const float* __restrict__ drhodt, float* __restrict__ drhodt_in,
+    float* __restrict__ p, const float* __restrict__ refd, const float* csp,
+    const float* p0p, float* __restrict__ binormal)
+{
+    using V = typename std::conditional<D == 2, float2, float4>::type;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t j = perm[i];
+    const uint32_t id_j = id_in[j], st = iset_in[j];
+    const int im = imove_in[j];
+    const V r_j = reinterpret_cast<const V*>(r_in)[j];
+    const V n_j = reinterpret_cast<const V*>(normal_in)[j];
+    const V t_j = reinterpret_cast<const V*>(tangent_in)[j];
+    const float rho_j = rho_in[j], m_j = m_in[j], dr_j = drhodt[j];
+    const V u_j = reinterpret_cast<const V*>(u_in)[j];
+    const V a_j = reinterpret_cast<const V*>(dudt)[j];
+    id[i] = id_j;
+    iset[i] = st;
+    imove[i] = im;
+    reinterpret_cast<V*>(r)[i] = r_j;
+    reinterpret_cast<V*>(normal)[i] = n_j;
+    rho[i] = rho_j;
+    m[i] = m_j;
+    reinterpret_cast<V*>(u)[i] = u_j;
+    reinterpret_cast<V*>(dudt_in)[i] = a_j;
+    drhodt_in[i] = dr_j;
+    // basic/EOS.cl
+    if (!((im <= 0) && (im != -1))) {
+        const float cs = *csp;
+        p[i] = (*p0p) + cs * cs * (rho_j - refd[st]);
+    }
+    // basic/Binormal.cl
+    if constexpr (D == 2) {
+        reinterpret_cast<float2*>(binormal)[i] = make_float2(0.f, 0.f);
+        reinterpret_cast<float2*>(tangent)[i] = make_float2(n_j.y, -n_j.x);
+    } else {
+        reinterpret_cast<float4*>(tangent)[i] = t_j;
+        reinterpret_cast<float4*>(binormal)[i] = make_float4(
+            n_j.y * t_j.z - n_j.z * t_j.y, n_j.z * t_j.x - n_j.x * t_j.z,
+            n_j.x * t_j.y - n_j.y * t_j.x, 0.f);
     }
 }
 
@@ -1549,6 +1614,24 @@ void launch_step_head(int dims, int32_t* imove, const float* r,
     AQ_DISPATCH(dims, step_head_k, AQ_GRID(N), imove, r, u, dudt, rho, drhodt,
                 r_in, u_in, dudt_in, rho_in, drhodt_in, m, N, dt, status,
                 dmin, dmax);
+}
+void launch_sort_chain(int dims, const uint32_t* perm, uint32_t N,
+                       const uint32_t* id_in, uint32_t* id,
+                       const uint32_t* iset_in, uint32_t* iset,
+                       const int* imove_in, int* imove, const float* r_in,
+                       float* r, const float* normal_in, float* normal,
+                       const float* tangent_in, float* tangent,
+                       const float* rho_in, float* rho, const float* m_in,
+                       float* m, const float* u_in, float* u,
+                       const float* dudt, float* dudt_in,
+                       const float* drhodt, float* drhodt_in, float* p,
+                       const float* refd, const float* cs, const float* p0,
+                       float* binormal, hipStream_t s)
+{
+    AQ_DISPATCH(dims, sort_chain_k, AQ_GRID(N), perm, N, id_in, id, iset_in,
+                iset, imove_in, imove, r_in, r, normal_in, normal, tangent_in,
+                tangent, rho_in, rho, m_in, m, u_in, u, dudt, dudt_in, drhodt,
+                drhodt_in, p, refd, cs, p0, binormal);
 }
 void launch_domain(int dims, int32_t* imove, float* r_in, float* u_in,
                    float* dudt_in, float* m, uint32_t N, const float* dmin,

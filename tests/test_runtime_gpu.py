@@ -852,7 +852,9 @@ def test_walks_are_bit_identical(cuda, tmp_path, dims, bi, walk, bi_walk,
         a.step(3)
         b.step(3, graph=True)
         for f in ("r", "u", "rho", "p", "shepard", "grad_p", "lap_u",
-                  "div_u", "lap_p", "dudt", "drhodt"):
+                  "div_u", "lap_p", "dudt", "drhodt", "id", "iset", "imove",
+                  "m", "normal", "tangent", "binormal", "dudt_in",
+                  "drhodt_in"):
             np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
 
 
@@ -931,10 +933,11 @@ def _dense_case(dims, n, seed):
 @pytest.mark.parametrize("dims,n", [(2, 4000), (3, 12000)])
 def test_step_head_and_tail_passes_are_bit_identical(cuda, tmp_path, dims, n,
                                                      monkeypatch):
-    """The predictor + Domain pass and the Rates + delta-SPH rate +
-    corrector pass give the state of their separate launches
-    (AQ_NO_FUSION=1) bit for bit, with particles leaving the domain (their
-    imove, mass and backed-up fields set by the Domain part)."""
+    """The predictor + Domain pass, the reorder + EOS + Binormal pass and
+    the Rates + delta-SPH rate + corrector pass give the state of their
+    separate launches (AQ_NO_FUSION=1) bit for bit, with particles leaving
+    the domain (their imove, mass and backed-up fields set by the Domain
+    part)."""
     case = _dense_case(dims, n, 5)
     case["params"]["domain_max"] = [0.97] * dims
     presets = tuple(p for p in casefiles.PBI_PRESETS
@@ -949,7 +952,8 @@ def test_step_head_and_tail_passes_are_bit_identical(cuda, tmp_path, dims, n,
         b.step(2, graph=graph)
         for f in ("imove", "m", "r", "u", "rho", "r_in", "u_in", "dudt_in",
                   "rho_in", "drhodt_in", "dudt", "drhodt", "grad_p",
-                  "lap_u", "div_u", "lap_p"):
+                  "lap_u", "div_u", "lap_p", "id", "iset", "p", "normal",
+                  "tangent", "binormal", "id_in", "imove_in", "m_in"):
             np.testing.assert_array_equal(a.get(f), b.get(f), err_msg=f)
     assert np.count_nonzero(b.get("imove") == -256) > 0
     a.close()
