@@ -202,6 +202,29 @@ def measured_traffic(kernel):
     return None
 
 
+def streaming_counters(dr):
+    """The committed HBM counters of the per-particle passes, the sort and
+    the link-list kernels on the 3D dam break at dr = 3 mm (rocprofv3 --pmc
+    FETCH_SIZE and WRITE_SIZE runs, copy-kernel scaled): {kernel: {MB per
+    launch, us, TB/s}}, or None at another size or without the summary."""
+    path = os.path.join(PROFILE_DIR, "streaming_kernels_dambreak3d_pmc.md")
+    if abs(dr - 0.003) > 1e-9 or not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) != 9 or not cells[0].startswith("`"):
+            continue
+        name = cells[0].strip("`").replace("void ", "")
+        try:
+            out[name] = {"traffic_mb": float(cells[4]),
+                         "avg_us": float(cells[6]),
+                         "tbps": float(cells[7])}
+        except ValueError:
+            continue
+    return out or None
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A plain device-to-device copy of known size, for scaling the
     FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
@@ -416,6 +439,11 @@ def main():
         "roofline": roof,
         "tools_ms": {n: round(m, 4) for n, _, m in top[:8]},
         "step_gpu_ms_sum": step_gpu_ms,
+        "hbm_counters": streaming_counters(a.dr),
+        "hbm_counters_source": "profiles/streaming_kernels_dambreak3d_pmc.md "
+                               "(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of "
+                               "each kernel on this workload, copy-kernel "
+                               "scaled; not measured in this run)",
     }
     sim.close()
     if not a.no_sizes:
