@@ -1597,7 +1597,7 @@ template <int D> struct PolBiShepard {
 // cfd/Boundary/BI/LapU.cl:58-123 (freeslip)
 template <int D> struct PolBiLapU {
     static constexpr int BATCH = 4;  // candidates loaded together
-    static constexpr bool STATIC_CELLS = false;  // coop walk hand-out
+    static constexpr bool STATIC_CELLS = true;  // coop walk hand-out
     static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
@@ -1669,7 +1669,7 @@ template <int D> struct PolBiLapU {
 // cfd/Boundary/BI/Interpolation.cl:54-107
 template <int D> struct PolBiInterp {
     static constexpr int BATCH = 4;  // candidates loaded together
-    static constexpr bool STATIC_CELLS = false;  // coop walk hand-out
+    static constexpr bool STATIC_CELLS = true;  // coop walk hand-out
     static constexpr int COOP_B = 2;  // per lane and round (coop walk)
     static constexpr int KIND = 0;
     static constexpr uint32_t OWNERS = AQ_CELL_ELEM;
