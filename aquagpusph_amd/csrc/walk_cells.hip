@@ -1057,6 +1057,13 @@ template <class Pol, bool NRQ> struct BatchPair {
     }
 };
 
+// Pol::STATIC_CELLS when the policy declares it (else false): static
+// hand-out of the listed cells in cell_kernel_k
+template <class P, class = void> struct StaticCells : std::false_type {};
+template <class P>
+struct StaticCells<P, std::void_t<decltype(P::STATIC_CELLS)>>
+    : std::integral_constant<bool, P::STATIC_CELLS> {};
+
 // PER: a periodic box; the policy's pair_p<NRQ>(i, j, shift, r_i, st) gets
 // the shift of j's neighbour cell.  NB > 1: batched consume (ring_walk).
 // olist: walk the cells it lists ((first particle, cell, cell_end)
@@ -1076,8 +1083,19 @@ __global__ void __launch_bounds__(64 * WC_WAVES)
     const uint32_t n_heads =
         olist ? uni(w.n_heads[count_word]) : uni(*w.n_heads);
     uint32_t range = blockIdx.x % w.ranges;
+    // static hand-out of a policy's listed cells (wave v takes v, v + W,
+    // ...; no atomic per cell)
+    const bool stat = StaticCells<Pol>::value && olist != nullptr;
+    const uint32_t nwv = gridDim.x * WC_WAVES;
+    uint32_t hs = blockIdx.x * WC_WAVES + (threadIdx.x >> 6);
     while (true) {
-        const uint32_t hk = next_cell(w, n_heads, lane, range);
+        uint32_t hk;
+        if (stat) {
+            hk = hs;
+            hs += nwv;
+        } else {
+            hk = next_cell(w, n_heads, lane, range);
+        }
         if (hk >= n_heads) break;
         uint32_t cstart, c_i, ce;
         if (olist) {
@@ -1723,6 +1741,8 @@ template <int D> struct PolBiInterp {
 // element inside the support (near flags of the fused walk) or all of them
 template <int D> struct PolBiInteractions {
     static constexpr int BATCH = 4;  // candidates loaded together
+    // the near cells cost about alike: static hand-out over the near list
+    static constexpr bool STATIC_CELLS = true;
     static constexpr int KIND = 1;
     static constexpr uint32_t OWNERS = AQ_CELL_FLUID;
     const float *r_, *normal, *u, *rho, *m, *p;
